@@ -1,0 +1,281 @@
+/*
+ * tblis.h -- C ABI of the MI355X tensor-contraction library (libtblis_amd.so).
+ *
+ * Drop-in for TBLIS 1.2.0's tensor-contraction entry point.  The types and
+ * functions in the first half keep the reference's names, layouts and
+ * argument meaning (citations are into the TBLIS source tree); the second
+ * half holds the extensions this library adds (DPD and batched C entry
+ * points, stream control, introspection for tests).
+ *
+ * Memory contract: tensor data pointers are device pointers (hipMalloc /
+ * torch CUDA tensors) or managed memory on the current HIP device.  Host
+ * pointers are rejected with an error.  Length, stride and label arrays are
+ * host memory, read during the call only.
+ */
+#ifndef TBLIS_AMD_TBLIS_H
+#define TBLIS_AMD_TBLIS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+#include <complex>
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Types (src/util/basic_types.h:53-77, 154-208, 471-545)                  */
+/* ---------------------------------------------------------------------- */
+
+typedef ptrdiff_t len_type;    /* TBLIS_LEN_TYPE default (configure.ac:91-123) */
+typedef ptrdiff_t stride_type; /* TBLIS_STRIDE_TYPE default */
+typedef char label_type;       /* TBLIS_LABEL_TYPE default */
+
+typedef enum
+{
+    TYPE_SINGLE   = 0,
+    TYPE_FLOAT    = TYPE_SINGLE,
+    TYPE_DOUBLE   = 1,
+    TYPE_SCOMPLEX = 2,
+    TYPE_DCOMPLEX = 3
+} type_t;
+
+#ifdef __cplusplus
+typedef std::complex<float> scomplex;
+typedef std::complex<double> dcomplex;
+#else
+typedef float _Complex scomplex;
+typedef double _Complex dcomplex;
+#endif
+
+/* 24 bytes: value union at 0, type at 16 (basic_types.h:154-208). */
+typedef struct tblis_scalar
+{
+    union
+    {
+        float s;
+        double d;
+        float c[2];
+        double z[2];
+    } data;
+    type_t type;
+} tblis_scalar;
+
+/* 64 bytes: type@0 conj@4 scalar@8 data@32 ndim@40 len@48 stride@56
+ * (basic_types.h:471-545). */
+typedef struct tblis_tensor
+{
+    type_t type;
+    int conj;
+    tblis_scalar scalar;
+    void* data;
+    unsigned ndim;
+    len_type* len;
+    stride_type* stride;
+} tblis_tensor;
+
+/* tci_comm, 24 bytes (src/external/tci/tci/communicator.h:8-15). */
+typedef struct tblis_comm
+{
+    void* context;
+    unsigned ngang;
+    unsigned gid;
+    unsigned nthread;
+    unsigned tid;
+} tblis_comm;
+
+/* Opaque (basic_types.h:51); here it selects the contraction algorithm. */
+typedef struct tblis_config_s tblis_config;
+
+/* Single-thread communicator (src/util/thread.h:14). */
+extern const tblis_comm* const tblis_single;
+
+/* ---------------------------------------------------------------------- */
+/* Descriptor setup (basic_types.h:549-579; basic_types.cxx:184-262)       */
+/* ---------------------------------------------------------------------- */
+
+void tblis_init_scalar_s(tblis_scalar* s, float value);
+void tblis_init_scalar_d(tblis_scalar* s, double value);
+void tblis_init_scalar_c(tblis_scalar* s, scomplex value);
+void tblis_init_scalar_z(tblis_scalar* s, dcomplex value);
+
+void tblis_init_tensor_scaled_s(tblis_tensor* t, float scalar, unsigned ndim, len_type* len,
+                                float* data, stride_type* stride);
+void tblis_init_tensor_scaled_d(tblis_tensor* t, double scalar, unsigned ndim, len_type* len,
+                                double* data, stride_type* stride);
+void tblis_init_tensor_scaled_c(tblis_tensor* t, scomplex scalar, unsigned ndim, len_type* len,
+                                scomplex* data, stride_type* stride);
+void tblis_init_tensor_scaled_z(tblis_tensor* t, dcomplex scalar, unsigned ndim, len_type* len,
+                                dcomplex* data, stride_type* stride);
+void tblis_init_tensor_s(tblis_tensor* t, unsigned ndim, len_type* len, float* data,
+                         stride_type* stride);
+void tblis_init_tensor_d(tblis_tensor* t, unsigned ndim, len_type* len, double* data,
+                         stride_type* stride);
+void tblis_init_tensor_c(tblis_tensor* t, unsigned ndim, len_type* len, scomplex* data,
+                         stride_type* stride);
+void tblis_init_tensor_z(tblis_tensor* t, unsigned ndim, len_type* len, dcomplex* data,
+                         stride_type* stride);
+
+/* ---------------------------------------------------------------------- */
+/* The contraction (src/iface/3t/mult.h:17-20, mult.cxx:24-165)             */
+/* ---------------------------------------------------------------------- */
+
+/* C <- alpha_A*alpha_B * A.B + alpha_C * C over label-matched indices.
+ * Repeated labels in one tensor take a diagonal, length-1 dimensions are
+ * ignored, labels present in one tensor only are an error.  beta == 0
+ * overwrites C without reading it; alpha == 0 reads neither A nor B.  After
+ * the call C->scalar is 1 and C->conj is 0.  comm may be NULL or any
+ * communicator (the GPU grid replaces the thread team).  cfg NULL selects
+ * the default algorithm for the type (FP64: FP64 MFMA; FP32: 3-term
+ * split-BF16 MFMA). */
+void tblis_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
+                       const tblis_tensor* A, const label_type* idx_A,
+                       const tblis_tensor* B, const label_type* idx_B,
+                       tblis_tensor* C, const label_type* idx_C);
+
+/* Thread-count controls (src/util/thread.h:21-23).  The GPU path does not
+ * use host threads; the value is stored and returned for compatibility. */
+unsigned tblis_get_num_threads(void);
+void tblis_set_num_threads(unsigned num_threads);
+
+/* Configuration lookup by name (src/util/configs.h:11).  Names:
+ *   "mi355x"        default: FP64 MFMA for double, split-BF16 x3 for float
+ *   "fp64_mfma"     float contractions widened to FP64 MFMA
+ *   "bf16x3"        3-term split-BF16 (6 products), FP32 accumulation
+ *   "bf16x3_fast"   3-term split-BF16 without the compensated k-blocking
+ *   "reference"     plain FMA kernel (no MFMA), for cross-checks
+ * Unknown names abort with a message, as in the reference. */
+const tblis_config* tblis_get_config(const char* name);
+
+/* ---------------------------------------------------------------------- */
+/* Extension: DPD (irrep block-sparse) contraction.                        */
+/* The reference exposes this only as the C++ template                     */
+/* mult<T>(alpha, dpd_varray_view A, idx_A, ..., beta, dpd_varray_view C)  */
+/* (src/iface/3t/mult.h:53-72, mult.cxx:170-290).  This C descriptor       */
+/* carries what a freshly laid out MArray dpd_varray holds: irrep,         */
+/* nirrep, per-irrep lengths [dim][irrep] and the layout code              */
+/* (src/external/marray/include/utility.hpp:100-122).                      */
+/* ---------------------------------------------------------------------- */
+
+typedef enum
+{
+    TBLIS_DPD_BALANCED_COLUMN_MAJOR = 0,
+    TBLIS_DPD_BALANCED_ROW_MAJOR = 1,
+    TBLIS_DPD_BLOCKED_COLUMN_MAJOR = 2,
+    TBLIS_DPD_BLOCKED_ROW_MAJOR = 3,
+    TBLIS_DPD_PREFIX_COLUMN_MAJOR = 4,
+    TBLIS_DPD_PREFIX_ROW_MAJOR = 5
+} tblis_dpd_layout;
+
+typedef struct tblis_dpd_tensor
+{
+    type_t type;
+    int conj;
+    tblis_scalar scalar;
+    void* data;
+    unsigned ndim;
+    unsigned irrep;
+    unsigned nirrep;      /* 1, 2, 4 or 8 */
+    int layout;           /* tblis_dpd_layout */
+    const len_type* len;  /* ndim x nirrep, row major: len[dim*nirrep + irrep] */
+} tblis_dpd_tensor;
+
+void tblis_init_dpd_tensor_d(tblis_dpd_tensor* t, double scalar, unsigned irrep, unsigned nirrep,
+                             unsigned ndim, const len_type* len, double* data, int layout);
+void tblis_init_dpd_tensor_s(tblis_dpd_tensor* t, float scalar, unsigned irrep, unsigned nirrep,
+                             unsigned ndim, const len_type* len, float* data, int layout);
+
+/* Number of stored elements of a DPD tensor (dpd_varray_base::size). */
+stride_type tblis_dpd_tensor_size(const tblis_dpd_tensor* t);
+
+/* DPD contraction; same semantics as the dense call, block by block.  All
+ * symmetry-allowed irrep blocks of every irrep_AB run in one persistent
+ * launch. */
+void tblis_dpd_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
+                           const tblis_dpd_tensor* A, const label_type* idx_A,
+                           const tblis_dpd_tensor* B, const label_type* idx_B,
+                           tblis_dpd_tensor* C, const label_type* idx_C);
+
+/* ---------------------------------------------------------------------- */
+/* Extension: a batch of independent contractions of uneven shapes, run   */
+/* in one persistent launch.  The reference has no counterpart; its        */
+/* equivalent is a host loop of tblis_tensor_mult calls.                   */
+/* ---------------------------------------------------------------------- */
+void tblis_tensor_mult_batch(const tblis_comm* comm, const tblis_config* cfg, len_type count,
+                             const tblis_tensor* const* A, const label_type* const* idx_A,
+                             const tblis_tensor* const* B, const label_type* const* idx_B,
+                             tblis_tensor* const* C, const label_type* const* idx_C);
+
+/* ---------------------------------------------------------------------- */
+/* Extension: HIP execution control and error reporting.                   */
+/* ---------------------------------------------------------------------- */
+
+/* Stream used by the calls of this host thread (NULL = default stream). */
+void tblis_hip_set_stream(void* hip_stream);
+void* tblis_hip_get_stream(void);
+/* 1: calls return after enqueueing; 0 (default): calls return when done. */
+void tblis_hip_set_async(int async);
+/* 0 (default): invalid arguments abort with a message, like the
+ * reference's debug build.  1: they set the error string and return. */
+void tblis_hip_set_error_mode(int return_on_error);
+/* Last error of this host thread ("" if none); cleared by the next call. */
+const char* tblis_hip_get_last_error(void);
+/* Drop all cached contraction plans (device buffers are freed). */
+void tblis_hip_clear_plan_cache(void);
+/* Algorithm selected by the last contraction on this thread (config name). */
+const char* tblis_hip_last_algorithm(void);
+
+/* Introspection (host only, no device work): the canonical form of a
+ * contraction as computed by this library's front end.
+ * out_groups receives, for AB, AC, BC, ABC in order: ndim, then the folded
+ * lengths and the per-tensor strides, as
+ *   [n, len[n], sA[n], sB[n]]       for AB
+ *   [n, len[n], sA[n], sC[n]]       for AC
+ *   [n, len[n], sB[n], sC[n]]       for BC
+ *   [n, len[n], sA[n], sB[n], sC[n]] for ABC
+ * followed by the m/n/k/l matrix views after the stride sort and pack_3d
+ * rotation ([n, len[n], s1[n], s2[n]] for m: A, C; n: B, C; k: A, B;
+ * l: [n, len, sA, sB, sC]) and the three pack_3d flags.  Returns the number
+ * of int64 written, or -1 for an invalid contraction. */
+int64_t tblis_hip_debug_canonical(const tblis_tensor* A, const label_type* idx_A,
+                                  const tblis_tensor* B, const label_type* idx_B,
+                                  const tblis_tensor* C, const label_type* idx_C,
+                                  int64_t* out, int64_t max_out);
+
+/* Block-scatter vector and block strides of one matrix dimension given as
+ * (len, stride) lists, at block size BS, rows [off, off+size), with the
+ * pack_3d cache-line order for element size elem_size (same meaning as the
+ * reference's block_scatter_matrix::fill_block_scatter). */
+void tblis_hip_debug_fill_block_scatter(int ndim, const len_type* len, const stride_type* stride,
+                                        len_type BS, len_type off, len_type size,
+                                        stride_type* scat, stride_type* bs, int pack_3d,
+                                        int elem_size);
+
+/* DPD block address (offset from the data pointer, lengths, strides) of the
+ * block with the given irreps, from this library's DPD layout code. */
+void tblis_hip_debug_dpd_block(const tblis_dpd_tensor* t, const unsigned* irreps,
+                               len_type* len, stride_type* offset, stride_type* stride);
+
+/* Patch list of the DPD GEMM of one irrep_AB, for matrix `which`
+ * (0: A, 1: B, 2: C), block sizes BS_row/BS_col: number of patches and, per
+ * patch (row-patch major), rows, cols, base offset and the concatenated
+ * row/col scatter vectors.  Returns the number of patches, or -1. */
+int tblis_hip_debug_dpd_patches(const tblis_dpd_tensor* A, const label_type* idx_A,
+                                const tblis_dpd_tensor* B, const label_type* idx_B,
+                                const tblis_dpd_tensor* C, const label_type* idx_C,
+                                int which, unsigned irrep_ab, len_type BS_row, len_type BS_col,
+                                int* npatch, len_type* rows, len_type* cols, stride_type* base,
+                                stride_type* rscat, stride_type* cscat, int64_t max_patch,
+                                int64_t max_scat);
+
+/* One 16x16x4 FP64 MFMA in the operand layout the GEMM kernels assume:
+ * A[i][k] = 1000*i + k + 1, B[k][j] = (k+1)*(j+1); writes the 16x16 result
+ * row-major to out_device (256 doubles, device memory) using the output
+ * lane layout the kernels assume.  Returns 0 on success. */
+int tblis_hip_debug_mfma_probe(double* out_device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

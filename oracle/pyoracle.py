@@ -1,0 +1,298 @@
+"""ctypes bindings for the oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: the checker the HIP library is compared against.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+MAXDIM = 16
+I64 = C.c_int64
+P64 = C.POINTER(C.c_int64)
+PD = C.POINTER(C.c_double)
+PF = C.POINTER(C.c_float)
+
+
+class Groups(C.Structure):
+    _fields_ = [("valid", C.c_int), ("n_ab", C.c_int), ("n_ac", C.c_int), ("n_bc", C.c_int),
+                ("n_abc", C.c_int)] + \
+        [(n, C.c_byte * MAXDIM) for n in ("idx_ab", "idx_ac", "idx_bc", "idx_abc")] + \
+        [(n, I64 * MAXDIM) for n in ("len_ab", "len_ac", "len_bc", "len_abc", "sA_ab", "sB_ab",
+                                     "sA_ac", "sC_ac", "sB_bc", "sC_bc", "sA_abc", "sB_abc",
+                                     "sC_abc")]
+
+    def as_dict(self):
+        d = {"valid": self.valid}
+        for g in ("ab", "ac", "bc", "abc"):
+            n = getattr(self, "n_" + g)
+            d["idx_" + g] = bytes(x & 0xFF for x in getattr(self, "idx_" + g)[:n]).decode("latin1")
+            d["len_" + g] = list(getattr(self, "len_" + g))[:n]
+        for f in ("sA_ab", "sB_ab", "sA_ac", "sC_ac", "sB_bc", "sC_bc", "sA_abc", "sB_abc",
+                  "sC_abc"):
+            g = f.split("_")[1]
+            d[f] = list(getattr(self, f))[:getattr(self, "n_" + g)]
+        return d
+
+
+class TMat(C.Structure):
+    _fields_ = [("nm", C.c_int), ("nn", C.c_int), ("nk", C.c_int), ("nl", C.c_int)] + \
+        [(n, I64 * MAXDIM) for n in ("len_m", "len_n", "len_k", "len_l", "sA_m", "sA_k", "sB_k",
+                                     "sB_n", "sC_m", "sC_n", "sA_l", "sB_l", "sC_l")] + \
+        [("pack_m_3d", C.c_int), ("pack_n_3d", C.c_int), ("pack_k_3d", C.c_int)]
+
+    def as_dict(self):
+        d = {}
+        for k, n in (("m", self.nm), ("n", self.nn), ("k", self.nk), ("l", self.nl)):
+            d["len_" + k] = list(getattr(self, "len_" + k))[:n]
+        for f, n in (("sA_m", self.nm), ("sA_k", self.nk), ("sB_k", self.nk), ("sB_n", self.nn),
+                     ("sC_m", self.nm), ("sC_n", self.nn), ("sA_l", self.nl), ("sB_l", self.nl),
+                     ("sC_l", self.nl)):
+            d[f] = list(getattr(self, f))[:n]
+        d["pack_3d"] = (self.pack_m_3d, self.pack_n_3d, self.pack_k_3d)
+        return d
+
+
+class Dpd(C.Structure):
+    _fields_ = [("ndim", C.c_int), ("nirrep", C.c_int), ("irrep", C.c_int), ("row_major", C.c_int),
+                ("len", (I64 * 8) * MAXDIM), ("ext", (I64 * 8) * MAXDIM),
+                ("leaf", C.c_int * MAXDIM), ("parent", C.c_int * (2 * MAXDIM)),
+                ("perm", C.c_int * MAXDIM), ("depth", C.c_int * MAXDIM),
+                ("size", (I64 * 8) * (2 * MAXDIM))]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(path)
+        L.orc_canonicalize.restype = C.c_int
+        L.orc_diagonal.restype = C.c_int
+        L.orc_fold.restype = C.c_int
+        L.orc_unit_dim.restype = C.c_int
+        L.orc_free_idx.restype = C.c_char
+        L.orc_gcd.restype = I64
+        L.orc_gcd.argtypes = [I64, I64]
+        L.orc_lcm.restype = I64
+        L.orc_lcm.argtypes = [I64, I64]
+        L.orc_dpd_size.restype = I64
+        L.orc_dpd_patch_scatter.restype = C.c_int
+        L.orc_blis_contract_d.restype = C.c_double
+        _LIB = L
+    return _LIB
+
+
+def _i64(a):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+    return a, a.ctypes.data_as(P64)
+
+
+def _lab(s):
+    return C.c_char_p(s.encode("latin1") if isinstance(s, str) else bytes(s))
+
+
+def canonicalize(lenA, strA, idxA, lenB, strB, idxB, lenC, strC, idxC):
+    g = Groups()
+    args = []
+    keep = []
+    for l, s, i in ((lenA, strA, idxA), (lenB, strB, idxB), (lenC, strC, idxC)):
+        la, lp = _i64(l)
+        sa, sp = _i64(s)
+        keep += [la, sa]
+        args += [len(i), lp, sp, _lab(i)]
+    ok = lib().orc_canonicalize(*args, C.byref(g))
+    return g if ok else None
+
+
+def blis_reorder(g):
+    t = TMat()
+    lib().orc_blis_reorder(C.byref(g), C.byref(t))
+    return t
+
+
+def diagonal(lens, strides, idx):
+    n = len(idx)
+    lo = np.zeros(n, np.int64)
+    so = np.zeros(n, np.int64)
+    io = C.create_string_buffer(n + 1)
+    la, lp = _i64(lens)
+    sa, sp = _i64(strides)
+    r = lib().orc_diagonal(n, lp, sp, _lab(idx), lo.ctypes.data_as(P64), so.ctypes.data_as(P64), io)
+    if r < 0:
+        return None
+    return list(lo[:r]), list(so[:r]), io.raw[:r].decode("latin1")
+
+
+def fold(lens, idx, strides_list):
+    n = len(idx)
+    la = np.ascontiguousarray(np.asarray(lens, np.int64))
+    ib = C.create_string_buffer(idx.encode("latin1"), n + 1)
+    sarrs = [np.ascontiguousarray(np.asarray(s, np.int64)) for s in strides_list]
+    ptrs = (P64 * len(sarrs))(*[s.ctypes.data_as(P64) for s in sarrs])
+    r = lib().orc_fold(n, la.ctypes.data_as(P64), ib, len(sarrs), ptrs)
+    return list(la[:r]), ib.raw[:r].decode("latin1"), [list(s[:r]) for s in sarrs]
+
+
+def sort_by_stride(strides_list):
+    n = len(strides_list[0])
+    sarrs = [np.ascontiguousarray(np.asarray(s, np.int64)) for s in strides_list]
+    ptrs = (P64 * len(sarrs))(*[s.ctypes.data_as(P64) for s in sarrs])
+    perm = (C.c_int * max(n, 1))()
+    lib().orc_sort_by_stride(n, len(sarrs), ptrs, perm)
+    return list(perm)[:n]
+
+
+def free_idx(labels):
+    return lib().orc_free_idx(len(labels), _lab(labels))
+
+
+def fill_scatter(lens, strides, BS, off, size, pack_3d=False, elem_size=8):
+    out = np.zeros(max(size, 1), np.int64)
+    la, lp = _i64(lens)
+    sa, sp = _i64(strides)
+    lib().orc_fill_scatter(len(lens), lp, sp, I64(BS), I64(off), I64(size),
+                           out.ctypes.data_as(P64), int(bool(pack_3d)), elem_size)
+    return out[:size]
+
+
+def fill_block_stride(BS, scat):
+    scat = np.ascontiguousarray(np.asarray(scat, np.int64))
+    bs = np.zeros(max(len(scat), 1), np.int64)
+    lib().orc_fill_block_stride(I64(BS), I64(len(scat)), scat.ctypes.data_as(P64),
+                                bs.ctypes.data_as(P64))
+    return bs[:len(scat)]
+
+
+def gcd(a, b):
+    return lib().orc_gcd(a, b)
+
+
+def lcm(a, b):
+    return lib().orc_lcm(a, b)
+
+
+def _tensor_args(lens, strides, idx):
+    la, lp = _i64(lens)
+    sa, sp = _i64(strides)
+    return (la, sa), [len(idx), lp, sp, _lab(idx)]
+
+
+def tensor_mult(alpha, A, lenA, strA, idxA, B, lenB, strB, idxB, beta, C_, lenC, strC, idxC,
+                acc=1):
+    """C_ (numpy, modified in place for float64) <- alpha*A.B + beta*C_.
+
+    For float32 inputs returns a float64 array (the exact-ish result)."""
+    kA, aA = _tensor_args(lenA, strA, idxA)
+    kB, aB = _tensor_args(lenB, strB, idxB)
+    kC, aC = _tensor_args(lenC, strC, idxC)
+    if A.dtype == np.float64:
+        lib().orc_tensor_mult_d(acc, C.c_double(alpha), *aA, A.ctypes.data_as(PD), *aB,
+                                B.ctypes.data_as(PD), C.c_double(beta), *aC,
+                                C_.ctypes.data_as(PD))
+        return C_
+    out = C_.astype(np.float64)
+    lib().orc_tensor_mult_s(C.c_double(alpha), *aA, A.ctypes.data_as(PF), *aB,
+                            B.ctypes.data_as(PF), C.c_double(beta), *aC,
+                            out.ctypes.data_as(PD), C_.ctypes.data_as(PF))
+    return out
+
+
+def mult_entries(g, alpha, A, B, beta, C0, ent_l, ent_m, ent_n):
+    """Extended-precision values of selected entries (l, m, n flat logical
+    indices of the canonical form g)."""
+    el, pl = _i64(ent_l)
+    em, pm = _i64(ent_m)
+    en, pn = _i64(ent_n)
+    out = np.zeros(len(el), np.float64)
+    is_float = int(A.dtype == np.float32)
+    lib().orc_mult_entries_d(C.byref(g), C.c_double(alpha), C.c_void_p(A.ctypes.data),
+                             C.c_void_p(B.ctypes.data), C.c_double(beta),
+                             C.c_void_p(C0.ctypes.data), is_float, I64(len(el)), pl, pm, pn,
+                             out.ctypes.data_as(PD))
+    return out
+
+
+def blis_contract(nthreads, alpha, A, lenA, strA, idxA, B, lenB, strB, idxB, beta, C_, lenC, strC,
+                  idxC, n_limit=0):
+    kA, aA = _tensor_args(lenA, strA, idxA)
+    kB, aB = _tensor_args(lenB, strB, idxB)
+    kC, aC = _tensor_args(lenC, strC, idxC)
+    return lib().orc_blis_contract_d(nthreads, C.c_double(alpha), *aA, A.ctypes.data_as(PD), *aB,
+                                     B.ctypes.data_as(PD), C.c_double(beta), *aC,
+                                     C_.ctypes.data_as(PD), I64(n_limit))
+
+
+# ---------------------------------------------------------------- DPD
+def dpd_init(irrep, nirrep, lens, layout):
+    """lens: ndim x nirrep nested list.  layout codes: 0 BALANCED_COL,
+    1 BALANCED_ROW, 2 BLOCKED_COL, 3 BLOCKED_ROW, 4 PREFIX_COL, 5 PREFIX_ROW."""
+    d = Dpd()
+    la = np.ascontiguousarray(np.asarray(lens, np.int64).reshape(-1))
+    lib().orc_dpd_init(C.byref(d), irrep, nirrep, len(lens), la.ctypes.data_as(P64), layout)
+    return d
+
+
+def dpd_size(d):
+    return lib().orc_dpd_size(C.byref(d))
+
+
+def dpd_get_block(d, irreps):
+    n = d.ndim
+    ir = (C.c_int * n)(*irreps)
+    ln = np.zeros(n, np.int64)
+    st = np.zeros(n, np.int64)
+    off = I64()
+    lib().orc_dpd_get_block(C.byref(d), ir, ln.ctypes.data_as(P64), C.byref(off),
+                            st.ctypes.data_as(P64))
+    return list(ln), off.value, list(st)
+
+
+def dpd_mult(alpha, dA, idxA, A, dB, idxB, B, beta, dC, idxC, Cd, mode=1):
+    """mode 0: double block-wise, 1: extended block-wise, 2: full dense
+    expansion.  float64 arrays: Cd updated in place.  float32: returns a
+    float64 array."""
+    if A.dtype == np.float64:
+        lib().orc_dpd_mult_d(mode, C.c_double(alpha), C.byref(dA), _lab(idxA), A.ctypes.data_as(PD),
+                             C.byref(dB), _lab(idxB), B.ctypes.data_as(PD), C.c_double(beta),
+                             C.byref(dC), _lab(idxC), Cd.ctypes.data_as(PD))
+        return Cd
+    out = Cd.astype(np.float64)
+    lib().orc_dpd_mult_s(C.c_double(alpha), C.byref(dA), _lab(idxA), A.ctypes.data_as(PF),
+                         C.byref(dB), _lab(idxB), B.ctypes.data_as(PF), C.c_double(beta),
+                         C.byref(dC), _lab(idxC), out.ctypes.data_as(PD), Cd.ctypes.data_as(PF))
+    return out
+
+
+def dpd_patch_scatter(dA, idxA, dB, idxB, dC, idxC, which, irrep_ab, BS_row, BS_col,
+                      max_scat=1 << 22):
+    npatch = (C.c_int * 2)()
+    maxp = 1 << 16
+    pr = np.zeros(maxp, np.int64)
+    pc = np.zeros(maxp, np.int64)
+    pb = np.zeros(maxp, np.int64)
+    rs = np.zeros(max_scat, np.int64)
+    cs = np.zeros(max_scat, np.int64)
+    n = lib().orc_dpd_patch_scatter(C.byref(dA), _lab(idxA), C.byref(dB), _lab(idxB), C.byref(dC),
+                                    _lab(idxC), which, irrep_ab, I64(BS_row), I64(BS_col), npatch,
+                                    pr.ctypes.data_as(P64), pc.ctypes.data_as(P64),
+                                    pb.ctypes.data_as(P64), rs.ctypes.data_as(P64),
+                                    cs.ctypes.data_as(P64), I64(max_scat))
+    if n < 0:
+        raise ValueError("dpd_patch_scatter failed: %d" % n)
+    nr = int(pr[:n].sum())
+    nc = int(pc[:n].sum())
+    return {"npatch": (npatch[0], npatch[1]), "rows": pr[:n].copy(), "cols": pc[:n].copy(),
+            "base": pb[:n].copy(), "rscat": rs[:nr].copy(), "cscat": cs[:nc].copy()}
+
+
+def fill_uniform(n, seed):
+    x = np.empty(n, np.float64)
+    lib().orc_fill_uniform_d(x.ctypes.data_as(PD), I64(n), C.c_uint64(seed))
+    return x

@@ -1,0 +1,255 @@
+"""Python binding of libtblis_amd (ctypes over the C ABI in include/tblis.h).
+
+Mirrors the reference's C++ front end ``tblis::mult<T>(alpha, A, idx_A, B,
+idx_B, beta, C, idx_C)`` (src/iface/3t/mult.h:28-37) for torch tensors on a
+HIP device: lengths and strides are taken from the tensors, labels are
+strings, and the call goes through ``tblis_tensor_mult``.  ``dpd_mult`` and
+``mult_batch`` wrap the DPD and batched extensions.  There is no CPU
+fallback: a missing library is an error.
+"""
+import ctypes as C
+import os
+
+__all__ = ["lib", "mult", "dpd_mult", "mult_batch", "Tensor", "DpdTensor", "get_config",
+           "set_stream", "set_async", "last_error", "last_algorithm", "DPD_LAYOUTS", "TblisError"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtblis_amd.so")
+
+TYPE_SINGLE, TYPE_DOUBLE, TYPE_SCOMPLEX, TYPE_DCOMPLEX = 0, 1, 2, 3
+
+DPD_LAYOUTS = {"BALANCED_COLUMN_MAJOR": 0, "BALANCED_ROW_MAJOR": 1, "BLOCKED_COLUMN_MAJOR": 2,
+               "BLOCKED_ROW_MAJOR": 3, "PREFIX_COLUMN_MAJOR": 4, "PREFIX_ROW_MAJOR": 5}
+
+
+class TblisError(RuntimeError):
+    pass
+
+
+class Scalar(C.Structure):
+    class _U(C.Union):
+        _fields_ = [("s", C.c_float), ("d", C.c_double), ("c", C.c_float * 2), ("z", C.c_double * 2)]
+    _fields_ = [("data", _U), ("type", C.c_int)]
+
+
+class Tensor(C.Structure):
+    _fields_ = [("type", C.c_int), ("conj", C.c_int), ("scalar", Scalar), ("data", C.c_void_p),
+                ("ndim", C.c_uint), ("len", C.POINTER(C.c_ssize_t)),
+                ("stride", C.POINTER(C.c_ssize_t))]
+
+
+class DpdTensor(C.Structure):
+    _fields_ = [("type", C.c_int), ("conj", C.c_int), ("scalar", Scalar), ("data", C.c_void_p),
+                ("ndim", C.c_uint), ("irrep", C.c_uint), ("nirrep", C.c_uint), ("layout", C.c_int),
+                ("len", C.POINTER(C.c_ssize_t))]
+
+
+assert C.sizeof(Scalar) == 24 and C.sizeof(Tensor) == 64
+
+_LIB = None
+
+# every symbol include/tblis.h declares
+EXPORTED = [
+    "tblis_single", "tblis_init_scalar_s", "tblis_init_scalar_d", "tblis_init_scalar_c",
+    "tblis_init_scalar_z", "tblis_init_tensor_scaled_s", "tblis_init_tensor_scaled_d",
+    "tblis_init_tensor_scaled_c", "tblis_init_tensor_scaled_z", "tblis_init_tensor_s",
+    "tblis_init_tensor_d", "tblis_init_tensor_c", "tblis_init_tensor_z", "tblis_tensor_mult",
+    "tblis_get_num_threads", "tblis_set_num_threads", "tblis_get_config",
+    "tblis_init_dpd_tensor_d", "tblis_init_dpd_tensor_s", "tblis_dpd_tensor_size",
+    "tblis_dpd_tensor_mult", "tblis_tensor_mult_batch", "tblis_hip_set_stream",
+    "tblis_hip_get_stream", "tblis_hip_set_async", "tblis_hip_set_error_mode",
+    "tblis_hip_get_last_error", "tblis_hip_clear_plan_cache", "tblis_hip_last_algorithm",
+    "tblis_hip_debug_canonical", "tblis_hip_debug_fill_block_scatter", "tblis_hip_debug_dpd_block",
+    "tblis_hip_debug_dpd_patches", "tblis_hip_debug_mfma_probe",
+]
+
+
+def lib():
+    """Load libtblis_amd.so (built in-tree by __graft_entry__.build())."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise TblisError("libtblis_amd.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        T, D = C.POINTER(Tensor), C.POINTER(DpdTensor)
+        P = C.POINTER
+        L.tblis_tensor_mult.argtypes = [C.c_void_p, C.c_void_p, T, C.c_char_p, T, C.c_char_p, T,
+                                        C.c_char_p]
+        L.tblis_dpd_tensor_mult.argtypes = [C.c_void_p, C.c_void_p, D, C.c_char_p, D, C.c_char_p, D,
+                                            C.c_char_p]
+        L.tblis_tensor_mult_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_ssize_t, P(T),
+                                              P(C.c_char_p), P(T), P(C.c_char_p), P(T),
+                                              P(C.c_char_p)]
+        L.tblis_get_config.restype = C.c_void_p
+        L.tblis_get_config.argtypes = [C.c_char_p]
+        L.tblis_hip_get_last_error.restype = C.c_char_p
+        L.tblis_hip_last_algorithm.restype = C.c_char_p
+        L.tblis_hip_get_stream.restype = C.c_void_p
+        L.tblis_hip_set_stream.argtypes = [C.c_void_p]
+        L.tblis_dpd_tensor_size.restype = C.c_ssize_t
+        L.tblis_dpd_tensor_size.argtypes = [D]
+        L.tblis_hip_debug_canonical.restype = C.c_int64
+        L.tblis_hip_debug_canonical.argtypes = [T, C.c_char_p, T, C.c_char_p, T, C.c_char_p,
+                                                P(C.c_int64), C.c_int64]
+        L.tblis_hip_debug_fill_block_scatter.argtypes = [C.c_int, P(C.c_ssize_t), P(C.c_ssize_t),
+                                                         C.c_ssize_t, C.c_ssize_t, C.c_ssize_t,
+                                                         P(C.c_ssize_t), P(C.c_ssize_t), C.c_int,
+                                                         C.c_int]
+        L.tblis_hip_debug_dpd_block.argtypes = [D, P(C.c_uint), P(C.c_ssize_t), P(C.c_ssize_t),
+                                                P(C.c_ssize_t)]
+        L.tblis_hip_debug_dpd_patches.restype = C.c_int
+        L.tblis_hip_debug_dpd_patches.argtypes = [D, C.c_char_p, D, C.c_char_p, D, C.c_char_p,
+                                                  C.c_int, C.c_uint, C.c_ssize_t, C.c_ssize_t,
+                                                  P(C.c_int), P(C.c_ssize_t), P(C.c_ssize_t),
+                                                  P(C.c_ssize_t), P(C.c_ssize_t), P(C.c_ssize_t),
+                                                  C.c_int64, C.c_int64]
+        L.tblis_hip_debug_mfma_probe.argtypes = [C.c_void_p]
+        L.tblis_hip_debug_mfma_probe.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def get_config(name):
+    return lib().tblis_get_config(name.encode())
+
+
+def set_stream(stream_ptr):
+    lib().tblis_hip_set_stream(stream_ptr)
+
+
+def set_async(flag):
+    lib().tblis_hip_set_async(int(bool(flag)))
+
+
+def set_error_mode(return_on_error):
+    lib().tblis_hip_set_error_mode(int(bool(return_on_error)))
+
+
+def last_error():
+    return lib().tblis_hip_get_last_error().decode()
+
+
+def last_algorithm():
+    return lib().tblis_hip_last_algorithm().decode()
+
+
+def _type_code(dtype):
+    import torch
+    if dtype == torch.float32:
+        return TYPE_SINGLE
+    if dtype == torch.float64:
+        return TYPE_DOUBLE
+    raise TblisError("unsupported dtype %s" % dtype)
+
+
+def _set_scalar(s, value, type_code):
+    if type_code == TYPE_SINGLE:
+        s.data.s = value
+    else:
+        s.data.d = value
+    s.type = type_code
+
+
+class _Desc:
+    """tblis_tensor over a torch tensor (keeps the length/stride arrays alive)."""
+
+    def __init__(self, t, scalar=1.0):
+        self.t = t
+        n = t.dim()
+        self.len = (C.c_ssize_t * max(n, 1))(*t.shape)
+        self.stride = (C.c_ssize_t * max(n, 1))(*t.stride())
+        self.d = Tensor()
+        self.d.type = _type_code(t.dtype)
+        self.d.conj = 0
+        _set_scalar(self.d.scalar, scalar, self.d.type)
+        self.d.data = C.c_void_p(t.data_ptr())
+        self.d.ndim = n
+        self.d.len = C.cast(self.len, C.POINTER(C.c_ssize_t))
+        self.d.stride = C.cast(self.stride, C.POINTER(C.c_ssize_t))
+
+
+def _labels(s, ndim, name):
+    if len(s) != ndim:
+        raise TblisError("%s: %d labels for %d dimensions" % (name, len(s), ndim))
+    return s.encode("latin1")
+
+
+def _check(fn):
+    err = last_error()
+    if err:
+        raise TblisError("%s: %s" % (fn, err))
+
+
+def mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None):
+    """C_ <- alpha * A.B + beta * C_ (tblis::mult).  Tensors: torch float32
+    or float64 on the current HIP device, any strides."""
+    dA, dB, dC = _Desc(A, alpha), _Desc(B), _Desc(C_, beta)
+    cfg = get_config(config) if config else None
+    L = lib()
+    L.tblis_hip_set_error_mode(1)
+    L.tblis_tensor_mult(None, cfg, C.byref(dA.d), _labels(idx_A, A.dim(), "A"), C.byref(dB.d),
+                        _labels(idx_B, B.dim(), "B"), C.byref(dC.d), _labels(idx_C, C_.dim(), "C"))
+    _check("tblis_tensor_mult")
+    return C_
+
+
+class _DpdDesc:
+    def __init__(self, t, irrep, nirrep, lens, layout, scalar=1.0):
+        self.t = t
+        ndim = len(lens)
+        flat = [int(x) for row in lens for x in row]
+        self.len = (C.c_ssize_t * max(len(flat), 1))(*flat)
+        self.d = DpdTensor()
+        self.d.type = _type_code(t.dtype)
+        self.d.conj = 0
+        _set_scalar(self.d.scalar, scalar, self.d.type)
+        self.d.data = C.c_void_p(t.data_ptr())
+        self.d.ndim = ndim
+        self.d.irrep = irrep
+        self.d.nirrep = nirrep
+        self.d.layout = DPD_LAYOUTS[layout] if isinstance(layout, str) else int(layout)
+        self.d.len = C.cast(self.len, C.POINTER(C.c_ssize_t))
+
+
+def dpd_size(irrep, nirrep, lens, layout="PREFIX_ROW_MAJOR"):
+    """Stored elements of a DPD tensor (lens: ndim x nirrep)."""
+    import torch
+    d = _DpdDesc(torch.empty(0, dtype=torch.float64), irrep, nirrep, lens, layout)
+    return lib().tblis_dpd_tensor_size(C.byref(d.d))
+
+
+def dpd_mult(alpha, A, irrep_A, idx_A, B, irrep_B, idx_B, beta, C_, irrep_C, idx_C, nirrep,
+             lens_A, lens_B, lens_C, layout="PREFIX_ROW_MAJOR", config=None):
+    """DPD contraction over 1-D storage tensors A, B, C_ (tblis::mult over
+    dpd_varray_view, src/iface/3t/mult.h:53-72)."""
+    dA = _DpdDesc(A, irrep_A, nirrep, lens_A, layout, alpha)
+    dB = _DpdDesc(B, irrep_B, nirrep, lens_B, layout)
+    dC = _DpdDesc(C_, irrep_C, nirrep, lens_C, layout, beta)
+    cfg = get_config(config) if config else None
+    L = lib()
+    L.tblis_hip_set_error_mode(1)
+    L.tblis_dpd_tensor_mult(None, cfg, C.byref(dA.d), idx_A.encode("latin1"), C.byref(dB.d),
+                            idx_B.encode("latin1"), C.byref(dC.d), idx_C.encode("latin1"))
+    _check("tblis_dpd_tensor_mult")
+    return C_
+
+
+def mult_batch(alpha, As, idx_As, Bs, idx_Bs, beta, Cs, idx_Cs, config=None):
+    """Independent contractions C_i <- alpha*A_i.B_i + beta*C_i, one launch."""
+    n = len(As)
+    dA = [_Desc(a, alpha) for a in As]
+    dB = [_Desc(b) for b in Bs]
+    dC = [_Desc(c, beta) for c in Cs]
+    T = C.POINTER(Tensor)
+    pa = (T * n)(*[C.pointer(d.d) for d in dA])
+    pb = (T * n)(*[C.pointer(d.d) for d in dB])
+    pc = (T * n)(*[C.pointer(d.d) for d in dC])
+    la = (C.c_char_p * n)(*[_labels(s, a.dim(), "A") for s, a in zip(idx_As, As)])
+    lb = (C.c_char_p * n)(*[_labels(s, b.dim(), "B") for s, b in zip(idx_Bs, Bs)])
+    lc = (C.c_char_p * n)(*[_labels(s, c.dim(), "C") for s, c in zip(idx_Cs, Cs)])
+    cfg = get_config(config) if config else None
+    L = lib()
+    L.tblis_hip_set_error_mode(1)
+    L.tblis_tensor_mult_batch(None, cfg, n, pa, la, pb, lb, pc, lc)
+    _check("tblis_tensor_mult_batch")
+    return Cs

@@ -1,0 +1,352 @@
+// Dense and batched contraction drivers, algorithm selection, introspection.
+#include "contract.hpp"
+
+#include <cstring>
+
+#include "plan.hpp"
+
+namespace tblis_amd
+{
+
+namespace
+{
+
+template <typename T>
+void put(std::string& k, const T& v)
+{
+    k.append((const char*)&v, sizeof(T));
+}
+
+void tensor_key(std::string& k, const tblis_tensor& t, const label_type* idx)
+{
+    put(k, (int32_t)t.type);
+    put(k, (uint32_t)t.ndim);
+    k.append((const char*)t.len, sizeof(len_type) * t.ndim);
+    k.append((const char*)t.stride, sizeof(stride_type) * t.ndim);
+    k.append(idx, t.ndim);
+}
+
+int elem_size(type_t t)
+{
+    switch (t)
+    {
+        case TYPE_SINGLE: return 4;
+        case TYPE_DOUBLE: return 8;
+        case TYPE_SCOMPLEX: return 8;
+        case TYPE_DCOMPLEX: return 16;
+    }
+    return 0;
+}
+
+void check_real(type_t t)
+{
+    if (t != TYPE_SINGLE && t != TYPE_DOUBLE)
+        fail("complex tensor types are not supported by this library yet");
+}
+
+double scalar_value(const tblis_scalar& s, type_t t)
+{
+    return t == TYPE_SINGLE ? (double)s.data.s : s.data.d;
+}
+
+// alpha = alpha_A * alpha_B in the tensor type (src/iface/3t/mult.cxx:123).
+double product_scalar(const tblis_scalar& a, const tblis_scalar& b, type_t t)
+{
+    if (t == TYPE_SINGLE) return (double)(a.data.s * b.data.s);
+    return a.data.d * b.data.d;
+}
+
+void reset_scalar(tblis_tensor* C)
+{
+    if (C->type == TYPE_SINGLE)
+        C->scalar.data.s = 1.0f;
+    else
+        C->scalar.data.d = 1.0;
+    C->conj = 0;
+}
+
+void check_tensor(const tblis_tensor* t, const label_type* idx, const char* name)
+{
+    if (!t) fail(std::string(name) + ": null tensor descriptor");
+    if (t->ndim > 0 && (!t->len || !t->stride || !idx))
+        fail(std::string(name) + ": null length, stride or label array");
+    if (t->ndim > 32) fail(std::string(name) + ": too many dimensions");
+}
+
+// Odometer offsets of every position of a (len, stride) list.
+std::vector<int64_t> all_offsets(const Group& g, int slot)
+{
+    int64_t n = prod_len(g);
+    std::vector<int64_t> out(std::max<int64_t>(n, 0));
+    std::vector<int64_t> digit(g.size(), 0);
+    int64_t off = 0;
+    for (int64_t i = 0; i < n; i++)
+    {
+        out[i] = off;
+        for (size_t d = 0; d < g.size(); d++)
+        {
+            if (digit[d] + 1 < g[d].len)
+            {
+                digit[d]++;
+                off += g[d].s[slot];
+                break;
+            }
+            off -= digit[d] * g[d].s[slot];
+            digit[d] = 0;
+        }
+    }
+    return out;
+}
+
+// The GEMM of one canonical dense contraction (m = AC, n = BC, k = AB,
+// batch = ABC), with bases added for batched calls.
+bool dense_gemm(const Canonical& c, GemmDesc& g, int64_t base_a, int64_t base_b, int64_t base_c)
+{
+    MatrixViews v = blis_views(c);
+    g.m = prod_len(v.m);
+    g.n = prod_len(v.n);
+    int64_t k = prod_len(v.k);
+    int64_t L = prod_len(v.l);
+    if (g.m == 0 || g.n == 0 || L == 0) return false;
+    g.c_base = base_c;
+    g.c_rows = side_of(v.m, 1, v.pack_m_3d);
+    g.c_cols = side_of(v.n, 1, v.pack_n_3d);
+    if (k > 0)
+    {
+        GemmSeg s;
+        s.a_base = base_a;
+        s.b_base = base_b;
+        s.a_rows = side_of(v.m, 0, v.pack_m_3d);
+        s.a_cols = side_of(v.k, 0, v.pack_k_3d);
+        s.b_rows = side_of(v.k, 1, v.pack_k_3d);
+        s.b_cols = side_of(v.n, 0, v.pack_n_3d);
+        s.k = k;
+        g.segs.push_back(s);
+    }
+    if (!v.l.empty())
+    {
+        g.l_a = all_offsets(v.l, 0);
+        g.l_b = all_offsets(v.l, 1);
+        g.l_c = all_offsets(v.l, 2);
+    }
+    return true;
+}
+
+int dtype_code(type_t t) { return t == TYPE_DOUBLE ? 0 : 1; }
+
+}  // namespace
+
+const char* algo_name(Algo a)
+{
+    switch (a)
+    {
+        case Algo::Default: return "mi355x";
+        case Algo::Fp64Mfma: return "fp64_mfma";
+        case Algo::Bf16x3: return "bf16x3";
+        case Algo::Bf16x3Fast: return "bf16x3_fast";
+        case Algo::Reference: return "reference";
+    }
+    return "?";
+}
+
+Algo resolve_algo(const tblis_config* cfg, type_t type)
+{
+    Algo a = cfg ? cfg->algo : Algo::Default;
+    if (a == Algo::Reference) return a;
+    if (type == TYPE_DOUBLE) return Algo::Fp64Mfma;
+    if (a == Algo::Default) return Algo::Bf16x3;
+    return a;
+}
+
+namespace
+{
+
+// Kernel family for an algorithm (the split-BF16 path is not wired to the
+// drivers yet: it runs the FP64 MFMA kernel and reports that).
+Algo effective(Algo a)
+{
+    if (a == Algo::Bf16x3 || a == Algo::Bf16x3Fast) return Algo::Fp64Mfma;
+    return a;
+}
+
+std::shared_ptr<Plan> empty_plan() { return std::make_shared<Plan>(); }
+
+}  // namespace
+
+void tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_type* idx_A,
+                 const tblis_tensor* B, const label_type* idx_B, tblis_tensor* C,
+                 const label_type* idx_C)
+{
+    check_tensor(A, idx_A, "A");
+    check_tensor(B, idx_B, "B");
+    check_tensor(C, idx_C, "C");
+    if (A->type != B->type || A->type != C->type) fail("A, B and C must have the same type");
+    check_real(A->type);
+    const type_t type = A->type;
+    Algo algo = effective(resolve_algo(cfg, type));
+    thread_state().last_algo = algo_name(algo);
+
+    std::string key;
+    put(key, (int32_t)algo);
+    tensor_key(key, *A, idx_A);
+    tensor_key(key, *B, idx_B);
+    tensor_key(key, *C, idx_C);
+
+    std::shared_ptr<Plan> plan = cache_get(key);
+    if (!plan)
+    {
+        Canonical c;
+        std::string why;
+        if (!canonicalize(*A, idx_A, *B, idx_B, *C, idx_C, c, why)) fail("invalid contraction: " + why);
+        GemmDesc g;
+        if (dense_gemm(c, g, 0, 0, 0))
+        {
+            PlanBuilder pb(elem_size(type), dtype_code(type), algo != Algo::Reference);
+            pb.add(g);
+            plan = pb.build();
+        }
+        if (!plan) plan = empty_plan();
+        cache_put(key, plan);
+    }
+
+    const double alpha = product_scalar(A->scalar, B->scalar, type);
+    const double beta = scalar_value(C->scalar, type);
+    if (plan->ntasks > 0)
+    {
+        const bool use_k = alpha != 0.0;
+        if (use_k || beta != 1.0)
+        {
+            check_device_pointer(C->data, "C");
+            if (use_k)
+            {
+                check_device_pointer(A->data, "A");
+                check_device_pointer(B->data, "B");
+            }
+            run_plan(*plan, A->data, B->data, C->data, alpha, beta, use_k);
+        }
+    }
+    reset_scalar(C);
+}
+
+void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tensor* const* A,
+                       const label_type* const* idx_A, const tblis_tensor* const* B,
+                       const label_type* const* idx_B, tblis_tensor* const* C,
+                       const label_type* const* idx_C)
+{
+    if (count < 0) fail("batch: negative count");
+    if (count == 0) return;
+    if (!A || !B || !C || !idx_A || !idx_B || !idx_C) fail("batch: null array argument");
+    const type_t type = A[0]->type;
+    check_real(type);
+    const int es = elem_size(type);
+    double alpha = 0, beta = 0;
+    for (int64_t i = 0; i < count; i++)
+    {
+        check_tensor(A[i], idx_A[i], "A");
+        check_tensor(B[i], idx_B[i], "B");
+        check_tensor(C[i], idx_C[i], "C");
+        if (A[i]->type != type || B[i]->type != type || C[i]->type != type)
+            fail("batch: every tensor must have the same type");
+        double al = product_scalar(A[i]->scalar, B[i]->scalar, type);
+        double be = scalar_value(C[i]->scalar, type);
+        if (i == 0)
+        {
+            alpha = al;
+            beta = be;
+        }
+        else if (al != alpha || be != beta)
+            fail("batch: every instance must use the same alpha and beta");
+    }
+    Algo algo = effective(resolve_algo(cfg, type));
+    thread_state().last_algo = algo_name(algo);
+
+    auto delta = [&](const void* p, const void* p0) -> int64_t {
+        intptr_t d = (const char*)p - (const char*)p0;
+        if (d % es) fail("batch: tensor data pointers are not aligned to the element size");
+        return (int64_t)(d / es);
+    };
+
+    std::string key;
+    put(key, (int32_t)algo);
+    put(key, count);
+    for (int64_t i = 0; i < count; i++)
+    {
+        tensor_key(key, *A[i], idx_A[i]);
+        tensor_key(key, *B[i], idx_B[i]);
+        tensor_key(key, *C[i], idx_C[i]);
+        put(key, delta(A[i]->data, A[0]->data));
+        put(key, delta(B[i]->data, B[0]->data));
+        put(key, delta(C[i]->data, C[0]->data));
+    }
+
+    std::shared_ptr<Plan> plan = cache_get(key);
+    if (!plan)
+    {
+        PlanBuilder pb(es, dtype_code(type), algo != Algo::Reference);
+        for (int64_t i = 0; i < count; i++)
+        {
+            Canonical c;
+            std::string why;
+            if (!canonicalize(*A[i], idx_A[i], *B[i], idx_B[i], *C[i], idx_C[i], c, why))
+                fail("batch: invalid contraction " + std::to_string(i) + ": " + why);
+            GemmDesc g;
+            if (dense_gemm(c, g, delta(A[i]->data, A[0]->data), delta(B[i]->data, B[0]->data),
+                           delta(C[i]->data, C[0]->data)))
+                pb.add(g);
+        }
+        plan = pb.build();
+        if (!plan) plan = empty_plan();
+        cache_put(key, plan);
+    }
+    if (plan->ntasks > 0)
+    {
+        const bool use_k = alpha != 0.0;
+        if (use_k || beta != 1.0)
+        {
+            check_device_pointer(C[0]->data, "C");
+            if (use_k)
+            {
+                check_device_pointer(A[0]->data, "A");
+                check_device_pointer(B[0]->data, "B");
+            }
+            run_plan(*plan, A[0]->data, B[0]->data, C[0]->data, alpha, beta, use_k);
+        }
+    }
+    for (int64_t i = 0; i < count; i++) reset_scalar(C[i]);
+}
+
+int64_t debug_canonical(const tblis_tensor* A, const label_type* idx_A, const tblis_tensor* B,
+                        const label_type* idx_B, const tblis_tensor* C, const label_type* idx_C,
+                        int64_t* out, int64_t max_out)
+{
+    check_tensor(A, idx_A, "A");
+    check_tensor(B, idx_B, "B");
+    check_tensor(C, idx_C, "C");
+    Canonical c;
+    std::string why;
+    if (!canonicalize(*A, idx_A, *B, idx_B, *C, idx_C, c, why)) return -1;
+    MatrixViews v = blis_views(c);
+    std::vector<int64_t> o;
+    auto emit = [&](const Group& g, int nslot) {
+        o.push_back((int64_t)g.size());
+        for (auto& d : g) o.push_back(d.len);
+        for (int s = 0; s < nslot; s++)
+            for (auto& d : g) o.push_back(d.s[s]);
+    };
+    emit(c.ab, 2);
+    emit(c.ac, 2);
+    emit(c.bc, 2);
+    emit(c.abc, 3);
+    emit(v.m, 2);
+    emit(v.n, 2);
+    emit(v.k, 2);
+    emit(v.l, 3);
+    o.push_back(v.pack_m_3d);
+    o.push_back(v.pack_n_3d);
+    o.push_back(v.pack_k_3d);
+    if ((int64_t)o.size() > max_out) return -2;
+    std::memcpy(out, o.data(), o.size() * sizeof(int64_t));
+    return (int64_t)o.size();
+}
+
+}  // namespace tblis_amd

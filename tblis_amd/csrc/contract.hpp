@@ -1,0 +1,36 @@
+// High-level contraction entry points behind the C ABI.
+#pragma once
+
+#include "common.hpp"
+
+namespace tblis_amd
+{
+
+void tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_type* idx_A,
+                 const tblis_tensor* B, const label_type* idx_B, tblis_tensor* C,
+                 const label_type* idx_C);
+
+void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tensor* const* A,
+                       const label_type* const* idx_A, const tblis_tensor* const* B,
+                       const label_type* const* idx_B, tblis_tensor* const* C,
+                       const label_type* const* idx_C);
+
+void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const label_type* idx_A,
+                     const tblis_dpd_tensor* B, const label_type* idx_B, tblis_dpd_tensor* C,
+                     const label_type* idx_C);
+
+int64_t debug_canonical(const tblis_tensor* A, const label_type* idx_A, const tblis_tensor* B,
+                        const label_type* idx_B, const tblis_tensor* C, const label_type* idx_C,
+                        int64_t* out, int64_t max_out);
+
+int debug_dpd_patches(const tblis_dpd_tensor* A, const label_type* idx_A,
+                      const tblis_dpd_tensor* B, const label_type* idx_B,
+                      const tblis_dpd_tensor* C, const label_type* idx_C, int which,
+                      unsigned irrep_ab, int64_t BS_row, int64_t BS_col, int* npatch,
+                      int64_t* rows, int64_t* cols, int64_t* base, int64_t* rscat,
+                      int64_t* cscat, int64_t max_patch, int64_t max_scat);
+
+// Selected algorithm for a type under a config.
+Algo resolve_algo(const tblis_config* cfg, type_t type);
+
+}  // namespace tblis_amd

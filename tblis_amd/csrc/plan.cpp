@@ -1,0 +1,349 @@
+// Planner, plan cache and launch.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <list>
+#include <mutex>
+#include <unordered_map>
+
+namespace tblis_amd
+{
+
+namespace
+{
+
+bool unit_first(const Side& s) { return !s.len.empty() && s.stride[0] == 1 && s.len[0] > 1; }
+
+int64_t first_stride(const Side& s)
+{
+    if (s.len.empty()) return 0;
+    return s.stride[0] < 0 ? -s.stride[0] : s.stride[0];
+}
+
+// Coalesce loads along the rows of a (rows x cols) operand tile?
+bool rows_fast(const Side& rows, const Side& cols)
+{
+    if (unit_first(rows)) return true;
+    if (unit_first(cols)) return false;
+    if (rows.len.empty()) return false;
+    if (cols.len.empty()) return true;
+    return first_stride(rows) <= first_stride(cols);
+}
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) fail(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+Plan::~Plan()
+{
+    if (dev)
+    {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(device);
+        (void)hipFree(dev);
+        (void)hipSetDevice(cur);
+    }
+}
+
+PlanBuilder::PlanBuilder(int elem_size, int dtype, int use_mfma)
+    : elem_size_(elem_size), dtype_(dtype), use_mfma_(use_mfma)
+{
+}
+
+SideRef PlanBuilder::side_ref(const Side& s, int64_t BS, int64_t start)
+{
+    if (s.len.size() <= 1) return SideRef{start, s.len.empty() ? 0 : s.stride[0]};
+    std::vector<int64_t> key;
+    key.push_back(BS);
+    key.push_back(s.pack_3d);
+    key.insert(key.end(), s.len.begin(), s.len.end());
+    key.insert(key.end(), s.stride.begin(), s.stride.end());
+    auto it = dedupe_.find(key);
+    int64_t at;
+    if (it != dedupe_.end())
+    {
+        at = it->second;
+    }
+    else
+    {
+        int64_t total = 1;
+        for (auto l : s.len) total *= l;
+        at = (int64_t)pool_.size();
+        pool_.resize(pool_.size() + std::max<int64_t>(total, 1));
+        fill_scatter(s, BS, 0, total, pool_.data() + at, elem_size_);
+        dedupe_[key] = at;
+    }
+    return SideRef{at + start, SCATTERED};
+}
+
+std::shared_ptr<Plan> PlanBuilder::build()
+{
+    if (gemms_.empty()) return nullptr;
+    auto plan = std::make_shared<Plan>();
+
+    // Orientation: the MFMA output puts consecutive lanes along n, so make
+    // n the unit-stride side of C when C is contiguous along m.
+    const GemmDesc& g0 = gemms_[0];
+    plan->swap = unit_first(g0.c_rows) && !unit_first(g0.c_cols);
+    const bool sw = plan->swap;
+
+    // Thread maps from the first segment that exists.
+    plan->key.a_mfast = 1;
+    plan->key.b_nfast = 1;
+    for (auto& g : gemms_)
+    {
+        if (g.segs.empty()) continue;
+        const GemmSeg& s = g.segs[0];
+        const Side& ar = sw ? s.b_cols : s.a_rows;
+        const Side& ac = sw ? s.b_rows : s.a_cols;
+        const Side& br = sw ? s.a_cols : s.b_rows;
+        const Side& bc = sw ? s.a_rows : s.b_cols;
+        plan->key.a_mfast = rows_fast(ar, ac);
+        plan->key.b_nfast = rows_fast(bc, br);
+        break;
+    }
+    plan->key.dtype = dtype_;
+    plan->key.use_mfma = use_mfma_;
+
+    // Tile: 128x128 when there are enough big tiles to fill the chip and the
+    // padding stays small; else 64x64.
+    int64_t tiles128 = 0, area = 0, area128 = 0;
+    for (auto& g : gemms_)
+    {
+        int64_t m = sw ? g.n : g.m, n = sw ? g.m : g.n;
+        int64_t L = g.l_c.empty() ? 1 : (int64_t)g.l_c.size();
+        tiles128 += ceil_div(m, 128) * ceil_div(n, 128) * L;
+        area += m * n * L;
+        area128 += ceil_div(m, 128) * 128 * ceil_div(n, 128) * 128 * L;
+    }
+    bool big = use_mfma_ && tiles128 >= device_cu_count() && area128 <= area * 115 / 100;
+    plan->key.tile = big ? 0 : 1;
+    const int BM = gemm_f64_tile_m(plan->key.tile), BN = gemm_f64_tile_n(plan->key.tile);
+    const int BK = 16;
+
+    std::vector<TaskDesc> tasks;
+    std::vector<SegDesc> segs;
+    std::vector<int64_t> cost;
+    for (auto& g : gemms_)
+    {
+        const int64_t m = sw ? g.n : g.m, n = sw ? g.m : g.n;
+        if (m == 0 || n == 0) continue;
+        const Side& cr = sw ? g.c_cols : g.c_rows;
+        const Side& cc = sw ? g.c_rows : g.c_cols;
+        const int64_t L = g.l_c.empty() ? 1 : (int64_t)g.l_c.size();
+        int64_t ksum = 0;
+        for (auto& s : g.segs) ksum += s.k;
+        for (int64_t l = 0; l < L; l++)
+        {
+            int64_t oa = g.l_a.empty() ? 0 : g.l_a[l];
+            int64_t ob = g.l_b.empty() ? 0 : g.l_b[l];
+            int64_t oc = g.l_c.empty() ? 0 : g.l_c[l];
+            if (sw) std::swap(oa, ob);
+            plan->useful_flops += 2 * g.m * g.n * ksum;
+            for (int64_t tm = 0; tm < ceil_div(m, BM); tm++)
+                for (int64_t tn = 0; tn < ceil_div(n, BN); tn++)
+                {
+                    TaskDesc t{};
+                    t.c_base = g.c_base + oc;
+                    t.c_row = side_ref(cr, BM, tm * BM);
+                    t.c_col = side_ref(cc, BN, tn * BN);
+                    t.mlen = (int32_t)std::min<int64_t>(BM, m - tm * BM);
+                    t.nlen = (int32_t)std::min<int64_t>(BN, n - tn * BN);
+                    t.seg0 = (int32_t)segs.size();
+                    t.nseg = 0;
+                    int64_t kpad = 0;
+                    for (auto& s : g.segs)
+                    {
+                        if (s.k == 0) continue;
+                        SegDesc d{};
+                        const Side& ar = sw ? s.b_cols : s.a_rows;
+                        const Side& ac = sw ? s.b_rows : s.a_cols;
+                        const Side& br = sw ? s.a_cols : s.b_rows;
+                        const Side& bc = sw ? s.a_rows : s.b_cols;
+                        d.a_base = (sw ? s.b_base : s.a_base) + oa;
+                        d.b_base = (sw ? s.a_base : s.b_base) + ob;
+                        d.a_row = side_ref(ar, BM, tm * BM);
+                        d.a_col = side_ref(ac, BK, 0);
+                        d.b_row = side_ref(br, BK, 0);
+                        d.b_col = side_ref(bc, BN, tn * BN);
+                        d.klen = s.k;
+                        segs.push_back(d);
+                        t.nseg++;
+                        kpad += ceil_div(s.k, BK) * BK;
+                    }
+                    plan->mfma_flops += 2 * (int64_t)BM * BN * kpad;
+                    tasks.push_back(t);
+                    cost.push_back((int64_t)t.mlen * t.nlen * std::max<int64_t>(kpad, 1));
+                }
+        }
+    }
+    if (tasks.empty()) return nullptr;
+
+    // Uneven work (DPD, batches): longest tasks first so the static
+    // round-robin over the persistent grid balances.
+    bool uniform = std::all_of(cost.begin(), cost.end(), [&](int64_t c) { return c == cost[0]; });
+    if (!uniform)
+    {
+        std::vector<int> order(tasks.size());
+        for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+        std::vector<TaskDesc> t2;
+        t2.reserve(tasks.size());
+        for (int i : order) t2.push_back(tasks[i]);
+        tasks.swap(t2);
+    }
+    if (tasks.size() > (size_t)INT32_MAX || segs.size() > (size_t)INT32_MAX)
+        fail("contraction too large: task list exceeds 2^31 entries");
+
+    plan->ntasks = (int)tasks.size();
+    int occ = gemm_f64_occupancy(plan->key);
+    plan->grid = (int)std::min<int64_t>(plan->ntasks, (int64_t)device_cu_count() * std::max(occ, 1));
+
+    size_t tb = tasks.size() * sizeof(TaskDesc);
+    size_t sb = segs.size() * sizeof(SegDesc);
+    size_t pb = std::max<size_t>(pool_.size(), 1) * sizeof(int64_t);
+    size_t off_s = (tb + 255) / 256 * 256;
+    size_t off_p = off_s + (sb + 255) / 256 * 256;
+    plan->bytes = off_p + pb;
+    hip_check(hipGetDevice(&plan->device), "hipGetDevice");
+    hip_check(hipMalloc(&plan->dev, plan->bytes), "hipMalloc (plan)");
+    char* base = (char*)plan->dev;
+    plan->d_tasks = (TaskDesc*)base;
+    plan->d_segs = (SegDesc*)(base + off_s);
+    plan->d_pool = (int64_t*)(base + off_p);
+    hip_check(hipMemcpy(plan->d_tasks, tasks.data(), tb, hipMemcpyHostToDevice), "hipMemcpy");
+    if (sb) hip_check(hipMemcpy(plan->d_segs, segs.data(), sb, hipMemcpyHostToDevice), "hipMemcpy");
+    if (!pool_.empty())
+        hip_check(hipMemcpy(plan->d_pool, pool_.data(), pool_.size() * sizeof(int64_t),
+                            hipMemcpyHostToDevice),
+                  "hipMemcpy");
+    return plan;
+}
+
+// ------------------------------------------------------------------ state
+
+ThreadState& thread_state()
+{
+    static thread_local ThreadState ts;
+    return ts;
+}
+
+namespace
+{
+
+struct Cache
+{
+    std::mutex mu;
+    std::list<std::pair<std::string, std::shared_ptr<Plan>>> lru;
+    std::unordered_map<std::string, decltype(lru)::iterator> index;
+    size_t max_entries = 256;
+};
+
+// Never destroyed: plans hold device memory that must not be freed after
+// the HIP runtime has shut down at process exit.
+Cache& cache()
+{
+    static Cache* c = new Cache();
+    return *c;
+}
+
+}  // namespace
+
+std::shared_ptr<Plan> cache_get(const std::string& key)
+{
+    Cache& c = cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    auto it = c.index.find(key);
+    if (it == c.index.end()) return nullptr;
+    c.lru.splice(c.lru.begin(), c.lru, it->second);
+    return it->second->second;
+}
+
+void cache_put(const std::string& key, std::shared_ptr<Plan> p)
+{
+    Cache& c = cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    auto it = c.index.find(key);
+    if (it != c.index.end())
+    {
+        c.lru.erase(it->second);
+        c.index.erase(it);
+    }
+    c.lru.emplace_front(key, std::move(p));
+    c.index[key] = c.lru.begin();
+    while (c.lru.size() > c.max_entries)
+    {
+        c.index.erase(c.lru.back().first);
+        c.lru.pop_back();
+    }
+}
+
+void cache_clear()
+{
+    Cache& c = cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    c.index.clear();
+    c.lru.clear();
+}
+
+int device_cu_count()
+{
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cus[dev])
+    {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        cus[dev] = n;
+    }
+    return cus[dev];
+}
+
+void check_device_pointer(const void* p, const char* what)
+{
+    if (!p) fail(std::string(what) + ": null data pointer");
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess)
+    {
+        (void)hipGetLastError();
+        fail(std::string(what) +
+             ": data is not device-accessible memory (host pointers are not supported; "
+             "allocate with hipMalloc, hipHostMalloc or a torch CUDA tensor)");
+    }
+    if (attr.type == hipMemoryTypeHost && attr.devicePointer == nullptr)
+        fail(std::string(what) + ": host memory that is not mapped for the device");
+}
+
+void run_plan(const Plan& p, const void* A, const void* B, void* C, double alpha, double beta,
+              bool use_k)
+{
+    ThreadState& ts = thread_state();
+    GemmLaunch L;
+    L.tasks = p.d_tasks;
+    L.ntasks = p.ntasks;
+    L.segs = p.d_segs;
+    L.pool = p.d_pool;
+    L.A = p.swap ? B : A;
+    L.B = p.swap ? A : B;
+    L.C = C;
+    L.alpha = alpha;
+    L.beta = beta;
+    L.use_k = use_k ? 1 : 0;
+    L.grid = p.grid;
+    L.stream = ts.stream;
+    hip_check(launch_gemm_f64(L, p.key), "kernel launch");
+    if (!ts.async) hip_check(hipStreamSynchronize(ts.stream), "hipStreamSynchronize");
+}
+
+}  // namespace tblis_amd

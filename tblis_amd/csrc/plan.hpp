@@ -1,0 +1,97 @@
+// Planner: turns one or many (sub-)GEMMs over scatter-described matrices
+// into the task/segment lists the kernels walk, and owns their device copy.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "plan_types.hpp"
+
+namespace tblis_amd
+{
+
+// One k segment of a GEMM: A (m x k) and B (k x n) patches.
+struct GemmSeg
+{
+    int64_t a_base = 0, b_base = 0;
+    Side a_rows, a_cols, b_rows, b_cols;
+    int64_t k = 0;
+};
+
+// C (m x n) patch, optionally repeated over a batch of base offsets, with
+// its k segments (summed into the same C patch).
+struct GemmDesc
+{
+    int64_t c_base = 0;
+    Side c_rows, c_cols;
+    int64_t m = 0, n = 0;
+    std::vector<GemmSeg> segs;
+    // batch: extra offsets added to the bases (empty = one instance)
+    std::vector<int64_t> l_a, l_b, l_c;
+};
+
+struct Plan
+{
+    KernelKey key{};
+    bool swap = false;  // kernel computes C^T = B^T A^T
+    int ntasks = 0;
+    int grid = 0;
+    int64_t useful_flops = 0;
+    int64_t mfma_flops = 0;  // incl. tile padding at 16-row MFMA granularity
+    void* dev = nullptr;     // one allocation: tasks | segs | pool
+    TaskDesc* d_tasks = nullptr;
+    SegDesc* d_segs = nullptr;
+    int64_t* d_pool = nullptr;
+    size_t bytes = 0;
+    int device = 0;
+    ~Plan();
+};
+
+class PlanBuilder
+{
+  public:
+    PlanBuilder(int elem_size, int dtype, int use_mfma);
+    void add(const GemmDesc& g) { gemms_.push_back(g); }
+    // Choose swap/tile/thread maps, emit tasks, upload.  Returns nullptr if
+    // there is nothing to launch.
+    std::shared_ptr<Plan> build();
+
+  private:
+    SideRef side_ref(const Side& s, int64_t BS, int64_t start);
+    int elem_size_, dtype_, use_mfma_;
+    std::vector<GemmDesc> gemms_;
+    std::vector<int64_t> pool_;
+    std::map<std::vector<int64_t>, int64_t> dedupe_;
+};
+
+// Per-thread execution settings.
+struct ThreadState
+{
+    hipStream_t stream = nullptr;
+    bool async = false;
+    bool return_on_error = false;
+    std::string last_error;
+    std::string last_algo;
+};
+ThreadState& thread_state();
+
+// Plan cache (process-wide, thread-safe).
+std::shared_ptr<Plan> cache_get(const std::string& key);
+void cache_put(const std::string& key, std::shared_ptr<Plan> p);
+void cache_clear();
+
+// Launch a plan on the current thread's stream and wait unless async.
+void run_plan(const Plan& p, const void* A, const void* B, void* C, double alpha, double beta,
+              bool use_k);
+
+// Device pointer check for a tensor that will be accessed.
+void check_device_pointer(const void* p, const char* what);
+
+int device_cu_count();
+
+}  // namespace tblis_amd

@@ -1,0 +1,59 @@
+// Device-side description of a contraction: a list of C tiles (tasks), each
+// with a list of k segments.  Shared by the host planner and the kernels.
+//
+// Every matrix element is addressed as
+//     data + base + row_offset(i) + col_offset(j)
+// where a side's offset is either affine, (start + i) * stride, when that
+// side of the matrix view is a single (folded) dimension, or read from the
+// scatter pool, pool[start + i], which holds the block-scatter vectors of
+// every operand and patch.  This is the reference's block-stride fast path
+// (src/matrix/block_scatter_matrix.hpp:232-247, src/nodes/gemm_ukr.hpp:279-363)
+// applied to whole sides.  A dense contraction has one patch per operand; a
+// DPD contraction one per irrep block; a batch one per instance.
+#pragma once
+
+#include <cstdint>
+
+namespace tblis_amd
+{
+
+constexpr int64_t SCATTERED = INT64_MIN;
+
+struct SideRef
+{
+    int64_t start;   // pool index (scattered) or first logical index (affine)
+    int64_t stride;  // SCATTERED, or the element stride of the side
+};
+
+struct TaskDesc
+{
+    int64_t c_base;  // element offset of this C patch
+    SideRef c_row;   // C rows of the tile (start already at the tile's first row)
+    SideRef c_col;   // C cols of the tile
+    int32_t mlen;    // valid rows in the tile (1..BM)
+    int32_t nlen;    // valid cols in the tile (1..BN)
+    int32_t seg0;    // first k segment
+    int32_t nseg;    // number of k segments (0: C <- beta*C)
+};
+
+struct SegDesc
+{
+    int64_t a_base, b_base;  // element offsets of the A and B patches
+    SideRef a_row;           // A rows of the tile
+    SideRef a_col;           // A cols (k) of the segment
+    SideRef b_row;           // B rows (k) of the segment
+    SideRef b_col;           // B cols of the tile
+    int64_t klen;            // k extent of the segment (>= 1)
+};
+
+// Kernel variant selector.
+struct KernelKey
+{
+    int tile;      // 0: 128x128, 1: 64x64
+    int a_mfast;   // A tile loads coalesced along m (1) or k (0)
+    int b_nfast;   // B tile loads coalesced along n (1) or k (0)
+    int dtype;     // 0: double in/out, 1: float in/out (widened to FP64)
+    int use_mfma;  // 1: v_mfma_f64_16x16x4; 0: plain FMA (cross-check)
+};
+
+}  // namespace tblis_amd

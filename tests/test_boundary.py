@@ -1,0 +1,300 @@
+"""The C-ABI boundary on the CPU: the library loads, exports every declared
+symbol, and its host-side integer work (canonical forms, block-scatter
+vectors, DPD layout and patch lists) is bit-exact with the oracle.  No
+device work is started here."""
+import ctypes as C
+import itertools
+import json
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import tblis_amd as T
+from oracle import pyoracle as po
+from tests.helpers import layout_for, random_contraction, storage_size
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "tblis.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(tblis_[a-z0-9_]+)\s*\(", text))
+    names |= set(re.findall(r"extern\s+const\s+tblis_comm\s*\*\s*const\s+(tblis_\w+)", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = T.lib()
+    declared = _declared_symbols()
+    assert declared == set(T.EXPORTED)
+    for name in declared:
+        assert hasattr(L, name), name
+
+
+def test_abi_layout():
+    assert C.sizeof(T.Scalar) == 24 and C.sizeof(T.Tensor) == 64
+    assert T.Tensor.scalar.offset == 8 and T.Tensor.data.offset == 32
+    assert T.Tensor.ndim.offset == 40 and T.Tensor.len.offset == 48 and T.Tensor.stride.offset == 56
+    assert T.Scalar.type.offset == 16
+
+
+def test_init_tensor_and_threads():
+    L = T.lib()
+    lens = (C.c_ssize_t * 2)(3, 4)
+    strides = (C.c_ssize_t * 2)(1, 3)
+    t = T.Tensor()
+    L.tblis_init_tensor_scaled_d(C.byref(t), C.c_double(2.5), 2, lens, None, strides)
+    assert t.type == T.TYPE_DOUBLE and t.scalar.data.d == 2.5 and t.ndim == 2 and t.conj == 0
+    L.tblis_init_tensor_s(C.byref(t), 1, lens, None, strides)
+    assert t.type == T.TYPE_SINGLE and t.scalar.data.s == 1.0
+    L.tblis_set_num_threads(7)
+    assert L.tblis_get_num_threads() == 7
+    L.tblis_set_num_threads(0)
+
+
+class _HostDesc:
+    """A tblis_tensor with host arrays only (data never touched)."""
+
+    def __init__(self, lens, strides, dtype=T.TYPE_DOUBLE):
+        n = len(lens)
+        self.l = (C.c_ssize_t * max(n, 1))(*lens)
+        self.s = (C.c_ssize_t * max(n, 1))(*strides)
+        self.d = T.Tensor()
+        self.d.type = dtype
+        self.d.scalar.data.d = 1.0
+        self.d.ndim = n
+        self.d.len = C.cast(self.l, C.POINTER(C.c_ssize_t))
+        self.d.stride = C.cast(self.s, C.POINTER(C.c_ssize_t))
+
+
+def _product_canonical(lA, sA, iA, lB, sB, iB, lC, sC, iC):
+    a, b, c = _HostDesc(lA, sA), _HostDesc(lB, sB), _HostDesc(lC, sC)
+    out = (C.c_int64 * 4096)()
+    T.lib().tblis_hip_set_error_mode(1)
+    n = T.lib().tblis_hip_debug_canonical(C.byref(a.d), iA.encode("latin1"), C.byref(b.d),
+                                          iB.encode("latin1"), C.byref(c.d), iC.encode("latin1"),
+                                          out, 4096)
+    if n < 0:
+        return None
+    v = list(out)[:n]
+    res, pos = [], 0
+
+    def take(nslot):
+        nonlocal pos
+        k = v[pos]
+        pos += 1
+        lens = v[pos:pos + k]
+        pos += k
+        st = []
+        for _ in range(nslot):
+            st.append(v[pos:pos + k])
+            pos += k
+        return lens, st
+
+    for nslot in (2, 2, 2, 3, 2, 2, 2, 3):
+        res.append(take(nslot))
+    res.append(tuple(v[pos:pos + 3]))
+    return res
+
+
+def _oracle_canonical(lA, sA, iA, lB, sB, iB, lC, sC, iC):
+    g = po.canonicalize(lA, sA, iA, lB, sB, iB, lC, sC, iC)
+    if g is None:
+        return None
+    d = g.as_dict()
+    t = po.blis_reorder(g).as_dict()
+    return [(d["len_ab"], [d["sA_ab"], d["sB_ab"]]), (d["len_ac"], [d["sA_ac"], d["sC_ac"]]),
+            (d["len_bc"], [d["sB_bc"], d["sC_bc"]]),
+            (d["len_abc"], [d["sA_abc"], d["sB_abc"], d["sC_abc"]]),
+            (t["len_m"], [t["sA_m"], t["sC_m"]]), (t["len_n"], [t["sB_n"], t["sC_n"]]),
+            (t["len_k"], [t["sA_k"], t["sB_k"]]), (t["len_l"], [t["sA_l"], t["sB_l"], t["sC_l"]]),
+            tuple(t["pack_3d"])]
+
+
+@pytest.mark.parametrize("seed", range(150))
+def test_canonical_form_bit_exact(seed):
+    rng = np.random.default_rng(1000 + seed)
+    iA, iB, iC, ext = random_contraction(rng, max_dims=3, max_len=5, allow_diag=seed % 4 == 0)
+    kinds = ["col", "row", "random"]
+    (lA, sA), (lB, sB), (lC, sC) = [layout_for(i, ext, rng, kinds[(seed + j) % 3])
+                                    for j, i in enumerate((iA, iB, iC))]
+    assert _product_canonical(lA, sA, iA, lB, sB, iB, lC, sC, iC) == \
+        _oracle_canonical(lA, sA, iA, lB, sB, iB, lC, sC, iC)
+
+
+def test_canonical_workloads_bit_exact():
+    cases = [("abef", "efij", "abij", dict(a=24, b=24, e=24, f=24, i=24, j=24)),
+             ("aebf", "fjei", "aibj", dict(a=64, b=64, e=64, f=64, i=32, j=32)),
+             ("aebf", "ecfd", "abcd", dict(a=96, b=96, c=96, d=96, e=96, f=96))]
+    from tests.helpers import col_major
+    for iA, iB, iC, e in cases:
+        args = []
+        for idx in (iA, iB, iC):
+            lens = [e[c] for c in idx]
+            args += [lens, col_major(lens), idx]
+        assert _product_canonical(*args) == _oracle_canonical(*args)
+
+
+def test_invalid_contraction_reports_error():
+    a, b, c = _HostDesc([2, 3], [1, 2]), _HostDesc([3], [1]), _HostDesc([3], [1])
+    L = T.lib()
+    L.tblis_hip_set_error_mode(1)
+    L.tblis_tensor_mult(None, None, C.byref(a.d), b"ab", C.byref(b.d), b"c", C.byref(c.d), b"c")
+    assert "only one tensor" in T.last_error()
+    # unequal lengths for a shared label
+    a, b, c = _HostDesc([2, 3], [1, 2]), _HostDesc([4], [1]), _HostDesc([2], [1])
+    L.tblis_tensor_mult(None, None, C.byref(a.d), b"ab", C.byref(b.d), b"b", C.byref(c.d), b"a")
+    assert "unequal" in T.last_error()
+    # mixed types
+    a2 = _HostDesc([2], [1], T.TYPE_SINGLE)
+    L.tblis_tensor_mult(None, None, C.byref(a2.d), b"a", C.byref(b.d), b"b", C.byref(c.d), b"ab")
+    assert "same type" in T.last_error()
+
+
+def test_abort_mode_and_unknown_config():
+    code = ("import ctypes as C, tblis_amd as T; L=T.lib(); L.tblis_get_config(b'nope')")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True)
+    assert r.returncode != 0 and "No configuration named nope" in r.stderr
+    code = ("import ctypes as C, tblis_amd as T\n"
+            "from tests.test_boundary import _HostDesc\n"
+            "a=_HostDesc([2,3],[1,2]); b=_HostDesc([3],[1]); c=_HostDesc([3],[1])\n"
+            "T.lib().tblis_tensor_mult(None,None,C.byref(a.d),b'ab',C.byref(b.d),b'c',"
+            "C.byref(c.d),b'c')\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True)
+    assert r.returncode != 0 and "tblis: invalid contraction" in r.stderr
+    for name in ("mi355x", "fp64_mfma", "bf16x3", "bf16x3_fast", "reference"):
+        assert T.get_config(name)
+
+
+def _product_scatter(lens, strides, BS, off, size, pack_3d, es):
+    n = len(lens)
+    out = (C.c_ssize_t * max(size, 1))()
+    bs = (C.c_ssize_t * max(size, 1))()
+    T.lib().tblis_hip_set_error_mode(1)
+    T.lib().tblis_hip_debug_fill_block_scatter(n, (C.c_ssize_t * max(n, 1))(*lens),
+                                               (C.c_ssize_t * max(n, 1))(*strides), BS, off, size,
+                                               out, bs, int(pack_3d), es)
+    assert T.last_error() == ""
+    return np.array(out[:size]), np.array(bs[:size])
+
+
+@pytest.mark.parametrize("seed", range(200))
+def test_block_scatter_bit_exact(seed):
+    rng = np.random.default_rng(2000 + seed)
+    pack = seed % 2 == 1
+    nd = int(rng.integers(2 if pack else 0, 6))
+    lens = [int(rng.integers(1, 14)) for _ in range(nd)]
+    strides = [int(rng.integers(-50, 5000)) for _ in range(nd)]
+    tot = int(np.prod(lens)) if nd else 1
+    off = int(rng.integers(0, tot)) if seed % 3 else 0
+    size = int(rng.integers(1, tot - off + 1)) if seed % 5 else tot - off
+    if nd == 0:
+        off, size = 0, 1
+    BS = int(rng.choice([1, 2, 3, 4, 6, 8, 16, 32, 64, 128]))
+    es = int(rng.choice([4, 8]))
+    got, gbs = _product_scatter(lens, strides, BS, off, size, pack, es)
+    want = po.fill_scatter(lens, strides, BS, off, size, pack, es)
+    assert (got == want).all()
+    wbs = po.fill_block_stride(BS, want)
+    assert (gbs[::BS] == wbs[::BS]).all()
+
+
+def test_block_scatter_rejects_reference_crash_case():
+    T.lib().tblis_hip_set_error_mode(1)
+    out = (C.c_ssize_t * 30)()
+    T.lib().tblis_hip_debug_fill_block_scatter(2, (C.c_ssize_t * 2)(10, 3), (C.c_ssize_t * 2)(1, 10),
+                                               5, 0, 30, out, None, 1, 8)
+    assert "gcd" in T.last_error()
+
+
+# ---------------------------------------------------------------- DPD
+def _dpd_desc(irrep, nirrep, lens, layout):
+    import torch
+    return T._DpdDesc(torch.empty(0, dtype=torch.float64), irrep, nirrep, lens, layout)
+
+
+def test_dpd_layout_known_answers_product():
+    kat = json.load(open(os.path.join(GOLD, "dpd_varray_kat.json")))
+    for j, code in enumerate(kat["layout_codes"]):
+        d = _dpd_desc(kat["irrep"], kat["nirrep"], kat["len"], code)
+        assert T.lib().tblis_dpd_tensor_size(C.byref(d.d)) == kat["total_size"]
+        for i, irr in enumerate(kat["irreps"]):
+            ln = (C.c_ssize_t * 4)()
+            st = (C.c_ssize_t * 4)()
+            off = C.c_ssize_t()
+            T.lib().tblis_hip_debug_dpd_block(C.byref(d.d), (C.c_uint * 4)(*irr), ln, C.byref(off), st)
+            assert list(ln) == kat["lengths"][i]
+            assert list(st) == kat["strides"][j][i]
+            assert off.value == kat["offsets"][j][i]
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_dpd_layout_matches_oracle(seed):
+    rng = np.random.default_rng(3000 + seed)
+    nirrep = [1, 2, 4, 8][seed % 4]
+    ndim = int(rng.integers(1, 6))
+    lens = [[int(rng.integers(0, 5)) for _ in range(nirrep)] for _ in range(ndim)]
+    irrep = int(rng.integers(0, nirrep))
+    for layout in range(6):
+        d = _dpd_desc(irrep, nirrep, lens, layout)
+        od = po.dpd_init(irrep, nirrep, lens, layout)
+        assert T.lib().tblis_dpd_tensor_size(C.byref(d.d)) == po.dpd_size(od)
+        for combo in itertools.islice(itertools.product(range(nirrep), repeat=ndim), 64):
+            if np.bitwise_xor.reduce(combo) != irrep:
+                continue
+            ln = (C.c_ssize_t * ndim)()
+            st = (C.c_ssize_t * ndim)()
+            off = C.c_ssize_t()
+            T.lib().tblis_hip_debug_dpd_block(C.byref(d.d), (C.c_uint * ndim)(*combo), ln,
+                                              C.byref(off), st)
+            assert (list(ln), off.value, list(st)) == po.dpd_get_block(od, list(combo))
+
+
+def _product_patches(dA, iA, dB, iB, dC, iC, which, r, BSr, BSc):
+    npatch = (C.c_int * 2)()
+    mp, ms = 1 << 14, 1 << 20
+    rows = (C.c_ssize_t * mp)()
+    cols = (C.c_ssize_t * mp)()
+    base = (C.c_ssize_t * mp)()
+    rs = (C.c_ssize_t * ms)()
+    cs = (C.c_ssize_t * ms)()
+    T.lib().tblis_hip_set_error_mode(1)
+    n = T.lib().tblis_hip_debug_dpd_patches(C.byref(dA.d), iA.encode(), C.byref(dB.d), iB.encode(),
+                                            C.byref(dC.d), iC.encode(), which, r, BSr, BSc, npatch,
+                                            rows, cols, base, rs, cs, mp, ms)
+    assert n >= 0, T.last_error()
+    nr, nc = sum(rows[:n]), sum(cols[:n])
+    return {"npatch": (npatch[0], npatch[1]), "rows": np.array(rows[:n]),
+            "cols": np.array(cols[:n]), "base": np.array(base[:n]), "rscat": np.array(rs[:nr]),
+            "cscat": np.array(cs[:nc])}
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_dpd_patch_scatter_bit_exact(seed):
+    rng = np.random.default_rng(4000 + seed)
+    nirrep = [2, 4, 8][seed % 3]
+    layout = seed % 6
+    cases = [("abef", "efij", "abij"), ("aebf", "fjei", "aibj"), ("abe", "eij", "abij")]
+    iA, iB, iC = cases[seed % 3]
+    ext = {c: [int(rng.integers(0, 4)) for _ in range(nirrep)] for c in set(iA + iB + iC)}
+    irA, irB = int(rng.integers(0, nirrep)), int(rng.integers(0, nirrep))
+    irC = irA ^ irB
+    lens = lambda idx: [ext[c] for c in idx]
+    oA, oB, oC = (po.dpd_init(irA, nirrep, lens(iA), layout), po.dpd_init(irB, nirrep, lens(iB), layout),
+                  po.dpd_init(irC, nirrep, lens(iC), layout))
+    pA, pB, pC = (_dpd_desc(irA, nirrep, lens(iA), layout), _dpd_desc(irB, nirrep, lens(iB), layout),
+                  _dpd_desc(irC, nirrep, lens(iC), layout))
+    for which in range(3):
+        for r in range(nirrep):
+            BSr, BSc = [(4, 2), (8, 4), (2, 8)][which]
+            want = po.dpd_patch_scatter(oA, iA, oB, iB, oC, iC, which, r, BSr, BSc)
+            got = _product_patches(pA, iA, pB, iB, pC, iC, which, r, BSr, BSc)
+            assert got["npatch"] == want["npatch"]
+            for k in ("rows", "cols", "base", "rscat", "cscat"):
+                assert (got[k] == want[k]).all(), (which, r, k)

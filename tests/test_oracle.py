@@ -1,0 +1,358 @@
+"""Pins the oracle (oracle/oracle.c) against the reference's own known-answer
+tests, closed forms, brute force and invariants.  CPU only."""
+import itertools
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.helpers import col_major, layout_for, random_contraction, row_major, storage_size
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---------------------------------------------------------------- DPD layout
+def test_dpd_layout_known_answers():
+    """src/external/marray/test/dpd_varray.cxx:7-55: 6 layouts x 8 blocks."""
+    kat = json.load(open(os.path.join(GOLD, "dpd_varray_kat.json")))
+    for j, code in enumerate(kat["layout_codes"]):
+        d = po.dpd_init(kat["irrep"], kat["nirrep"], kat["len"], code)
+        assert po.dpd_size(d) == kat["total_size"]
+        assert list(d.perm)[:4] == kat["perms"][j]
+        for i, irr in enumerate(kat["irreps"]):
+            ln, off, st = po.dpd_get_block(d, irr)
+            assert ln == kat["lengths"][i], (kat["layouts"][j], irr)
+            assert st == kat["strides"][j][i], (kat["layouts"][j], irr)
+            assert off == kat["offsets"][j][i], (kat["layouts"][j], irr)
+
+
+def _closed_form_size(irrep, lens, nirrep):
+    # dpd_marray_base.hpp:460-496
+    size = [lens[0][r] for r in range(nirrep)]
+    for row in lens[1:]:
+        size = [sum(size[r1 ^ r2] * row[r2] for r2 in range(nirrep)) for r1 in range(nirrep)]
+    return size[irrep]
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_dpd_layout_tiles_storage(seed):
+    """Every allowed block is a dense box; the blocks tile [0, size) exactly."""
+    rng = np.random.default_rng(seed)
+    nirrep = [1, 2, 4, 8][seed % 4]
+    ndim = int(rng.integers(1, 5))
+    lens = [[int(rng.integers(0, 4)) for _ in range(nirrep)] for _ in range(ndim)]
+    irrep = int(rng.integers(0, nirrep))
+    for layout in range(6):
+        d = po.dpd_init(irrep, nirrep, lens, layout)
+        total = po.dpd_size(d)
+        assert total == _closed_form_size(irrep, lens, nirrep)
+        seen = np.zeros(total, np.int64)
+        for combo in itertools.product(range(nirrep), repeat=ndim):
+            if np.bitwise_xor.reduce(combo) != irrep:
+                continue
+            ln, off, st = po.dpd_get_block(d, list(combo))
+            assert ln == [lens[i][combo[i]] for i in range(ndim)]
+            if 0 in ln:
+                continue
+            for idx in itertools.product(*[range(l) for l in ln]):
+                seen[off + sum(i * s for i, s in zip(idx, st))] += 1
+        assert (seen == 1).all()
+
+
+# ---------------------------------------------------------------- scatter
+def _plain_offsets(lens, strides, off, size):
+    out = []
+    for i in range(off, off + size):
+        f, o = i, 0
+        for l, s in zip(lens, strides):
+            o += (f % l) * s
+            f //= l
+        out.append(o)
+    return np.array(out, np.int64)
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_fill_scatter_plain_closed_form(seed):
+    rng = np.random.default_rng(seed)
+    nd = int(rng.integers(0, 5))
+    lens = [int(rng.integers(1, 6)) for _ in range(nd)]
+    strides = [int(rng.integers(-20, 40)) for _ in range(nd)]
+    tot = int(np.prod(lens)) if nd else 1
+    off = int(rng.integers(0, tot))
+    size = int(rng.integers(1, tot - off + 1))
+    if nd == 0:
+        off, size = 0, 1
+    got = po.fill_scatter(lens, strides, int(rng.integers(1, 9)), off, size, False)
+    assert (got == _plain_offsets(lens, strides, off, size)).all()
+
+
+def test_fill_scatter_pack3d_hand_example():
+    """m0=10, m1=3 column-major, BS=2, CL=8 doubles: BS0=min(lcm(2,8),10)=8,
+    BS1=min(8,max(2,2))=2; full 8x2 and 8x1 tiles first, then the two
+    leftover columns of dim 0 row by row."""
+    got = po.fill_scatter([10, 3], [1, 10], 2, 0, 30, True, 8)
+    want = list(range(0, 8)) + list(range(10, 18)) + list(range(20, 28)) + [8, 9, 18, 19, 28, 29]
+    assert got.tolist() == want
+
+
+@pytest.mark.parametrize("seed", range(60))
+def test_fill_scatter_pack3d_is_permutation_of_range(seed):
+    """The pack_3d order covers exactly the multi-indices of [off, off+size)."""
+    rng = np.random.default_rng(100 + seed)
+    nd = int(rng.integers(2, 5))
+    lens = [int(rng.integers(1, 13)) for _ in range(nd)]
+    strides = col_major(lens)  # unique offsets
+    tot = int(np.prod(lens))
+    off = int(rng.integers(0, tot)) if seed % 2 else 0
+    size = int(rng.integers(1, tot - off + 1)) if seed % 3 else tot - off
+    BS = int(rng.choice([1, 2, 3, 4, 6, 8, 16]))
+    es = int(rng.choice([4, 8, 16]))
+    got = po.fill_scatter(lens, strides, BS, off, size, True, es)
+    want = _plain_offsets(lens, strides, off, size)
+    assert sorted(got.tolist()) == sorted(want.tolist())
+
+
+@pytest.mark.parametrize("seed", range(30))
+def test_fill_block_stride_brute_force(seed):
+    rng = np.random.default_rng(200 + seed)
+    n = int(rng.integers(1, 40))
+    scat = np.cumsum(rng.choice([1, 1, 1, 2, 5], size=n)) - 1
+    BS = int(rng.integers(1, 9))
+    bs = po.fill_block_stride(BS, scat)
+    for i in range(0, n, BS):
+        blk = scat[i:i + BS]
+        if len(blk) == 1:
+            assert bs[i] == 1
+        else:
+            d = np.diff(blk)
+            assert bs[i] == (d[0] if (d == d[0]).all() else 0)
+
+
+def test_gcd_lcm():
+    """The reference's binary gcd (block_scatter_matrix.hpp:18-42) shifts once
+    per subtraction.  Worked by hand: gcd(7,5): 7,5 -> 1,5 -> 1,2 -> 1,0 ->
+    0,0 -> 0.  With a power-of-two second argument (the cache-line length it
+    is used with) it is either exact or 0."""
+    assert po.gcd(12, 18) == 6 and po.gcd(64, 8) == 8 and po.gcd(0, 9) == 9
+    assert po.gcd(6, 8) == 2 and po.gcd(3, 8) == 1 and po.gcd(24, 8) == 8
+    assert po.gcd(7, 5) == 0 and po.gcd(5, 8) == 0
+    for CL in (4, 8, 16):
+        for BS in range(1, 129):
+            assert po.gcd(BS, CL) in (0, math.gcd(BS, CL))
+            if BS & (BS - 1) == 0:
+                assert po.lcm(BS, CL) == max(BS, CL)
+    assert po.lcm(5, 8) == 0
+
+
+# ---------------------------------------------------------------- canonical form
+def _ext(**kw):
+    return kw
+
+
+def _T(idx, ext, layout=col_major):
+    lens = [ext[c] for c in idx]
+    return lens, layout(lens), idx
+
+
+def test_canonical_worked_forms():
+    """Hand-derived canonical forms of the dense workloads."""
+    e = _ext(a=24, b=24, e=24, f=24, i=24, j=24)
+    t = po.blis_reorder(po.canonicalize(*_T("abef", e), *_T("efij", e), *_T("abij", e))).as_dict()
+    assert (t["len_m"], t["len_n"], t["len_k"]) == ([576], [576], [576])
+    assert (t["sA_m"], t["sC_m"], t["sB_n"], t["sC_n"], t["sA_k"], t["sB_k"]) == \
+        ([1], [1], [576], [576], [576], [1])
+    e = _ext(a=64, b=64, e=64, f=64, i=32, j=32)
+    t = po.blis_reorder(po.canonicalize(*_T("aebf", e), *_T("fjei", e), *_T("aibj", e))).as_dict()
+    assert t["len_m"] == [64, 64] and t["sA_m"] == [1, 4096] and t["sC_m"] == [1, 2048]
+    assert t["len_n"] == [32, 32] and t["sB_n"] == [131072, 64] and t["sC_n"] == [64, 131072]
+    assert t["len_k"] == [64, 64] and t["sA_k"] == [262144, 64] and t["sB_k"] == [1, 2048]
+    assert t["pack_3d"] == (0, 0, 0)
+    e = _ext(a=96, b=96, c=96, d=96, e=96, f=96)
+    t = po.blis_reorder(po.canonicalize(*_T("aebf", e), *_T("ecfd", e), *_T("abcd", e))).as_dict()
+    assert t["sA_m"] == [1, 9216] and t["sC_m"] == [1, 96]
+    assert t["sB_n"] == [96, 884736] and t["sC_n"] == [9216, 884736]
+    assert t["sA_k"] == [96, 884736] and t["sB_k"] == [1, 9216]
+
+
+def test_canonical_dummy_dims_and_invalid():
+    # C holds only batch labels: dummy labels \0 (A, C) and \1 (B, C) added
+    g = po.canonicalize([3], [1], "a", [3], [1], "a", [3], [1], "a").as_dict()
+    assert g["idx_abc"] == "a" and g["idx_ac"] == "\x00" and g["idx_bc"] == "\x01"
+    assert g["len_ac"] == [1] and g["len_bc"] == [1]
+    # A-only label
+    assert po.canonicalize([2, 3], [1, 2], "ab", [3], [1], "c", [3], [1], "c") is None
+    # free_idx
+    assert po.free_idx("abc") == b"\x00"
+    assert po.free_idx("\x00\x01\x03") == b"\x02"
+
+
+def test_diagonal_and_fold():
+    ln, st, ix = po.diagonal([3, 1, 3, 4], [1, 3, 3, 9], "aeab")
+    assert (ln, st, ix) == ([3, 4], [4, 9], "ab")
+    ln, ix, ss = po.fold([4, 5, 6], "abc", [[1, 4, 20], [1, 4, 99]])
+    assert (ln, ix, ss) == ([20, 6], "ac", [[1, 20], [1, 99]])
+    assert po.sort_by_stride([[5, 1, 5], [2, 9, 3]]) == [1, 0, 2]
+
+
+def _element_triples(lensA, sA, iA, lensB, sB, iB, lensC, sC, iC):
+    labels = sorted(set(iA) | set(iB) | set(iC))
+    ext = {}
+    for lens, idx in ((lensA, iA), (lensB, iB), (lensC, iC)):
+        for l, c in zip(lens, idx):
+            ext[c] = l
+    out = []
+    for vals in itertools.product(*[range(ext[c]) for c in labels]):
+        v = dict(zip(labels, vals))
+        out.append((sum(v[c] * s for c, s in zip(iA, sA)), sum(v[c] * s for c, s in zip(iB, sB)),
+                    sum(v[c] * s for c, s in zip(iC, sC))))
+    return sorted(out)
+
+
+def _canonical_triples(g):
+    out = []
+    def offs(lens, strides):
+        res = []
+        for vals in itertools.product(*[range(l) for l in lens]):
+            res.append(sum(v * s for v, s in zip(vals, strides)))
+        return res
+    d = g.as_dict()
+    for ab in zip(offs(d["len_ab"], d["sA_ab"]), offs(d["len_ab"], d["sB_ab"])):
+        for ac in zip(offs(d["len_ac"], d["sA_ac"]), offs(d["len_ac"], d["sC_ac"])):
+            for bc in zip(offs(d["len_bc"], d["sB_bc"]), offs(d["len_bc"], d["sC_bc"])):
+                for abc in zip(offs(d["len_abc"], d["sA_abc"]), offs(d["len_abc"], d["sB_abc"]),
+                               offs(d["len_abc"], d["sC_abc"])):
+                    out.append((ab[0] + ac[0] + abc[0], ab[1] + bc[0] + abc[1],
+                                ac[1] + bc[1] + abc[2]))
+    return sorted(out)
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_canonical_preserves_every_element_address(seed):
+    """diagonal + group split + fold keep the (A, B, C) address triple of every
+    term of the contraction."""
+    rng = np.random.default_rng(300 + seed)
+    iA, iB, iC, ext = random_contraction(rng, max_dims=2, max_len=3)
+    (lA, sA), (lB, sB), (lC, sC) = [layout_for(i, ext, rng) for i in (iA, iB, iC)]
+    g = po.canonicalize(lA, sA, iA, lB, sB, iB, lC, sC, iC)
+    assert g is not None
+    assert _canonical_triples(g) == _element_triples(lA, sA, iA, lB, sB, iB, lC, sC, iC)
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_canonical_invariant_under_dim_permutation(seed):
+    rng = np.random.default_rng(400 + seed)
+    iA, iB, iC, ext = random_contraction(rng)
+    (lA, sA), (lB, sB), (lC, sC) = [layout_for(i, ext, rng) for i in (iA, iB, iC)]
+    g1 = po.canonicalize(lA, sA, iA, lB, sB, iB, lC, sC, iC).as_dict()
+    p = rng.permutation(len(iA))
+    g2 = po.canonicalize([lA[i] for i in p], [sA[i] for i in p], "".join(iA[i] for i in p),
+                         lB, sB, iB, lC, sC, iC).as_dict()
+    assert g1 == g2
+
+
+# ---------------------------------------------------------------- contraction
+@pytest.mark.parametrize("seed", range(30))
+def test_tensor_mult_matches_einsum(seed):
+    rng = np.random.default_rng(500 + seed)
+    iA, iB, iC, ext = random_contraction(rng, allow_diag=True)
+    (lA, sA), (lB, sB), (lC, sC) = [layout_for(i, ext, rng) for i in (iA, iB, iC)]
+    A = rng.uniform(-1, 1, storage_size(lA, sA))
+    B = rng.uniform(-1, 1, storage_size(lB, sB))
+    C0 = rng.uniform(-1, 1, storage_size(lC, sC))
+    alpha, beta = 0.75, [0.0, 1.0, -0.5][seed % 3]
+    view = lambda X, l, s: np.lib.stride_tricks.as_strided(X, shape=l, strides=[8 * x for x in s])
+    ref = alpha * np.einsum("%s,%s->%s" % (iA, iB, iC), view(A, lA, sA), view(B, lB, sB))
+    ref = ref + beta * view(C0, lC, sC)
+    C = C0.copy()
+    po.tensor_mult(alpha, A, lA, sA, iA, B, lB, sB, iB, beta, C, lC, sC, iC)
+    np.testing.assert_allclose(view(C, lC, sC), ref, rtol=1e-13, atol=1e-13)
+
+
+def test_tensor_mult_special_rules():
+    e = dict(a=3, b=4, k=5)
+    lA, sA = [3, 5], [1, 3]
+    lB, sB = [5, 4], [1, 5]
+    lC, sC = [3, 4], [1, 3]
+    rng = np.random.default_rng(7)
+    A, B = rng.uniform(-1, 1, 15), rng.uniform(-1, 1, 20)
+    # beta == 0 overwrites without reading C
+    C = np.full(12, np.nan)
+    po.tensor_mult(1.0, A, lA, sA, "ak", B, lB, sB, "kb", 0.0, C, lC, sC, "ab")
+    assert np.isfinite(C).all()
+    # alpha == 0 reads neither A nor B
+    An = np.full(15, np.nan)
+    C = np.ones(12)
+    po.tensor_mult(0.0, An, lA, sA, "ak", B, lB, sB, "kb", 2.0, C, lC, sC, "ab")
+    assert (C == 2.0).all()
+    # k == 0: C <- beta*C
+    C = np.ones(12)
+    po.tensor_mult(1.0, A, [3, 0], sA, "ak", B, [0, 4], sB, "kb", 3.0, C, lC, sC, "ab")
+    assert (C == 3.0).all()
+    # empty m: C untouched
+    C = np.ones(12)
+    po.tensor_mult(1.0, A, [0, 5], sA, "ak", B, lB, sB, "kb", 0.0, C, [0, 4], sC, "ab")
+    assert (C == 1.0).all()
+
+
+# ---------------------------------------------------------------- DPD contraction
+def _dpd_problem(rng, nirrep, ndims=(2, 2, 2), maxlen=3, layout=5, irreps=None):
+    iA, iB, iC = "abef", "efij", "abij"
+    if ndims != (2, 2, 2):
+        iA, iB, iC = "aef", "efi", "ai"
+    labels = sorted(set(iA + iB + iC))
+    ext = {c: [int(rng.integers(0, maxlen + 1)) for _ in range(nirrep)] for c in labels}
+    irA, irB = (irreps or (int(rng.integers(0, nirrep)), int(rng.integers(0, nirrep))))[:2]
+    irC = irA ^ irB
+    dA = po.dpd_init(irA, nirrep, [ext[c] for c in iA], layout)
+    dB = po.dpd_init(irB, nirrep, [ext[c] for c in iB], layout)
+    dC = po.dpd_init(irC, nirrep, [ext[c] for c in iC], layout)
+    return iA, iB, iC, ext, dA, dB, dC
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_dpd_mult_blockwise_matches_full_expansion(seed):
+    rng = np.random.default_rng(600 + seed)
+    nirrep = [1, 2, 4, 8][seed % 4]
+    iA, iB, iC, ext, dA, dB, dC = _dpd_problem(rng, nirrep, layout=seed % 6,
+                                               ndims=(2, 2, 2) if seed % 3 else (3, 3, 2))
+    A = rng.uniform(-1, 1, max(po.dpd_size(dA), 1))
+    B = rng.uniform(-1, 1, max(po.dpd_size(dB), 1))
+    C0 = rng.uniform(-1, 1, max(po.dpd_size(dC), 1))
+    C1, C2 = C0.copy(), C0.copy()
+    po.dpd_mult(0.5, dA, iA, A, dB, iB, B, -1.5, dC, iC, C1, mode=1)
+    po.dpd_mult(0.5, dA, iA, A, dB, iB, B, -1.5, dC, iC, C2, mode=2)
+    np.testing.assert_allclose(C1, C2, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_dpd_patch_scatter_addresses(seed):
+    """Every patch of the irrep_AB GEMM addresses exactly the elements of
+    the allowed blocks with that column irrep."""
+    rng = np.random.default_rng(700 + seed)
+    nirrep = [2, 4, 8][seed % 3]
+    iA, iB, iC, ext, dA, dB, dC = _dpd_problem(rng, nirrep, layout=5, irreps=(0, 0))
+    for r in range(nirrep):
+        p = po.dpd_patch_scatter(dA, iA, dB, iB, dC, iC, 0, r, 4, 2)
+        addr = []
+        ro = co = 0
+        for n in range(len(p["rows"])):
+            rs = p["rscat"][ro:ro + p["rows"][n]]
+            cs = p["cscat"][co:co + p["cols"][n]]
+            addr += (p["base"][n] + rs[:, None] + cs[None, :]).ravel().tolist()
+            ro += p["rows"][n]
+            co += p["cols"][n]
+        want = []
+        for combo in itertools.product(range(nirrep), repeat=4):
+            if combo[0] ^ combo[1] ^ combo[2] ^ combo[3]:
+                continue
+            if combo[2] ^ combo[3] != r:
+                continue
+            ln, off, st = po.dpd_get_block(dA, list(combo))
+            if 0 in ln:
+                continue
+            for idx in itertools.product(*[range(l) for l in ln]):
+                want.append(off + sum(i * s for i, s in zip(idx, st)))
+        assert sorted(addr) == sorted(want)

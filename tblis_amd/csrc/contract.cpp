@@ -65,6 +65,33 @@ void reset_scalar(tblis_tensor* C)
     C->conj = 0;
 }
 
+// Element offset range [lo, hi] spanned by a strided tensor (lo > hi if empty).
+void tensor_extent(const tblis_tensor& t, int64_t& lo, int64_t& hi)
+{
+    lo = hi = 0;
+    for (unsigned i = 0; i < t.ndim; i++)
+    {
+        if (t.len[i] == 0)
+        {
+            lo = 0;
+            hi = -1;
+            return;
+        }
+        int64_t span = (t.len[i] - 1) * t.stride[i];
+        if (span < 0)
+            lo += span;
+        else
+            hi += span;
+    }
+}
+
+void check_data(const tblis_tensor* t, const char* name)
+{
+    int64_t lo, hi;
+    tensor_extent(*t, lo, hi);
+    check_device_pointer(t->data, name, lo, hi, elem_size(t->type));
+}
+
 void check_tensor(const tblis_tensor* t, const label_type* idx, const char* name)
 {
     if (!t) fail(std::string(name) + ": null tensor descriptor");
@@ -216,11 +243,11 @@ void tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_typ
         const bool use_k = alpha != 0.0;
         if (use_k || beta != 1.0)
         {
-            check_device_pointer(C->data, "C");
-            if (use_k)
+            check_data(C, "C");
+            if (use_k && plan->reads_ab)
             {
-                check_device_pointer(A->data, "A");
-                check_device_pointer(B->data, "B");
+                check_data(A, "A");
+                check_data(B, "B");
             }
             run_plan(*plan, A->data, B->data, C->data, alpha, beta, use_k);
         }
@@ -303,11 +330,14 @@ void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tenso
         const bool use_k = alpha != 0.0;
         if (use_k || beta != 1.0)
         {
-            check_device_pointer(C[0]->data, "C");
-            if (use_k)
+            for (int64_t i = 0; i < count; i++)
             {
-                check_device_pointer(A[0]->data, "A");
-                check_device_pointer(B[0]->data, "B");
+                check_data(C[i], "C");
+                if (use_k && plan->reads_ab)
+                {
+                    check_data(A[i], "A");
+                    check_data(B[i], "B");
+                }
             }
             run_plan(*plan, A[0]->data, B[0]->data, C[0]->data, alpha, beta, use_k);
         }

@@ -420,11 +420,17 @@ void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const l
         const bool use_k = alpha != 0.0;
         if (use_k || beta != 1.0)
         {
-            check_device_pointer(C->data, "C");
-            if (use_k)
+            const int es = type == TYPE_DOUBLE ? 8 : 4;
+            auto check = [&](const tblis_dpd_tensor* t, const char* nm) {
+                DpdLayout L;
+                L.init(t->irrep, t->nirrep, t->ndim, t->len, t->layout);
+                check_device_pointer(t->data, nm, 0, L.total() - 1, es);
+            };
+            check(C, "C");
+            if (use_k && plan->reads_ab)
             {
-                check_device_pointer(A->data, "A");
-                check_device_pointer(B->data, "B");
+                check(A, "A");
+                check(B, "B");
             }
             run_plan(*plan, A->data, B->data, C->data, alpha, beta, use_k);
         }

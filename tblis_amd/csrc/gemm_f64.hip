@@ -9,6 +9,10 @@
 // epilogue applies alpha/beta and stores through the C scatter vectors;
 // beta == 0 overwrites C without reading it.
 //
+// v_mfma_f64_16x16x4_f64 operand layouts (checked by the lane-layout probe
+// test): A lane l holds A[l%16][l/16], B lane l holds B[l/16][l%16], and
+// accumulator register r of lane l is D[l/16 + 4r][l%16].
+//
 // Thread maps for the loads are chosen per operand by the planner: a "fast"
 // side with unit stride makes consecutive lanes touch consecutive elements
 // (coalesced), the other side is a per-thread gather.
@@ -234,7 +238,7 @@ gemm_f64_kernel(const TaskDesc* __restrict__ tasks, int ntasks, const SegDesc* _
                                 double bv = Bs[parity][kk][wn0 + j * 16 + lane % 16];
 #pragma unroll
                                 for (int r = 0; r < 4; r++)
-                                    acc[i][j][r] = fma(As[parity][kk][wm0 + i * 16 + 4 * (lane / 16) + r],
+                                    acc[i][j][r] = fma(As[parity][kk][wm0 + i * 16 + lane / 16 + 4 * r],
                                                        bv, acc[i][j][r]);
                             }
                         }
@@ -263,7 +267,7 @@ gemm_f64_kernel(const TaskDesc* __restrict__ tasks, int ntasks, const SegDesc* _
 #pragma unroll
             for (int r = 0; r < 4; r++)
             {
-                int row = wm0 + i * 16 + 4 * (lane / 16) + r;
+                int row = wm0 + i * 16 + lane / 16 + 4 * r;
                 if (row >= task.mlen) continue;
                 int64_t ro = task.c_base + side(task.c_row, row);
 #pragma unroll
@@ -339,7 +343,8 @@ int gemm_f64_occupancy(const KernelKey& k)
 
 // One 16x16x4 FP64 MFMA with A[i][k] = 1000*i + k + 1, B[k][j] = (k+1)*(j+1)
 // loaded in the operand layout the GEMM assumes; each lane writes its four
-// D values at the (row, col) positions the epilogue assumes.  Lets a test
+// D values at the (row, col) positions the epilogue assumes
+// (row l/16 + 4r, col l%16).  Lets a test
 // check the lane layout against a host product.
 __global__ void mfma_probe_kernel(double* out)
 {
@@ -349,7 +354,7 @@ __global__ void mfma_probe_kernel(double* out)
     double b = (kq + 1.0) * (i + 1.0);
     dbl4 d = {0.0, 0.0, 0.0, 0.0};
     d = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d, 0, 0, 0);
-    for (int r = 0; r < 4; r++) out[(4 * (lane / 16) + r) * 16 + lane % 16] = d[r];
+    for (int r = 0; r < 4; r++) out[(lane / 16 + 4 * r) * 16 + lane % 16] = d[r];
 }
 
 hipError_t launch_mfma_probe(double* out, hipStream_t s)

@@ -186,6 +186,8 @@ std::shared_ptr<Plan> PlanBuilder::build()
         }
     }
     if (tasks.empty()) return nullptr;
+    plan->reads_ab = !segs.empty();
+    plan->elem = elem_size_;
 
     // Uneven work (DPD, batches): longest tasks first so the static
     // round-robin over the persistent grid balances.
@@ -309,7 +311,7 @@ int device_cu_count()
     return cus[dev];
 }
 
-void check_device_pointer(const void* p, const char* what)
+void check_device_pointer(const void* p, const char* what, int64_t lo, int64_t hi, int elem)
 {
     if (!p) fail(std::string(what) + ": null data pointer");
     hipPointerAttribute_t attr;
@@ -317,12 +319,46 @@ void check_device_pointer(const void* p, const char* what)
     if (e != hipSuccess)
     {
         (void)hipGetLastError();
-        fail(std::string(what) +
-             ": data is not device-accessible memory (host pointers are not supported; "
-             "allocate with hipMalloc, hipHostMalloc or a torch CUDA tensor)");
+        fail(std::string(what) + ": data is not HIP-visible memory");
     }
-    if (attr.type == hipMemoryTypeHost && attr.devicePointer == nullptr)
-        fail(std::string(what) + ": host memory that is not mapped for the device");
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    switch (attr.type)
+    {
+        case hipMemoryTypeDevice:
+            if (attr.device != cur)
+                fail(std::string(what) + ": data lives on HIP device " + std::to_string(attr.device) +
+                     ", not on the current device " + std::to_string(cur));
+            break;
+        case hipMemoryTypeManaged:
+        case hipMemoryTypeUnified:
+            break;
+        case hipMemoryTypeHost:
+            if (attr.devicePointer) break;
+            fail(std::string(what) + ": host memory that is not mapped for the device");
+        default:
+            fail(std::string(what) +
+                 ": data is not device-accessible memory (host pointers are not supported; "
+                 "allocate with hipMalloc, hipHostMalloc or a torch CUDA tensor)");
+    }
+    // Every element the plan touches must lie inside the allocation.
+    if (attr.type == hipMemoryTypeDevice && lo <= hi)
+    {
+        hipDeviceptr_t base = nullptr;
+        size_t size = 0;
+        if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) == hipSuccess && base)
+        {
+            const char* b = (const char*)base;
+            const char* q = (const char*)p;
+            if (q + lo * elem < b || q + (hi + 1) * elem > b + size)
+                fail(std::string(what) + ": the tensor's lengths and strides reach outside its "
+                     "allocation");
+        }
+        else
+        {
+            (void)hipGetLastError();
+        }
+    }
 }
 
 void run_plan(const Plan& p, const void* A, const void* B, void* C, double alpha, double beta,
@@ -339,7 +375,7 @@ void run_plan(const Plan& p, const void* A, const void* B, void* C, double alpha
     L.C = C;
     L.alpha = alpha;
     L.beta = beta;
-    L.use_k = use_k ? 1 : 0;
+    L.use_k = use_k && p.reads_ab ? 1 : 0;
     L.grid = p.grid;
     L.stream = ts.stream;
     hip_check(launch_gemm_f64(L, p.key), "kernel launch");

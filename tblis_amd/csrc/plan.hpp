@@ -42,6 +42,8 @@ struct Plan
     int ntasks = 0;
     int grid = 0;
     int64_t useful_flops = 0;
+    bool reads_ab = false;   // some task has k segments
+    int elem = 8;            // element size of A, B, C
     int64_t mfma_flops = 0;  // incl. tile padding at 16-row MFMA granularity
     void* dev = nullptr;     // one allocation: tasks | segs | pool
     TaskDesc* d_tasks = nullptr;
@@ -89,8 +91,10 @@ void cache_clear();
 void run_plan(const Plan& p, const void* A, const void* B, void* C, double alpha, double beta,
               bool use_k);
 
-// Device pointer check for a tensor that will be accessed.
-void check_device_pointer(const void* p, const char* what);
+// Device pointer check for a tensor that will be accessed: the memory must
+// be device-accessible on the current device and, for device allocations,
+// element offsets [lo, hi] (in elements of size elem) must lie inside it.
+void check_device_pointer(const void* p, const char* what, int64_t lo, int64_t hi, int elem);
 
 int device_cu_count();
 

@@ -97,6 +97,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-cols", type=int, default=256, help="n columns in the CPU sample")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run one plain device copy of MIB MiB (reference point for "
+                         "FETCH_SIZE/WRITE_SIZE counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -114,9 +117,16 @@ def main():
 
     wl, flops, nbytes, lens = permuted_workload(dev)
     a, ia, b, ib, c, ic, alpha, beta = wl
-    fn = lambda: T.mult(alpha, a, ia, b, ib, beta, c, ic)
+    fn = T.prepare_mult(alpha, a, ia, b, ib, beta, c, ic)
     ms_step, ms_kernel, ms_kernel_min = time_steps(fn, args.steps, args.warmup, stream)
     T.set_async(False)
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 8
+        src = torch.rand(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
     value = flops / (ms_step * 1e-3) / 1e12
     achieved = flops / (ms_kernel * 1e-3) / 1e12
 

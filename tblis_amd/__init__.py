@@ -14,7 +14,8 @@ __all__ = ["lib", "mult", "dpd_mult", "mult_batch", "Tensor", "DpdTensor", "get_
            "set_stream", "set_async", "last_error", "last_algorithm", "DPD_LAYOUTS", "TblisError"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libtblis_amd.so")
+# TBLIS_AMD_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("TBLIS_AMD_LIB", os.path.join(_HERE, "libtblis_amd.so"))
 
 TYPE_SINGLE, TYPE_DOUBLE, TYPE_SCOMPLEX, TYPE_DCOMPLEX = 0, 1, 2, 3
 
@@ -180,16 +181,46 @@ def _check(fn):
         raise TblisError("%s: %s" % (fn, err))
 
 
+class _Prepared:
+    """A contraction whose descriptors are built once; calling it re-issues
+    the C-ABI call (the tensors' storage must stay the same)."""
+
+    def __init__(self, fn, args, keep):
+        self._fn, self._args, self._keep = fn, args, keep
+
+    def __call__(self):
+        L = lib()
+        L.tblis_hip_set_error_mode(1)
+        self._fn(*self._args)
+        err = last_error()
+        if err:
+            raise TblisError(err)
+
+
+def prepare_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None):
+    """Build the descriptors of mult() once; returns a callable."""
+    dA, dB, dC = _Desc(A, alpha), _Desc(B), _Desc(C_, beta)
+    cfg = get_config(config) if config else None
+    args = (None, cfg, C.byref(dA.d), _labels(idx_A, A.dim(), "A"), C.byref(dB.d),
+            _labels(idx_B, B.dim(), "B"), C.byref(dC.d), _labels(idx_C, C_.dim(), "C"))
+    fn = lib().tblis_tensor_mult
+
+    def run():
+        # the library resets C's scalar to 1 after a call (as the reference
+        # does): restore beta for the next call
+        _set_scalar(dC.d.scalar, beta, dC.d.type)
+        fn(*args)
+
+    return _Prepared(run, (), (dA, dB, dC))
+
+
 def mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None):
     """C_ <- alpha * A.B + beta * C_ (tblis::mult).  Tensors: torch float32
     or float64 on the current HIP device, any strides."""
-    dA, dB, dC = _Desc(A, alpha), _Desc(B), _Desc(C_, beta)
-    cfg = get_config(config) if config else None
-    L = lib()
-    L.tblis_hip_set_error_mode(1)
-    L.tblis_tensor_mult(None, cfg, C.byref(dA.d), _labels(idx_A, A.dim(), "A"), C.byref(dB.d),
-                        _labels(idx_B, B.dim(), "B"), C.byref(dC.d), _labels(idx_C, C_.dim(), "C"))
-    _check("tblis_tensor_mult")
+    try:
+        prepare_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config)()
+    except TblisError as e:
+        raise TblisError("tblis_tensor_mult: %s" % e) from None
     return C_
 
 
@@ -234,8 +265,8 @@ def dpd_mult(alpha, A, irrep_A, idx_A, B, irrep_B, idx_B, beta, C_, irrep_C, idx
     return C_
 
 
-def mult_batch(alpha, As, idx_As, Bs, idx_Bs, beta, Cs, idx_Cs, config=None):
-    """Independent contractions C_i <- alpha*A_i.B_i + beta*C_i, one launch."""
+def prepare_batch(alpha, As, idx_As, Bs, idx_Bs, beta, Cs, idx_Cs, config=None):
+    """Build the descriptor arrays of mult_batch() once; returns a callable."""
     n = len(As)
     dA = [_Desc(a, alpha) for a in As]
     dB = [_Desc(b) for b in Bs]
@@ -248,8 +279,21 @@ def mult_batch(alpha, As, idx_As, Bs, idx_Bs, beta, Cs, idx_Cs, config=None):
     lb = (C.c_char_p * n)(*[_labels(s, b.dim(), "B") for s, b in zip(idx_Bs, Bs)])
     lc = (C.c_char_p * n)(*[_labels(s, c.dim(), "C") for s, c in zip(idx_Cs, Cs)])
     cfg = get_config(config) if config else None
-    L = lib()
-    L.tblis_hip_set_error_mode(1)
-    L.tblis_tensor_mult_batch(None, cfg, n, pa, la, pb, lb, pc, lc)
-    _check("tblis_tensor_mult_batch")
+    args = (None, cfg, n, pa, la, pb, lb, pc, lc)
+
+    def run():
+        # the library resets each C scalar to 1 after a call: restore beta
+        for d in dC:
+            _set_scalar(d.d.scalar, beta, d.d.type)
+        lib().tblis_tensor_mult_batch(*args)
+
+    return _Prepared(run, (), (dA, dB, dC, pa, pb, pc, la, lb, lc))
+
+
+def mult_batch(alpha, As, idx_As, Bs, idx_Bs, beta, Cs, idx_Cs, config=None):
+    """Independent contractions C_i <- alpha*A_i.B_i + beta*C_i, one launch."""
+    try:
+        prepare_batch(alpha, As, idx_As, Bs, idx_Bs, beta, Cs, idx_Cs, config)()
+    except TblisError as e:
+        raise TblisError("tblis_tensor_mult_batch: %s" % e) from None
     return Cs

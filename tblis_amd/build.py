@@ -18,11 +18,12 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
             "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def _compile(src):
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = ["hipcc"] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+def _compile(src, build_dir=None, defines=()):
+    obj = os.path.join(build_dir or BUILD, os.path.splitext(src)[0] + ".o")
+    flags = HIPFLAGS + ["-D" + d for d in defines]
+    cmd = ["hipcc"] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
     if src.endswith(".cpp"):
-        cmd = ["hipcc", "-x", "hip"] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = ["hipcc", "-x", "hip"] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:
@@ -30,21 +31,25 @@ def _compile(src):
     return obj, err
 
 
-def build(verbose=False):
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose=False, out=OUT, defines=()):
+    """Compile every source and link `out`.  `defines` (NAME=VALUE strings)
+    build a variant of the kernels for tuning runs."""
+    build_dir = BUILD if not defines else BUILD + "_" + "_".join(
+        d.replace("=", "") for d in defines)
+    os.makedirs(build_dir, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        results = list(ex.map(_compile, SOURCES))
+        results = list(ex.map(lambda s: _compile(s, build_dir, defines), SOURCES))
     for _, err in results:
         if verbose and err.strip():
             print(err, file=sys.stderr)
-    tmp = OUT + ".tmp"
+    tmp = out + ".tmp"
     cmd = ["hipcc", "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + \
         [o for o, _ in results]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), r.stderr))
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

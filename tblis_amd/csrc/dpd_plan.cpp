@@ -426,11 +426,16 @@ void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const l
                 L.init(t->irrep, t->nirrep, t->ndim, t->len, t->layout);
                 check_device_pointer(t->data, nm, 0, L.total() - 1, es);
             };
-            check(C, "C");
-            if (use_k && plan->reads_ab)
+            const bool k = use_k && plan->reads_ab;
+            if (!plan->already_verified(A->data, B->data, C->data, k))
             {
-                check(A, "A");
-                check(B, "B");
+                check(C, "C");
+                if (k)
+                {
+                    check(A, "A");
+                    check(B, "B");
+                }
+                plan->mark_verified(A->data, B->data, C->data, k);
             }
             run_plan(*plan, A->data, B->data, C->data, alpha, beta, use_k);
         }

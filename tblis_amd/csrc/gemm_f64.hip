@@ -29,24 +29,46 @@ namespace tblis_amd
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
+#ifndef TBLIS_SCHED_BARRIER
+#define TBLIS_SCHED_BARRIER 1
+#endif
+
 namespace
 {
 
-constexpr int NT = 256;
+// offset of logical index i of one matrix side
+__device__ __forceinline__ int64_t side_off(const int64_t* __restrict__ pool, const SideRef& r, int64_t i)
+{
+    return r.stride == SCATTERED ? pool[r.start + i] : (r.start + i) * r.stride;
+}
 
-template <int BM, int BN, int BK, bool A_MFAST, bool B_NFAST, typename T, bool MFMA>
-__global__ void __launch_bounds__(NT)
+// (offset of the first k of tile kt, block stride or SCATTERED) of a k side;
+// uniform across the workgroup.
+__device__ __forceinline__ void k_block(const int64_t* __restrict__ pool, const SideRef& r,
+                                        int64_t tbl, int64_t kt, int BK, int64_t& base, int64_t& stride)
+{
+    if (r.stride != SCATTERED)
+    {
+        base = (r.start + kt * BK) * r.stride;
+        stride = r.stride;
+    }
+    else
+    {
+        base = pool[tbl + 2 * kt];
+        stride = pool[tbl + 2 * kt + 1];
+    }
+}
+
+template <int BM, int BN, int BK, int WGM, int WGN, bool A_MFAST, bool B_NFAST, typename T, bool MFMA>
+__global__ void __launch_bounds__(64 * WGM * WGN)
 gemm_f64_kernel(const TaskDesc* __restrict__ tasks, int ntasks, const SegDesc* __restrict__ segs,
                 const int64_t* __restrict__ pool, const T* __restrict__ A, const T* __restrict__ B,
                 T* __restrict__ C, double alpha, double beta, int use_k)
 {
-    // offset of logical index i of one matrix side
-    auto side = [pool](const SideRef& r, int64_t i) -> int64_t {
-        return r.stride == SCATTERED ? pool[r.start + i] : (r.start + i) * r.stride;
-    };
-
+    constexpr int NT = 64 * WGM * WGN;
     constexpr int PAD = 1;
-    constexpr int WM = BM / 2, WN = BN / 2;
+    constexpr int WM = BM / WGM, WN = BN / WGN;
+    static_assert(WM % 16 == 0 && WN % 16 == 0, "wave tile");
     constexpr int FM = WM / 16, FN = WN / 16;
     constexpr int EA = BM * BK / NT, EB = BN * BK / NT;
     static_assert(EA >= 1 && EB >= 1, "tile too small");
@@ -58,8 +80,16 @@ gemm_f64_kernel(const TaskDesc* __restrict__ tasks, int ntasks, const SegDesc* _
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int wm0 = (wave & 1) * WM;
-    const int wn0 = (wave >> 1) * WN;
+    const int wm0 = (wave % WGM) * WM;
+    const int wn0 = (wave / WGM) * WN;
+
+    // this thread's tile coordinates for the loads (m-fast / k-fast maps)
+    const int a_i = A_MFAST ? tid % BM : tid / BK;   // first row
+    const int a_k = A_MFAST ? tid / BM : tid % BK;   // first k
+    const int b_j = B_NFAST ? tid % BN : tid / BK;   // first col
+    const int b_k = B_NFAST ? tid / BN : tid % BK;   // first k
+    constexpr int A_RSTEP = A_MFAST ? 0 : NT / BK, A_KSTEP = A_MFAST ? NT / BM : 0;
+    constexpr int B_CSTEP = B_NFAST ? 0 : NT / BK, B_KSTEP = B_NFAST ? NT / BN : 0;
 
     int parity = 0;
 
@@ -72,139 +102,81 @@ gemm_f64_kernel(const TaskDesc* __restrict__ tasks, int ntasks, const SegDesc* _
 #pragma unroll
             for (int j = 0; j < FN; j++) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
 
-        if (use_k && task.nseg > 0)
+        const int nseg = use_k ? task.nseg : 0;
+        for (int s = 0; s < nseg; s++)
         {
-            SegDesc seg = segs[task.seg0];
-            // Offsets fixed for a whole segment: A rows (m-fast: one row per
-            // thread; k-fast: EA rows per thread), B cols likewise.
-            int64_t a_fix[A_MFAST ? 1 : EA];
-            int64_t b_fix[B_NFAST ? 1 : EB];
-            auto load_fixed = [&]() {
-                if (A_MFAST)
-                {
-                    a_fix[0] = side(seg.a_row, min(tid % BM, task.mlen - 1));
-                }
-                else
-                {
+            const SegDesc seg = segs[task.seg0 + s];
+            const int64_t klen = seg.klen;
+            const int64_t nkt = (klen + BK - 1) / BK;
+
+            // per-thread row offsets of A and column offsets of B, fixed for
+            // the segment (rows past the tile edge repeat the last valid one)
+            int64_t a_fix[EA], b_fix[EB];
 #pragma unroll
-                    for (int j = 0; j < EA; j++)
-                        a_fix[j] = side(seg.a_row, min(tid / BK + j * (NT / BK), task.mlen - 1));
-                }
-                if (B_NFAST)
-                {
-                    b_fix[0] = side(seg.b_col, min(tid % BN, task.nlen - 1));
-                }
-                else
-                {
+            for (int j = 0; j < EA; j++)
+                a_fix[j] = seg.a_base + side_off(pool, seg.a_row, min(a_i + j * A_RSTEP, task.mlen - 1));
 #pragma unroll
-                    for (int j = 0; j < EB; j++)
-                        b_fix[j] = side(seg.b_col, min(tid / BK + j * (NT / BK), task.nlen - 1));
-                }
-            };
+            for (int j = 0; j < EB; j++)
+                b_fix[j] = seg.b_base + side_off(pool, seg.b_col, min(b_j + j * B_CSTEP, task.nlen - 1));
 
             double ra[EA], rb[EB];
-            auto load_tile = [&](int64_t k0) {
-                const int64_t klen = seg.klen;
-                if (A_MFAST)
-                {
+
+            // global -> registers for k tile kt
+            auto load = [&](int64_t kt) {
+                int64_t akb, aks, bkb, bks;
+                k_block(pool, seg.a_col, seg.a_kblk, kt, BK, akb, aks);
+                k_block(pool, seg.b_row, seg.b_kblk, kt, BK, bkb, bks);
+                const int64_t k0 = kt * BK;
+                const bool full = k0 + BK <= klen;
 #pragma unroll
-                    for (int j = 0; j < EA; j++)
-                    {
-                        int64_t kg = k0 + tid / BM + j * (NT / BM);
-                        bool ok = kg < klen;
-                        int64_t co = side(seg.a_col, ok ? kg : klen - 1);
-                        double v = (double)A[seg.a_base + a_fix[0] + co];
-                        ra[j] = ok ? v : 0.0;
-                    }
+                for (int j = 0; j < EA; j++)
+                {
+                    const int kk = a_k + j * A_KSTEP;
+                    const bool ok = full || k0 + kk < klen;
+                    const int64_t kc = ok ? kk : klen - 1 - k0;
+                    const int64_t co = aks != SCATTERED ? akb + kc * aks : pool[seg.a_col.start + k0 + kc];
+                    const double v = (double)A[a_fix[j] + co];
+                    ra[j] = ok ? v : 0.0;
                 }
-                else
-                {
-                    int64_t kg = k0 + tid % BK;
-                    bool ok = kg < klen;
-                    int64_t co = side(seg.a_col, ok ? kg : klen - 1);
 #pragma unroll
-                    for (int j = 0; j < EA; j++)
-                    {
-                        double v = (double)A[seg.a_base + a_fix[j] + co];
-                        ra[j] = ok ? v : 0.0;
-                    }
-                }
-                if (B_NFAST)
+                for (int j = 0; j < EB; j++)
                 {
-#pragma unroll
-                    for (int j = 0; j < EB; j++)
-                    {
-                        int64_t kg = k0 + tid / BN + j * (NT / BN);
-                        bool ok = kg < klen;
-                        int64_t ro = side(seg.b_row, ok ? kg : klen - 1);
-                        double v = (double)B[seg.b_base + ro + b_fix[0]];
-                        rb[j] = ok ? v : 0.0;
-                    }
-                }
-                else
-                {
-                    int64_t kg = k0 + tid % BK;
-                    bool ok = kg < klen;
-                    int64_t ro = side(seg.b_row, ok ? kg : klen - 1);
-#pragma unroll
-                    for (int j = 0; j < EB; j++)
-                    {
-                        double v = (double)B[seg.b_base + ro + b_fix[j]];
-                        rb[j] = ok ? v : 0.0;
-                    }
+                    const int kk = b_k + j * B_KSTEP;
+                    const bool ok = full || k0 + kk < klen;
+                    const int64_t kc = ok ? kk : klen - 1 - k0;
+                    const int64_t ro = bks != SCATTERED ? bkb + kc * bks : pool[seg.b_row.start + k0 + kc];
+                    const double v = (double)B[b_fix[j] + ro];
+                    rb[j] = ok ? v : 0.0;
                 }
             };
 
-            load_fixed();
-            load_tile(0);
-            int s = 0;
-            int64_t kt = 0;
-            while (true)
+            load(0);
+            for (int64_t kt = 0; kt < nkt; kt++)
             {
-                // registers -> LDS[parity]
-                if (A_MFAST)
-                {
 #pragma unroll
-                    for (int j = 0; j < EA; j++) As[parity][tid / BM + j * (NT / BM)][tid % BM] = ra[j];
+                for (int j = 0; j < EA; j++)
+                {
+                    if (A_MFAST)
+                        As[parity][a_k + j * A_KSTEP][a_i] = ra[j];
+                    else
+                        As[parity][a_k][a_i + j * A_RSTEP] = ra[j];
                 }
-                else
-                {
 #pragma unroll
-                    for (int j = 0; j < EA; j++) As[parity][tid % BK][tid / BK + j * (NT / BK)] = ra[j];
-                }
-                if (B_NFAST)
+                for (int j = 0; j < EB; j++)
                 {
-#pragma unroll
-                    for (int j = 0; j < EB; j++) Bs[parity][tid / BN + j * (NT / BN)][tid % BN] = rb[j];
-                }
-                else
-                {
-#pragma unroll
-                    for (int j = 0; j < EB; j++) Bs[parity][tid % BK][tid / BK + j * (NT / BK)] = rb[j];
+                    if (B_NFAST)
+                        Bs[parity][b_k + j * B_KSTEP][b_j] = rb[j];
+                    else
+                        Bs[parity][b_k][b_j + j * B_CSTEP] = rb[j];
                 }
                 __syncthreads();
+                if (kt + 1 < nkt) load(kt + 1);
+#if TBLIS_SCHED_BARRIER
+                // keep the next tile's global loads ahead of the MFMA block so
+                // their latency hides behind it
+                __builtin_amdgcn_sched_barrier(0);
+#endif
 
-                // prefetch the next tile (possibly of the next segment)
-                int64_t nkt = (seg.klen + BK - 1) / BK;
-                int ns = s;
-                int64_t nk = kt + 1;
-                if (nk >= nkt)
-                {
-                    ns = s + 1;
-                    nk = 0;
-                }
-                const bool more = ns < task.nseg;
-                if (more)
-                {
-                    if (ns != s)
-                    {
-                        seg = segs[task.seg0 + ns];
-                        load_fixed();
-                    }
-                    load_tile(nk * BK);
-                }
-
-                // compute from LDS[parity]
                 if (MFMA)
                 {
 #pragma unroll
@@ -245,9 +217,6 @@ gemm_f64_kernel(const TaskDesc* __restrict__ tasks, int ntasks, const SegDesc* _
                     }
                 }
                 parity ^= 1;
-                if (!more) break;
-                s = ns;
-                kt = nk;
             }
         }
 
@@ -259,7 +228,7 @@ gemm_f64_kernel(const TaskDesc* __restrict__ tasks, int ntasks, const SegDesc* _
         {
             int col = wn0 + j * 16 + lane % 16;
             col_ok[j] = col < task.nlen;
-            col_off[j] = side(task.c_col, col_ok[j] ? col : 0);
+            col_off[j] = side_off(pool, task.c_col, col_ok[j] ? col : 0);
         }
 #pragma unroll
         for (int i = 0; i < FM; i++)
@@ -269,7 +238,7 @@ gemm_f64_kernel(const TaskDesc* __restrict__ tasks, int ntasks, const SegDesc* _
             {
                 int row = wm0 + i * 16 + lane / 16 + 4 * r;
                 if (row >= task.mlen) continue;
-                int64_t ro = task.c_base + side(task.c_row, row);
+                int64_t ro = task.c_base + side_off(pool, task.c_row, row);
 #pragma unroll
                 for (int j = 0; j < FN; j++)
                 {
@@ -284,22 +253,39 @@ gemm_f64_kernel(const TaskDesc* __restrict__ tasks, int ntasks, const SegDesc* _
     }
 }
 
-template <int BM, int BN, typename T, bool MFMA>
+// Tile table: (id, BM, BN, BK, waves along m, waves along n).
+#define TBLIS_F64_TILES(X)      \
+    X(0, 128, 128, 16, 2, 2)    \
+    X(1, 64, 64, 16, 2, 2)      \
+    X(2, 128, 64, 16, 2, 2)     \
+    X(3, 64, 128, 16, 2, 2)     \
+    X(4, 128, 128, 16, 2, 4)    \
+    X(5, 32, 32, 16, 2, 2)      \
+    X(6, 64, 64, 32, 2, 2)      \
+    X(7, 128, 64, 16, 4, 2)
+
+template <int BM, int BN, int BK, int WGM, int WGN, typename T, bool MFMA>
 const void* select_layout(const KernelKey& k)
 {
-    constexpr int BK = 16;
-    if (k.a_mfast && k.b_nfast) return (const void*)&gemm_f64_kernel<BM, BN, BK, true, true, T, MFMA>;
-    if (k.a_mfast) return (const void*)&gemm_f64_kernel<BM, BN, BK, true, false, T, MFMA>;
-    if (k.b_nfast) return (const void*)&gemm_f64_kernel<BM, BN, BK, false, true, T, MFMA>;
-    return (const void*)&gemm_f64_kernel<BM, BN, BK, false, false, T, MFMA>;
+    if (k.a_mfast && k.b_nfast)
+        return (const void*)&gemm_f64_kernel<BM, BN, BK, WGM, WGN, true, true, T, MFMA>;
+    if (k.a_mfast) return (const void*)&gemm_f64_kernel<BM, BN, BK, WGM, WGN, true, false, T, MFMA>;
+    if (k.b_nfast) return (const void*)&gemm_f64_kernel<BM, BN, BK, WGM, WGN, false, true, T, MFMA>;
+    return (const void*)&gemm_f64_kernel<BM, BN, BK, WGM, WGN, false, false, T, MFMA>;
 }
 
 template <typename T>
 const void* select_typed(const KernelKey& k)
 {
-    if (!k.use_mfma) return select_layout<64, 64, T, false>(k);
-    if (k.tile == 0) return select_layout<128, 128, T, true>(k);
-    return select_layout<64, 64, T, true>(k);
+    if (!k.use_mfma) return select_layout<64, 64, 16, 2, 2, T, false>(k);
+    switch (k.tile)
+    {
+#define TBLIS_CASE(id, bm, bn, bk, wm, wn) \
+    case id: return select_layout<bm, bn, bk, wm, wn, T, true>(k);
+        TBLIS_F64_TILES(TBLIS_CASE)
+#undef TBLIS_CASE
+    }
+    return nullptr;
 }
 
 const void* select_kernel(const KernelKey& k)
@@ -307,14 +293,36 @@ const void* select_kernel(const KernelKey& k)
     return k.dtype == 0 ? select_typed<double>(k) : select_typed<float>(k);
 }
 
+struct TileShape
+{
+    int bm, bn, bk, threads;
+};
+
+TileShape tile_shape(int tile, int use_mfma)
+{
+    if (!use_mfma) return {64, 64, 16, 256};
+    switch (tile)
+    {
+#define TBLIS_CASE(id, bm, bn, bk, wm, wn) \
+    case id: return {bm, bn, bk, 64 * wm * wn};
+        TBLIS_F64_TILES(TBLIS_CASE)
+#undef TBLIS_CASE
+    }
+    return {64, 64, 16, 256};
+}
+
 }  // namespace
 
-int gemm_f64_tile_m(int tile) { return tile == 0 ? 128 : 64; }
-int gemm_f64_tile_n(int tile) { return tile == 0 ? 128 : 64; }
+int gemm_f64_num_tiles() { return 8; }
+int gemm_f64_tile_m(int tile) { return tile_shape(tile, 1).bm; }
+int gemm_f64_tile_n(int tile) { return tile_shape(tile, 1).bn; }
+int gemm_f64_tile_k(int tile) { return tile_shape(tile, 1).bk; }
 
 hipError_t launch_gemm_f64(const GemmLaunch& L, const KernelKey& k)
 {
     if (L.ntasks == 0) return hipSuccess;
+    const void* f = select_kernel(k);
+    if (!f) return hipErrorInvalidValue;
     const TaskDesc* tasks = L.tasks;
     int ntasks = L.ntasks;
     const SegDesc* segs = L.segs;
@@ -325,15 +333,19 @@ hipError_t launch_gemm_f64(const GemmLaunch& L, const KernelKey& k)
     double alpha = L.alpha, beta = L.beta;
     int use_k = L.use_k;
     void* args[] = {&tasks, &ntasks, &segs, &pool, &A, &B, &C, &alpha, &beta, &use_k};
-    hipError_t e = hipLaunchKernel(select_kernel(k), dim3(L.grid), dim3(NT), args, 0, L.stream);
+    hipError_t e = hipLaunchKernel(f, dim3(L.grid), dim3(tile_shape(k.tile, k.use_mfma).threads), args,
+                                   0, L.stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 
 int gemm_f64_occupancy(const KernelKey& k)
 {
+    const void* f = select_kernel(k);
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, select_kernel(k), NT, 0) != hipSuccess)
+    if (!f ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, tile_shape(k.tile, k.use_mfma).threads, 0) !=
+            hipSuccess)
     {
         (void)hipGetLastError();
         return 1;

@@ -23,7 +23,9 @@ struct GemmLaunch
     hipStream_t stream;
 };
 
+int gemm_f64_num_tiles();
 int gemm_f64_tile_m(int tile);
+int gemm_f64_tile_k(int tile);
 int gemm_f64_tile_n(int tile);
 hipError_t launch_gemm_f64(const GemmLaunch& L, const KernelKey& k);
 int gemm_f64_occupancy(const KernelKey& k);

@@ -84,6 +84,36 @@ SideRef PlanBuilder::side_ref(const Side& s, int64_t BS, int64_t start)
     return SideRef{at + start, SCATTERED};
 }
 
+int64_t PlanBuilder::kblock_table(const Side& s, int64_t BK)
+{
+    if (s.len.size() <= 1) return -1;
+    std::vector<int64_t> key;
+    key.push_back(-BK);  // distinct from scatter keys
+    key.push_back(s.pack_3d);
+    key.insert(key.end(), s.len.begin(), s.len.end());
+    key.insert(key.end(), s.stride.begin(), s.stride.end());
+    auto it = dedupe_.find(key);
+    if (it != dedupe_.end()) return it->second;
+    SideRef r = side_ref(s, BK, 0);
+    int64_t total = 1;
+    for (auto l : s.len) total *= l;
+    int64_t nblk = (total + BK - 1) / BK;
+    int64_t at = (int64_t)pool_.size();
+    pool_.resize(pool_.size() + 2 * std::max<int64_t>(nblk, 1));
+    const int64_t* scat = pool_.data() + r.start;
+    for (int64_t b = 0; b < nblk; b++)
+    {
+        int64_t k0 = b * BK, n = std::min(BK, total - k0);
+        int64_t stride = n > 1 ? scat[k0 + 1] - scat[k0] : 0;
+        for (int64_t j = k0 + 1; j + 1 < k0 + n; j++)
+            if (scat[j + 1] - scat[j] != stride) stride = SCATTERED;
+        pool_[at + 2 * b] = scat[k0];
+        pool_[at + 2 * b + 1] = stride;
+    }
+    dedupe_[key] = at;
+    return at;
+}
+
 std::shared_ptr<Plan> PlanBuilder::build()
 {
     if (gemms_.empty()) return nullptr;
@@ -113,21 +143,59 @@ std::shared_ptr<Plan> PlanBuilder::build()
     plan->key.dtype = dtype_;
     plan->key.use_mfma = use_mfma_;
 
-    // Tile: 128x128 when there are enough big tiles to fill the chip and the
-    // padding stays small; else 64x64.
-    int64_t tiles128 = 0, area = 0, area128 = 0;
-    for (auto& g : gemms_)
+    // Tile choice: a cost model over the candidate tiles.  Per resident
+    // workgroup each tile sustains a rate measured on MI355X (FP64 MFMA,
+    // GFLOP/s per workgroup at full occupancy); the estimated time is the
+    // padded work spread over min(tasks, resident slots) workgroups, but no
+    // less than the largest single task.
+    struct Cand
     {
-        int64_t m = sw ? g.n : g.m, n = sw ? g.m : g.n;
-        int64_t L = g.l_c.empty() ? 1 : (int64_t)g.l_c.size();
-        tiles128 += ceil_div(m, 128) * ceil_div(n, 128) * L;
-        area += m * n * L;
-        area128 += ceil_div(m, 128) * 128 * ceil_div(n, 128) * 128 * L;
+        int tile;
+        double gflops_per_wg;
+    };
+    static const Cand cands[] = {{1, 33.8}, {5, 11.1}, {7, 66.0}, {4, 113.0}};
+    plan->key.tile = 1;
+    if (use_mfma_)
+    {
+        double best = 0;
+        for (const Cand& c : cands)
+        {
+            KernelKey k = plan->key;
+            k.tile = c.tile;
+            const int64_t bm = gemm_f64_tile_m(c.tile), bn = gemm_f64_tile_n(c.tile);
+            const int64_t bk = gemm_f64_tile_k(c.tile);
+            double work = 0, maxwork = 0;
+            int64_t ntask = 0;
+            for (auto& g : gemms_)
+            {
+                int64_t m = sw ? g.n : g.m, n = sw ? g.m : g.n;
+                int64_t L = g.l_c.empty() ? 1 : (int64_t)g.l_c.size();
+                int64_t kpad = 0;
+                for (auto& s : g.segs) kpad += ceil_div(s.k, bk) * bk;
+                double tw = 2.0 * bm * bn * std::max<int64_t>(kpad, bk);
+                int64_t nt = ceil_div(m, bm) * ceil_div(n, bn) * L;
+                ntask += nt;
+                work += tw * nt;
+                maxwork = std::max(maxwork, tw);
+            }
+            int64_t slots = (int64_t)device_cu_count() * std::max(1, gemm_f64_occupancy(k));
+            double rate = c.gflops_per_wg * 1e9;
+            double t = std::max(work / (std::min(ntask, slots) * rate), maxwork / rate);
+            if (best == 0 || t < best)
+            {
+                best = t;
+                plan->key.tile = c.tile;
+            }
+        }
     }
-    bool big = use_mfma_ && tiles128 >= device_cu_count() && area128 <= area * 115 / 100;
-    plan->key.tile = big ? 0 : 1;
+    // TBLIS_HIP_TILE=<id> forces the tile (tuning runs)
+    if (const char* e = std::getenv("TBLIS_HIP_TILE"))
+    {
+        int t = std::atoi(e);
+        if (use_mfma_ && e[0] >= '0' && e[0] <= '9' && t < gemm_f64_num_tiles()) plan->key.tile = t;
+    }
     const int BM = gemm_f64_tile_m(plan->key.tile), BN = gemm_f64_tile_n(plan->key.tile);
-    const int BK = 16;
+    const int BK = gemm_f64_tile_k(plan->key.tile);
 
     std::vector<TaskDesc> tasks;
     std::vector<SegDesc> segs;
@@ -175,6 +243,8 @@ std::shared_ptr<Plan> PlanBuilder::build()
                         d.b_row = side_ref(br, BK, 0);
                         d.b_col = side_ref(bc, BN, tn * BN);
                         d.klen = s.k;
+                        d.a_kblk = kblock_table(ac, BK);
+                        d.b_kblk = kblock_table(br, BK);
                         segs.push_back(d);
                         t.nseg++;
                         kpad += ceil_div(s.k, BK) * BK;

@@ -44,6 +44,11 @@ struct SegDesc
     SideRef b_row;           // B rows (k) of the segment
     SideRef b_col;           // B cols of the tile
     int64_t klen;            // k extent of the segment (>= 1)
+    // Scattered k sides: pool index of a table of (offset of the block's
+    // first k, block stride or SCATTERED) pairs, one per BK-block of k, so
+    // a k tile inside which the side is affine is addressed without a
+    // per-lane lookup.  Unused (-1) for affine sides.
+    int64_t a_kblk, b_kblk;
 };
 
 // Kernel variant selector.

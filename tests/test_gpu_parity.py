@@ -286,3 +286,23 @@ def test_batch_uneven():
     for i in range(n):
         got = torch.as_strided(Cs[i], (Cs[i].numel(),), (1,)).cpu().numpy()
         assert rel_fro(got, refs[i]) <= FP64_TOL, i
+
+
+@pytest.mark.parametrize("tile", range(8))
+def test_every_tile_variant(tile, monkeypatch):
+    """Each tile shape of the FP64 kernel (forced through TBLIS_HIP_TILE)
+    agrees with the oracle on contractions with ragged edges, scattered k
+    and a DPD problem (several k segments)."""
+    monkeypatch.setenv("TBLIS_HIP_TILE", str(tile))
+    T.lib().tblis_hip_clear_plan_cache()
+    try:
+        rng = np.random.default_rng(30000 + tile)
+        _dense_case(rng, "aebf", "fjei", "aibj", dict(a=13, b=11, e=9, f=7, i=5, j=14),
+                    ["col"] * 3, alpha=0.5, beta=1.0)
+        _dense_case(rng, "abef", "efij", "abij", dict(a=37, b=3, e=70, f=1, i=29, j=2),
+                    ["row", "col", "random"])
+        ext = {c: [3, 5, 0, 2] for c in "abefij"}
+        _dpd_case(rng, 4, "abef", "efij", "abij", ext, 1, 3, 5, alpha=1.0, beta=0.5)
+    finally:
+        monkeypatch.delenv("TBLIS_HIP_TILE")
+        T.lib().tblis_hip_clear_plan_cache()

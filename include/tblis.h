@@ -274,6 +274,11 @@ int tblis_hip_debug_dpd_patches(const tblis_dpd_tensor* A, const label_type* idx
  * lane layout the kernels assume.  Returns 0 on success. */
 int tblis_hip_debug_mfma_probe(double* out_device);
 
+/* One 32x32x16 BF16 MFMA in the operand and output layouts the split-BF16
+ * kernels assume: A (32x16) and B (16x32) row-major FP32 in device memory
+ * (values exactly representable in BF16), D (32x32) row-major FP32 out. */
+int tblis_hip_debug_mfma_probe_bf16(const float* A, const float* B, float* D);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,7 +61,7 @@ EXPORTED = [
     "tblis_hip_get_stream", "tblis_hip_set_async", "tblis_hip_set_error_mode",
     "tblis_hip_get_last_error", "tblis_hip_clear_plan_cache", "tblis_hip_last_algorithm",
     "tblis_hip_debug_canonical", "tblis_hip_debug_fill_block_scatter", "tblis_hip_debug_dpd_block",
-    "tblis_hip_debug_dpd_patches", "tblis_hip_debug_mfma_probe",
+    "tblis_hip_debug_dpd_patches", "tblis_hip_debug_mfma_probe", "tblis_hip_debug_mfma_probe_bf16",
 ]
 
 
@@ -106,6 +106,8 @@ def lib():
                                                   C.c_int64, C.c_int64]
         L.tblis_hip_debug_mfma_probe.argtypes = [C.c_void_p]
         L.tblis_hip_debug_mfma_probe.restype = C.c_int
+        L.tblis_hip_debug_mfma_probe_bf16.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.tblis_hip_debug_mfma_probe_bf16.restype = C.c_int
         _LIB = L
     return _LIB
 

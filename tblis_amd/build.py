@@ -11,7 +11,7 @@ BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("TBLIS_AMD_ARCH", "gfx950")
 
 SOURCES = ["canon.cpp", "scatter.cpp", "dpd.cpp", "plan.cpp", "contract.cpp", "dpd_plan.cpp",
-           "api.cpp", "gemm_f64.hip"]
+           "api.cpp", "gemm_f64.hip", "gemm_bf16x3.hip"]
 
 # Packed-FP32 instructions are disabled for every device pass.
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",

@@ -314,4 +314,16 @@ int tblis_hip_debug_mfma_probe(double* out_device)
         -1);
 }
 
+int tblis_hip_debug_mfma_probe_bf16(const float* A, const float* B, float* D)
+{
+    return guarded(
+        [&] {
+            hipError_t e = launch_bf16_probe(A, B, D, thread_state().stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(thread_state().stream);
+            if (e != hipSuccess) fail(std::string("bf16 mfma probe: ") + hipGetErrorString(e));
+            return 0;
+        },
+        -1);
+}
+
 }  // extern "C"

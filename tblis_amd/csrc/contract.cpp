@@ -188,17 +188,18 @@ Algo resolve_algo(const tblis_config* cfg, type_t type)
 namespace
 {
 
-// Kernel family for an algorithm (the split-BF16 path is not wired to the
-// drivers yet: it runs the FP64 MFMA kernel and reports that).
-Algo effective(Algo a)
+int family_of(Algo a)
 {
-    if (a == Algo::Bf16x3 || a == Algo::Bf16x3Fast) return Algo::Fp64Mfma;
-    return a;
+    if (a == Algo::Bf16x3) return FAMILY_BF16X3;
+    if (a == Algo::Bf16x3Fast) return FAMILY_BF16X3_FAST;
+    return FAMILY_F64;
 }
 
 std::shared_ptr<Plan> empty_plan() { return std::make_shared<Plan>(); }
 
 }  // namespace
+
+int algo_family(Algo a) { return family_of(a); }
 
 void tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_type* idx_A,
                  const tblis_tensor* B, const label_type* idx_B, tblis_tensor* C,
@@ -210,7 +211,7 @@ void tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_typ
     if (A->type != B->type || A->type != C->type) fail("A, B and C must have the same type");
     check_real(A->type);
     const type_t type = A->type;
-    Algo algo = effective(resolve_algo(cfg, type));
+    Algo algo = resolve_algo(cfg, type);
     thread_state().last_algo = algo_name(algo);
 
     std::string key;
@@ -228,7 +229,7 @@ void tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_typ
         GemmDesc g;
         if (dense_gemm(c, g, 0, 0, 0))
         {
-            PlanBuilder pb(elem_size(type), dtype_code(type), algo != Algo::Reference);
+            PlanBuilder pb(elem_size(type), dtype_code(type), algo != Algo::Reference, family_of(algo));
             pb.add(g);
             plan = pb.build();
         }
@@ -289,7 +290,7 @@ void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tenso
         else if (al != alpha || be != beta)
             fail("batch: every instance must use the same alpha and beta");
     }
-    Algo algo = effective(resolve_algo(cfg, type));
+    Algo algo = resolve_algo(cfg, type);
     thread_state().last_algo = algo_name(algo);
 
     auto delta = [&](const void* p, const void* p0) -> int64_t {
@@ -314,7 +315,7 @@ void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tenso
     std::shared_ptr<Plan> plan = cache_get(key);
     if (!plan)
     {
-        PlanBuilder pb(es, dtype_code(type), algo != Algo::Reference);
+        PlanBuilder pb(es, dtype_code(type), algo != Algo::Reference, family_of(algo));
         for (int64_t i = 0; i < count; i++)
         {
             Canonical c;

@@ -32,5 +32,7 @@ int debug_dpd_patches(const tblis_dpd_tensor* A, const label_type* idx_A,
 
 // Selected algorithm for a type under a config.
 Algo resolve_algo(const tblis_config* cfg, type_t type);
+// Kernel family (plan_types.hpp FAMILY_*) of an algorithm.
+int algo_family(Algo a);
 
 }  // namespace tblis_amd

@@ -376,7 +376,6 @@ void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const l
         fail("complex tensor types are not supported by this library yet");
     const type_t type = A->type;
     Algo algo = resolve_algo(cfg, type);
-    if (algo == Algo::Bf16x3 || algo == Algo::Bf16x3Fast) algo = Algo::Fp64Mfma;
     thread_state().last_algo = algo_name(algo);
 
     std::string key;
@@ -390,7 +389,8 @@ void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const l
     {
         DpdProblem P;
         P.init(A, idx_A, B, idx_B, C, idx_C);
-        PlanBuilder pb(type == TYPE_DOUBLE ? 8 : 4, dtype_code(type), algo != Algo::Reference);
+        PlanBuilder pb(type == TYPE_DOUBLE ? 8 : 4, dtype_code(type), algo != Algo::Reference,
+                       algo_family(algo));
         build_gemms(P, pb);
         plan = pb.build();
         if (!plan) plan = std::make_shared<Plan>();

@@ -31,4 +31,31 @@ hipError_t launch_gemm_f64(const GemmLaunch& L, const KernelKey& k);
 int gemm_f64_occupancy(const KernelKey& k);
 hipError_t launch_mfma_probe(double* out, hipStream_t s);
 
+int gemm_bf16_num_tiles();
+int gemm_bf16_tile_m(int tile);
+int gemm_bf16_tile_n(int tile);
+hipError_t launch_gemm_bf16x3(const GemmLaunch& L, const KernelKey& k);
+int gemm_bf16x3_occupancy(const KernelKey& k);
+hipError_t launch_bf16_probe(const float* A, const float* B, float* D, hipStream_t s);
+
+// family-generic helpers
+inline int tile_m(const KernelKey& k)
+{
+    return k.family == FAMILY_F64 ? gemm_f64_tile_m(k.tile) : gemm_bf16_tile_m(k.tile);
+}
+inline int tile_n(const KernelKey& k)
+{
+    return k.family == FAMILY_F64 ? gemm_f64_tile_n(k.tile) : gemm_bf16_tile_n(k.tile);
+}
+inline int tile_k(const KernelKey& k) { return k.family == FAMILY_F64 ? gemm_f64_tile_k(k.tile) : 16; }
+inline int num_tiles(int family) { return family == FAMILY_F64 ? gemm_f64_num_tiles() : gemm_bf16_num_tiles(); }
+inline int occupancy(const KernelKey& k)
+{
+    return k.family == FAMILY_F64 ? gemm_f64_occupancy(k) : gemm_bf16x3_occupancy(k);
+}
+inline hipError_t launch_gemm(const GemmLaunch& L, const KernelKey& k)
+{
+    return k.family == FAMILY_F64 ? launch_gemm_f64(L, k) : launch_gemm_bf16x3(L, k);
+}
+
 }  // namespace tblis_amd

@@ -53,8 +53,8 @@ Plan::~Plan()
     }
 }
 
-PlanBuilder::PlanBuilder(int elem_size, int dtype, int use_mfma)
-    : elem_size_(elem_size), dtype_(dtype), use_mfma_(use_mfma)
+PlanBuilder::PlanBuilder(int elem_size, int dtype, int use_mfma, int family)
+    : elem_size_(elem_size), dtype_(dtype), use_mfma_(use_mfma), family_(family)
 {
 }
 
@@ -142,6 +142,7 @@ std::shared_ptr<Plan> PlanBuilder::build()
     }
     plan->key.dtype = dtype_;
     plan->key.use_mfma = use_mfma_;
+    plan->key.family = family_;
 
     // Tile choice: a cost model over the candidate tiles.  Per resident
     // workgroup each tile sustains a rate measured on MI355X (FP64 MFMA,
@@ -153,17 +154,20 @@ std::shared_ptr<Plan> PlanBuilder::build()
         int tile;
         double gflops_per_wg;
     };
-    static const Cand cands[] = {{1, 33.8}, {5, 11.1}, {7, 66.0}, {4, 113.0}};
-    plan->key.tile = 1;
+    static const Cand f64_cands[] = {{1, 33.8}, {5, 11.1}, {7, 66.0}, {4, 113.0}};
+    static const Cand bf16_cands[] = {{0, 600.0}, {1, 180.0}, {2, 350.0}};
+    const Cand* cands = family_ == FAMILY_F64 ? f64_cands : bf16_cands;
+    const int ncand = family_ == FAMILY_F64 ? 4 : 3;
+    plan->key.tile = family_ == FAMILY_F64 ? 1 : 0;
     if (use_mfma_)
     {
         double best = 0;
-        for (const Cand& c : cands)
+        for (int ci = 0; ci < ncand; ci++)
         {
+            const Cand& c = cands[ci];
             KernelKey k = plan->key;
             k.tile = c.tile;
-            const int64_t bm = gemm_f64_tile_m(c.tile), bn = gemm_f64_tile_n(c.tile);
-            const int64_t bk = gemm_f64_tile_k(c.tile);
+            const int64_t bm = tile_m(k), bn = tile_n(k), bk = tile_k(k);
             double work = 0, maxwork = 0;
             int64_t ntask = 0;
             for (auto& g : gemms_)
@@ -178,7 +182,7 @@ std::shared_ptr<Plan> PlanBuilder::build()
                 work += tw * nt;
                 maxwork = std::max(maxwork, tw);
             }
-            int64_t slots = (int64_t)device_cu_count() * std::max(1, gemm_f64_occupancy(k));
+            int64_t slots = (int64_t)device_cu_count() * std::max(1, occupancy(k));
             double rate = c.gflops_per_wg * 1e9;
             double t = std::max(work / (std::min(ntask, slots) * rate), maxwork / rate);
             if (best == 0 || t < best)
@@ -192,10 +196,10 @@ std::shared_ptr<Plan> PlanBuilder::build()
     if (const char* e = std::getenv("TBLIS_HIP_TILE"))
     {
         int t = std::atoi(e);
-        if (use_mfma_ && e[0] >= '0' && e[0] <= '9' && t < gemm_f64_num_tiles()) plan->key.tile = t;
+        if (use_mfma_ && e[0] >= '0' && e[0] <= '9' && t < num_tiles(family_)) plan->key.tile = t;
     }
-    const int BM = gemm_f64_tile_m(plan->key.tile), BN = gemm_f64_tile_n(plan->key.tile);
-    const int BK = gemm_f64_tile_k(plan->key.tile);
+    const int BM = tile_m(plan->key), BN = tile_n(plan->key);
+    const int BK = tile_k(plan->key);
 
     std::vector<TaskDesc> tasks;
     std::vector<SegDesc> segs;
@@ -276,7 +280,7 @@ std::shared_ptr<Plan> PlanBuilder::build()
         fail("contraction too large: task list exceeds 2^31 entries");
 
     plan->ntasks = (int)tasks.size();
-    int occ = gemm_f64_occupancy(plan->key);
+    int occ = occupancy(plan->key);
     plan->grid = (int)std::min<int64_t>(plan->ntasks, (int64_t)device_cu_count() * std::max(occ, 1));
 
     size_t tb = tasks.size() * sizeof(TaskDesc);
@@ -448,7 +452,7 @@ void run_plan(const Plan& p, const void* A, const void* B, void* C, double alpha
     L.use_k = use_k && p.reads_ab ? 1 : 0;
     L.grid = p.grid;
     L.stream = ts.stream;
-    hip_check(launch_gemm_f64(L, p.key), "kernel launch");
+    hip_check(launch_gemm(L, p.key), "kernel launch");
     if (!ts.async) hip_check(hipStreamSynchronize(ts.stream), "hipStreamSynchronize");
 }
 

@@ -78,7 +78,7 @@ struct Plan
 class PlanBuilder
 {
   public:
-    PlanBuilder(int elem_size, int dtype, int use_mfma);
+    PlanBuilder(int elem_size, int dtype, int use_mfma, int family = FAMILY_F64);
     void add(const GemmDesc& g) { gemms_.push_back(g); }
     // Choose swap/tile/thread maps, emit tasks, upload.  Returns nullptr if
     // there is nothing to launch.
@@ -87,7 +87,7 @@ class PlanBuilder
   private:
     SideRef side_ref(const Side& s, int64_t BS, int64_t start);
     int64_t kblock_table(const Side& s, int64_t BK);
-    int elem_size_, dtype_, use_mfma_;
+    int elem_size_, dtype_, use_mfma_, family_;
     std::vector<GemmDesc> gemms_;
     std::vector<int64_t> pool_;
     std::map<std::vector<int64_t>, int64_t> dedupe_;

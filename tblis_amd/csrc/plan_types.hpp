@@ -51,10 +51,16 @@ struct SegDesc
     int64_t a_kblk, b_kblk;
 };
 
+// Kernel families.
+constexpr int FAMILY_F64 = 0;          // FP64 MFMA (FP32 data widened)
+constexpr int FAMILY_BF16X3 = 1;       // FP32 as 3-term split BF16, chunked accumulation
+constexpr int FAMILY_BF16X3_FAST = 2;  // same, single accumulator
+
 // Kernel variant selector.
 struct KernelKey
 {
-    int tile;      // 0: 128x128, 1: 64x64
+    int family;    // FAMILY_*
+    int tile;      // index into the family's tile table
     int a_mfast;   // A tile loads coalesced along m (1) or k (0)
     int b_nfast;   // B tile loads coalesced along n (1) or k (0)
     int dtype;     // 0: double in/out, 1: float in/out (widened to FP64)

@@ -1,0 +1,122 @@
+/*
+ * tblis.hpp -- C++ front end of libtblis_amd, mirroring the reference's
+ * tblis::mult<T> templates (src/iface/3t/mult.h:28-72).
+ *
+ * The reference's templates take MArray views (varray_view / dpd_varray_view)
+ * and build tblis_tensor descriptors for the C call; these do the same over
+ * two small view types that carry exactly what the descriptors need, so a
+ * caller holding MArray views passes view.data(), view.lengths() and
+ * view.strides() (see INTEGRATION.md for the adapter).
+ */
+#ifndef TBLIS_AMD_TBLIS_HPP
+#define TBLIS_AMD_TBLIS_HPP
+
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "tblis.h"
+
+namespace tblis
+{
+
+// Strided dense tensor view (device memory), like MArray's varray_view.
+template <typename T>
+struct varray_view
+{
+    T* data;
+    std::vector<len_type> len;
+    std::vector<stride_type> stride;
+};
+
+// DPD tensor view: irrep, nirrep, per-irrep lengths [dim][irrep], layout.
+template <typename T>
+struct dpd_varray_view
+{
+    T* data;
+    unsigned irrep;
+    unsigned nirrep;
+    std::vector<std::vector<len_type>> len;
+    int layout = TBLIS_DPD_PREFIX_ROW_MAJOR;
+};
+
+namespace detail
+{
+
+template <typename T>
+struct type_tag;
+template <>
+struct type_tag<float>
+{
+    static constexpr type_t value = TYPE_SINGLE;
+};
+template <>
+struct type_tag<double>
+{
+    static constexpr type_t value = TYPE_DOUBLE;
+};
+
+template <typename T>
+tblis_tensor make_tensor(T scalar, const varray_view<T>& v)
+{
+    tblis_tensor t;
+    t.type = type_tag<T>::value;
+    t.conj = 0;
+    t.scalar.type = t.type;
+    if (std::is_same<T, float>::value)
+        t.scalar.data.s = (float)scalar;
+    else
+        t.scalar.data.d = (double)scalar;
+    t.data = (void*)v.data;
+    t.ndim = (unsigned)v.len.size();
+    t.len = const_cast<len_type*>(v.len.data());
+    t.stride = const_cast<stride_type*>(v.stride.data());
+    return t;
+}
+
+template <typename T>
+tblis_dpd_tensor make_dpd(T scalar, const dpd_varray_view<T>& v, std::vector<len_type>& flat)
+{
+    flat.clear();
+    for (auto& row : v.len) flat.insert(flat.end(), row.begin(), row.end());
+    tblis_dpd_tensor t;
+    if (std::is_same<T, float>::value)
+        tblis_init_dpd_tensor_s(&t, (float)scalar, v.irrep, v.nirrep, (unsigned)v.len.size(), flat.data(),
+                                (float*)v.data, v.layout);
+    else
+        tblis_init_dpd_tensor_d(&t, (double)scalar, v.irrep, v.nirrep, (unsigned)v.len.size(), flat.data(),
+                                (double*)v.data, v.layout);
+    return t;
+}
+
+}  // namespace detail
+
+// C <- alpha * A.B + beta * C (src/iface/3t/mult.h:28-37).
+template <typename T>
+void mult(T alpha, const varray_view<T>& A, const label_type* idx_A, const varray_view<T>& B,
+          const label_type* idx_B, T beta, const varray_view<T>& C, const label_type* idx_C,
+          const tblis_config* cfg = nullptr)
+{
+    tblis_tensor a = detail::make_tensor(alpha, A);
+    tblis_tensor b = detail::make_tensor(T(1), B);
+    tblis_tensor c = detail::make_tensor(beta, C);
+    tblis_tensor_mult(nullptr, cfg, &a, idx_A, &b, idx_B, &c, idx_C);
+}
+
+// DPD contraction (src/iface/3t/mult.h:53-72).
+template <typename T>
+void mult(T alpha, const dpd_varray_view<T>& A, const label_type* idx_A, const dpd_varray_view<T>& B,
+          const label_type* idx_B, T beta, const dpd_varray_view<T>& C, const label_type* idx_C,
+          const tblis_config* cfg = nullptr)
+{
+    std::vector<len_type> fa, fb, fc;
+    tblis_dpd_tensor a = detail::make_dpd(alpha, A, fa);
+    tblis_dpd_tensor b = detail::make_dpd(T(1), B, fb);
+    tblis_dpd_tensor c = detail::make_dpd(beta, C, fc);
+    tblis_dpd_tensor_mult(nullptr, cfg, &a, idx_A, &b, idx_B, &c, idx_C);
+}
+
+}  // namespace tblis
+
+#endif

@@ -381,3 +381,252 @@ hipError_t launch_bf16_probe(const float* A, const float* B, float* D, hipStream
 }
 
 }  // namespace tblis_amd
+
+// ---------------------------------------------------------------------------
+// Packed path: split once into BF16 panels, then a GEMM over the panels.
+// ---------------------------------------------------------------------------
+namespace tblis_amd
+{
+namespace
+{
+
+constexpr int PR = 128;  // panel rows
+constexpr int PK = 16;   // panel k per chunk
+constexpr int PCH = 3 * PR * PK;  // BF16 elements per chunk
+
+// One workgroup (256 threads) packs one chunk (p, kt): each thread gathers
+// 8 consecutive k of one row (k-fast: lane pairs share a row; rows-fast:
+// consecutive lanes take consecutive rows), splits, and writes 3 x 16 bytes.
+template <bool KFAST>
+__global__ void __launch_bounds__(256)
+pack_bf16x3_kernel(PackDesc d, const int64_t* __restrict__ pool, const float* __restrict__ X,
+                   __bf16* __restrict__ out)
+{
+    const int tid = threadIdx.x;
+    const int p = blockIdx.x;
+    const int64_t kt = blockIdx.y;
+    const int row = KFAST ? tid / 2 : tid % PR;
+    const int kh = KFAST ? tid % 2 : tid / PR;
+    const int64_t r = (int64_t)p * PR + row;
+    const bool rok = r < d.nrows;
+    const int64_t ro = d.base + side_off(pool, d.rows, rok ? r : d.nrows - 1);
+    int64_t kb, ks;
+    k_block(pool, d.kside, d.ktbl, kt, PK, kb, ks);
+    const int64_t k0 = kt * PK;
+    bf16x8 h, m, l;
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+    {
+        const int kk = 8 * kh + j;
+        const bool ok = rok && k0 + kk < d.klen;
+        const int64_t kc = k0 + kk < d.klen ? kk : d.klen - 1 - k0;
+        const int64_t ko = ks != SCATTERED ? kb + kc * ks : pool[d.kside.start + k0 + kc];
+        const float v = ok ? X[ro + ko] : 0.0f;
+        __bf16 a, b, c;
+        split3(v, a, b, c);
+        h[j] = a;
+        m[j] = b;
+        l[j] = c;
+    }
+    __bf16* dst = out + d.out + ((int64_t)p * d.nkt + kt) * PCH + row * PK + 8 * kh;
+    *(bf16x8*)dst = h;
+    *(bf16x8*)(dst + PR * PK) = m;
+    *(bf16x8*)(dst + 2 * PR * PK) = l;
+}
+
+// GEMM over packed panels: 128x128 C tile, 4 waves of 64x64 (2x2 blocks of
+// 32x32x16 BF16 MFMA), 6 products per block, chunked FP32 accumulation.
+// SegDesc.a_base / b_base hold the BF16 offsets of the task's A and B
+// panels, klen the k extent.
+template <bool CHUNKED>
+__global__ void __launch_bounds__(256, 2)
+gemm_bf16x3_packed_kernel(const TaskDesc* __restrict__ tasks, int ntasks,
+                          const SegDesc* __restrict__ segs, const int64_t* __restrict__ pool,
+                          const __bf16* __restrict__ Ap, const __bf16* __restrict__ Bp,
+                          float* __restrict__ C, double alpha, double beta, int use_k)
+{
+    constexpr int KP = PK + 8;
+    constexpr int CHUNK = 16;
+    __shared__ __bf16 As[2][3][PR][KP];
+    __shared__ __bf16 Bs[2][3][PR][KP];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm0 = (wave % 2) * 64;
+    const int wn0 = (wave / 2) * 64;
+    // copy map: piece (tid + 256 i) of a chunk = plane i, row tid/2, k half tid%2
+    const int crow = tid / 2, ckh = tid % 2;
+
+    int parity = 0;
+    for (int t = blockIdx.x; t < ntasks; t += gridDim.x)
+    {
+        const TaskDesc task = tasks[t];
+        f32x16 acc[2][2], sum[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                for (int r = 0; r < 16; r++)
+                {
+                    acc[i][j][r] = 0.0f;
+                    sum[i][j][r] = 0.0f;
+                }
+        int chunk_left = CHUNK;
+
+        if (use_k && task.nseg > 0)
+        {
+            const SegDesc seg = segs[task.seg0];
+            const int64_t nkt = (seg.klen + PK - 1) / PK;
+            const uint4* ga = (const uint4*)(Ap + seg.a_base) + tid;
+            const uint4* gb = (const uint4*)(Bp + seg.b_base) + tid;
+            uint4 ra[3], rb[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+            {
+                ra[i] = ga[i * 256];
+                rb[i] = gb[i * 256];
+            }
+            for (int64_t kt = 0; kt < nkt; kt++)
+            {
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+                {
+                    *(uint4*)&As[parity][i][crow][8 * ckh] = ra[i];
+                    *(uint4*)&Bs[parity][i][crow][8 * ckh] = rb[i];
+                }
+                __syncthreads();
+                if (kt + 1 < nkt)
+                {
+                    ga += PCH / 8;
+                    gb += PCH / 8;
+#pragma unroll
+                    for (int i = 0; i < 3; i++)
+                    {
+                        ra[i] = ga[i * 256];
+                        rb[i] = gb[i * 256];
+                    }
+                }
+                bf16x8 af[2][3], bf[2][3];
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int p = 0; p < 3; p++)
+                    {
+                        af[i][p] = *(const bf16x8*)&As[parity][p][wm0 + i * 32 + lane % 32][8 * (lane / 32)];
+                        bf[i][p] = *(const bf16x8*)&Bs[parity][p][wn0 + i * 32 + lane % 32][8 * (lane / 32)];
+                    }
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++)
+                    {
+                        f32x16 c = acc[i][j];
+                        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i][1], bf[j][1], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i][0], bf[j][2], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i][2], bf[j][0], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i][0], bf[j][1], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i][1], bf[j][0], c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i][0], bf[j][0], c, 0, 0, 0);
+                        acc[i][j] = c;
+                    }
+                parity ^= 1;
+                if (CHUNKED && --chunk_left == 0)
+                {
+                    chunk_left = CHUNK;
+#pragma unroll
+                    for (int i = 0; i < 2; i++)
+#pragma unroll
+                        for (int j = 0; j < 2; j++)
+                        {
+                            sum[i][j] += acc[i][j];
+                            for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
+                        }
+                }
+            }
+        }
+
+        int64_t col_off[2];
+        bool col_ok[2];
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+        {
+            const int col = wn0 + j * 32 + lane % 32;
+            col_ok[j] = col < task.nlen;
+            col_off[j] = side_off(pool, task.c_col, col_ok[j] ? col : 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+        {
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+            {
+                const int row = wm0 + i * 32 + 8 * (r / 4) + 4 * (lane / 32) + r % 4;
+                if (row >= task.mlen) continue;
+                const int64_t ro = task.c_base + side_off(pool, task.c_row, row);
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                {
+                    if (!col_ok[j]) continue;
+                    float* p = C + ro + col_off[j];
+                    const float res = CHUNKED ? sum[i][j][r] + acc[i][j][r] : acc[i][j][r];
+                    double v = alpha * (double)res;
+                    if (beta != 0.0) v = fma(beta, (double)*p, v);
+                    *p = (float)v;
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+int pack_bf16_rows() { return PR; }
+int pack_bf16_k() { return PK; }
+
+hipError_t launch_pack_bf16x3(const PackDesc& d, bool kfast, const int64_t* pool, const float* X,
+                              __bf16* out, hipStream_t s)
+{
+    if (d.nrt == 0 || d.nkt == 0) return hipSuccess;
+    dim3 grid(d.nrt, d.nkt);
+    if (kfast)
+        hipLaunchKernelGGL(pack_bf16x3_kernel<true>, grid, dim3(256), 0, s, d, pool, X, out);
+    else
+        hipLaunchKernelGGL(pack_bf16x3_kernel<false>, grid, dim3(256), 0, s, d, pool, X, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_bf16x3_packed(const GemmLaunch& L, bool chunked)
+{
+    if (L.ntasks == 0) return hipSuccess;
+    const void* f = chunked ? (const void*)&gemm_bf16x3_packed_kernel<true>
+                            : (const void*)&gemm_bf16x3_packed_kernel<false>;
+    const TaskDesc* tasks = L.tasks;
+    int ntasks = L.ntasks;
+    const SegDesc* segs = L.segs;
+    const int64_t* pool = L.pool;
+    const void* A = L.A;
+    const void* B = L.B;
+    void* C = L.C;
+    double alpha = L.alpha, beta = L.beta;
+    int use_k = L.use_k;
+    void* args[] = {&tasks, &ntasks, &segs, &pool, &A, &B, &C, &alpha, &beta, &use_k};
+    hipError_t e = hipLaunchKernel(f, dim3(L.grid), dim3(256), args, 0, L.stream);
+    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+int gemm_bf16x3_packed_occupancy(bool chunked)
+{
+    const void* f = chunked ? (const void*)&gemm_bf16x3_packed_kernel<true>
+                            : (const void*)&gemm_bf16x3_packed_kernel<false>;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 256, 0) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return n;
+}
+
+}  // namespace tblis_amd

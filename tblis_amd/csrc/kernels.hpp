@@ -38,6 +38,13 @@ hipError_t launch_gemm_bf16x3(const GemmLaunch& L, const KernelKey& k);
 int gemm_bf16x3_occupancy(const KernelKey& k);
 hipError_t launch_bf16_probe(const float* A, const float* B, float* D, hipStream_t s);
 
+int pack_bf16_rows();
+int pack_bf16_k();
+hipError_t launch_pack_bf16x3(const PackDesc& d, bool kfast, const int64_t* pool, const float* X,
+                              __bf16* out, hipStream_t s);
+hipError_t launch_gemm_bf16x3_packed(const GemmLaunch& L, bool chunked);
+int gemm_bf16x3_packed_occupancy(bool chunked);
+
 // family-generic helpers
 inline int tile_m(const KernelKey& k)
 {

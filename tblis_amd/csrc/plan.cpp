@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <list>
+#include <map>
 #include <mutex>
 #include <unordered_map>
 
@@ -114,6 +115,77 @@ int64_t PlanBuilder::kblock_table(const Side& s, int64_t BK)
     return at;
 }
 
+// A single dense GEMM in a split-BF16 family with enough tiles that every
+// A' element would otherwise be re-split by many workgroups: pack both
+// operands once, then run the panel GEMM.
+bool PlanBuilder::try_packed(Plan& plan, std::vector<TaskDesc>& tasks, std::vector<SegDesc>& segs)
+{
+    if (family_ == FAMILY_F64 || !use_mfma_ || gemms_.size() != 1) return false;
+    if (const char* e = std::getenv("TBLIS_HIP_NO_PACK"))
+        if (e[0] == '1') return false;
+    const GemmDesc& g = gemms_[0];
+    if (g.segs.size() != 1 || !g.l_c.empty()) return false;
+    const bool sw = plan.swap;
+    const int64_t m = sw ? g.n : g.m, n = sw ? g.m : g.n, k = g.segs[0].k;
+    const int64_t PR = pack_bf16_rows(), PK = pack_bf16_k();
+    if (m < 2 * PR || n < 2 * PR || k < PK) return false;
+
+    const GemmSeg& s = g.segs[0];
+    const Side& ar = sw ? s.b_cols : s.a_rows;
+    const Side& ac = sw ? s.b_rows : s.a_cols;
+    const Side& br = sw ? s.a_cols : s.b_rows;
+    const Side& bc = sw ? s.a_rows : s.b_cols;
+    const int64_t nkt = ceil_div(k, PK);
+    const int64_t chunk = 3 * PR * PK;
+    const Side* rows[2] = {&ar, &bc};
+    const Side* ks[2] = {&ac, &br};
+    const int64_t nr[2] = {m, n};
+    const int64_t base[2] = {sw ? s.b_base : s.a_base, sw ? s.a_base : s.b_base};
+    int64_t out = 0;
+    for (int o = 0; o < 2; o++)
+    {
+        PackDesc& d = plan.pack[o];
+        d.base = base[o];
+        d.rows = side_ref(*rows[o], PR, 0);
+        d.kside = side_ref(*ks[o], PK, 0);
+        d.ktbl = kblock_table(*ks[o], PK);
+        d.nrows = nr[o];
+        d.klen = k;
+        d.nrt = (int32_t)ceil_div(nr[o], PR);
+        d.nkt = (int32_t)nkt;
+        d.out = out;
+        plan.pack_kfast[o] = !rows_fast(*rows[o], *ks[o]);
+        out += (int64_t)d.nrt * nkt * chunk;
+    }
+    plan.ws_bytes = (size_t)out * 2;
+
+    const Side& cr = sw ? g.c_cols : g.c_rows;
+    const Side& cc = sw ? g.c_rows : g.c_cols;
+    for (int64_t tm = 0; tm < plan.pack[0].nrt; tm++)
+        for (int64_t tn = 0; tn < plan.pack[1].nrt; tn++)
+        {
+            TaskDesc t{};
+            t.c_base = g.c_base;
+            t.c_row = side_ref(cr, PR, tm * PR);
+            t.c_col = side_ref(cc, PR, tn * PR);
+            t.mlen = (int32_t)std::min<int64_t>(PR, m - tm * PR);
+            t.nlen = (int32_t)std::min<int64_t>(PR, n - tn * PR);
+            t.seg0 = (int32_t)segs.size();
+            t.nseg = 1;
+            SegDesc d{};
+            d.a_base = tm * nkt * chunk;
+            d.b_base = tn * nkt * chunk;
+            d.klen = k;
+            d.a_kblk = d.b_kblk = -1;
+            segs.push_back(d);
+            tasks.push_back(t);
+        }
+    plan.packed = true;
+    plan.useful_flops = 2 * g.m * g.n * k;
+    plan.mfma_flops = 2 * PR * PR * nkt * PK * (int64_t)tasks.size() * 6;
+    return true;
+}
+
 std::shared_ptr<Plan> PlanBuilder::build()
 {
     if (gemms_.empty()) return nullptr;
@@ -204,8 +276,10 @@ std::shared_ptr<Plan> PlanBuilder::build()
     std::vector<TaskDesc> tasks;
     std::vector<SegDesc> segs;
     std::vector<int64_t> cost;
+    const bool packed = try_packed(*plan, tasks, segs);
     for (auto& g : gemms_)
     {
+        if (packed) break;
         const int64_t m = sw ? g.n : g.m, n = sw ? g.m : g.n;
         if (m == 0 || n == 0) continue;
         const Side& cr = sw ? g.c_cols : g.c_rows;
@@ -265,7 +339,8 @@ std::shared_ptr<Plan> PlanBuilder::build()
 
     // Uneven work (DPD, batches): longest tasks first so the static
     // round-robin over the persistent grid balances.
-    bool uniform = std::all_of(cost.begin(), cost.end(), [&](int64_t c) { return c == cost[0]; });
+    bool uniform = cost.empty() ||
+                   std::all_of(cost.begin(), cost.end(), [&](int64_t c) { return c == cost[0]; });
     if (!uniform)
     {
         std::vector<int> order(tasks.size());
@@ -280,7 +355,7 @@ std::shared_ptr<Plan> PlanBuilder::build()
         fail("contraction too large: task list exceeds 2^31 entries");
 
     plan->ntasks = (int)tasks.size();
-    int occ = occupancy(plan->key);
+    int occ = packed ? gemm_bf16x3_packed_occupancy(family_ == FAMILY_BF16X3) : occupancy(plan->key);
     plan->grid = (int)std::min<int64_t>(plan->ntasks, (int64_t)device_cu_count() * std::max(occ, 1));
 
     size_t tb = tasks.size() * sizeof(TaskDesc);
@@ -435,6 +510,41 @@ void check_device_pointer(const void* p, const char* what, int64_t lo, int64_t h
     }
 }
 
+namespace
+{
+
+// Grow-only device workspace per (device, stream), for packed operands.
+// Calls on one stream are ordered, so they can share it.
+void* workspace(size_t bytes, hipStream_t s)
+{
+    struct Entry
+    {
+        void* ptr = nullptr;
+        size_t size = 0;
+    };
+    static std::mutex mu;
+    static auto* ws = new std::map<std::pair<int, hipStream_t>, Entry>();
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> g(mu);
+    Entry& e = (*ws)[{dev, s}];
+    if (e.size < bytes)
+    {
+        if (e.ptr)
+        {
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+            hip_check(hipFree(e.ptr), "hipFree (workspace)");
+        }
+        e.ptr = nullptr;
+        e.size = 0;
+        hip_check(hipMalloc(&e.ptr, bytes), "hipMalloc (workspace)");
+        e.size = bytes;
+    }
+    return e.ptr;
+}
+
+}  // namespace
+
 void run_plan(const Plan& p, const void* A, const void* B, void* C, double alpha, double beta,
               bool use_k)
 {
@@ -452,7 +562,27 @@ void run_plan(const Plan& p, const void* A, const void* B, void* C, double alpha
     L.use_k = use_k && p.reads_ab ? 1 : 0;
     L.grid = p.grid;
     L.stream = ts.stream;
-    hip_check(launch_gemm(L, p.key), "kernel launch");
+    if (p.packed && L.use_k)
+    {
+        __bf16* ws = (__bf16*)workspace(p.ws_bytes, ts.stream);
+        const void* src[2] = {L.A, L.B};
+        for (int o = 0; o < 2; o++)
+            hip_check(launch_pack_bf16x3(p.pack[o], p.pack_kfast[o], p.d_pool, (const float*)src[o], ws,
+                                         ts.stream),
+                      "pack launch");
+        L.A = ws + p.pack[0].out;
+        L.B = ws + p.pack[1].out;
+        hip_check(launch_gemm_bf16x3_packed(L, p.key.family == FAMILY_BF16X3), "kernel launch");
+    }
+    else if (p.packed)
+    {
+        // alpha == 0: only C <- beta*C; the panel kernel skips its k loop
+        hip_check(launch_gemm_bf16x3_packed(L, p.key.family == FAMILY_BF16X3), "kernel launch");
+    }
+    else
+    {
+        hip_check(launch_gemm(L, p.key), "kernel launch");
+    }
     if (!ts.async) hip_check(hipStreamSynchronize(ts.stream), "hipStreamSynchronize");
 }
 

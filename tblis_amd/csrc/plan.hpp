@@ -45,6 +45,12 @@ struct Plan
     int64_t useful_flops = 0;
     bool reads_ab = false;   // some task has k segments
     int elem = 8;            // element size of A, B, C
+    // Packed split-BF16 plan: A' and B' are packed into BF16 panels in a
+    // workspace of ws_bytes, then the panel GEMM runs.
+    bool packed = false;
+    PackDesc pack[2] = {};
+    bool pack_kfast[2] = {false, false};
+    size_t ws_bytes = 0;
     // Data pointers (A, B, C of the first instance) whose memory checks
     // passed on the last call; a call with the same pointers skips them.
     // The key of a batched plan fixes every other instance's pointer
@@ -85,6 +91,7 @@ class PlanBuilder
     std::shared_ptr<Plan> build();
 
   private:
+    bool try_packed(Plan& plan, std::vector<TaskDesc>& tasks, std::vector<SegDesc>& segs);
     SideRef side_ref(const Side& s, int64_t BS, int64_t start);
     int64_t kblock_table(const Side& s, int64_t BK);
     int elem_size_, dtype_, use_mfma_, family_;

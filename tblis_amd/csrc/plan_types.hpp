@@ -51,6 +51,21 @@ struct SegDesc
     int64_t a_kblk, b_kblk;
 };
 
+// One operand (A or B after the orientation swap) packed into split-BF16
+// panels: rows (m or n) x k, gathered through the scatter description and
+// written as chunks [3 planes][PR rows][PK k] BF16, chunk (p, kt) at
+// out + ((p * nkt + kt) * 3 * PR * PK), zero-padded past nrows and klen.
+struct PackDesc
+{
+    int64_t base;   // element offset of the operand patch in its tensor
+    SideRef rows;   // row side (start 0)
+    SideRef kside;  // k side (start 0)
+    int64_t ktbl;   // k block table (pool index) or -1
+    int64_t nrows, klen;
+    int64_t out;    // BF16 element offset of the panels in the workspace
+    int32_t nrt, nkt;
+};
+
 // Kernel families.
 constexpr int FAMILY_F64 = 0;          // FP64 MFMA (FP32 data widened)
 constexpr int FAMILY_BF16X3 = 1;       // FP32 as 3-term split BF16, chunked accumulation

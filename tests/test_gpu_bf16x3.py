@@ -151,3 +151,30 @@ def test_float_dpd_and_batch():
     for i in range(20):
         got = torch.as_strided(Cs[i], (Cs[i].numel(),), (1,)).cpu().numpy()
         assert rel_fro(got, refs[i]) <= SPLIT_TOL, i
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_packed_split_path(seed):
+    """Shapes big enough for the packed panel path (m, n >= 256), with
+    ragged edges, k not a multiple of 16 and multi-dimensional (scattered)
+    sides, against the same contraction with packing switched off."""
+    rng = np.random.default_rng(42000 + seed)
+    ext = [dict(a=20, b=15, e=37, f=1, i=13, j=20), dict(a=17, b=19, e=3, f=11, i=16, j=21),
+           dict(a=64, b=5, e=16, f=16, i=3, j=100), dict(a=9, b=31, e=7, f=5, i=29, j=11)][seed]
+    kinds = [["col"] * 3, ["row", "col", "random"], ["random", "random", "col"], ["col", "row", "row"]][seed]
+    _float_case(rng, "aebf", "fjei", "aibj", ext, kinds, outlier_normal, alpha=0.5,
+                beta=[0.0, 1.0, -1.0, 0.25][seed])
+
+
+def test_packed_alpha_zero_and_beta():
+    rng = np.random.default_rng(43)
+    A = torch.from_numpy(uniform(rng, 300 * 40)).to(DEV).view(300, 40)
+    B = torch.from_numpy(uniform(rng, 40 * 280)).to(DEV).view(40, 280)
+    C0 = torch.from_numpy(uniform(rng, 300 * 280)).to(DEV).view(300, 280)
+    Cm = C0.clone()
+    T.mult(0.0, A, "ak", B, "kb", 2.0, Cm, "ab")
+    assert torch.equal(Cm, C0 * 2.0)
+    Cm = C0.clone()
+    T.mult(1.0, A, "ak", B, "kb", 0.5, Cm, "ab")
+    ref = (A.double() @ B.double() + 0.5 * C0.double()).cpu().numpy()
+    assert rel_fro(Cm.cpu().numpy(), ref) <= SPLIT_TOL

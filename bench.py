@@ -1,17 +1,28 @@
 """Benchmark of the MI355X tensor-contraction path (one JSON line on stdout).
 
-Workload: the permuted-layout FP64 contraction
+Headline workload (value, roofline, cpu_baseline): the permuted-layout FP64
+contraction
     C[aibj] += 0.5 * A[aebf] * B[fjei],  a=b=e=f=64, i=j=32,
 column-major, entries uniform[-1,1], C initialised uniform[-1,1], through
 tblis_tensor_mult (FP64 MFMA).  A step is one contraction with inputs
-resident in HBM.  Reported:
-  value      useful TFLOP/s (2*m*n*k / wall time per step)
-  roofline   the GEMM kernel's achieved FLOP/s from HIP events around each
-             launch on the library's stream, against the FP64 MFMA peak
-  cpu_baseline  the oracle's multithreaded blocked port timed on the host
-             cores on a column sample of the same contraction
-Extra fields report the other workloads (dense 24^4, DPD, batch) when
---all is given.
+resident in HBM.
+  value         useful TFLOP/s (2*m*n*k / wall time per step)
+  roofline      the GEMM kernel's FLOP/s from HIP events around each launch
+                on the library's stream, against the FP64 MFMA peak
+  cpu_baseline  the oracle's multithreaded blocked port on the host cores,
+                on a column sample of the same contraction
+
+Further legs (fields of the same line; --only runs one of them alone and
+makes it the headline):
+  fp32_96      C[abcd] = A[aebf]*B[ecfd], extents 96, FP32 normal(0,1) with
+               1% of entries x1e4: split-BF16 MFMA vs FP64 MFMA, relative
+               Frobenius error of each against a float64 torch product
+  dpd          8-irrep DPD C[abij] = A[abef]*B[efij], per-irrep extents
+               [12,9,7,5,11,8,6,4]: FP64 and split-BF16, and the same kernels
+               on the dense 62^4 contraction (dense-equivalent throughput)
+  batch        4096 independent C[abij] = A[abef]*B[efij], extents drawn from
+               4..40 per instance, FP64 and split-BF16, one hipGraph replay
+               per step
 """
 import argparse
 import json
@@ -25,7 +36,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "useful TFLOP/s + %MFMA peak; split-BF16 vs FP64-MFMA at stated rel error"
-PEAK_FP64 = 78.6  # TFLOP/s, MI355X dense FP64 matrix (datasheet)
+PEAK_FP64 = 78.6    # TFLOP/s, MI355X dense FP64 matrix (datasheet)
+PEAK_BF16 = 2500.0  # TFLOP/s, MI355X dense BF16 matrix (datasheet)
+SPLIT_PRODUCTS = 6  # BF16 MFMA products per split-BF16 multiply-add
 
 
 def colmaj(lens):
@@ -36,24 +49,14 @@ def colmaj(lens):
     return s
 
 
-def permuted_workload(dev, seed=42):
+def strided(flat, lens):
     import torch
-    ext = dict(a=64, b=64, e=64, f=64, i=32, j=32)
-    lens = lambda s: [ext[c] for c in s]
-    lA, lB, lC = lens("aebf"), lens("fjei"), lens("aibj")
-    g = torch.Generator(device=dev).manual_seed(seed)
-    A = torch.rand(int(np.prod(lA)), dtype=torch.float64, device=dev, generator=g) * 2 - 1
-    B = torch.rand(int(np.prod(lB)), dtype=torch.float64, device=dev, generator=g) * 2 - 1
-    C = torch.rand(int(np.prod(lC)), dtype=torch.float64, device=dev, generator=g) * 2 - 1
-    a = torch.as_strided(A, lA, colmaj(lA))
-    b = torch.as_strided(B, lB, colmaj(lB))
-    c = torch.as_strided(C, lC, colmaj(lC))
-    flops = 2.0 * (64 * 64) * (32 * 32) * (64 * 64)
-    nbytes = 8 * (A.numel() + B.numel() + 2 * C.numel())
-    return (a, "aebf", b, "fjei", c, "aibj", 0.5, 1.0), flops, nbytes, (lA, lB, lC)
+    return torch.as_strided(flat, lens, colmaj(lens))
 
 
 def time_steps(fn, steps, warmup, stream):
+    """Wall ms per step, and mean/min of HIP-event time around each call on
+    the library's stream."""
     import torch
     for _ in range(warmup):
         fn()
@@ -70,6 +73,183 @@ def time_steps(fn, steps, warmup, stream):
     kern = [s.elapsed_time(e) for s, e in zip(starts, ends)]
     return wall / steps * 1e3, float(np.mean(kern)), float(np.min(kern))
 
+
+def rel_fro(got, want):
+    import torch
+    got, want = got.double(), want.double()
+    return float(torch.linalg.vector_norm(got - want) / torch.linalg.vector_norm(want))
+
+
+def tflops(flops, ms):
+    return round(flops / (ms * 1e-3) / 1e12, 3)
+
+
+# ---------------------------------------------------------------- permuted
+
+def permuted_leg(T, dev, stream, steps, warmup):
+    import torch
+    ext = dict(a=64, b=64, e=64, f=64, i=32, j=32)
+    lens = lambda s: [ext[c] for c in s]
+    lA, lB, lC = lens("aebf"), lens("fjei"), lens("aibj")
+    g = torch.Generator(device=dev).manual_seed(42)
+    A = torch.rand(int(np.prod(lA)), dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    B = torch.rand(int(np.prod(lB)), dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    C = torch.rand(int(np.prod(lC)), dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    flops = 2.0 * (64 * 64) * (32 * 32) * (64 * 64)
+    nbytes = 8 * (A.numel() + B.numel() + 2 * C.numel())
+    fn = T.prepare_mult(0.5, strided(A, lA), "aebf", strided(B, lB), "fjei", 1.0, strided(C, lC),
+                        "aibj")
+    ms_step, ms_kern, ms_min = time_steps(fn, steps, warmup, stream)
+    achieved = flops / (ms_kern * 1e-3) / 1e12
+    return {
+        "value": tflops(flops, ms_step), "ms_per_step": round(ms_step, 4), "dtype": "f64",
+        "config": {"workload": "C[aibj] += 0.5*A[aebf]*B[fjei], a=b=e=f=64, i=j=32, column-major, "
+                               "FP64 MFMA (tblis_tensor_mult)",
+                   "m": 4096, "n": 1024, "k": 4096, "algorithm": T.last_algorithm()},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64, 4), "traffic": None,
+                     "kernel_ms_mean": round(ms_kern, 4), "kernel_ms_min": round(ms_min, 4),
+                     "algorithmic_flops": flops, "algorithmic_bytes": nbytes},
+    }, (lA, lB, lC)
+
+
+# ---------------------------------------------------------------- fp32 96^4
+
+def outlier_normal(dev, n, seed):
+    """normal(0,1) FP32 with 1% of the entries scaled by 1e4."""
+    import torch
+    g = torch.Generator(device=dev).manual_seed(seed)
+    x = torch.randn(n, dtype=torch.float32, device=dev, generator=g)
+    x[torch.rand(n, device=dev, generator=g) < 0.01] *= 1e4
+    return x
+
+
+def fp32_leg(T, dev, stream, steps, warmup):
+    import torch
+    n4, L = 96 ** 4, [96] * 4
+    A, B = outlier_normal(dev, n4, 42), outlier_normal(dev, n4, 142)
+    Cs = torch.zeros(n4, dtype=torch.float32, device=dev)
+    Cd = torch.zeros(n4, dtype=torch.float32, device=dev)
+    a, b = strided(A, L), strided(B, L)
+    flops = 2.0 * 9216 ** 3
+    out = {"workload": "C[abcd] = A[aebf]*B[ecfd], extents 96, FP32 normal(0,1) with 1% x1e4",
+           "useful_flops": flops}
+    for name, cfg, Cx in (("split_bf16", None, Cs), ("fp64_mfma", "fp64_mfma", Cd)):
+        fn = T.prepare_mult(1.0, a, "aebf", b, "ecfd", 0.0, strided(Cx, L), "abcd", config=cfg)
+        ms_step, ms_kern, _ = time_steps(fn, steps, warmup, stream)
+        out[name] = {"algorithm": T.last_algorithm(), "ms": round(ms_step, 4),
+                     "useful_tflops": tflops(flops, ms_step), "call_ms_events": round(ms_kern, 4)}
+    # float64 reference: A(ab,ef) @ B(ef,cd) with A[aebf] -> (a,b,e,f), B[ecfd] -> (e,f,c,d)
+    A2 = a.double().permute(0, 2, 1, 3).reshape(9216, 9216)
+    B2 = b.double().permute(0, 2, 1, 3).reshape(9216, 9216)
+    ref = (A2 @ B2).reshape(96, 96, 96, 96)
+    del A2, B2
+    for name, Cx in (("split_bf16", Cs), ("fp64_mfma", Cd)):
+        out[name]["rel_err"] = float("%.3e" % rel_fro(strided(Cx, L), ref))
+    ms_s, ms_d = out["split_bf16"]["ms"], out["fp64_mfma"]["ms"]
+    out["speedup_split_vs_fp64"] = round(ms_d / ms_s, 3)
+    out["split_bf16"]["mfma_tflops"] = tflops(SPLIT_PRODUCTS * flops, ms_s)
+    out["split_bf16"]["mfma_frac"] = round(SPLIT_PRODUCTS * flops / (ms_s * 1e-3) / 1e12 / PEAK_BF16, 4)
+    out["fp64_mfma"]["mfma_frac"] = round(flops / (ms_d * 1e-3) / 1e12 / PEAK_FP64, 4)
+    out["reference"] = "float64 torch matmul of the same FP32 inputs"
+    return out
+
+
+# ---------------------------------------------------------------- DPD
+
+DPD_EXT = [12, 9, 7, 5, 11, 8, 6, 4]
+
+
+def dpd_leg(T, dev, stream, steps, warmup):
+    import torch
+    nirrep = 8
+    lens = [DPD_EXT] * 4
+    n = T.dpd_size(0, nirrep, lens)
+    g = torch.Generator(device=dev).manual_seed(7)
+    A = torch.rand(n, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    B = torch.rand(n, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    Cd = torch.zeros(n, dtype=torch.float64, device=dev)
+    Cf = torch.zeros(n, dtype=torch.float32, device=dev)
+    s = [sum(DPD_EXT[h] * DPD_EXT[h ^ r] for h in range(nirrep)) for r in range(nirrep)]
+    flops = sum(2.0 * x ** 3 for x in s)
+    out = {"workload": "8-irrep DPD C[abij] = A[abef]*B[efij], per-irrep extents %s, irrep 0, "
+                       "PREFIX_ROW_MAJOR" % DPD_EXT, "useful_flops": flops}
+    legs = (("fp64", A, B, Cd), ("split_bf16", A.float(), B.float(), Cf))
+    for name, a, b, c in legs:
+        fn = T.prepare_dpd_mult(1.0, a, 0, "abef", b, 0, "efij", 0.0, c, 0, "abij", nirrep,
+                                lens, lens, lens)
+        ms_step, ms_kern, _ = time_steps(fn, steps, warmup, stream)
+        out[name] = {"algorithm": T.last_algorithm(), "ms": round(ms_step, 4),
+                     "useful_tflops": tflops(flops, ms_step), "call_ms_events": round(ms_kern, 4)}
+    out["split_bf16"]["rel_err_vs_fp64"] = float("%.3e" % rel_fro(Cf, Cd))
+    # dense-equivalent: the same kernels on C[abij] = A[abef]*B[efij], extents 62
+    L = [62] * 4
+    dflops = 2.0 * 3844 ** 3
+    for name, dt in (("fp64", torch.float64), ("split_bf16", torch.float32)):
+        gd = torch.Generator(device=dev).manual_seed(8)
+        X = [torch.rand(62 ** 4, dtype=dt, device=dev, generator=gd) * 2 - 1 for _ in range(3)]
+        fn = T.prepare_mult(1.0, strided(X[0], L), "abef", strided(X[1], L), "efij", 0.0,
+                            strided(X[2], L), "abij")
+        ms_step, _, _ = time_steps(fn, steps, warmup, stream)
+        dense = tflops(dflops, ms_step)
+        out[name]["dense62_useful_tflops"] = dense
+        out[name]["frac_of_dense"] = round(out[name]["useful_tflops"] / dense, 3)
+    return out
+
+
+# ---------------------------------------------------------------- batch
+
+def capture(T, fn, stream):
+    """Run fn once eagerly on a side stream (plan + workspace), then capture
+    one call into a hipGraph; the library's stream is restored after."""
+    import torch
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    T.set_stream(side.cuda_stream)
+    with torch.cuda.stream(side):
+        fn()
+    side.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=side):
+        fn()
+    T.set_stream(stream.cuda_stream)
+    return graph
+
+
+def batch_leg(T, dev, stream, steps, warmup, count=4096, seed=7):
+    import torch
+    rng = np.random.default_rng(seed)
+    exts = rng.integers(4, 41, size=(count, 6))  # a b e f i j
+    flops = float(sum(2.0 * np.prod(e.astype(np.float64)) for e in exts))
+    g = torch.Generator(device=dev).manual_seed(seed)
+    insts = {"fp64": ([], [], []), "split_bf16": ([], [], [])}
+    for e in exts:
+        a, b, ef, f, i, j = (int(x) for x in e)
+        shapes = ([a, b, ef, f], [ef, f, i, j], [a, b, i, j])
+        for t, sh in enumerate(shapes):
+            x = torch.rand(int(np.prod(sh)), dtype=torch.float64, device=dev, generator=g) * 2 - 1
+            if t == 2:
+                x.zero_()
+            insts["fp64"][t].append(strided(x, sh))
+            insts["split_bf16"][t].append(strided(x.float(), sh))
+    out = {"workload": "%d independent C[abij] = A[abef]*B[efij], extents uniform in 4..40 "
+                       "(numpy default_rng(%d)), column-major" % (count, seed),
+           "useful_flops": flops}
+    for name in ("fp64", "split_bf16"):
+        As, Bs, Cs = insts[name]
+        fn = T.prepare_batch(1.0, As, ["abef"] * count, Bs, ["efij"] * count, 0.0, Cs,
+                             ["abij"] * count)
+        graph = capture(T, fn, stream)
+        ms_step, ms_kern, _ = time_steps(graph.replay, steps, warmup, torch.cuda.current_stream())
+        out[name] = {"algorithm": T.last_algorithm(), "ms": round(ms_step, 4),
+                     "useful_tflops": tflops(flops, ms_step), "launch": "one hipGraph replay"}
+    got = torch.cat([c.reshape(-1) for c in insts["split_bf16"][2]])
+    want = torch.cat([c.reshape(-1) for c in insts["fp64"][2]])
+    out["split_bf16"]["rel_err_vs_fp64"] = float("%.3e" % rel_fro(got, want))
+    return out
+
+
+# ---------------------------------------------------------------- CPU
 
 def cpu_baseline(lA, lB, lC, sample_cols, threads):
     """Oracle port (multithreaded blocked GEMM over scatter vectors) on the
@@ -95,6 +275,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--only", choices=["permuted", "fp32_96", "dpd", "batch"],
+                    help="run one leg alone (for profiling) and report it as the headline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-cols", type=int, default=256, help="n columns in the CPU sample")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -102,8 +284,8 @@ def main():
                          "FETCH_SIZE/WRITE_SIZE counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(see DESIGN.md for how the contraction would shard); use --gpus 1",
+        print("bench.py measures one GPU: this build has no multi-GPU path (the contraction "
+              "would shard over C's columns, DPD irreps or batch instances); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 
@@ -114,11 +296,22 @@ def main():
     stream = torch.cuda.current_stream()
     T.set_stream(stream.cuda_stream)
     T.set_async(True)
+    few = max(3, args.steps // 10)
 
-    wl, flops, nbytes, lens = permuted_workload(dev)
-    a, ia, b, ib, c, ic, alpha, beta = wl
-    fn = T.prepare_mult(alpha, a, ia, b, ib, beta, c, ic)
-    ms_step, ms_kernel, ms_kernel_min = time_steps(fn, args.steps, args.warmup, stream)
+    head, lens = None, None
+    if args.only in (None, "permuted"):
+        head, lens = permuted_leg(T, dev, stream, args.steps, args.warmup)
+    legs = {}
+    if args.only in (None, "fp32_96"):
+        legs["fp32_96"] = fp32_leg(T, dev, stream, few if args.only is None else args.steps,
+                                   min(args.warmup, 2))
+        torch.cuda.empty_cache()
+    if args.only in (None, "dpd"):
+        legs["dpd"] = dpd_leg(T, dev, stream, args.steps, args.warmup)
+    if args.only in (None, "batch"):
+        legs["batch"] = batch_leg(T, dev, stream, few if args.only is None else args.steps,
+                                  min(args.warmup, 2))
+        torch.cuda.empty_cache()
     T.set_async(False)
     if args.calibrate_copy:
         n = args.calibrate_copy * (1 << 20) // 8
@@ -127,31 +320,32 @@ def main():
         torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
-    value = flops / (ms_step * 1e-3) / 1e12
-    achieved = flops / (ms_kernel * 1e-3) / 1e12
 
+    if head is None:
+        leg = legs[args.only]
+        first = "split_bf16" if "split_bf16" in leg else next(k for k in leg if isinstance(leg[k], dict))
+        head = {"value": leg[first]["useful_tflops"], "ms_per_step": leg[first]["ms"],
+                "dtype": "f32" if first == "split_bf16" else "f64",
+                "config": {"workload": leg["workload"], "algorithm": leg[first]["algorithm"]}}
     out = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": head["value"],
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic: uniform[-1,1] from a seeded generator, resident in HBM",
-        "config": {"workload": "C[aibj] += 0.5*A[aebf]*B[fjei], a=b=e=f=64, i=j=32, column-major, "
-                               "FP64 MFMA (tblis_tensor_mult)",
-                   "m": 4096, "n": 1024, "k": 4096, "algorithm": T.last_algorithm()},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64, 4), "traffic": None,
-                     "kernel_ms_mean": round(ms_kernel, 4), "kernel_ms_min": round(ms_kernel_min, 4),
-                     "algorithmic_bytes": nbytes},
+        "dtype": head["dtype"],
+        "data": "synthetic: seeded torch generators on the device, resident in HBM",
+        "config": head["config"],
     }
-    if not args.no_cpu:
+    if "roofline" in head:
+        out["roofline"] = head["roofline"]
+    out.update(legs)
+    if not args.no_cpu and lens is not None:
         threads = min(os.cpu_count() or 1, 16)
         gflops, fl, dt = cpu_baseline(*lens, args.cpu_cols, threads)
         out["cpu_baseline"] = {"value": round(gflops / 1e3, 5), "unit": "TFLOP/s", "cores": threads,

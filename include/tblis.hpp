@@ -6,7 +6,8 @@
  * and build tblis_tensor descriptors for the C call; these do the same over
  * two small view types that carry exactly what the descriptors need, so a
  * caller holding MArray views passes view.data(), view.lengths() and
- * view.strides() (see INTEGRATION.md for the adapter).
+ * view.strides() into varray_view, and for a DPD view its data(), irrep(),
+ * num_irreps() and per-dimension lengths into dpd_varray_view.
  */
 #ifndef TBLIS_AMD_TBLIS_HPP
 #define TBLIS_AMD_TBLIS_HPP

@@ -10,7 +10,8 @@ fallback: a missing library is an error.
 import ctypes as C
 import os
 
-__all__ = ["lib", "mult", "dpd_mult", "mult_batch", "Tensor", "DpdTensor", "get_config",
+__all__ = ["lib", "mult", "dpd_mult", "mult_batch", "prepare_mult", "prepare_dpd_mult",
+           "prepare_batch", "Tensor", "DpdTensor", "get_config",
            "set_stream", "set_async", "last_error", "last_algorithm", "DPD_LAYOUTS", "TblisError"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -251,19 +252,33 @@ def dpd_size(irrep, nirrep, lens, layout="PREFIX_ROW_MAJOR"):
     return lib().tblis_dpd_tensor_size(C.byref(d.d))
 
 
-def dpd_mult(alpha, A, irrep_A, idx_A, B, irrep_B, idx_B, beta, C_, irrep_C, idx_C, nirrep,
-             lens_A, lens_B, lens_C, layout="PREFIX_ROW_MAJOR", config=None):
-    """DPD contraction over 1-D storage tensors A, B, C_ (tblis::mult over
-    dpd_varray_view, src/iface/3t/mult.h:53-72)."""
+def prepare_dpd_mult(alpha, A, irrep_A, idx_A, B, irrep_B, idx_B, beta, C_, irrep_C, idx_C, nirrep,
+                     lens_A, lens_B, lens_C, layout="PREFIX_ROW_MAJOR", config=None):
+    """Build the descriptors of dpd_mult() once; returns a callable."""
     dA = _DpdDesc(A, irrep_A, nirrep, lens_A, layout, alpha)
     dB = _DpdDesc(B, irrep_B, nirrep, lens_B, layout)
     dC = _DpdDesc(C_, irrep_C, nirrep, lens_C, layout, beta)
     cfg = get_config(config) if config else None
-    L = lib()
-    L.tblis_hip_set_error_mode(1)
-    L.tblis_dpd_tensor_mult(None, cfg, C.byref(dA.d), idx_A.encode("latin1"), C.byref(dB.d),
-                            idx_B.encode("latin1"), C.byref(dC.d), idx_C.encode("latin1"))
-    _check("tblis_dpd_tensor_mult")
+    args = (None, cfg, C.byref(dA.d), idx_A.encode("latin1"), C.byref(dB.d),
+            idx_B.encode("latin1"), C.byref(dC.d), idx_C.encode("latin1"))
+    fn = lib().tblis_dpd_tensor_mult
+
+    def run():
+        _set_scalar(dC.d.scalar, beta, dC.d.type)
+        fn(*args)
+
+    return _Prepared(run, (), (dA, dB, dC))
+
+
+def dpd_mult(alpha, A, irrep_A, idx_A, B, irrep_B, idx_B, beta, C_, irrep_C, idx_C, nirrep,
+             lens_A, lens_B, lens_C, layout="PREFIX_ROW_MAJOR", config=None):
+    """DPD contraction over 1-D storage tensors A, B, C_ (tblis::mult over
+    dpd_varray_view, src/iface/3t/mult.h:53-72)."""
+    try:
+        prepare_dpd_mult(alpha, A, irrep_A, idx_A, B, irrep_B, idx_B, beta, C_, irrep_C, idx_C,
+                         nirrep, lens_A, lens_B, lens_C, layout, config)()
+    except TblisError as e:
+        raise TblisError("tblis_dpd_tensor_mult: %s" % e) from None
     return C_
 
 

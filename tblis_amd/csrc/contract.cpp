@@ -255,7 +255,7 @@ void tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_typ
                 }
                 plan->mark_verified(A->data, B->data, C->data, k);
             }
-            run_plan(*plan, A->data, B->data, C->data, alpha, beta, use_k);
+            run_plan(plan, A->data, B->data, C->data, alpha, beta, use_k);
         }
     }
     reset_scalar(C);
@@ -350,7 +350,7 @@ void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tenso
                 }
                 plan->mark_verified(A[0]->data, B[0]->data, C[0]->data, k);
             }
-            run_plan(*plan, A[0]->data, B[0]->data, C[0]->data, alpha, beta, use_k);
+            run_plan(plan, A[0]->data, B[0]->data, C[0]->data, alpha, beta, use_k);
         }
     }
     for (int64_t i = 0; i < count; i++) reset_scalar(C[i]);

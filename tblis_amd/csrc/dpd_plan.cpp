@@ -437,7 +437,7 @@ void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const l
                 }
                 plan->mark_verified(A->data, B->data, C->data, k);
             }
-            run_plan(*plan, A->data, B->data, C->data, alpha, beta, use_k);
+            run_plan(plan, A->data, B->data, C->data, alpha, beta, use_k);
         }
     }
     if (type == TYPE_SINGLE)

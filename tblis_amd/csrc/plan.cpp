@@ -514,7 +514,8 @@ namespace
 {
 
 // Grow-only device workspace per (device, stream), for packed operands.
-// Calls on one stream are ordered, so they can share it.
+// Calls on one stream are ordered, so they can share it.  A buffer that is
+// outgrown is retired, not freed: a captured graph may still refer to it.
 void* workspace(size_t bytes, hipStream_t s)
 {
     struct Entry
@@ -530,25 +531,38 @@ void* workspace(size_t bytes, hipStream_t s)
     Entry& e = (*ws)[{dev, s}];
     if (e.size < bytes)
     {
-        if (e.ptr)
-        {
-            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-            hip_check(hipFree(e.ptr), "hipFree (workspace)");
-        }
-        e.ptr = nullptr;
-        e.size = 0;
-        hip_check(hipMalloc(&e.ptr, bytes), "hipMalloc (workspace)");
-        e.size = bytes;
+        size_t grow = e.size + e.size / 2;
+        size_t want = bytes > grow ? bytes : grow;
+        void* p = nullptr;
+        hip_check(hipMalloc(&p, want), "hipMalloc (workspace)");
+        e.ptr = p;
+        e.size = want;
     }
     return e.ptr;
 }
 
+// Plans used while a stream was being captured (never released).
+void pin_plan(const std::shared_ptr<Plan>& p)
+{
+    static std::mutex mu;
+    static auto* pinned = new std::vector<std::shared_ptr<Plan>>();
+    std::lock_guard<std::mutex> g(mu);
+    for (auto& q : *pinned)
+        if (q == p) return;
+    pinned->push_back(p);
+}
+
 }  // namespace
 
-void run_plan(const Plan& p, const void* A, const void* B, void* C, double alpha, double beta,
-              bool use_k)
+void run_plan(const std::shared_ptr<Plan>& pp, const void* A, const void* B, void* C, double alpha,
+              double beta, bool use_k)
 {
+    const Plan& p = *pp;
     ThreadState& ts = thread_state();
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    hip_check(hipStreamIsCapturing(ts.stream, &cap), "hipStreamIsCapturing");
+    const bool capturing = cap != hipStreamCaptureStatusNone;
+    if (capturing) pin_plan(pp);
     GemmLaunch L;
     L.tasks = p.d_tasks;
     L.ntasks = p.ntasks;
@@ -583,7 +597,7 @@ void run_plan(const Plan& p, const void* A, const void* B, void* C, double alpha
     {
         hip_check(launch_gemm(L, p.key), "kernel launch");
     }
-    if (!ts.async) hip_check(hipStreamSynchronize(ts.stream), "hipStreamSynchronize");
+    if (!ts.async && !capturing) hip_check(hipStreamSynchronize(ts.stream), "hipStreamSynchronize");
 }
 
 }  // namespace tblis_amd

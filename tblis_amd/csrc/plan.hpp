@@ -117,8 +117,11 @@ void cache_put(const std::string& key, std::shared_ptr<Plan> p);
 void cache_clear();
 
 // Launch a plan on the current thread's stream and wait unless async.
-void run_plan(const Plan& p, const void* A, const void* B, void* C, double alpha, double beta,
-              bool use_k);
+// During stream capture (hipGraph) the call never waits, and the plan is
+// pinned for the life of the process: a captured graph keeps referring to
+// its device tables after the cache would have evicted them.
+void run_plan(const std::shared_ptr<Plan>& p, const void* A, const void* B, void* C, double alpha,
+              double beta, bool use_k);
 
 // Device pointer check for a tensor that will be accessed: the memory must
 // be device-accessible on the current device and, for device allocations,

@@ -11,7 +11,12 @@ BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("TBLIS_AMD_ARCH", "gfx950")
 
 SOURCES = ["canon.cpp", "scatter.cpp", "dpd.cpp", "plan.cpp", "contract.cpp", "dpd_plan.cpp",
-           "api.cpp", "gemm_f64.hip", "gemm_bf16x3.hip"]
+           "api.cpp", "panel.cpp", "gemm_f64.hip", "gemm_bf16x3.hip", "gemm_panel.hip"]
+
+# Per-source device flags.  The panel GEMM keeps its MFMA accumulators in
+# ordinary VGPRs (the unified gfx950 register file); the default AGPR form
+# copies every accumulator in and out of AGPRs once per k iteration.
+EXTRA = {"gemm_panel.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 # Packed-FP32 instructions are disabled for every device pass.
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
@@ -20,7 +25,7 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
 
 def _compile(src, build_dir=None, defines=()):
     obj = os.path.join(build_dir or BUILD, os.path.splitext(src)[0] + ".o")
-    flags = HIPFLAGS + ["-D" + d for d in defines]
+    flags = HIPFLAGS + EXTRA.get(src, []) + ["-D" + d for d in defines]
     cmd = ["hipcc"] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
     if src.endswith(".cpp"):
         cmd = ["hipcc", "-x", "hip"] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]

@@ -17,6 +17,7 @@
 
 #include "contract.hpp"
 #include "dpd.hpp"
+#include "panel.hpp"
 #include "plan.hpp"
 
 namespace tblis_amd
@@ -360,6 +361,119 @@ void build_gemms(const DpdProblem& P, PlanBuilder& pb)
     }
 }
 
+// Irrep combinations of `nd` dimensions whose XOR is x, in the reference's
+// irrep_iterator order (src/internal/1t/dpd/util.hpp:14-60): dimensions
+// 1.. count as an odometer, dimension 0 takes the remaining irrep.
+std::vector<std::vector<unsigned>> irrep_combos(int nd, int nir, unsigned x)
+{
+    std::vector<std::vector<unsigned>> out;
+    if (nd == 0)
+    {
+        if (x == 0) out.push_back({});
+        return out;
+    }
+    int64_t n = 1;
+    for (int i = 1; i < nd; i++) n *= nir;
+    for (int64_t c = 0; c < n; c++)
+    {
+        std::vector<unsigned> v(nd);
+        int64_t y = c;
+        v[0] = x;
+        for (int i = 1; i < nd; i++)
+        {
+            v[i] = (unsigned)(y % nir);
+            y /= nir;
+            v[0] ^= v[i];
+        }
+        out.push_back(v);
+    }
+    return out;
+}
+
+// Panel path: one GEMM per contracted irrep r (XOR of the AB irreps).  Its
+// rows are every AC irrep combination allowed with r, its columns every BC
+// combination, its k every AB combination; A, B and C blocks sit at the
+// crossings.  This is the reference's loop over irrep_AB with
+// dpd_tensor_matrix views (src/internal/3t/dpd/mult.cxx:224-356,
+// src/matrix/dpd_tensor_matrix.hpp:41-111) as a single dense GEMM per irrep
+// instead of one per block.  Requires no AB C labels and
+// irrep(C) = irrep(A) ^ irrep(B).
+bool build_panel_gemms(const DpdProblem& P, PlanBuilder& pb)
+{
+    if (!P.a_abc.empty() || P.C->irrep != (P.A->irrep ^ P.B->irrep)) return false;
+    const int nir = P.nirrep;
+    const int nac = (int)P.a_ac.size(), nbc = (int)P.b_bc.size(), nab = (int)P.a_ab.size();
+    for (int r = 0; r < nir; r++)
+    {
+        auto rows = irrep_combos(nac, nir, P.A->irrep ^ (unsigned)r);
+        auto cols = irrep_combos(nbc, nir, P.B->irrep ^ (unsigned)r);
+        auto ks = irrep_combos(nab, nir, (unsigned)r);
+        // drop empty combinations
+        auto len_of = [](const DpdLayout& L, const std::vector<int>& dims, const std::vector<unsigned>& irr) {
+            int64_t n = 1;
+            for (size_t i = 0; i < dims.size(); i++) n *= L.len[dims[i]][irr[i]];
+            return n;
+        };
+        auto prune = [&](std::vector<std::vector<unsigned>>& v, const DpdLayout& L,
+                         const std::vector<int>& dims, std::vector<int64_t>& lens) {
+            std::vector<std::vector<unsigned>> keep;
+            for (auto& c : v)
+            {
+                int64_t n = len_of(L, dims, c);
+                if (n == 0) continue;
+                keep.push_back(c);
+                lens.push_back(n);
+            }
+            v.swap(keep);
+        };
+        PanelGemm g;
+        prune(rows, P.la, P.a_ac, g.mlen);
+        prune(cols, P.lb, P.b_bc, g.nlen);
+        prune(ks, P.la, P.a_ab, g.klen);
+        if (rows.empty() || cols.empty()) continue;
+        unsigned ai[DPD_MAXDIM], bi[DPD_MAXDIM], ci[DPD_MAXDIM];
+        int64_t len[DPD_MAXDIM], str[DPD_MAXDIM], off;
+        for (auto& rc : rows)
+            for (auto& kc : ks)
+            {
+                for (int i = 0; i < nac; i++) ai[P.a_ac[i]] = rc[i];
+                for (int i = 0; i < nab; i++) ai[P.a_ab[i]] = kc[i];
+                P.la.block(ai, len, &off, str);
+                PBlock b;
+                b.base = off;
+                b.rows = DpdProblem::side(len, str, P.a_ac, false);
+                b.k = DpdProblem::side(len, str, P.a_ab, false);
+                g.a.push_back(b);
+            }
+        for (auto& cc : cols)
+            for (auto& kc : ks)
+            {
+                for (int i = 0; i < nbc; i++) bi[P.b_bc[i]] = cc[i];
+                for (int i = 0; i < nab; i++) bi[P.b_ab[i]] = kc[i];
+                P.lb.block(bi, len, &off, str);
+                PBlock b;
+                b.base = off;
+                b.rows = DpdProblem::side(len, str, P.b_bc, false);
+                b.k = DpdProblem::side(len, str, P.b_ab, false);
+                g.b.push_back(b);
+            }
+        for (auto& rc : rows)
+            for (auto& cc : cols)
+            {
+                for (int i = 0; i < nac; i++) ci[P.c_ac[i]] = rc[i];
+                for (int i = 0; i < nbc; i++) ci[P.c_bc[i]] = cc[i];
+                P.lc.block(ci, len, &off, str);
+                CBlock b;
+                b.base = off;
+                b.rows = DpdProblem::side(len, str, P.c_ac, false);
+                b.cols = DpdProblem::side(len, str, P.c_bc, false);
+                g.c.push_back(b);
+            }
+        pb.add_panel(g);
+    }
+    return true;
+}
+
 int dtype_code(type_t t) { return t == TYPE_DOUBLE ? 0 : 1; }
 
 double scalar_of(const tblis_scalar& s, type_t t) { return t == TYPE_SINGLE ? (double)s.data.s : s.data.d; }
@@ -391,7 +505,7 @@ void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const l
         P.init(A, idx_A, B, idx_B, C, idx_C);
         PlanBuilder pb(type == TYPE_DOUBLE ? 8 : 4, dtype_code(type), algo != Algo::Reference,
                        algo_family(algo));
-        build_gemms(P, pb);
+        if (!pb.panel_route() || !build_panel_gemms(P, pb)) build_gemms(P, pb);
         plan = pb.build();
         if (!plan) plan = std::make_shared<Plan>();
         cache_put(key, plan);

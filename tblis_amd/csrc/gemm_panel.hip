@@ -1,0 +1,300 @@
+// FP64 MFMA over fragment-ordered panels (gfx950).
+//
+// Three kernels per contraction:
+//   pack    gathers every operand block through its row and k scatter
+//           vectors (the reference's block-scatter packing,
+//           src/matrix/block_scatter_matrix.hpp:353-431 and
+//           src/kernels/3m/packm.hpp:293-404) into panels whose 1 KiB
+//           fragment groups are the exact lane order of v_mfma_f64_16x16x4,
+//           zero-filling padding;
+//   gemm    one workgroup per C tile; each wave streams its row blocks of
+//           A' and B' with one coalesced 16-byte load per lane and per
+//           block, straight into registers (no LDS, no barriers), two
+//           k groups in flight, and accumulates with v_mfma_f64_16x16x4;
+//           the epilogue applies alpha/beta through C's scatter vectors
+//           (beta == 0 does not read C, src/kernels/3m/gemm.hpp:54-58), or
+//           stores a dense C panel when the GEMM spans several C blocks;
+//   unpack  (DPD) scatters C panels into the C blocks with alpha/beta.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace tblis_amd
+{
+
+namespace
+{
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------ pack
+
+template <typename T>
+__global__ __launch_bounds__(256) void pack_panel_kernel(const PanelJob* __restrict__ jobs,
+                                                         const PanelChunk* __restrict__ chunks,
+                                                         const int64_t* __restrict__ pool,
+                                                         const T* __restrict__ A,
+                                                         const T* __restrict__ B,
+                                                         double* __restrict__ ws)
+{
+    const PanelChunk ch = chunks[blockIdx.x];
+    const PanelJob j = jobs[ch.job];
+    const T* __restrict__ X = j.operand ? B : A;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const bool zero = j.src == PANEL_MISSING;
+    const int ng = ch.nrb * ch.nkg;
+    for (int g = wave; g < ng; g += 4)
+    {
+        const int rb = ch.rb0 + g / ch.nkg;
+        const int kg = ch.kg0 + g % ch.nkg;
+        const int row = rb * 16 + lr - j.r0;
+        const bool rin = row >= 0 && row < j.nr;
+        const int k0 = kg * 8 + lk - j.k0, k1 = k0 + 4;
+        const bool in0 = rin && k0 >= 0 && k0 < j.nk;
+        const bool in1 = rin && k1 >= 0 && k1 < j.nk;
+        double v0 = 0.0, v1 = 0.0;
+        if (!zero && rin)
+        {
+            const int64_t roff = j.src + pool[j.rowmap + row];
+            if (in0) v0 = (double)X[roff + pool[j.kmap + k0]];
+            if (in1) v1 = (double)X[roff + pool[j.kmap + k1]];
+        }
+        double* out = ws + j.out + ((int64_t)rb * j.pnkg + kg) * 128 + lane * 2;
+        if (in0 && in1)
+        {
+            double2 v;
+            v.x = v0;
+            v.y = v1;
+            *(double2*)out = v;
+        }
+        else
+        {
+            if (in0) out[0] = v0;
+            if (in1) out[1] = v1;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ gemm
+
+template <int MB, int NB>
+__device__ __forceinline__ void load_groups(double2 (&a)[MB], double2 (&b)[NB], const double2* pa,
+                                            const double2* pb, int64_t rs, int kg)
+{
+#pragma unroll
+    for (int i = 0; i < MB; i++) a[i] = pa[i * rs + kg * 64];
+#pragma unroll
+    for (int j = 0; j < NB; j++) b[j] = pb[j * rs + kg * 64];
+}
+
+template <int MB, int NB>
+__device__ __forceinline__ void mma_group(v4d (&acc)[MB][NB], const double2 (&a)[MB],
+                                          const double2 (&b)[NB])
+{
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].x, acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].y, acc[i][j], 0, 0, 0);
+}
+
+template <int BM, int BN, int WGM, int WGN, typename TC>
+__global__ __launch_bounds__(64 * WGM * WGN) void gemm_panel_kernel(
+    const PanelTask* __restrict__ tasks, const int64_t* __restrict__ pool,
+    const double* __restrict__ ws, double* __restrict__ cws, TC* __restrict__ C, double alpha,
+    double beta, int use_k)
+{
+    constexpr int WM = BM / WGM, WN = BN / WGN, MB = WM / 16, NB = WN / 16;
+    const PanelTask t = tasks[blockIdx.x];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = wave % WGM, wn = wave / WGM;
+    if (wm * WM >= t.mlen || wn * WN >= t.nlen) return;  // no barriers below
+
+    v4d acc[MB][NB];
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+
+    const int nk = use_k ? t.nkg : 0;  // even
+    if (nk > 0)
+    {
+        const int64_t rs = (int64_t)t.nkg * 64;  // double2 per row block
+        const double2* pa = (const double2*)(ws + t.a) + (int64_t)wm * MB * rs + lane;
+        const double2* pb = (const double2*)(ws + t.b) + (int64_t)wn * NB * rs + lane;
+        double2 a0[MB], b0[NB], a1[MB], b1[NB];
+        load_groups<MB, NB>(a0, b0, pa, pb, rs, 0);
+        load_groups<MB, NB>(a1, b1, pa, pb, rs, 1);
+        for (int kg = 0; kg < nk; kg += 2)
+        {
+            // the last iteration re-reads the final groups (in bounds, unused)
+            const int nx = kg + 2 < nk ? kg + 2 : nk - 2;
+            mma_group<MB, NB>(acc, a0, b0);
+            load_groups<MB, NB>(a0, b0, pa, pb, rs, nx);
+            mma_group<MB, NB>(acc, a1, b1);
+            load_groups<MB, NB>(a1, b1, pa, pb, rs, nx + 1);
+        }
+    }
+
+    // D register r of lane l holds D[l / 16 + 4 r][l % 16] of each block
+    const int cl = lane & 15, rl = lane >> 4;
+    if (t.c_ld)
+    {
+#pragma unroll
+        for (int i = 0; i < MB; i++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+            {
+                const int row = wm * WM + i * 16 + rl + 4 * r;
+                if (row >= t.mlen) continue;
+#pragma unroll
+                for (int j = 0; j < NB; j++)
+                {
+                    const int col = wn * WN + j * 16 + cl;
+                    if (col < t.nlen) cws[t.c_src + (int64_t)row * t.c_ld + col] = acc[i][j][r];
+                }
+            }
+        return;
+    }
+    int64_t coff[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+    {
+        const int col = wn * WN + j * 16 + cl;
+        coff[j] = col < t.nlen ? pool[t.c_colmap + col] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+        {
+            const int row = wm * WM + i * 16 + rl + 4 * r;
+            if (row >= t.mlen) continue;
+            const int64_t roff = t.c_src + pool[t.c_rowmap + row];
+#pragma unroll
+            for (int j = 0; j < NB; j++)
+            {
+                const int col = wn * WN + j * 16 + cl;
+                if (col >= t.nlen) continue;
+                TC* p = C + roff + coff[j];
+                double v = alpha * acc[i][j][r];
+                if (beta != 0.0) v += beta * (double)*p;
+                *p = (TC)v;
+            }
+        }
+}
+
+// ------------------------------------------------------------------ unpack
+
+template <typename TC>
+__global__ __launch_bounds__(256) void unpack_panel_kernel(const UnpackJob* __restrict__ jobs,
+                                                           const UnpackChunk* __restrict__ chunks,
+                                                           const int64_t* __restrict__ pool,
+                                                           const double* __restrict__ ws,
+                                                           TC* __restrict__ C, double alpha,
+                                                           double beta)
+{
+    const UnpackChunk ch = chunks[blockIdx.x];
+    const UnpackJob j = jobs[ch.job];
+    for (int e = threadIdx.x; e < 64 * 64; e += 256)
+    {
+        const int i = ch.r0 + (j.rows_fast ? (e & 63) : (e >> 6));
+        const int c = ch.c0 + (j.rows_fast ? (e >> 6) : (e & 63));
+        if (i >= j.nr || c >= j.nc) continue;
+        TC* p = C + j.dst + pool[j.rowmap + i] + pool[j.colmap + c];
+        double v = alpha * ws[j.panel + (int64_t)i * j.ld + c];
+        if (beta != 0.0) v += beta * (double)*p;
+        *p = (TC)v;
+    }
+}
+
+// ------------------------------------------------------------------ table
+
+struct PanelTile
+{
+    int bm, bn, wgm, wgn;
+};
+
+#define TBLIS_PANEL_TILES(X) \
+    X(0, 128, 128, 2, 2)     \
+    X(1, 64, 64, 2, 2)       \
+    X(2, 64, 64, 1, 1)       \
+    X(3, 128, 64, 2, 1)      \
+    X(4, 64, 128, 1, 2)      \
+    X(5, 32, 32, 1, 1)
+
+constexpr PanelTile kTiles[] = {
+#define X(id, bm, bn, wgm, wgn) {bm, bn, wgm, wgn},
+    TBLIS_PANEL_TILES(X)
+#undef X
+};
+constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
+
+template <typename TC>
+hipError_t launch_gemm_t(const PanelLaunch& L)
+{
+    const dim3 grid(L.ntasks), block(64 * kTiles[L.tile].wgm * kTiles[L.tile].wgn);
+    switch (L.tile)
+    {
+#define X(id, bm, bn, wgm, wgn)                                                                       \
+    case id:                                                                                          \
+        hipLaunchKernelGGL((gemm_panel_kernel<bm, bn, wgm, wgn, TC>), grid, block, 0, L.stream,       \
+                           L.tasks, L.pool, (const double*)L.ws, L.ws, (TC*)L.C, L.alpha, L.beta,     \
+                           L.use_k);                                                                  \
+        break;
+        TBLIS_PANEL_TILES(X)
+#undef X
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int panel_num_tiles() { return kNumTiles; }
+int panel_tile_m(int t) { return kTiles[t].bm; }
+int panel_tile_n(int t) { return kTiles[t].bn; }
+int panel_tile_waves(int t) { return kTiles[t].wgm * kTiles[t].wgn; }
+
+hipError_t launch_panel_pack(const PanelLaunch& L)
+{
+    if (L.nchunks == 0) return hipSuccess;
+    if (L.dtype == 0)
+        hipLaunchKernelGGL(pack_panel_kernel<double>, dim3(L.nchunks), dim3(256), 0, L.stream, L.jobs,
+                           L.chunks, L.pool, (const double*)L.A, (const double*)L.B, L.ws);
+    else
+        hipLaunchKernelGGL(pack_panel_kernel<float>, dim3(L.nchunks), dim3(256), 0, L.stream, L.jobs,
+                           L.chunks, L.pool, (const float*)L.A, (const float*)L.B, L.ws);
+    return hipGetLastError();
+}
+
+hipError_t launch_panel_gemm(const PanelLaunch& L)
+{
+    if (L.ntasks == 0) return hipSuccess;
+    if (L.tile < 0 || L.tile >= kNumTiles) return hipErrorInvalidValue;
+    return L.dtype == 0 ? launch_gemm_t<double>(L) : launch_gemm_t<float>(L);
+}
+
+hipError_t launch_panel_unpack(const PanelLaunch& L)
+{
+    if (L.nuchunks == 0) return hipSuccess;
+    if (L.dtype == 0)
+        hipLaunchKernelGGL(unpack_panel_kernel<double>, dim3(L.nuchunks), dim3(256), 0, L.stream,
+                           L.ujobs, L.uchunks, L.pool, (const double*)L.ws, (double*)L.C, L.alpha,
+                           L.beta);
+    else
+        hipLaunchKernelGGL(unpack_panel_kernel<float>, dim3(L.nuchunks), dim3(256), 0, L.stream,
+                           L.ujobs, L.uchunks, L.pool, (const double*)L.ws, (float*)L.C, L.alpha,
+                           L.beta);
+    return hipGetLastError();
+}
+
+}  // namespace tblis_amd

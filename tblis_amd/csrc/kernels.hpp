@@ -45,6 +45,36 @@ hipError_t launch_pack_bf16x3(const PackDesc& d, bool kfast, const int64_t* pool
 hipError_t launch_gemm_bf16x3_packed(const GemmLaunch& L, bool chunked);
 int gemm_bf16x3_packed_occupancy(bool chunked);
 
+// FP64 panel path
+int panel_num_tiles();
+int panel_tile_m(int tile);
+int panel_tile_n(int tile);
+int panel_tile_waves(int tile);
+struct PanelLaunch
+{
+    const PanelJob* jobs;
+    const PanelChunk* chunks;
+    int nchunks;
+    const PanelTask* tasks;
+    int ntasks;
+    const UnpackJob* ujobs;
+    const UnpackChunk* uchunks;
+    int nuchunks;
+    const int64_t* pool;
+    const void* A;
+    const void* B;
+    void* C;
+    double* ws;
+    double alpha, beta;
+    int use_k;
+    int dtype;  // 0: double, 1: float
+    int tile;
+    hipStream_t stream;
+};
+hipError_t launch_panel_pack(const PanelLaunch& L);
+hipError_t launch_panel_gemm(const PanelLaunch& L);
+hipError_t launch_panel_unpack(const PanelLaunch& L);
+
 // family-generic helpers
 inline int tile_m(const KernelKey& k)
 {

@@ -1,0 +1,449 @@
+// Planner of the FP64 panel path.
+//
+// A plan holds, in one device allocation:
+//   pack jobs + chunks   every operand block -> its place in the A' / B' panels
+//   tasks                one per C tile, longest first
+//   unpack jobs + chunks C panels -> C blocks (GEMMs spanning several C blocks)
+//   pool                 explicit offset vectors (rows, k, C rows / cols) in
+//                        the odometer order of the reference's scatter vectors
+//                        (src/matrix/block_scatter_matrix.hpp:68-113)
+// and the workspace size for the panels.
+#include "panel.hpp"
+
+#include <algorithm>
+#include <cstdlib>
+#include <map>
+
+namespace tblis_amd
+{
+
+namespace
+{
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) fail(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+bool unit_first(const Side& s) { return !s.len.empty() && s.stride[0] == 1 && s.len[0] > 1; }
+
+int64_t side_len(const Side& s)
+{
+    int64_t n = 1;
+    for (auto l : s.len) n *= l;
+    return n;
+}
+
+// Offset vectors, deduplicated by (lengths, strides).
+class Pool
+{
+  public:
+    int64_t offsets(const Side& s)
+    {
+        std::vector<int64_t> key(s.len);
+        key.insert(key.end(), s.stride.begin(), s.stride.end());
+        key.push_back((int64_t)s.len.size());
+        auto it = index_.find(key);
+        if (it != index_.end()) return it->second;
+        const int64_t at = (int64_t)data_.size();
+        const int64_t n = side_len(s);
+        data_.resize(data_.size() + std::max<int64_t>(n, 1));
+        // odometer, first dimension fastest
+        std::vector<int64_t> idx(s.len.size(), 0);
+        int64_t off = 0;
+        for (int64_t i = 0; i < n; i++)
+        {
+            data_[at + i] = off;
+            for (size_t d = 0; d < s.len.size(); d++)
+            {
+                off += s.stride[d];
+                if (++idx[d] < s.len[d]) break;
+                off -= s.stride[d] * s.len[d];
+                idx[d] = 0;
+            }
+        }
+        index_[key] = at;
+        return at;
+    }
+    std::vector<int64_t>& data() { return data_; }
+
+  private:
+    std::vector<int64_t> data_;
+    std::map<std::vector<int64_t>, int64_t> index_;
+};
+
+struct TileModel
+{
+    int tile;
+    double eff;  // fraction of the per-SIMD MFMA rate one wave of this shape sustains
+    int occ;     // waves per SIMD (register limited)
+};
+
+// Per-wave efficiency by wave tile (64x64: 4x4 MFMA blocks, 32x32: 2x2).
+const TileModel kModels[] = {{0, 0.85, 2}, {2, 0.80, 2}, {3, 0.82, 2}, {4, 0.82, 2}, {1, 0.50, 6},
+                             {5, 0.45, 6}};
+
+// Estimated run time of the GEMM tasks for a tile.
+double tile_cost(const std::vector<PanelGemm>& gemms, const TileModel& tm, int cus)
+{
+    const int64_t BM = panel_tile_m(tm.tile), BN = panel_tile_n(tm.tile);
+    const int W = panel_tile_waves(tm.tile);
+    const double simd = 60e9;  // FP64 MFMA flop/s per SIMD at the sustained clock
+    double total = 0, maxtask = 0;
+    int64_t waves = 0;
+    for (auto& g : gemms)
+    {
+        int64_t m = 0, n = 0, k = 0;
+        for (auto x : g.mlen) m += x;
+        for (auto x : g.nlen) n += x;
+        for (auto x : g.klen) k += x;
+        if (!m || !n) continue;
+        const int64_t kp = std::max<int64_t>(ceil_div(k, 16) * 16, 16);
+        const int64_t nt = ceil_div(m, BM) * ceil_div(n, BN);
+        const double tw = 2.0 * BM * BN * kp;
+        total += tw * nt;
+        maxtask = std::max(maxtask, tw);
+        waves += nt * W;
+    }
+    const int64_t S = 4LL * cus;
+    const double per_simd = simd * std::min(1.0, tm.eff * (waves > S ? 1.2 : 1.0));
+    const double busy = (double)std::min<int64_t>(waves, S);
+    return std::max(total / (busy * per_simd), maxtask / (W * simd * tm.eff));
+}
+
+}  // namespace
+
+PanelGemm transpose_gemm(const PanelGemm& g)
+{
+    PanelGemm t;
+    t.mlen = g.nlen;
+    t.nlen = g.mlen;
+    t.klen = g.klen;
+    t.a = g.b;
+    t.b = g.a;
+    const size_t nm = g.mlen.size(), nn = g.nlen.size();
+    t.c.resize(g.c.size());
+    for (size_t p = 0; p < nm; p++)
+        for (size_t s = 0; s < nn; s++)
+        {
+            const CBlock& c = g.c[p * nn + s];
+            CBlock& d = t.c[s * nm + p];
+            d.base = c.base;
+            d.rows = c.cols;
+            d.cols = c.rows;
+        }
+    return t;
+}
+
+void panel_from_desc(const GemmDesc& g, std::vector<PanelGemm>& out)
+{
+    const int64_t L = g.l_c.empty() ? 1 : (int64_t)g.l_c.size();
+    for (int64_t l = 0; l < L; l++)
+    {
+        const int64_t oa = g.l_a.empty() ? 0 : g.l_a[l];
+        const int64_t ob = g.l_b.empty() ? 0 : g.l_b[l];
+        const int64_t oc = g.l_c.empty() ? 0 : g.l_c[l];
+        PanelGemm p;
+        p.mlen = {g.m};
+        p.nlen = {g.n};
+        for (auto& s : g.segs)
+        {
+            if (s.k == 0) continue;
+            p.klen.push_back(s.k);
+            PBlock a, b;
+            a.base = s.a_base + oa;
+            a.rows = s.a_rows;
+            a.k = s.a_cols;
+            b.base = s.b_base + ob;
+            b.rows = s.b_cols;
+            b.k = s.b_rows;
+            p.a.push_back(a);
+            p.b.push_back(b);
+        }
+        CBlock c;
+        c.base = g.c_base + oc;
+        c.rows = g.c_rows;
+        c.cols = g.c_cols;
+        p.c.push_back(c);
+        out.push_back(std::move(p));
+    }
+}
+
+bool panel_enabled(int family, int use_mfma)
+{
+    if (family != FAMILY_F64 || !use_mfma) return false;
+    if (const char* e = std::getenv("TBLIS_HIP_PANEL"))
+        if (e[0] == '0') return false;
+    return true;
+}
+
+bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T)
+{
+    // drop GEMMs without rows or columns
+    gemms.erase(std::remove_if(gemms.begin(), gemms.end(),
+                               [](const PanelGemm& g) {
+                                   int64_t m = 0, n = 0;
+                                   for (auto x : g.mlen) m += x;
+                                   for (auto x : g.nlen) n += x;
+                                   return m == 0 || n == 0;
+                               }),
+                gemms.end());
+    if (gemms.empty()) return false;
+
+    // Orientation: lanes of the MFMA output run along n, so make n the
+    // unit-stride side of C when C is contiguous along m.
+    const CBlock& c0 = gemms[0].c[0];
+    T.swap = unit_first(c0.rows) && !unit_first(c0.cols);
+    if (T.swap)
+        for (auto& g : gemms) g = transpose_gemm(g);
+
+    // Tile
+    const int cus = device_cu_count();
+    int tile = 0;
+    double best = -1;
+    for (const TileModel& tm : kModels)
+    {
+        double c = tile_cost(gemms, tm, cus);
+        if (best < 0 || c < best)
+        {
+            best = c;
+            tile = tm.tile;
+        }
+    }
+    if (const char* e = std::getenv("TBLIS_HIP_PANEL_TILE"))
+    {
+        int t = std::atoi(e);
+        if (e[0] >= '0' && e[0] <= '9' && t < panel_num_tiles()) tile = t;
+    }
+    const int64_t BM = panel_tile_m(tile), BN = panel_tile_n(tile);
+
+    Pool pool;
+    std::vector<PanelJob>& jobs = T.jobs;
+    std::vector<PanelChunk>& chunks = T.chunks;
+    std::vector<PanelTask>& tasks = T.tasks;
+    std::vector<int64_t> cost;
+    std::vector<UnpackJob>& ujobs = T.ujobs;
+    std::vector<UnpackChunk>& uchunks = T.uchunks;
+    int64_t ws = 0;  // doubles
+    T.useful_flops = 0;
+    T.mfma_flops = 0;
+    T.reads_ab = false;
+
+    auto add_job = [&](const PanelJob& j) {
+        if (j.nr <= 0 || j.nk <= 0) return;
+        const int32_t id = (int32_t)jobs.size();
+        jobs.push_back(j);
+        const int64_t rb0 = j.r0 / 16, rb1 = ceil_div(j.r0 + j.nr, 16);
+        const int64_t kg0 = j.k0 / 8, kg1 = ceil_div(j.k0 + j.nk, 8);
+        for (int64_t rb = rb0; rb < rb1; rb += 4)
+            for (int64_t kg = kg0; kg < kg1; kg += 16)
+            {
+                PanelChunk c{};
+                c.job = id;
+                c.rb0 = (int32_t)rb;
+                c.nrb = (int32_t)std::min<int64_t>(4, rb1 - rb);
+                c.kg0 = (int32_t)kg;
+                c.nkg = (int32_t)std::min<int64_t>(16, kg1 - kg);
+                chunks.push_back(c);
+            }
+    };
+
+    for (const PanelGemm& g : gemms)
+    {
+        const size_t NM = g.mlen.size(), NN = g.nlen.size(), NK = g.klen.size();
+        std::vector<int64_t> m0(NM + 1, 0), n0(NN + 1, 0), k0(NK + 1, 0);
+        for (size_t i = 0; i < NM; i++) m0[i + 1] = m0[i] + g.mlen[i];
+        for (size_t i = 0; i < NN; i++) n0[i + 1] = n0[i] + g.nlen[i];
+        for (size_t i = 0; i < NK; i++) k0[i + 1] = k0[i] + g.klen[i];
+        const int64_t m = m0[NM], n = n0[NN], k = k0[NK];
+        if (m == 0 || n == 0) continue;
+        const int64_t mp = ceil_div(m, BM) * BM, np = ceil_div(n, BN) * BN;
+        const int64_t kp = ceil_div(k, 16) * 16;
+        const int64_t nkg = kp / 8;
+        if (m > INT32_MAX || n > INT32_MAX || kp > INT32_MAX) fail("contraction too large for one GEMM");
+        T.useful_flops += 2 * m * n * k;
+        T.mfma_flops += 2 * mp * np * kp;
+        if (k > 0) T.reads_ab = true;
+
+        // panels
+        const int64_t oa = ws;
+        ws += mp * kp;
+        const int64_t ob = ws;
+        ws += np * kp;
+        const std::vector<PBlock>* blocks[2] = {&g.a, &g.b};
+        const std::vector<int64_t>* rpre[2] = {&m0, &n0};
+        const int64_t rtot[2] = {m, n}, rpad[2] = {mp, np}, outs[2] = {oa, ob};
+        for (int o = 0; o < 2 && kp > 0; o++)
+        {
+            const size_t NR = (o == 0 ? NM : NN);
+            for (size_t p = 0; p < NR; p++)
+                for (size_t q = 0; q < NK; q++)
+                {
+                    const PBlock& blk = (*blocks[o])[p * NK + q];
+                    PanelJob j{};
+                    j.src = blk.base;
+                    j.r0 = (int32_t)(*rpre[o])[p];
+                    j.nr = (int32_t)((*rpre[o])[p + 1] - (*rpre[o])[p]);
+                    j.k0 = (int32_t)k0[q];
+                    j.nk = (int32_t)g.klen[q];
+                    j.out = outs[o];
+                    j.pnkg = (int32_t)nkg;
+                    j.operand = o;
+                    if (j.nr == 0 || j.nk == 0) continue;
+                    if (blk.base != PANEL_MISSING)
+                    {
+                        j.rowmap = pool.offsets(blk.rows);
+                        j.kmap = pool.offsets(blk.k);
+                    }
+                    add_job(j);
+                }
+            // zero padding: rows [rtot, rpad) x all k, then k [k, kp) x rows
+            PanelJob z{};
+            z.src = PANEL_MISSING;
+            z.out = outs[o];
+            z.pnkg = (int32_t)nkg;
+            z.operand = o;
+            z.r0 = (int32_t)rtot[o];
+            z.nr = (int32_t)(rpad[o] - rtot[o]);
+            z.k0 = 0;
+            z.nk = (int32_t)kp;
+            add_job(z);
+            z.r0 = 0;
+            z.nr = (int32_t)rtot[o];
+            z.k0 = (int32_t)k;
+            z.nk = (int32_t)(kp - k);
+            add_job(z);
+        }
+
+        // C
+        const bool direct = g.c.size() == 1;
+        int64_t oc = 0, crow = 0, ccol = 0;
+        if (direct)
+        {
+            crow = pool.offsets(g.c[0].rows);
+            ccol = pool.offsets(g.c[0].cols);
+        }
+        else
+        {
+            oc = ws;
+            ws += mp * np;
+            for (size_t p = 0; p < NM; p++)
+                for (size_t s = 0; s < NN; s++)
+                {
+                    const CBlock& cb = g.c[p * NN + s];
+                    UnpackJob u{};
+                    u.dst = cb.base;
+                    u.rowmap = pool.offsets(cb.rows);
+                    u.colmap = pool.offsets(cb.cols);
+                    u.panel = oc + m0[p] * np + n0[s];
+                    u.nr = (int32_t)g.mlen[p];
+                    u.nc = (int32_t)g.nlen[s];
+                    u.ld = (int32_t)np;
+                    u.rows_fast = unit_first(cb.rows) && !unit_first(cb.cols);
+                    if (u.nr == 0 || u.nc == 0) continue;
+                    const int32_t id = (int32_t)ujobs.size();
+                    ujobs.push_back(u);
+                    for (int32_t r = 0; r < u.nr; r += 64)
+                        for (int32_t c = 0; c < u.nc; c += 64) uchunks.push_back(UnpackChunk{id, r, c, 0});
+                }
+        }
+        for (int64_t tm = 0; tm < mp / BM; tm++)
+            for (int64_t tn = 0; tn < np / BN; tn++)
+            {
+                PanelTask t{};
+                t.a = oa + tm * (BM / 16) * nkg * 128;
+                t.b = ob + tn * (BN / 16) * nkg * 128;
+                t.nkg = (int32_t)nkg;
+                t.mlen = (int32_t)std::min<int64_t>(BM, m - tm * BM);
+                t.nlen = (int32_t)std::min<int64_t>(BN, n - tn * BN);
+                if (direct)
+                {
+                    t.c_src = g.c[0].base;
+                    t.c_rowmap = crow + tm * BM;
+                    t.c_colmap = ccol + tn * BN;
+                    t.c_ld = 0;
+                }
+                else
+                {
+                    t.c_src = oc + tm * BM * np + tn * BN;
+                    t.c_ld = (int32_t)np;
+                }
+                tasks.push_back(t);
+                cost.push_back(std::max<int64_t>(nkg, 1) * t.mlen * t.nlen);
+            }
+    }
+    if (tasks.empty()) return false;
+    if (tasks.size() > (size_t)INT32_MAX || chunks.size() > (size_t)INT32_MAX ||
+        uchunks.size() > (size_t)INT32_MAX)
+        fail("contraction too large: panel task list exceeds 2^31 entries");
+
+    // longest tasks first (uneven batches and DPD irreps)
+    {
+        std::vector<int> order(tasks.size());
+        for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+        std::vector<PanelTask> t2;
+        t2.reserve(tasks.size());
+        for (int i : order) t2.push_back(tasks[i]);
+        tasks.swap(t2);
+    }
+
+    T.tile = tile;
+    T.ws = std::max<int64_t>(ws, 1);
+    T.pool = std::move(pool.data());
+    return true;
+}
+
+bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, int dtype)
+{
+    PanelTables T;
+    if (!plan_panel_tables(std::move(gemms), T)) return false;
+    plan.swap = T.swap;
+    plan.useful_flops = T.useful_flops;
+    plan.mfma_flops = T.mfma_flops;
+    plan.reads_ab = T.reads_ab;
+    plan.panel = true;
+    plan.panel_tile = T.tile;
+    plan.key.family = FAMILY_F64;
+    plan.key.dtype = dtype;
+    plan.key.use_mfma = 1;
+    plan.key.tile = T.tile;
+    plan.elem = elem_size;
+    plan.ntasks = (int)T.tasks.size();
+    plan.grid = plan.ntasks;
+    plan.p_nchunks = (int)T.chunks.size();
+    plan.p_nuchunks = (int)T.uchunks.size();
+    plan.ws_bytes = (size_t)T.ws * sizeof(double);
+
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t s_jobs = T.jobs.size() * sizeof(PanelJob), s_chunks = T.chunks.size() * sizeof(PanelChunk);
+    const size_t s_tasks = T.tasks.size() * sizeof(PanelTask), s_uj = T.ujobs.size() * sizeof(UnpackJob);
+    const size_t s_uc = T.uchunks.size() * sizeof(UnpackChunk);
+    const size_t s_pool = std::max<size_t>(T.pool.size(), 1) * 8;
+    const size_t o_chunks = al(s_jobs), o_tasks = o_chunks + al(s_chunks), o_uj = o_tasks + al(s_tasks);
+    const size_t o_uc = o_uj + al(s_uj), o_pool = o_uc + al(s_uc);
+    plan.bytes = o_pool + s_pool;
+    hip_check(hipGetDevice(&plan.device), "hipGetDevice");
+    hip_check(hipMalloc(&plan.dev, plan.bytes), "hipMalloc (plan)");
+    char* base = (char*)plan.dev;
+    plan.d_pjobs = (PanelJob*)base;
+    plan.d_pchunks = (PanelChunk*)(base + o_chunks);
+    plan.d_ptasks = (PanelTask*)(base + o_tasks);
+    plan.d_ujobs = (UnpackJob*)(base + o_uj);
+    plan.d_uchunks = (UnpackChunk*)(base + o_uc);
+    plan.d_pool = (int64_t*)(base + o_pool);
+    auto up = [&](void* dst, const void* src, size_t n) {
+        if (n) hip_check(hipMemcpy(dst, src, n, hipMemcpyHostToDevice), "hipMemcpy (plan)");
+    };
+    up(plan.d_pjobs, T.jobs.data(), s_jobs);
+    up(plan.d_pchunks, T.chunks.data(), s_chunks);
+    up(plan.d_ptasks, T.tasks.data(), s_tasks);
+    up(plan.d_ujobs, T.ujobs.data(), s_uj);
+    up(plan.d_uchunks, T.uchunks.data(), s_uc);
+    up(plan.d_pool, T.pool.data(), T.pool.size() * 8);
+    return true;
+}
+
+}  // namespace tblis_amd

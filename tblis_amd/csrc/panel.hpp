@@ -1,0 +1,70 @@
+// FP64 panel path: GEMMs assembled from operand blocks, packed once into
+// fragment-ordered panels (plan_types.hpp) and multiplied by the panel GEMM.
+#pragma once
+
+#include <vector>
+
+#include "plan.hpp"
+
+namespace tblis_amd
+{
+
+// Rows x k of one operand block; base == PANEL_MISSING means zeros.
+struct PBlock
+{
+    int64_t base = PANEL_MISSING;
+    Side rows, k;
+};
+
+struct CBlock
+{
+    int64_t base = 0;
+    Side rows, cols;
+};
+
+// One GEMM over blocks: m = sum(mlen), n = sum(nlen), k = sum(klen).  A block
+// (p, q) is a[p * nk + q] (rows along m), B block (s, q) is b[s * nk + q]
+// (rows along n), C block (p, s) is c[p * nn + s].  A dense contraction is
+// one block of each; a DPD irrep GEMM has one block per irrep combination.
+struct PanelGemm
+{
+    std::vector<int64_t> mlen, nlen, klen;
+    std::vector<PBlock> a, b;
+    std::vector<CBlock> c;
+};
+
+// C^T = B^T A^T.
+PanelGemm transpose_gemm(const PanelGemm& g);
+
+// One PanelGemm per batch element of a GemmDesc (one C patch with its k
+// segments).
+void panel_from_desc(const GemmDesc& g, std::vector<PanelGemm>& out);
+
+// The panel path serves the FP64 family on MFMA; TBLIS_HIP_PANEL=0 turns it
+// off (direct-gather kernels), =1 forces it for every FP64 plan.
+bool panel_enabled(int family, int use_mfma);
+
+// Host tables of a panel plan.
+struct PanelTables
+{
+    std::vector<PanelJob> jobs;
+    std::vector<PanelChunk> chunks;
+    std::vector<PanelTask> tasks;  // longest first
+    std::vector<UnpackJob> ujobs;
+    std::vector<UnpackChunk> uchunks;
+    std::vector<int64_t> pool;
+    int tile = 0;
+    bool swap = false;  // GEMMs were transposed: operand 0 is the B tensor
+    bool reads_ab = false;
+    int64_t ws = 0;     // workspace doubles
+    int64_t useful_flops = 0, mfma_flops = 0;
+};
+
+// Orientation, tile, pack jobs, tasks and C unpack jobs (host only).
+// Returns false when there is nothing to launch.
+bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T);
+
+// plan_panel_tables + upload into the plan.
+bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, int dtype);
+
+}  // namespace tblis_amd

@@ -1,6 +1,8 @@
 // Planner, plan cache and launch.
 #include "plan.hpp"
 
+#include "panel.hpp"
+
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -55,9 +57,19 @@ Plan::~Plan()
 }
 
 PlanBuilder::PlanBuilder(int elem_size, int dtype, int use_mfma, int family)
-    : elem_size_(elem_size), dtype_(dtype), use_mfma_(use_mfma), family_(family)
+    : elem_size_(elem_size),
+      dtype_(dtype),
+      use_mfma_(use_mfma),
+      family_(family),
+      panel_gemms_(new std::vector<PanelGemm>())
 {
 }
+
+PlanBuilder::~PlanBuilder() = default;
+
+void PlanBuilder::add_panel(const PanelGemm& g) { panel_gemms_->push_back(g); }
+
+bool PlanBuilder::panel_route() const { return panel_enabled(family_, use_mfma_); }
 
 SideRef PlanBuilder::side_ref(const Side& s, int64_t BS, int64_t start)
 {
@@ -188,8 +200,16 @@ bool PlanBuilder::try_packed(Plan& plan, std::vector<TaskDesc>& tasks, std::vect
 
 std::shared_ptr<Plan> PlanBuilder::build()
 {
-    if (gemms_.empty()) return nullptr;
+    if (gemms_.empty() && panel_gemms_->empty()) return nullptr;
     auto plan = std::make_shared<Plan>();
+    if (!panel_gemms_->empty() || panel_route())
+    {
+        if (!panel_route()) fail("internal: block GEMMs need the panel path");
+        std::vector<PanelGemm> pg(*panel_gemms_);
+        for (auto& g : gemms_) panel_from_desc(g, pg);
+        if (!build_panel_plan(*plan, std::move(pg), elem_size_, dtype_)) return nullptr;
+        return plan;
+    }
 
     // Orientation: the MFMA output puts consecutive lanes along n, so make
     // n the unit-stride side of C when C is contiguous along m.
@@ -576,7 +596,33 @@ void run_plan(const std::shared_ptr<Plan>& pp, const void* A, const void* B, voi
     L.use_k = use_k && p.reads_ab ? 1 : 0;
     L.grid = p.grid;
     L.stream = ts.stream;
-    if (p.packed && L.use_k)
+    if (p.panel)
+    {
+        PanelLaunch P{};
+        P.jobs = p.d_pjobs;
+        P.chunks = p.d_pchunks;
+        P.nchunks = p.p_nchunks;
+        P.tasks = p.d_ptasks;
+        P.ntasks = p.ntasks;
+        P.ujobs = p.d_ujobs;
+        P.uchunks = p.d_uchunks;
+        P.nuchunks = p.p_nuchunks;
+        P.pool = p.d_pool;
+        P.A = L.A;
+        P.B = L.B;
+        P.C = C;
+        P.ws = (double*)workspace(p.ws_bytes, ts.stream);
+        P.alpha = alpha;
+        P.beta = beta;
+        P.use_k = L.use_k;
+        P.dtype = p.key.dtype;
+        P.tile = p.panel_tile;
+        P.stream = ts.stream;
+        if (P.use_k) hip_check(launch_panel_pack(P), "pack launch");
+        hip_check(launch_panel_gemm(P), "kernel launch");
+        hip_check(launch_panel_unpack(P), "unpack launch");
+    }
+    else if (p.packed && L.use_k)
     {
         __bf16* ws = (__bf16*)workspace(p.ws_bytes, ts.stream);
         const void* src[2] = {L.A, L.B};

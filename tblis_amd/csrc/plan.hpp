@@ -51,6 +51,15 @@ struct Plan
     PackDesc pack[2] = {};
     bool pack_kfast[2] = {false, false};
     size_t ws_bytes = 0;
+    // FP64 panel plan (panel.hpp): pack jobs -> panel GEMM tasks -> C unpack
+    bool panel = false;
+    int panel_tile = 0;
+    int p_nchunks = 0, p_nuchunks = 0;
+    PanelJob* d_pjobs = nullptr;
+    PanelChunk* d_pchunks = nullptr;
+    PanelTask* d_ptasks = nullptr;
+    UnpackJob* d_ujobs = nullptr;
+    UnpackChunk* d_uchunks = nullptr;
     // Data pointers (A, B, C of the first instance) whose memory checks
     // passed on the last call; a call with the same pointers skips them.
     // The key of a batched plan fixes every other instance's pointer
@@ -81,11 +90,18 @@ struct Plan
     ~Plan();
 };
 
+struct PanelGemm;
+
 class PlanBuilder
 {
   public:
     PlanBuilder(int elem_size, int dtype, int use_mfma, int family = FAMILY_F64);
+    ~PlanBuilder();
     void add(const GemmDesc& g) { gemms_.push_back(g); }
+    // A GEMM assembled from blocks (panel path only).
+    void add_panel(const PanelGemm& g);
+    // Will build() take the panel path?
+    bool panel_route() const;
     // Choose swap/tile/thread maps, emit tasks, upload.  Returns nullptr if
     // there is nothing to launch.
     std::shared_ptr<Plan> build();
@@ -96,6 +112,7 @@ class PlanBuilder
     int64_t kblock_table(const Side& s, int64_t BK);
     int elem_size_, dtype_, use_mfma_, family_;
     std::vector<GemmDesc> gemms_;
+    std::unique_ptr<std::vector<PanelGemm>> panel_gemms_;
     std::vector<int64_t> pool_;
     std::map<std::vector<int64_t>, int64_t> dedupe_;
 };

@@ -66,6 +66,63 @@ struct PackDesc
     int32_t nrt, nkt;
 };
 
+// ---------------------------------------------------------------- panels
+// FP64 panel path.  Each GEMM operand (A': rows = m, B': rows = n, both
+// against k) is gathered once into a fragment-ordered panel: 16-row by
+// 8-k fragment groups of 128 doubles, group (rb, kg) at
+//     out + (rb * pnkg + kg) * 128,
+// lane l of a wave holding X[16 rb + l % 16][8 kg + l / 16] and
+// X[16 rb + l % 16][8 kg + 4 + l / 16] as one 16-byte pair -- exactly the
+// operand of two v_mfma_f64_16x16x4 steps.  Rows and k are padded with
+// zeros to the tile and to 16.
+constexpr int64_t PANEL_MISSING = INT64_MIN;
+
+// One block of one operand (a dense tensor has one block per operand, a
+// DPD irrep GEMM one per (row irreps, k irreps) pair): panel rows
+// [r0, r0 + nr) x k [k0, k0 + nk), element (i, kk) read from
+//     X[src + pool[rowmap + i] + pool[kmap + kk]],
+// or zeros when src == PANEL_MISSING (padding, absent blocks).
+struct PanelJob
+{
+    int64_t src;
+    int64_t rowmap, kmap;  // pool indices
+    int64_t out;           // panel offset (doubles) in the workspace
+    int32_t r0, nr, k0, nk;
+    int32_t pnkg;          // k groups of the whole panel
+    int32_t operand;       // 0: A pointer, 1: B pointer (after orientation)
+};
+
+// Rectangle of fragment groups of one job: rows blocks [rb0, rb0 + nrb),
+// k groups [kg0, kg0 + nkg).
+struct PanelChunk
+{
+    int32_t job, rb0, nrb, kg0, nkg, pad;
+};
+
+// One C tile of one panel GEMM.
+struct PanelTask
+{
+    int64_t a, b;        // workspace offsets (doubles) of the tile's first row block of A', B'
+    int64_t c_src;       // C: element offset of the C block; C panel: workspace offset
+    int64_t c_rowmap;    // pool index of the tile's first row offset (direct epilogue)
+    int64_t c_colmap;    // pool index of the tile's first col offset
+    int32_t nkg;         // k groups (even)
+    int32_t mlen, nlen;  // valid rows / cols of the tile
+    int32_t c_ld;        // 0: write C directly; else leading dimension of the C panel
+};
+
+// C panel -> C block: C[dst + rowmap[i] + colmap[j]] = alpha * P[i * ld + j] + beta * C[...]
+struct UnpackJob
+{
+    int64_t dst, rowmap, colmap, panel;
+    int32_t nr, nc, ld, rows_fast;
+};
+
+struct UnpackChunk
+{
+    int32_t job, r0, c0, pad;  // 64 x 64 element sub-block
+};
+
 // Kernel families.
 constexpr int FAMILY_F64 = 0;          // FP64 MFMA (FP32 data widened)
 constexpr int FAMILY_BF16X3 = 1;       // FP32 as 3-term split BF16, chunked accumulation

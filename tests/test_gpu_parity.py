@@ -290,9 +290,11 @@ def test_batch_uneven():
 
 @pytest.mark.parametrize("tile", range(8))
 def test_every_tile_variant(tile, monkeypatch):
-    """Each tile shape of the FP64 kernel (forced through TBLIS_HIP_TILE)
-    agrees with the oracle on contractions with ragged edges, scattered k
-    and a DPD problem (several k segments)."""
+    """Each tile shape of the direct-gather FP64 kernel (panel path off,
+    tile forced through TBLIS_HIP_TILE) agrees with the oracle on
+    contractions with ragged edges, scattered k and a DPD problem (several
+    k segments)."""
+    monkeypatch.setenv("TBLIS_HIP_PANEL", "0")
     monkeypatch.setenv("TBLIS_HIP_TILE", str(tile))
     T.lib().tblis_hip_clear_plan_cache()
     try:
@@ -305,4 +307,46 @@ def test_every_tile_variant(tile, monkeypatch):
         _dpd_case(rng, 4, "abef", "efij", "abij", ext, 1, 3, 5, alpha=1.0, beta=0.5)
     finally:
         monkeypatch.delenv("TBLIS_HIP_TILE")
+        T.lib().tblis_hip_clear_plan_cache()
+
+
+@pytest.mark.parametrize("tile", range(6))
+def test_every_panel_tile(tile, monkeypatch):
+    """Each tile of the FP64 panel GEMM (forced through TBLIS_HIP_PANEL_TILE):
+    dense contractions with ragged edges and scattered k, FP32 data through
+    FP64 MFMA, the 8-irrep DPD structure at small extents (merged irrep
+    GEMMs with a C panel), a DPD problem with a batch label, and a batch."""
+    monkeypatch.setenv("TBLIS_HIP_PANEL_TILE", str(tile))
+    T.lib().tblis_hip_clear_plan_cache()
+    try:
+        rng = np.random.default_rng(31000 + tile)
+        _dense_case(rng, "aebf", "fjei", "aibj", dict(a=13, b=11, e=9, f=7, i=5, j=14),
+                    ["col", "col", "col"])
+        _dense_case(rng, "abef", "efij", "abij", dict(a=24, b=7, e=19, f=5, i=33, j=3),
+                    ["random", "row", "random"], alpha=1.0, beta=0.0)
+        _dense_case(rng, "aebf", "fjei", "aibj", dict(a=9, b=13, e=7, f=6, i=11, j=5),
+                    ["col", "col", "col"], dtype=np.float32, config="fp64_mfma", tol=1e-6)
+        ext = {c: [4, 3, 2, 1, 5, 3, 2, 1] for c in "abefij"}
+        _dpd_case(rng, 8, "abef", "efij", "abij", ext, 0, 0, 5, alpha=0.5, beta=1.0)
+        ext = {c: [3, 0, 2, 4] for c in "aebfij"}
+        _dpd_case(rng, 4, "aebf", "fjei", "aibj", ext, 1, 3, 4, alpha=2.0, beta=-1.0)
+        ext = {c: [int(rng.integers(1, 4)) for _ in range(4)] for c in "abez"}
+        _dpd_case(rng, 4, "aez", "ebz", "abz", ext, 1, 2, 5, alpha=1.5, beta=0.5, mode=2)
+        test_batch_uneven()
+    finally:
+        monkeypatch.delenv("TBLIS_HIP_PANEL_TILE")
+        T.lib().tblis_hip_clear_plan_cache()
+
+
+def test_direct_path_dpd_and_batch(monkeypatch):
+    """The direct-gather kernels (panel path off) still serve DPD and batches."""
+    monkeypatch.setenv("TBLIS_HIP_PANEL", "0")
+    T.lib().tblis_hip_clear_plan_cache()
+    try:
+        rng = np.random.default_rng(32000)
+        ext = {c: [4, 3, 2, 1, 5, 3, 2, 1] for c in "abefij"}
+        _dpd_case(rng, 8, "abef", "efij", "abij", ext, 0, 0, 5, alpha=0.5, beta=1.0)
+        test_batch_uneven()
+    finally:
+        monkeypatch.delenv("TBLIS_HIP_PANEL")
         T.lib().tblis_hip_clear_plan_cache()

@@ -246,7 +246,7 @@ hipError_t launch_gemm_t(const PanelLaunch& L)
 #define X(id, bm, bn, wgm, wgn)                                                                       \
     case id:                                                                                          \
         hipLaunchKernelGGL((gemm_panel_kernel<bm, bn, wgm, wgn, TC>), grid, block, 0, L.stream,       \
-                           L.tasks, L.pool, (const double*)L.ws, L.ws, (TC*)L.C, L.alpha, L.beta,     \
+                           L.tasks, L.pool, (const double*)L.ws, L.cws, (TC*)L.C, L.alpha, L.beta,    \
                            L.use_k);                                                                  \
         break;
         TBLIS_PANEL_TILES(X)
@@ -259,26 +259,40 @@ hipError_t launch_gemm_t(const PanelLaunch& L)
 
 }  // namespace
 
-int panel_num_tiles() { return kNumTiles; }
-int panel_tile_m(int t) { return kTiles[t].bm; }
-int panel_tile_n(int t) { return kTiles[t].bn; }
-int panel_tile_waves(int t) { return kTiles[t].wgm * kTiles[t].wgn; }
+int panel_num_tiles(int family)
+{
+    return family == FAMILY_F64 ? kNumTiles : panel_bf16_num_tiles();
+}
+int panel_tile_m(int family, int t)
+{
+    return family == FAMILY_F64 ? kTiles[t].bm : panel_bf16_tile_m(t);
+}
+int panel_tile_n(int family, int t)
+{
+    return family == FAMILY_F64 ? kTiles[t].bn : panel_bf16_tile_n(t);
+}
+int panel_tile_waves(int family, int t)
+{
+    return family == FAMILY_F64 ? kTiles[t].wgm * kTiles[t].wgn : panel_bf16_tile_waves(t);
+}
 
 hipError_t launch_panel_pack(const PanelLaunch& L)
 {
     if (L.nchunks == 0) return hipSuccess;
+    if (L.family != FAMILY_F64) return launch_panel_pack_bf16(L);
     if (L.dtype == 0)
         hipLaunchKernelGGL(pack_panel_kernel<double>, dim3(L.nchunks), dim3(256), 0, L.stream, L.jobs,
-                           L.chunks, L.pool, (const double*)L.A, (const double*)L.B, L.ws);
+                           L.chunks, L.pool, (const double*)L.A, (const double*)L.B, (double*)L.ws);
     else
         hipLaunchKernelGGL(pack_panel_kernel<float>, dim3(L.nchunks), dim3(256), 0, L.stream, L.jobs,
-                           L.chunks, L.pool, (const float*)L.A, (const float*)L.B, L.ws);
+                           L.chunks, L.pool, (const float*)L.A, (const float*)L.B, (double*)L.ws);
     return hipGetLastError();
 }
 
 hipError_t launch_panel_gemm(const PanelLaunch& L)
 {
     if (L.ntasks == 0) return hipSuccess;
+    if (L.family != FAMILY_F64) return launch_panel_gemm_bf16(L);
     if (L.tile < 0 || L.tile >= kNumTiles) return hipErrorInvalidValue;
     return L.dtype == 0 ? launch_gemm_t<double>(L) : launch_gemm_t<float>(L);
 }
@@ -288,11 +302,11 @@ hipError_t launch_panel_unpack(const PanelLaunch& L)
     if (L.nuchunks == 0) return hipSuccess;
     if (L.dtype == 0)
         hipLaunchKernelGGL(unpack_panel_kernel<double>, dim3(L.nuchunks), dim3(256), 0, L.stream,
-                           L.ujobs, L.uchunks, L.pool, (const double*)L.ws, (double*)L.C, L.alpha,
+                           L.ujobs, L.uchunks, L.pool, (const double*)L.cws, (double*)L.C, L.alpha,
                            L.beta);
     else
         hipLaunchKernelGGL(unpack_panel_kernel<float>, dim3(L.nuchunks), dim3(256), 0, L.stream,
-                           L.ujobs, L.uchunks, L.pool, (const double*)L.ws, (float*)L.C, L.alpha,
+                           L.ujobs, L.uchunks, L.pool, (const double*)L.cws, (float*)L.C, L.alpha,
                            L.beta);
     return hipGetLastError();
 }

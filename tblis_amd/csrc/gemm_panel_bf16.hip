@@ -1,0 +1,311 @@
+// Split-BF16 MFMA over fragment-ordered panels (gfx950).
+//
+// FP32 operands are split once, at packing time, into three BF16 terms
+// x = hi + mid + lo (hi = RN(x), mid = RN(x - hi), lo = RN(x - hi - mid)),
+// and every C element accumulates the six products that carry FP32
+// accuracy, smallest first: mid*mid, hi*lo, lo*hi, hi*mid, mid*hi, hi*hi,
+// with v_mfma_f32_32x32x16_bf16 (FP32 accumulation).  With CHUNKED the
+// products of every 256 k are summed in a fresh accumulator that is then
+// added to a second one, which keeps the FP32 rounding of long k sums near
+// that of a 256-term sum.
+//
+// Panels use the FP64 path's jobs and tasks (plan_types.hpp) with a larger
+// fragment group: 32 rows x 16 k, three terms, group (rb, ks) at
+//     out + ((rb * pnkg + ks) * 3 + term) * 512,
+// lane l holding X[32 rb + l % 32][16 ks + 8 (l / 32) + j], j = 0..7, as one
+// 16-byte vector -- the A and B operand of v_mfma_f32_32x32x16_bf16.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace tblis_amd
+{
+
+namespace
+{
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
+{
+    h = (__bf16)x;
+    const float r1 = x - (float)h;
+    m = (__bf16)r1;
+    const float r2 = r1 - (float)m;
+    l = (__bf16)r2;
+}
+
+// ------------------------------------------------------------------ pack
+
+__global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PanelJob* __restrict__ jobs,
+                                                              const PanelChunk* __restrict__ chunks,
+                                                              const int64_t* __restrict__ pool,
+                                                              const float* __restrict__ A,
+                                                              const float* __restrict__ B,
+                                                              __bf16* __restrict__ ws)
+{
+    const PanelChunk ch = chunks[blockIdx.x];
+    const PanelJob j = jobs[ch.job];
+    const float* __restrict__ X = j.operand ? B : A;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 31, lh = lane >> 5;
+    const bool zero = j.src == PANEL_MISSING;
+    const int ng = ch.nrb * ch.nkg;
+    for (int g = wave; g < ng; g += 4)
+    {
+        const int rb = ch.rb0 + g / ch.nkg;
+        const int ks = ch.kg0 + g % ch.nkg;
+        const int row = rb * 32 + lr - j.r0;
+        const bool rin = row >= 0 && row < j.nr;
+        const int kb = ks * 16 + lh * 8 - j.k0;
+        int64_t roff = 0;
+        if (rin && !zero) roff = j.src + pool[j.rowmap + row];
+        bf16x8 h, m, l;
+        unsigned inmask = 0;
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+        {
+            const int k = kb + e;
+            const bool in = rin && k >= 0 && k < j.nk;
+            float v = 0.0f;
+            if (in && !zero) v = X[roff + pool[j.kmap + k]];
+            __bf16 a, b, c;
+            split3(v, a, b, c);
+            h[e] = a;
+            m[e] = b;
+            l[e] = c;
+            inmask |= (unsigned)in << e;
+        }
+        __bf16* out = ws + j.out + ((int64_t)rb * j.pnkg + ks) * 1536 + lane * 8;
+        if (inmask == 0xffu)
+        {
+            *(bf16x8*)out = h;
+            *(bf16x8*)(out + 512) = m;
+            *(bf16x8*)(out + 1024) = l;
+        }
+        else if (inmask)
+        {
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                if (inmask >> e & 1u)
+                {
+                    out[e] = h[e];
+                    out[512 + e] = m[e];
+                    out[1024 + e] = l[e];
+                }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ gemm
+
+template <int MB, int NB>
+__device__ __forceinline__ void load_frags(bf16x8 (&a)[MB][3], bf16x8 (&b)[NB][3], const uint4* pa,
+                                           const uint4* pb, int64_t rs, int ks)
+{
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int t = 0; t < 3; t++)
+        {
+            uint4 v = pa[i * rs + ks * 192 + t * 64];
+            a[i][t] = __builtin_bit_cast(bf16x8, v);
+        }
+#pragma unroll
+    for (int jb = 0; jb < NB; jb++)
+#pragma unroll
+        for (int t = 0; t < 3; t++)
+        {
+            uint4 v = pb[jb * rs + ks * 192 + t * 64];
+            b[jb][t] = __builtin_bit_cast(bf16x8, v);
+        }
+}
+
+// Six products per block pair, interleaved across blocks so that
+// consecutive MFMAs on one accumulator are MB*NB instructions apart.
+template <int MB, int NB>
+__device__ __forceinline__ void mma_frags(f32x16 (&acc)[MB][NB], const bf16x8 (&a)[MB][3],
+                                          const bf16x8 (&b)[NB][3])
+{
+    constexpr int PA[6] = {1, 0, 2, 0, 1, 0};
+    constexpr int PB[6] = {1, 2, 0, 1, 0, 0};
+#pragma unroll
+    for (int p = 0; p < 6; p++)
+#pragma unroll
+        for (int i = 0; i < MB; i++)
+#pragma unroll
+            for (int jb = 0; jb < NB; jb++)
+                acc[i][jb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i][PA[p]], b[jb][PB[p]],
+                                                                    acc[i][jb], 0, 0, 0);
+}
+
+template <int BM, int BN, int WGM, int WGN, bool CHUNKED>
+__global__ __launch_bounds__(64 * WGM * WGN) void gemm_bf16_panel_kernel(
+    const PanelTask* __restrict__ tasks, const int64_t* __restrict__ pool,
+    const __bf16* __restrict__ ws, double* __restrict__ cws, float* __restrict__ C, double alpha,
+    double beta, int use_k)
+{
+    constexpr int WM = BM / WGM, WN = BN / WGN, MB = WM / 32, NB = WN / 32;
+    constexpr int CHUNK = 16;  // k16 steps per accumulation chunk (256 k)
+    const PanelTask t = tasks[blockIdx.x];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = wave % WGM, wn = wave / WGM;
+    if (wm * WM >= t.mlen || wn * WN >= t.nlen) return;  // no barriers below
+
+    f32x16 acc[MB][NB], tot[CHUNKED ? MB : 1][CHUNKED ? NB : 1];
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int jb = 0; jb < NB; jb++)
+            for (int r = 0; r < 16; r++) acc[i][jb][r] = 0.0f;
+    if (CHUNKED)
+#pragma unroll
+        for (int i = 0; i < (CHUNKED ? MB : 1); i++)
+#pragma unroll
+            for (int jb = 0; jb < (CHUNKED ? NB : 1); jb++)
+                for (int r = 0; r < 16; r++) tot[i][jb][r] = 0.0f;
+
+    const int nk = use_k ? t.nkg : 0;  // even
+    if (nk > 0)
+    {
+        const int64_t rs = (int64_t)t.nkg * 192;  // uint4 per row block
+        const uint4* pa = (const uint4*)(ws + t.a) + (int64_t)wm * MB * rs + lane;
+        const uint4* pb = (const uint4*)(ws + t.b) + (int64_t)wn * NB * rs + lane;
+        bf16x8 a0[MB][3], b0[NB][3], a1[MB][3], b1[NB][3];
+        load_frags<MB, NB>(a0, b0, pa, pb, rs, 0);
+        load_frags<MB, NB>(a1, b1, pa, pb, rs, 1);
+        for (int ks = 0; ks < nk; ks += 2)
+        {
+            // the last iteration re-reads the final groups (in bounds, unused)
+            const int nx = ks + 2 < nk ? ks + 2 : nk - 2;
+            mma_frags<MB, NB>(acc, a0, b0);
+            load_frags<MB, NB>(a0, b0, pa, pb, rs, nx);
+            mma_frags<MB, NB>(acc, a1, b1);
+            load_frags<MB, NB>(a1, b1, pa, pb, rs, nx + 1);
+            if (CHUNKED && ((ks + 2) % CHUNK == 0 || ks + 2 >= nk))
+            {
+#pragma unroll
+                for (int i = 0; i < MB; i++)
+#pragma unroll
+                    for (int jb = 0; jb < NB; jb++)
+                    {
+                        tot[i % (CHUNKED ? MB : 1)][jb % (CHUNKED ? NB : 1)] += acc[i][jb];
+                        for (int r = 0; r < 16; r++) acc[i][jb][r] = 0.0f;
+                    }
+            }
+        }
+    }
+
+    // D register r of lane l holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]
+    const int cl = lane & 31, rh = lane >> 5;
+    int64_t coff[NB];
+    if (!t.c_ld)
+#pragma unroll
+        for (int jb = 0; jb < NB; jb++)
+        {
+            const int col = wn * WN + jb * 32 + cl;
+            coff[jb] = col < t.nlen ? pool[t.c_colmap + col] : 0;
+        }
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+        {
+            const int row = wm * WM + i * 32 + 8 * (r / 4) + 4 * rh + r % 4;
+            if (row >= t.mlen) continue;
+            const int64_t roff = t.c_ld ? 0 : t.c_src + pool[t.c_rowmap + row];
+#pragma unroll
+            for (int jb = 0; jb < NB; jb++)
+            {
+                const int col = wn * WN + jb * 32 + cl;
+                if (col >= t.nlen) continue;
+                const float s = CHUNKED ? tot[i % (CHUNKED ? MB : 1)][jb % (CHUNKED ? NB : 1)][r]
+                                        : acc[i][jb][r];
+                if (t.c_ld)
+                {
+                    cws[t.c_src + (int64_t)row * t.c_ld + col] = (double)s;
+                }
+                else
+                {
+                    float* p = C + roff + coff[jb];
+                    double v = alpha * (double)s;
+                    if (beta != 0.0) v += beta * (double)*p;
+                    *p = (float)v;
+                }
+            }
+        }
+}
+
+// ------------------------------------------------------------------ table
+
+struct Bf16Tile
+{
+    int bm, bn, wgm, wgn;
+    bool fast_only;  // the chunked accumulator does not fit the register file
+};
+
+// id, BM, BN, waves (WGM x WGN), fast only.  Registers per wave (gfx950):
+// 128x64 and 64x128 wave tiles 228 (single accumulator) / 410 (chunked),
+// 64x64 174 / 230, 128x128 512 (single accumulator only).
+#define TBLIS_PANEL_BF16_TILES(X) \
+    X(0, 256, 128, 2, 2, false)   \
+    X(1, 128, 256, 2, 2, false)   \
+    X(2, 128, 128, 2, 2, false)   \
+    X(3, 256, 256, 2, 2, true)
+
+constexpr Bf16Tile kBTiles[] = {
+#define X(id, bm, bn, wgm, wgn, fo) {bm, bn, wgm, wgn, fo},
+    TBLIS_PANEL_BF16_TILES(X)
+#undef X
+};
+constexpr int kNumBTiles = sizeof(kBTiles) / sizeof(kBTiles[0]);
+
+}  // namespace
+
+int panel_bf16_num_tiles() { return kNumBTiles; }
+int panel_bf16_tile_m(int t) { return kBTiles[t].bm; }
+int panel_bf16_tile_n(int t) { return kBTiles[t].bn; }
+int panel_bf16_tile_waves(int t) { return kBTiles[t].wgm * kBTiles[t].wgn; }
+bool panel_bf16_tile_fast_only(int t) { return kBTiles[t].fast_only; }
+
+hipError_t launch_panel_pack_bf16(const PanelLaunch& L)
+{
+    if (L.nchunks == 0) return hipSuccess;
+    if (L.dtype != 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pack_bf16_panel_kernel, dim3(L.nchunks), dim3(256), 0, L.stream, L.jobs,
+                       L.chunks, L.pool, (const float*)L.A, (const float*)L.B, (__bf16*)L.ws);
+    return hipGetLastError();
+}
+
+hipError_t launch_panel_gemm_bf16(const PanelLaunch& L)
+{
+    if (L.ntasks == 0) return hipSuccess;
+    if (L.tile < 0 || L.tile >= kNumBTiles || L.dtype != 1) return hipErrorInvalidValue;
+    const bool chunked = L.family == FAMILY_BF16X3;
+    if (chunked && kBTiles[L.tile].fast_only) return hipErrorInvalidValue;
+    const dim3 grid(L.ntasks), block(64 * kBTiles[L.tile].wgm * kBTiles[L.tile].wgn);
+    switch (L.tile * 2 + (chunked ? 1 : 0))
+    {
+#define X(id, bm, bn, wgm, wgn, fo)                                                                   \
+    case 2 * id:                                                                                      \
+        hipLaunchKernelGGL((gemm_bf16_panel_kernel<bm, bn, wgm, wgn, false>), grid, block, 0,         \
+                           L.stream, L.tasks, L.pool, (const __bf16*)L.ws, L.cws, (float*)L.C,        \
+                           L.alpha, L.beta, L.use_k);                                                 \
+        break;                                                                                        \
+    case 2 * id + 1:                                                                                  \
+        if (!fo)                                                                                      \
+            hipLaunchKernelGGL((gemm_bf16_panel_kernel<bm, bn, wgm, wgn, !fo>), grid, block, 0,       \
+                               L.stream, L.tasks, L.pool, (const __bf16*)L.ws, L.cws, (float*)L.C,    \
+                               L.alpha, L.beta, L.use_k);                                             \
+        break;
+        TBLIS_PANEL_BF16_TILES(X)
+#undef X
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace tblis_amd

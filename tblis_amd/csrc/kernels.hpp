@@ -45,11 +45,11 @@ hipError_t launch_pack_bf16x3(const PackDesc& d, bool kfast, const int64_t* pool
 hipError_t launch_gemm_bf16x3_packed(const GemmLaunch& L, bool chunked);
 int gemm_bf16x3_packed_occupancy(bool chunked);
 
-// FP64 panel path
-int panel_num_tiles();
-int panel_tile_m(int tile);
-int panel_tile_n(int tile);
-int panel_tile_waves(int tile);
+// Panel path (FP64 and split-BF16 families)
+int panel_num_tiles(int family);
+int panel_tile_m(int family, int tile);
+int panel_tile_n(int family, int tile);
+int panel_tile_waves(int family, int tile);
 struct PanelLaunch
 {
     const PanelJob* jobs;
@@ -64,16 +64,26 @@ struct PanelLaunch
     const void* A;
     const void* B;
     void* C;
-    double* ws;
+    void* ws;     // panels (double or BF16 values)
+    double* cws;  // C panels
     double alpha, beta;
     int use_k;
-    int dtype;  // 0: double, 1: float
+    int dtype;   // 0: double, 1: float
+    int family;  // FAMILY_F64 or a split-BF16 family
     int tile;
     hipStream_t stream;
 };
 hipError_t launch_panel_pack(const PanelLaunch& L);
 hipError_t launch_panel_gemm(const PanelLaunch& L);
 hipError_t launch_panel_unpack(const PanelLaunch& L);
+// split-BF16 panels (gemm_panel_bf16.hip)
+int panel_bf16_num_tiles();
+int panel_bf16_tile_m(int tile);
+int panel_bf16_tile_n(int tile);
+int panel_bf16_tile_waves(int tile);
+bool panel_bf16_tile_fast_only(int tile);
+hipError_t launch_panel_pack_bf16(const PanelLaunch& L);
+hipError_t launch_panel_gemm_bf16(const PanelLaunch& L);
 
 // family-generic helpers
 inline int tile_m(const KernelKey& k)

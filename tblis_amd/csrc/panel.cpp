@@ -74,6 +74,23 @@ class Pool
     std::map<std::vector<int64_t>, int64_t> index_;
 };
 
+// Fragment-group geometry of a family's panels.
+struct Geom
+{
+    int gr;          // rows per fragment group
+    int gk;          // k per fragment group
+    int terms;       // values stored per element (split-BF16: 3)
+    int kpad;        // k padded to a multiple of this (two groups: the GEMM
+                     // loop is unrolled by two)
+    int elem_bytes;  // bytes per stored value
+};
+
+Geom geom_of(int family)
+{
+    if (family == FAMILY_F64) return Geom{16, 8, 1, 16, 8};
+    return Geom{32, 16, 3, 32, 2};
+}
+
 struct TileModel
 {
     int tile;
@@ -81,16 +98,20 @@ struct TileModel
     int occ;     // waves per SIMD (register limited)
 };
 
-// Per-wave efficiency by wave tile (64x64: 4x4 MFMA blocks, 32x32: 2x2).
-const TileModel kModels[] = {{0, 0.85, 2}, {2, 0.80, 2}, {3, 0.82, 2}, {4, 0.82, 2}, {1, 0.50, 6},
-                             {5, 0.45, 6}};
+// Per-wave efficiency by wave tile.  FP64: 64x64 waves are 4x4 MFMA
+// blocks, 32x32 waves 2x2.  Split-BF16: per tile of the BF16 table.
+const TileModel kModelsF64[] = {{0, 0.85, 2}, {2, 0.80, 2}, {3, 0.82, 2}, {4, 0.82, 2},
+                                {1, 0.50, 6}, {5, 0.45, 6}};
+const TileModel kModelsBf16[] = {{0, 0.55, 1}, {1, 0.55, 1}, {2, 0.35, 2}, {3, 0.60, 1}};
 
 // Estimated run time of the GEMM tasks for a tile.
-double tile_cost(const std::vector<PanelGemm>& gemms, const TileModel& tm, int cus)
+double tile_cost(const std::vector<PanelGemm>& gemms, const TileModel& tm, int cus, int family)
 {
-    const int64_t BM = panel_tile_m(tm.tile), BN = panel_tile_n(tm.tile);
-    const int W = panel_tile_waves(tm.tile);
-    const double simd = 60e9;  // FP64 MFMA flop/s per SIMD at the sustained clock
+    const int64_t BM = panel_tile_m(family, tm.tile), BN = panel_tile_n(family, tm.tile);
+    const int W = panel_tile_waves(family, tm.tile);
+    // MFMA flop/s per SIMD at the sustained clock (split-BF16: MFMA flops
+    // of the 6 products)
+    const double simd = family == FAMILY_F64 ? 60e9 : 2200e9 / 6;
     double total = 0, maxtask = 0;
     int64_t waves = 0;
     for (auto& g : gemms)
@@ -173,14 +194,15 @@ void panel_from_desc(const GemmDesc& g, std::vector<PanelGemm>& out)
 
 bool panel_enabled(int family, int use_mfma)
 {
-    if (family != FAMILY_F64 || !use_mfma) return false;
+    if (!use_mfma) return false;
     if (const char* e = std::getenv("TBLIS_HIP_PANEL"))
         if (e[0] == '0') return false;
     return true;
 }
 
-bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T)
+bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family)
 {
+    const Geom G = geom_of(family);
     // drop GEMMs without rows or columns
     gemms.erase(std::remove_if(gemms.begin(), gemms.end(),
                                [](const PanelGemm& g) {
@@ -203,9 +225,13 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T)
     const int cus = device_cu_count();
     int tile = 0;
     double best = -1;
-    for (const TileModel& tm : kModels)
+    const TileModel* models = family == FAMILY_F64 ? kModelsF64 : kModelsBf16;
+    const int nmodels = family == FAMILY_F64 ? 6 : 4;
+    for (int mi = 0; mi < nmodels; mi++)
     {
-        double c = tile_cost(gemms, tm, cus);
+        const TileModel& tm = models[mi];
+        if (family == FAMILY_BF16X3 && panel_bf16_tile_fast_only(tm.tile)) continue;
+        double c = tile_cost(gemms, tm, cus, family);
         if (best < 0 || c < best)
         {
             best = c;
@@ -215,9 +241,11 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T)
     if (const char* e = std::getenv("TBLIS_HIP_PANEL_TILE"))
     {
         int t = std::atoi(e);
-        if (e[0] >= '0' && e[0] <= '9' && t < panel_num_tiles()) tile = t;
+        if (e[0] >= '0' && e[0] <= '9' && t < panel_num_tiles(family) &&
+            !(family == FAMILY_BF16X3 && panel_bf16_tile_fast_only(t)))
+            tile = t;
     }
-    const int64_t BM = panel_tile_m(tile), BN = panel_tile_n(tile);
+    const int64_t BM = panel_tile_m(family, tile), BN = panel_tile_n(family, tile);
 
     Pool pool;
     std::vector<PanelJob>& jobs = T.jobs;
@@ -226,7 +254,8 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T)
     std::vector<int64_t> cost;
     std::vector<UnpackJob>& ujobs = T.ujobs;
     std::vector<UnpackChunk>& uchunks = T.uchunks;
-    int64_t ws = 0;  // doubles
+    int64_t ws = 0;   // panel values (family type)
+    int64_t wsc = 0;  // C panel doubles
     T.useful_flops = 0;
     T.mfma_flops = 0;
     T.reads_ab = false;
@@ -235,8 +264,8 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T)
         if (j.nr <= 0 || j.nk <= 0) return;
         const int32_t id = (int32_t)jobs.size();
         jobs.push_back(j);
-        const int64_t rb0 = j.r0 / 16, rb1 = ceil_div(j.r0 + j.nr, 16);
-        const int64_t kg0 = j.k0 / 8, kg1 = ceil_div(j.k0 + j.nk, 8);
+        const int64_t rb0 = j.r0 / G.gr, rb1 = ceil_div(j.r0 + j.nr, G.gr);
+        const int64_t kg0 = j.k0 / G.gk, kg1 = ceil_div(j.k0 + j.nk, G.gk);
         for (int64_t rb = rb0; rb < rb1; rb += 4)
             for (int64_t kg = kg0; kg < kg1; kg += 16)
             {
@@ -260,8 +289,8 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T)
         const int64_t m = m0[NM], n = n0[NN], k = k0[NK];
         if (m == 0 || n == 0) continue;
         const int64_t mp = ceil_div(m, BM) * BM, np = ceil_div(n, BN) * BN;
-        const int64_t kp = ceil_div(k, 16) * 16;
-        const int64_t nkg = kp / 8;
+        const int64_t kp = ceil_div(k, G.kpad) * G.kpad;
+        const int64_t nkg = kp / G.gk;
         if (m > INT32_MAX || n > INT32_MAX || kp > INT32_MAX) fail("contraction too large for one GEMM");
         T.useful_flops += 2 * m * n * k;
         T.mfma_flops += 2 * mp * np * kp;
@@ -269,9 +298,9 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T)
 
         // panels
         const int64_t oa = ws;
-        ws += mp * kp;
+        ws += G.terms * mp * kp;
         const int64_t ob = ws;
-        ws += np * kp;
+        ws += G.terms * np * kp;
         const std::vector<PBlock>* blocks[2] = {&g.a, &g.b};
         const std::vector<int64_t>* rpre[2] = {&m0, &n0};
         const int64_t rtot[2] = {m, n}, rpad[2] = {mp, np}, outs[2] = {oa, ob};
@@ -327,8 +356,8 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T)
         }
         else
         {
-            oc = ws;
-            ws += mp * np;
+            oc = wsc;
+            wsc += mp * np;
             for (size_t p = 0; p < NM; p++)
                 for (size_t s = 0; s < NN; s++)
                 {
@@ -353,8 +382,9 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T)
             for (int64_t tn = 0; tn < np / BN; tn++)
             {
                 PanelTask t{};
-                t.a = oa + tm * (BM / 16) * nkg * 128;
-                t.b = ob + tn * (BN / 16) * nkg * 128;
+                const int64_t ge = (int64_t)G.terms * G.gr * G.gk;  // values per group
+                t.a = oa + tm * (BM / G.gr) * nkg * ge;
+                t.b = ob + tn * (BN / G.gr) * nkg * ge;
                 t.nkg = (int32_t)nkg;
                 t.mlen = (int32_t)std::min<int64_t>(BM, m - tm * BM);
                 t.nlen = (int32_t)std::min<int64_t>(BN, n - tn * BN);
@@ -391,22 +421,24 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T)
     }
 
     T.tile = tile;
-    T.ws = std::max<int64_t>(ws, 1);
+    T.ws_panel = ws;
+    T.ws_c = wsc;
+    T.family = family;
     T.pool = std::move(pool.data());
     return true;
 }
 
-bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, int dtype)
+bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, int dtype, int family)
 {
     PanelTables T;
-    if (!plan_panel_tables(std::move(gemms), T)) return false;
+    if (!plan_panel_tables(std::move(gemms), T, family)) return false;
     plan.swap = T.swap;
     plan.useful_flops = T.useful_flops;
     plan.mfma_flops = T.mfma_flops;
     plan.reads_ab = T.reads_ab;
     plan.panel = true;
     plan.panel_tile = T.tile;
-    plan.key.family = FAMILY_F64;
+    plan.key.family = family;
     plan.key.dtype = dtype;
     plan.key.use_mfma = 1;
     plan.key.tile = T.tile;
@@ -415,7 +447,9 @@ bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, i
     plan.grid = plan.ntasks;
     plan.p_nchunks = (int)T.chunks.size();
     plan.p_nuchunks = (int)T.uchunks.size();
-    plan.ws_bytes = (size_t)T.ws * sizeof(double);
+    const size_t pbytes = (size_t)T.ws_panel * geom_of(family).elem_bytes;
+    plan.ws_c_offset = (pbytes + 255) / 256 * 256;
+    plan.ws_bytes = std::max<size_t>(plan.ws_c_offset + (size_t)T.ws_c * sizeof(double), 256);
 
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
     const size_t s_jobs = T.jobs.size() * sizeof(PanelJob), s_chunks = T.chunks.size() * sizeof(PanelChunk);

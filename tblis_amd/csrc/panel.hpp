@@ -40,8 +40,8 @@ PanelGemm transpose_gemm(const PanelGemm& g);
 // segments).
 void panel_from_desc(const GemmDesc& g, std::vector<PanelGemm>& out);
 
-// The panel path serves the FP64 family on MFMA; TBLIS_HIP_PANEL=0 turns it
-// off (direct-gather kernels), =1 forces it for every FP64 plan.
+// The panel path serves every MFMA family; TBLIS_HIP_PANEL=0 turns it off
+// (direct-gather kernels).
 bool panel_enabled(int family, int use_mfma);
 
 // Host tables of a panel plan.
@@ -54,17 +54,19 @@ struct PanelTables
     std::vector<UnpackChunk> uchunks;
     std::vector<int64_t> pool;
     int tile = 0;
+    int family = 0;
     bool swap = false;  // GEMMs were transposed: operand 0 is the B tensor
     bool reads_ab = false;
-    int64_t ws = 0;     // workspace doubles
+    int64_t ws_panel = 0;  // panel values (double for FP64, BF16 for split-BF16)
+    int64_t ws_c = 0;      // C panel doubles
     int64_t useful_flops = 0, mfma_flops = 0;
 };
 
 // Orientation, tile, pack jobs, tasks and C unpack jobs (host only).
 // Returns false when there is nothing to launch.
-bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T);
+bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family);
 
 // plan_panel_tables + upload into the plan.
-bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, int dtype);
+bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, int dtype, int family);
 
 }  // namespace tblis_amd

@@ -207,7 +207,7 @@ std::shared_ptr<Plan> PlanBuilder::build()
         if (!panel_route()) fail("internal: block GEMMs need the panel path");
         std::vector<PanelGemm> pg(*panel_gemms_);
         for (auto& g : gemms_) panel_from_desc(g, pg);
-        if (!build_panel_plan(*plan, std::move(pg), elem_size_, dtype_)) return nullptr;
+        if (!build_panel_plan(*plan, std::move(pg), elem_size_, dtype_, family_)) return nullptr;
         return plan;
     }
 
@@ -611,7 +611,10 @@ void run_plan(const std::shared_ptr<Plan>& pp, const void* A, const void* B, voi
         P.A = L.A;
         P.B = L.B;
         P.C = C;
-        P.ws = (double*)workspace(p.ws_bytes, ts.stream);
+        char* ws = (char*)workspace(p.ws_bytes, ts.stream);
+        P.ws = ws;
+        P.cws = (double*)(ws + p.ws_c_offset);
+        P.family = p.key.family;
         P.alpha = alpha;
         P.beta = beta;
         P.use_k = L.use_k;

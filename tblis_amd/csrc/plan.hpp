@@ -54,6 +54,7 @@ struct Plan
     // FP64 panel plan (panel.hpp): pack jobs -> panel GEMM tasks -> C unpack
     bool panel = false;
     int panel_tile = 0;
+    size_t ws_c_offset = 0;  // byte offset of the C panels in the workspace
     int p_nchunks = 0, p_nuchunks = 0;
     PanelJob* d_pjobs = nullptr;
     PanelChunk* d_pchunks = nullptr;

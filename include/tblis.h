@@ -224,6 +224,17 @@ const char* tblis_hip_get_last_error(void);
 void tblis_hip_clear_plan_cache(void);
 /* Algorithm selected by the last contraction on this thread (config name). */
 const char* tblis_hip_last_algorithm(void);
+/* Autotuner (off unless TBLIS_HIP_AUTOTUNE=1): when on, building a plan for
+ * a new set of GEMM shapes times every candidate tile of the panel GEMM on
+ * scratch buffers and keeps the fastest; results are cached per process and,
+ * with TBLIS_HIP_TUNE_FILE=<path>, appended to / read from that file.  No
+ * measurement happens while the stream is being captured. */
+void tblis_hip_set_autotune(int on);
+/* Number of tuning measurements this process has made. */
+int tblis_hip_autotune_runs(void);
+/* Panel GEMM tile of the last contraction on this thread, -1 if it ran on
+ * the direct-gather kernels. */
+int tblis_hip_last_tile(void);
 
 /* Introspection (host only, no device work): the canonical form of a
  * contraction as computed by this library's front end.

@@ -61,6 +61,7 @@ EXPORTED = [
     "tblis_dpd_tensor_mult", "tblis_tensor_mult_batch", "tblis_hip_set_stream",
     "tblis_hip_get_stream", "tblis_hip_set_async", "tblis_hip_set_error_mode",
     "tblis_hip_get_last_error", "tblis_hip_clear_plan_cache", "tblis_hip_last_algorithm",
+    "tblis_hip_set_autotune", "tblis_hip_autotune_runs", "tblis_hip_last_tile",
     "tblis_hip_debug_canonical", "tblis_hip_debug_fill_block_scatter", "tblis_hip_debug_dpd_block",
     "tblis_hip_debug_dpd_patches", "tblis_hip_debug_mfma_probe", "tblis_hip_debug_mfma_probe_bf16",
 ]
@@ -135,6 +136,19 @@ def last_error():
 
 def last_algorithm():
     return lib().tblis_hip_last_algorithm().decode()
+
+
+def set_autotune(flag):
+    """Time the candidate panel tiles when a plan is built (see tblis.h)."""
+    lib().tblis_hip_set_autotune(int(bool(flag)))
+
+
+def autotune_runs():
+    return lib().tblis_hip_autotune_runs()
+
+
+def last_tile():
+    return lib().tblis_hip_last_tile()
 
 
 def _type_code(dtype):

@@ -9,6 +9,7 @@
 #include "contract.hpp"
 #include "dpd.hpp"
 #include "kernels.hpp"
+#include "panel.hpp"
 #include "plan.hpp"
 
 using namespace tblis_amd;
@@ -241,6 +242,9 @@ void tblis_hip_set_async(int a) { thread_state().async = a != 0; }
 void tblis_hip_set_error_mode(int r) { thread_state().return_on_error = r != 0; }
 const char* tblis_hip_get_last_error(void) { return thread_state().last_error.c_str(); }
 void tblis_hip_clear_plan_cache(void) { cache_clear(); }
+void tblis_hip_set_autotune(int on) { set_autotune(on); }
+int tblis_hip_last_tile(void) { return thread_state().last_tile; }
+int tblis_hip_autotune_runs(void) { return autotune_runs(); }
 const char* tblis_hip_last_algorithm(void) { return thread_state().last_algo.c_str(); }
 
 int64_t tblis_hip_debug_canonical(const tblis_tensor* A, const label_type* idx_A,

@@ -40,10 +40,12 @@ __global__ __launch_bounds__(256) void pack_panel_kernel(const PanelJob* __restr
     const PanelChunk ch = chunks[blockIdx.x];
     const PanelJob j = jobs[ch.job];
     const T* __restrict__ X = j.operand ? B : A;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lr = lane & 15, lk = lane >> 4;
     const bool zero = j.src == PANEL_MISSING;
     const int ng = ch.nrb * ch.nkg;
+    const int rbf = j.r0 / 16, kgf = j.k0 / 8;  // first groups of the job's tables
     for (int g = wave; g < ng; g += 4)
     {
         const int rb = ch.rb0 + g / ch.nkg;
@@ -54,11 +56,21 @@ __global__ __launch_bounds__(256) void pack_panel_kernel(const PanelJob* __restr
         const bool in0 = rin && k0 >= 0 && k0 < j.nk;
         const bool in1 = rin && k1 >= 0 && k1 < j.nk;
         double v0 = 0.0, v1 = 0.0;
-        if (!zero && rin)
+        if (!zero)
         {
-            const int64_t roff = j.src + pool[j.rowmap + row];
-            if (in0) v0 = (double)X[roff + pool[j.kmap + k0]];
-            if (in1) v1 = (double)X[roff + pool[j.kmap + k1]];
+            // group tables: wave-uniform (base, stride) of this row / k group
+            const int64_t rbase = pool[j.rowtab + 2 * (rb - rbf)];
+            const int64_t rstr = pool[j.rowtab + 2 * (rb - rbf) + 1];
+            const int64_t kbase = pool[j.ktab + 2 * (kg - kgf)];
+            const int64_t kstr = pool[j.ktab + 2 * (kg - kgf) + 1];
+            if (rin)
+            {
+                const int64_t roff =
+                    j.src + (rstr != SCATTERED ? rbase + lr * rstr : pool[j.rowmap + row]);
+                if (in0) v0 = (double)X[roff + (kstr != SCATTERED ? kbase + lk * kstr : pool[j.kmap + k0])];
+                if (in1)
+                    v1 = (double)X[roff + (kstr != SCATTERED ? kbase + (lk + 4) * kstr : pool[j.kmap + k1])];
+            }
         }
         double* out = ws + j.out + ((int64_t)rb * j.pnkg + kg) * 128 + lane * 2;
         if (in0 && in1)

@@ -48,10 +48,12 @@ __global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PanelJob* __
     const PanelChunk ch = chunks[blockIdx.x];
     const PanelJob j = jobs[ch.job];
     const float* __restrict__ X = j.operand ? B : A;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lr = lane & 31, lh = lane >> 5;
     const bool zero = j.src == PANEL_MISSING;
     const int ng = ch.nrb * ch.nkg;
+    const int rbf = j.r0 / 32, ksf = j.k0 / 16;  // first groups of the job's tables
     for (int g = wave; g < ng; g += 4)
     {
         const int rb = ch.rb0 + g / ch.nkg;
@@ -59,8 +61,16 @@ __global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PanelJob* __
         const int row = rb * 32 + lr - j.r0;
         const bool rin = row >= 0 && row < j.nr;
         const int kb = ks * 16 + lh * 8 - j.k0;
-        int64_t roff = 0;
-        if (rin && !zero) roff = j.src + pool[j.rowmap + row];
+        int64_t roff = 0, kbase = 0, kstr = 0;
+        if (!zero)
+        {
+            // group tables: wave-uniform (base, stride) of this row / k group
+            const int64_t rbase = pool[j.rowtab + 2 * (rb - rbf)];
+            const int64_t rstr = pool[j.rowtab + 2 * (rb - rbf) + 1];
+            kbase = pool[j.ktab + 2 * (ks - ksf)];
+            kstr = pool[j.ktab + 2 * (ks - ksf) + 1];
+            if (rin) roff = j.src + (rstr != SCATTERED ? rbase + lr * rstr : pool[j.rowmap + row]);
+        }
         bf16x8 h, m, l;
         unsigned inmask = 0;
 #pragma unroll
@@ -69,7 +79,8 @@ __global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PanelJob* __
             const int k = kb + e;
             const bool in = rin && k >= 0 && k < j.nk;
             float v = 0.0f;
-            if (in && !zero) v = X[roff + pool[j.kmap + k]];
+            if (in && !zero)
+                v = X[roff + (kstr != SCATTERED ? kbase + (lh * 8 + e) * kstr : pool[j.kmap + k])];
             __bf16 a, b, c;
             split3(v, a, b, c);
             h[e] = a;

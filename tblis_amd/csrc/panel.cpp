@@ -11,8 +11,13 @@
 #include "panel.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <cstdio>
 #include <cstdlib>
 #include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
 
 namespace tblis_amd
 {
@@ -65,6 +70,27 @@ class Pool
             }
         }
         index_[key] = at;
+        return at;
+    }
+    // (base, stride) per fragment group of size G covering job rows
+    // [0, n) placed at panel rows [r0, r0 + n); see PanelJob.
+    int64_t group_table(int64_t map, int64_t r0, int64_t n, int64_t G)
+    {
+        const int64_t g0 = r0 / G, g1 = (r0 + n + G - 1) / G;
+        const int64_t at = (int64_t)data_.size();
+        data_.resize(data_.size() + 2 * (g1 - g0));
+        for (int64_t g = g0; g < g1; g++)
+        {
+            const int64_t lo = std::max<int64_t>(g * G - r0, 0), hi = std::min<int64_t>(g * G - r0 + G, n);
+            const int64_t* off = data_.data() + map;
+            int64_t stride = hi - lo > 1 ? off[lo + 1] - off[lo] : 0;
+            for (int64_t i = lo + 1; i + 1 < hi; i++)
+                if (off[i + 1] - off[i] != stride) stride = SCATTERED;
+            int64_t base = 0;
+            if (stride != SCATTERED) base = off[lo] - (lo - (g * G - r0)) * stride;
+            data_[at + 2 * (g - g0)] = base;
+            data_[at + 2 * (g - g0) + 1] = stride;
+        }
         return at;
     }
     std::vector<int64_t>& data() { return data_; }
@@ -200,9 +226,16 @@ bool panel_enabled(int family, int use_mfma)
     return true;
 }
 
+namespace
+{
+
+bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T);
+int autotune_tile(const std::vector<PanelGemm>& gemms, int family, const std::vector<int>& cands);
+
+}  // namespace
+
 bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family)
 {
-    const Geom G = geom_of(family);
     // drop GEMMs without rows or columns
     gemms.erase(std::remove_if(gemms.begin(), gemms.end(),
                                [](const PanelGemm& g) {
@@ -217,20 +250,23 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family)
     // Orientation: lanes of the MFMA output run along n, so make n the
     // unit-stride side of C when C is contiguous along m.
     const CBlock& c0 = gemms[0].c[0];
-    T.swap = unit_first(c0.rows) && !unit_first(c0.cols);
-    if (T.swap)
+    const bool swap = unit_first(c0.rows) && !unit_first(c0.cols);
+    if (swap)
         for (auto& g : gemms) g = transpose_gemm(g);
 
-    // Tile
+    // Tile: the cost model, replaced by a measurement when the autotuner is
+    // on, and by TBLIS_HIP_PANEL_TILE when set.
     const int cus = device_cu_count();
     int tile = 0;
     double best = -1;
     const TileModel* models = family == FAMILY_F64 ? kModelsF64 : kModelsBf16;
     const int nmodels = family == FAMILY_F64 ? 6 : 4;
+    std::vector<int> cands;
     for (int mi = 0; mi < nmodels; mi++)
     {
         const TileModel& tm = models[mi];
         if (family == FAMILY_BF16X3 && panel_bf16_tile_fast_only(tm.tile)) continue;
+        cands.push_back(tm.tile);
         double c = tile_cost(gemms, tm, cus, family);
         if (best < 0 || c < best)
         {
@@ -238,6 +274,8 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family)
             tile = tm.tile;
         }
     }
+    const int tuned = autotune_tile(gemms, family, cands);
+    if (tuned >= 0) tile = tuned;
     if (const char* e = std::getenv("TBLIS_HIP_PANEL_TILE"))
     {
         int t = std::atoi(e);
@@ -245,6 +283,17 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family)
             !(family == FAMILY_BF16X3 && panel_bf16_tile_fast_only(t)))
             tile = t;
     }
+    if (!tables_for_tile(gemms, family, tile, T)) return false;
+    T.swap = swap;
+    return true;
+}
+
+namespace
+{
+
+bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T)
+{
+    const Geom G = geom_of(family);
     const int64_t BM = panel_tile_m(family, tile), BN = panel_tile_n(family, tile);
 
     Pool pool;
@@ -325,6 +374,8 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family)
                     {
                         j.rowmap = pool.offsets(blk.rows);
                         j.kmap = pool.offsets(blk.k);
+                        j.rowtab = pool.group_table(j.rowmap, j.r0, j.nr, G.gr);
+                        j.ktab = pool.group_table(j.kmap, j.k0, j.nk, G.gk);
                     }
                     add_job(j);
                 }
@@ -420,12 +471,214 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family)
         tasks.swap(t2);
     }
 
-    T.tile = tile;
     T.ws_panel = ws;
     T.ws_c = wsc;
     T.family = family;
+    T.tile = tile;
     T.pool = std::move(pool.data());
     return true;
+}
+
+// ------------------------------------------------------------------ autotuner
+//
+// Times every candidate tile of the panel GEMM on the device for this list
+// of GEMM shapes and keeps the fastest.  The measurement never touches the
+// caller's data: the GEMM runs on zero-filled scratch panels laid out for
+// the candidate and writes C tiles into a scratch buffer (1024 tile slots,
+// reused round-robin).  Pack and unpack costs do not depend on the tile and
+// are not timed.  Results are cached per (family, GEMM shapes) for the
+// process and, with TBLIS_HIP_TUNE_FILE, in a text file that later
+// processes read back.
+
+std::atomic<int> g_autotune{-1};  // -1: not yet read from TBLIS_HIP_AUTOTUNE
+
+bool autotune_on()
+{
+    int v = g_autotune.load();
+    if (v < 0)
+    {
+        const char* e = std::getenv("TBLIS_HIP_AUTOTUNE");
+        v = (e && e[0] == '1') ? 1 : 0;
+        g_autotune.store(v);
+    }
+    return v == 1;
+}
+
+struct TuneDb
+{
+    std::mutex mu;
+    std::unordered_map<uint64_t, int> best;
+    bool loaded = false;
+    int runs = 0;
+};
+
+TuneDb& tune_db()
+{
+    static TuneDb* d = new TuneDb();
+    return *d;
+}
+
+uint64_t fnv1a(const std::string& s)
+{
+    uint64_t h = 1469598103934665603ULL;
+    for (unsigned char c : s)
+    {
+        h ^= c;
+        h *= 1099511628211ULL;
+    }
+    return h;
+}
+
+uint64_t shape_hash(const std::vector<PanelGemm>& gemms, int family)
+{
+    std::string sig = "f" + std::to_string(family);
+    for (auto& g : gemms)
+    {
+        int64_t m = 0, n = 0, k = 0;
+        for (auto x : g.mlen) m += x;
+        for (auto x : g.nlen) n += x;
+        for (auto x : g.klen) k += x;
+        sig += ";" + std::to_string(m) + "," + std::to_string(n) + "," + std::to_string(k) +
+               (g.c.size() > 1 ? "p" : "");
+    }
+    return fnv1a(sig);
+}
+
+void load_tune_file(TuneDb& db)
+{
+    db.loaded = true;
+    const char* path = std::getenv("TBLIS_HIP_TUNE_FILE");
+    if (!path || !path[0]) return;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return;
+    char tag[32];
+    int fam, tile;
+    unsigned long long h;
+    while (std::fscanf(f, "%31s %d %llx %d", tag, &fam, &h, &tile) == 4)
+        if (std::string(tag) == "tblis-tune-v1" && tile >= 0 && tile < panel_num_tiles(fam))
+            db.best[(uint64_t)h ^ (uint64_t)fam] = tile;
+    std::fclose(f);
+}
+
+void save_tune(int family, uint64_t h, int tile)
+{
+    const char* path = std::getenv("TBLIS_HIP_TUNE_FILE");
+    if (!path || !path[0]) return;
+    FILE* f = std::fopen(path, "a");
+    if (!f) return;
+    std::fprintf(f, "tblis-tune-v1 %d %llx %d\n", family, (unsigned long long)h, tile);
+    std::fclose(f);
+}
+
+int autotune_tile(const std::vector<PanelGemm>& gemms, int family, const std::vector<int>& cands)
+{
+    if (!autotune_on() || cands.size() < 2) return -1;
+    hipStream_t s = thread_state().stream;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
+    {
+        (void)hipGetLastError();
+        return -1;  // no allocation or timing inside a graph capture
+    }
+    const uint64_t h = shape_hash(gemms, family);
+    TuneDb& db = tune_db();
+    std::lock_guard<std::mutex> lock(db.mu);
+    if (!db.loaded) load_tune_file(db);
+    auto it = db.best.find(h ^ (uint64_t)family);
+    if (it != db.best.end()) return it->second;
+
+    std::vector<PanelTables> tabs(cands.size());
+    size_t panel_bytes = 256, cslot = 0, ntask = 0;
+    const int eb = geom_of(family).elem_bytes;
+    for (size_t i = 0; i < cands.size(); i++)
+    {
+        if (!tables_for_tile(gemms, family, cands[i], tabs[i])) return -1;
+        panel_bytes = std::max(panel_bytes, (size_t)tabs[i].ws_panel * eb + 256);
+        cslot = std::max(cslot, (size_t)panel_tile_m(family, cands[i]) * panel_tile_n(family, cands[i]));
+        ntask = std::max(ntask, tabs[i].tasks.size());
+    }
+    constexpr size_t SLOTS = 1024;
+    void *panels = nullptr, *cbuf = nullptr, *dtasks = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto release = [&]() {
+        if (panels) (void)hipFree(panels);
+        if (cbuf) (void)hipFree(cbuf);
+        if (dtasks) (void)hipFree(dtasks);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipGetLastError();
+    };
+    if (hipMalloc(&panels, panel_bytes) != hipSuccess || hipMalloc(&cbuf, SLOTS * cslot * 8) != hipSuccess ||
+        hipMalloc(&dtasks, ntask * sizeof(PanelTask)) != hipSuccess ||
+        hipMemsetAsync(panels, 0, panel_bytes, s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess)
+    {
+        release();
+        return -1;  // too large to measure: keep the model's choice
+    }
+    int best_tile = -1;
+    float best_ms = 0;
+    std::vector<PanelTask> tasks;
+    for (size_t i = 0; i < cands.size(); i++)
+    {
+        const int BM = panel_tile_m(family, cands[i]), BN = panel_tile_n(family, cands[i]);
+        tasks = tabs[i].tasks;
+        for (size_t k = 0; k < tasks.size(); k++)
+        {
+            tasks[k].c_ld = BN;
+            tasks[k].c_src = (int64_t)(k % SLOTS) * BM * BN;
+        }
+        if (hipMemcpyAsync(dtasks, tasks.data(), tasks.size() * sizeof(PanelTask), hipMemcpyHostToDevice, s) !=
+                hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            break;
+        PanelLaunch L{};
+        L.tasks = (const PanelTask*)dtasks;
+        L.ntasks = (int)tasks.size();
+        L.ws = panels;
+        L.cws = (double*)cbuf;
+        L.alpha = 1.0;
+        L.beta = 0.0;
+        L.use_k = 1;
+        L.dtype = family == FAMILY_F64 ? 0 : 1;
+        L.family = family;
+        L.tile = cands[i];
+        L.stream = s;
+        float ms_min = -1;
+        bool ok = launch_panel_gemm(L) == hipSuccess;
+        for (int rep = 0; ok && rep < 2; rep++)
+        {
+            float ms = 0;
+            ok = hipEventRecord(e0, s) == hipSuccess && launch_panel_gemm(L) == hipSuccess &&
+                 hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+                 hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+            if (ok && (ms_min < 0 || ms < ms_min)) ms_min = ms;
+        }
+        if (!ok) break;
+        if (best_tile < 0 || ms_min < best_ms)
+        {
+            best_ms = ms_min;
+            best_tile = cands[i];
+        }
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) best_tile = -1;
+    release();
+    if (best_tile < 0) return -1;
+    db.best[h ^ (uint64_t)family] = best_tile;
+    db.runs++;
+    save_tune(family, h, best_tile);
+    return best_tile;
+}
+
+}  // namespace
+
+void set_autotune(int on) { g_autotune.store(on ? 1 : 0); }
+
+int autotune_runs()
+{
+    TuneDb& db = tune_db();
+    std::lock_guard<std::mutex> lock(db.mu);
+    return db.runs;
 }
 
 bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, int dtype, int family)

@@ -66,6 +66,11 @@ struct PanelTables
 // Returns false when there is nothing to launch.
 bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family);
 
+// Autotuner switch (initially TBLIS_HIP_AUTOTUNE) and the number of tuning
+// measurements made by this process.
+void set_autotune(int on);
+int autotune_runs();
+
 // plan_panel_tables + upload into the plan.
 bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, int dtype, int family);
 

@@ -583,6 +583,7 @@ void run_plan(const std::shared_ptr<Plan>& pp, const void* A, const void* B, voi
     hip_check(hipStreamIsCapturing(ts.stream, &cap), "hipStreamIsCapturing");
     const bool capturing = cap != hipStreamCaptureStatusNone;
     if (capturing) pin_plan(pp);
+    ts.last_tile = p.panel ? p.panel_tile : -1;
     GemmLaunch L;
     L.tasks = p.d_tasks;
     L.ntasks = p.ntasks;

@@ -126,6 +126,7 @@ struct ThreadState
     bool return_on_error = false;
     std::string last_error;
     std::string last_algo;
+    int last_tile = -1;  // panel tile of the last launched plan (-1: other kernels)
 };
 ThreadState& thread_state();
 

@@ -82,11 +82,18 @@ constexpr int64_t PANEL_MISSING = INT64_MIN;
 // [r0, r0 + nr) x k [k0, k0 + nk), element (i, kk) read from
 //     X[src + pool[rowmap + i] + pool[kmap + kk]],
 // or zeros when src == PANEL_MISSING (padding, absent blocks).
+// Group tables (rowtab, ktab) hold one (base, stride) pair per fragment
+// group the job touches, counted from group r0 / rows-per-group (k0 /
+// k-per-group): element i of the group is at base + i * stride, i counted
+// from the group's first panel row (k), or stride == SCATTERED and the
+// per-element maps apply -- the reference's block strides (rbs / cbs,
+// src/matrix/block_scatter_matrix.hpp:232-247) at fragment-group size.
 struct PanelJob
 {
     int64_t src;
     int64_t rowmap, kmap;  // pool indices
-    int64_t out;           // panel offset (doubles) in the workspace
+    int64_t rowtab, ktab;  // pool indices of the group tables
+    int64_t out;           // panel offset (values) in the workspace
     int32_t r0, nr, k0, nk;
     int32_t pnkg;          // k groups of the whole panel
     int32_t operand;       // 0: A pointer, 1: B pointer (after orientation)

@@ -1,0 +1,42 @@
+"""The panel-tile autotuner: with tuning on, a new GEMM shape is measured
+once (the tuning counter moves, the tuning file gets a line), later plans of
+the same shapes reuse the result, and the contraction it then runs matches
+the oracle (FP64: 1e-12, split-BF16: 1e-6 relative Frobenius)."""
+import numpy as np
+import pytest
+
+import tblis_amd as T
+from tests.test_gpu_bf16x3 import _float_case, uniform
+from tests.test_gpu_parity import _dense_case, _dpd_case
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autotune_picks_and_runs(tmp_path, monkeypatch):
+    path = tmp_path / "tune.txt"
+    monkeypatch.setenv("TBLIS_HIP_TUNE_FILE", str(path))
+    T.set_autotune(True)
+    T.lib().tblis_hip_clear_plan_cache()
+    try:
+        rng = np.random.default_rng(33000)
+        r0 = T.autotune_runs()
+        ext = dict(a=40, b=33, e=21, f=17, i=29, j=31)
+        _dense_case(rng, "aebf", "fjei", "aibj", ext, ["col"] * 3)
+        r1 = T.autotune_runs()
+        assert r1 == r0 + 1
+        assert T.last_tile() >= 0
+        # same shapes, new plan (cache cleared): no new measurement
+        T.lib().tblis_hip_clear_plan_cache()
+        _dense_case(rng, "aebf", "fjei", "aibj", ext, ["col"] * 3)
+        assert T.autotune_runs() == r1
+        dext = {c: [4, 3, 2, 1, 5, 3, 2, 1] for c in "abefij"}
+        _dpd_case(rng, 8, "abef", "efij", "abij", dext, 0, 0, 5, alpha=0.5, beta=1.0)
+        _float_case(rng, "aebf", "fjei", "aibj", dict(a=30, b=17, e=12, f=10, i=23, j=19), ["col"] * 3,
+                    uniform)
+        assert T.autotune_runs() >= r1 + 2
+        lines = path.read_text().splitlines()
+        assert len(lines) == T.autotune_runs() - r0
+        assert all(l.startswith("tblis-tune-v1 ") for l in lines)
+    finally:
+        T.set_autotune(False)
+        T.lib().tblis_hip_clear_plan_cache()

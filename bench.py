@@ -19,10 +19,13 @@ makes it the headline):
                Frobenius error of each against a float64 torch product
   dpd          8-irrep DPD C[abij] = A[abef]*B[efij], per-irrep extents
                [12,9,7,5,11,8,6,4]: FP64 and split-BF16, and the same kernels
-               on the dense 62^4 contraction (dense-equivalent throughput)
+               on the dense 62^4 contraction (dense-equivalent throughput),
+               each call replayed from a hipGraph
   batch        4096 independent C[abij] = A[abef]*B[efij], extents drawn from
                4..40 per instance, FP64 and split-BF16, one hipGraph replay
                per step
+Panel tiles are picked by the library's autotuner (on-device timing when a
+plan is first built, during warm-up) unless --no-autotune.
 """
 import argparse
 import json
@@ -105,7 +108,8 @@ def permuted_leg(T, dev, stream, steps, warmup):
         "value": tflops(flops, ms_step), "ms_per_step": round(ms_step, 4), "dtype": "f64",
         "config": {"workload": "C[aibj] += 0.5*A[aebf]*B[fjei], a=b=e=f=64, i=j=32, column-major, "
                                "FP64 MFMA (tblis_tensor_mult)",
-                   "m": 4096, "n": 1024, "k": 4096, "algorithm": T.last_algorithm()},
+                   "m": 4096, "n": 1024, "k": 4096, "algorithm": T.last_algorithm(),
+                   "tile": T.last_tile()},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64, 4), "traffic": None,
                      "kernel_ms_mean": round(ms_kern, 4), "kernel_ms_min": round(ms_min, 4),
@@ -137,7 +141,7 @@ def fp32_leg(T, dev, stream, steps, warmup):
     for name, cfg, Cx in (("split_bf16", None, Cs), ("fp64_mfma", "fp64_mfma", Cd)):
         fn = T.prepare_mult(1.0, a, "aebf", b, "ecfd", 0.0, strided(Cx, L), "abcd", config=cfg)
         ms_step, ms_kern, _ = time_steps(fn, steps, warmup, stream)
-        out[name] = {"algorithm": T.last_algorithm(), "ms": round(ms_step, 4),
+        out[name] = {"algorithm": T.last_algorithm(), "tile": T.last_tile(), "ms": round(ms_step, 4),
                      "useful_tflops": tflops(flops, ms_step), "call_ms_events": round(ms_kern, 4)}
     # float64 reference: A(ab,ef) @ B(ef,cd) with A[aebf] -> (a,b,e,f), B[ecfd] -> (e,f,c,d)
     A2 = a.double().permute(0, 2, 1, 3).reshape(9216, 9216)
@@ -178,9 +182,10 @@ def dpd_leg(T, dev, stream, steps, warmup):
     for name, a, b, c in legs:
         fn = T.prepare_dpd_mult(1.0, a, 0, "abef", b, 0, "efij", 0.0, c, 0, "abij", nirrep,
                                 lens, lens, lens)
-        ms_step, ms_kern, _ = time_steps(fn, steps, warmup, stream)
-        out[name] = {"algorithm": T.last_algorithm(), "ms": round(ms_step, 4),
-                     "useful_tflops": tflops(flops, ms_step), "call_ms_events": round(ms_kern, 4)}
+        graph = capture(T, fn, stream)
+        ms_step, ms_kern, _ = time_steps(graph.replay, steps, warmup, torch.cuda.current_stream())
+        out[name] = {"algorithm": T.last_algorithm(), "tile": T.last_tile(), "ms": round(ms_step, 4),
+                     "useful_tflops": tflops(flops, ms_step), "launch": "one hipGraph replay"}
     out["split_bf16"]["rel_err_vs_fp64"] = float("%.3e" % rel_fro(Cf, Cd))
     # dense-equivalent: the same kernels on C[abij] = A[abef]*B[efij], extents 62
     L = [62] * 4
@@ -190,7 +195,8 @@ def dpd_leg(T, dev, stream, steps, warmup):
         X = [torch.rand(62 ** 4, dtype=dt, device=dev, generator=gd) * 2 - 1 for _ in range(3)]
         fn = T.prepare_mult(1.0, strided(X[0], L), "abef", strided(X[1], L), "efij", 0.0,
                             strided(X[2], L), "abij")
-        ms_step, _, _ = time_steps(fn, steps, warmup, stream)
+        graph = capture(T, fn, stream)
+        ms_step, _, _ = time_steps(graph.replay, steps, warmup, torch.cuda.current_stream())
         dense = tflops(dflops, ms_step)
         out[name]["dense62_useful_tflops"] = dense
         out[name]["frac_of_dense"] = round(out[name]["useful_tflops"] / dense, 3)
@@ -241,7 +247,7 @@ def batch_leg(T, dev, stream, steps, warmup, count=4096, seed=7):
                              ["abij"] * count)
         graph = capture(T, fn, stream)
         ms_step, ms_kern, _ = time_steps(graph.replay, steps, warmup, torch.cuda.current_stream())
-        out[name] = {"algorithm": T.last_algorithm(), "ms": round(ms_step, 4),
+        out[name] = {"algorithm": T.last_algorithm(), "tile": T.last_tile(), "ms": round(ms_step, 4),
                      "useful_tflops": tflops(flops, ms_step), "launch": "one hipGraph replay"}
     got = torch.cat([c.reshape(-1) for c in insts["split_bf16"][2]])
     want = torch.cat([c.reshape(-1) for c in insts["fp64"][2]])
@@ -278,6 +284,8 @@ def main():
     ap.add_argument("--only", choices=["permuted", "fp32_96", "dpd", "batch"],
                     help="run one leg alone (for profiling) and report it as the headline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the cost model's tile choice (no on-device tile timing)")
     ap.add_argument("--cpu-cols", type=int, default=256, help="n columns in the CPU sample")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (reference point for "
@@ -296,6 +304,7 @@ def main():
     stream = torch.cuda.current_stream()
     T.set_stream(stream.cuda_stream)
     T.set_async(True)
+    T.set_autotune(not args.no_autotune)
     few = max(3, args.steps // 10)
 
     head, lens = None, None

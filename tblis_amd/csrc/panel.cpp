@@ -487,7 +487,7 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
 // the candidate and writes C tiles into a scratch buffer (1024 tile slots,
 // reused round-robin).  Pack and unpack costs do not depend on the tile and
 // are not timed.  Results are cached per (family, GEMM shapes) for the
-// process and, with TBLIS_HIP_TUNE_FILE, in a text file that later
+// process and, with TBLIS_HIP_TUNE_FILE, in a JSON-lines file that later
 // processes read back.
 
 std::atomic<int> g_autotune{-1};  // -1: not yet read from TBLIS_HIP_AUTOTUNE
@@ -551,12 +551,17 @@ void load_tune_file(TuneDb& db)
     if (!path || !path[0]) return;
     FILE* f = std::fopen(path, "r");
     if (!f) return;
-    char tag[32];
-    int fam, tile;
-    unsigned long long h;
-    while (std::fscanf(f, "%31s %d %llx %d", tag, &fam, &h, &tile) == 4)
-        if (std::string(tag) == "tblis-tune-v1" && tile >= 0 && tile < panel_num_tiles(fam))
+    // JSON lines written by save_tune
+    char line[256];
+    while (std::fgets(line, sizeof line, f))
+    {
+        int fam = -1, tile = -1;
+        unsigned long long h = 0;
+        if (std::sscanf(line, "{\"format\": \"tblis-tune-v1\", \"family\": %d, \"shapes\": \"%llx\", \"tile\": %d",
+                        &fam, &h, &tile) == 3 &&
+            fam >= 0 && fam <= FAMILY_BF16X3_FAST && tile >= 0 && tile < panel_num_tiles(fam))
             db.best[(uint64_t)h ^ (uint64_t)fam] = tile;
+    }
     std::fclose(f);
 }
 
@@ -566,7 +571,8 @@ void save_tune(int family, uint64_t h, int tile)
     if (!path || !path[0]) return;
     FILE* f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "tblis-tune-v1 %d %llx %d\n", family, (unsigned long long)h, tile);
+    std::fprintf(f, "{\"format\": \"tblis-tune-v1\", \"family\": %d, \"shapes\": \"%016llx\", \"tile\": %d}\n",
+                 family, (unsigned long long)h, tile);
     std::fclose(f);
 }
 

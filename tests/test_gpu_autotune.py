@@ -2,6 +2,8 @@
 once (the tuning counter moves, the tuning file gets a line), later plans of
 the same shapes reuse the result, and the contraction it then runs matches
 the oracle (FP64: 1e-12, split-BF16: 1e-6 relative Frobenius)."""
+import json
+
 import numpy as np
 import pytest
 
@@ -34,9 +36,9 @@ def test_autotune_picks_and_runs(tmp_path, monkeypatch):
         _float_case(rng, "aebf", "fjei", "aibj", dict(a=30, b=17, e=12, f=10, i=23, j=19), ["col"] * 3,
                     uniform)
         assert T.autotune_runs() >= r1 + 2
-        lines = path.read_text().splitlines()
+        lines = [json.loads(l) for l in path.read_text().splitlines()]
         assert len(lines) == T.autotune_runs() - r0
-        assert all(l.startswith("tblis-tune-v1 ") for l in lines)
+        assert all(l["format"] == "tblis-tune-v1" and l["tile"] >= 0 for l in lines)
     finally:
         T.set_autotune(False)
         T.lib().tblis_hip_clear_plan_cache()

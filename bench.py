@@ -255,11 +255,69 @@ def batch_leg(T, dev, stream, steps, warmup, count=4096, seed=7):
     return out
 
 
+# ---------------------------------------------------------------- traffic
+
+COPY_MIB = 256
+
+
+def measure_traffic(timeout=150):
+    """HBM bytes per launch of the permuted workload's kernels from two
+    rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE) over a short run
+    of this script, each scaled by what the same counter reads for a plain
+    256 MiB device copy in the same run.  Runs before this process touches
+    the GPU; returns None when rocprofv3 is unavailable or fails."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    if not shutil.which("rocprofv3"):
+        return None
+    per_kernel = {}
+    scale = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="tblis_pmc_")
+        cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "p", "--",
+               sys.executable, os.path.abspath(__file__), "--only", "permuted", "--steps", "2",
+               "--warmup", "1", "--no-cpu", "--no-traffic", "--calibrate-copy", str(COPY_MIB)]
+        try:
+            subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=timeout,
+                           check=False)
+            files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+            if not files:
+                return None
+            sums = {}
+            for r in csv.DictReader(open(files[0])):
+                if r.get("Counter_Name") != counter:
+                    continue
+                name = r["Kernel_Name"]
+                key = ("copy" if "copyBuffer" in name else "gemm" if "gemm" in name
+                       else "pack" if "pack" in name else None)
+                if key:
+                    sums.setdefault(key, []).append(float(r["Counter_Value"]))
+        except (subprocess.TimeoutExpired, OSError, KeyError, ValueError):
+            return None
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        if "copy" not in sums or "gemm" not in sums:
+            return None
+        copy_kib = max(sums["copy"])  # the calibration copy is the largest copy
+        scale[counter] = COPY_MIB * 1024.0 / copy_kib
+        for key in ("gemm", "pack"):
+            if key in sums:
+                v = sums[key][-1] * 1024.0 * scale[counter]  # last dispatch, bytes
+                per_kernel.setdefault(key, {})[counter] = v
+    out = {k: round(v.get("FETCH_SIZE", 0) + v.get("WRITE_SIZE", 0)) for k, v in per_kernel.items()}
+    out["scale"] = {k: round(v, 3) for k, v in scale.items()}
+    return out
+
+
 # ---------------------------------------------------------------- CPU
 
-def cpu_baseline(lA, lB, lC, sample_cols, threads):
+def cpu_baseline(lA, lB, lC, sample_cols, threads, seconds):
     """Oracle port (multithreaded blocked GEMM over scatter vectors) on the
-    first sample_cols columns (n) of the permuted contraction."""
+    first sample_cols columns (n) of the permuted contraction, repeated for
+    about `seconds` of wall time; returns GFLOP/s, flop per run, runs, time."""
     from oracle import pyoracle as po
     rng = np.random.default_rng(0)
     A = rng.uniform(-1, 1, int(np.prod(lA)))
@@ -267,13 +325,14 @@ def cpu_baseline(lA, lB, lC, sample_cols, threads):
     C = rng.uniform(-1, 1, int(np.prod(lC)))
     args = (0.5, A, lA, colmaj(lA), "aebf", B, lB, colmaj(lB), "fjei", 1.0, C, lC, colmaj(lC), "aibj")
     po.blis_contract(threads, *args, n_limit=8)  # warm-up
-    best, flops = None, 0.0
-    for _ in range(2):
-        t0 = time.perf_counter()
+    runs, flops, t0 = 0, 0.0, time.perf_counter()
+    while True:
         flops = po.blis_contract(threads, *args, n_limit=sample_cols)
+        runs += 1
         dt = time.perf_counter() - t0
-        best = dt if best is None else min(best, dt)
-    return flops / best / 1e9, flops, best
+        if dt >= seconds:
+            break
+    return flops * runs / dt / 1e9, flops, runs, dt
 
 
 def main():
@@ -284,9 +343,12 @@ def main():
     ap.add_argument("--only", choices=["permuted", "fp32_96", "dpd", "batch"],
                     help="run one leg alone (for profiling) and report it as the headline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 counter passes behind roofline.traffic")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the cost model's tile choice (no on-device tile timing)")
-    ap.add_argument("--cpu-cols", type=int, default=256, help="n columns in the CPU sample")
+    ap.add_argument("--cpu-cols", type=int, default=1024, help="n columns in the CPU sample (of 1024)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline duration")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (reference point for "
                          "FETCH_SIZE/WRITE_SIZE counter runs)")
@@ -296,6 +358,12 @@ def main():
               "would shard over C's columns, DPD irreps or batch instances); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+
+    # counter passes first: they run this script as rocprofv3 children, before
+    # this process initialises the GPU
+    traffic = None
+    if args.only in (None, "permuted") and not args.no_traffic:
+        traffic = measure_traffic()
 
     import torch
     import tblis_amd as T
@@ -353,15 +421,22 @@ def main():
     }
     if "roofline" in head:
         out["roofline"] = head["roofline"]
+        if traffic and "gemm" in traffic:
+            out["roofline"]["traffic"] = traffic["gemm"]
+            out["roofline"]["traffic_note"] = (
+                "HBM bytes per GEMM launch: rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a %d MiB "
+                "copy (factors %s); pack kernel %s bytes per launch" %
+                (COPY_MIB, traffic["scale"], traffic.get("pack")))
     out.update(legs)
     if not args.no_cpu and lens is not None:
         threads = min(os.cpu_count() or 1, 16)
-        gflops, fl, dt = cpu_baseline(*lens, args.cpu_cols, threads)
+        gflops, fl, runs, dt = cpu_baseline(*lens, args.cpu_cols, threads, args.cpu_seconds)
         out["cpu_baseline"] = {"value": round(gflops / 1e3, 5), "unit": "TFLOP/s", "cores": threads,
                                "kind": "port",
-                               "sample": "first %d of 1024 (i,j) columns of the same contraction "
-                                         "(%.3g flop, %.2f s), oracle blocked GEMM over scatter "
-                                         "vectors, best of 2" % (args.cpu_cols, fl, dt)}
+                               "sample": "%d of 1024 (i,j) columns of the same contraction (%.3g flop "
+                                         "per run), %d runs in %.1f s, oracle blocked GEMM over "
+                                         "scatter vectors on %d host threads" %
+                                         (args.cpu_cols, fl, runs, dt, threads)}
     print(json.dumps(out))
 
 

@@ -15,6 +15,8 @@
 //           (beta == 0 does not read C, src/kernels/3m/gemm.hpp:54-58), or
 //           stores a dense C panel when the GEMM spans several C blocks;
 //   unpack  (DPD) scatters C panels into the C blocks with alpha/beta.
+#include <algorithm>
+
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -227,6 +229,27 @@ __global__ __launch_bounds__(256) void unpack_panel_kernel(const UnpackJob* __re
     }
 }
 
+// ------------------------------------------------------------------ fill
+
+// Pseudo-random values in [-1, 1) (a 64-bit mix of the index) so that
+// timing runs see data with the switching activity of real operands.
+__global__ __launch_bounds__(256) void fill_random_kernel(void* p, int64_t n, int bf16)
+{
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+    {
+        uint64_t x = (uint64_t)i * 0x9E3779B97F4A7C15ULL;
+        x ^= x >> 31;
+        x *= 0xBF58476D1CE4E5B9ULL;
+        x ^= x >> 29;
+        const double v = (double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+        if (bf16)
+            ((__bf16*)p)[i] = (__bf16)(float)v;
+        else
+            ((double*)p)[i] = v;
+    }
+}
+
 // ------------------------------------------------------------------ table
 
 struct PanelTile
@@ -307,6 +330,14 @@ hipError_t launch_panel_gemm(const PanelLaunch& L)
     if (L.family != FAMILY_F64) return launch_panel_gemm_bf16(L);
     if (L.tile < 0 || L.tile >= kNumTiles) return hipErrorInvalidValue;
     return L.dtype == 0 ? launch_gemm_t<double>(L) : launch_gemm_t<float>(L);
+}
+
+hipError_t launch_fill_random(void* p, int64_t n, bool bf16, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_random_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n, bf16 ? 1 : 0);
+    return hipGetLastError();
 }
 
 hipError_t launch_panel_unpack(const PanelLaunch& L)

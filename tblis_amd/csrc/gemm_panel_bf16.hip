@@ -259,12 +259,12 @@ struct Bf16Tile
 
 // id, BM, BN, waves (WGM x WGN), fast only.  Registers per wave (gfx950):
 // 128x64 and 64x128 wave tiles 228 (single accumulator) / 410 (chunked),
-// 64x64 174 / 230, 128x128 512 (single accumulator only).
+// 64x64 174 / 230.  (A 128x128 wave tile needs all 512 registers for the
+// single accumulator and spills; it is not offered.)
 #define TBLIS_PANEL_BF16_TILES(X) \
     X(0, 256, 128, 2, 2, false)   \
     X(1, 128, 256, 2, 2, false)   \
-    X(2, 128, 128, 2, 2, false)   \
-    X(3, 256, 256, 2, 2, true)
+    X(2, 128, 128, 2, 2, false)
 
 constexpr Bf16Tile kBTiles[] = {
 #define X(id, bm, bn, wgm, wgn, fo) {bm, bn, wgm, wgn, fo},

@@ -76,6 +76,8 @@ struct PanelLaunch
 hipError_t launch_panel_pack(const PanelLaunch& L);
 hipError_t launch_panel_gemm(const PanelLaunch& L);
 hipError_t launch_panel_unpack(const PanelLaunch& L);
+// n pseudo-random values in [-1, 1) (double, or BF16 when bf16) at p
+hipError_t launch_fill_random(void* p, int64_t n, bool bf16, hipStream_t s);
 // split-BF16 panels (gemm_panel_bf16.hip)
 int panel_bf16_num_tiles();
 int panel_bf16_tile_m(int tile);

@@ -1,4 +1,4 @@
-// Planner of the FP64 panel path.
+// Planner of the panel path (FP64 and split-BF16 families).
 //
 // A plan holds, in one device allocation:
 //   pack jobs + chunks   every operand block -> its place in the A' / B' panels
@@ -128,7 +128,7 @@ struct TileModel
 // blocks, 32x32 waves 2x2.  Split-BF16: per tile of the BF16 table.
 const TileModel kModelsF64[] = {{0, 0.85, 2}, {2, 0.80, 2}, {3, 0.82, 2}, {4, 0.82, 2},
                                 {1, 0.50, 6}, {5, 0.45, 6}};
-const TileModel kModelsBf16[] = {{0, 0.55, 1}, {1, 0.55, 1}, {2, 0.35, 2}, {3, 0.60, 1}};
+const TileModel kModelsBf16[] = {{0, 0.55, 1}, {1, 0.55, 1}, {2, 0.45, 2}};
 
 // Estimated run time of the GEMM tasks for a tile.
 double tile_cost(const std::vector<PanelGemm>& gemms, const TileModel& tm, int cus, int family)
@@ -260,7 +260,7 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family)
     int tile = 0;
     double best = -1;
     const TileModel* models = family == FAMILY_F64 ? kModelsF64 : kModelsBf16;
-    const int nmodels = family == FAMILY_F64 ? 6 : 4;
+    const int nmodels = family == FAMILY_F64 ? 6 : 3;
     std::vector<int> cands;
     for (int mi = 0; mi < nmodels; mi++)
     {
@@ -483,9 +483,10 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
 //
 // Times every candidate tile of the panel GEMM on the device for this list
 // of GEMM shapes and keeps the fastest.  The measurement never touches the
-// caller's data: the GEMM runs on zero-filled scratch panels laid out for
-// the candidate and writes C tiles into a scratch buffer (1024 tile slots,
-// reused round-robin).  Pack and unpack costs do not depend on the tile and
+// caller's data: the GEMM runs on scratch panels of pseudo-random values
+// laid out for the candidate (all-zero operands would time the MFMA units at
+// an unrealistically low power draw) and writes C tiles into a scratch
+// buffer (1024 tile slots, reused round-robin).  Pack and unpack costs do not depend on the tile and
 // are not timed.  Results are cached per (family, GEMM shapes) for the
 // process and, with TBLIS_HIP_TUNE_FILE, in a JSON-lines file that later
 // processes read back.
@@ -604,28 +605,24 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, const std::ve
         ntask = std::max(ntask, tabs[i].tasks.size());
     }
     constexpr size_t SLOTS = 1024;
-    void *panels = nullptr, *cbuf = nullptr, *dtasks = nullptr;
+    const size_t nc = cands.size();
+    void *panels = nullptr, *cbuf = nullptr;
+    std::vector<void*> dtasks(nc, nullptr);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     auto release = [&]() {
         if (panels) (void)hipFree(panels);
         if (cbuf) (void)hipFree(cbuf);
-        if (dtasks) (void)hipFree(dtasks);
+        for (void* p : dtasks)
+            if (p) (void)hipFree(p);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         (void)hipGetLastError();
     };
-    if (hipMalloc(&panels, panel_bytes) != hipSuccess || hipMalloc(&cbuf, SLOTS * cslot * 8) != hipSuccess ||
-        hipMalloc(&dtasks, ntask * sizeof(PanelTask)) != hipSuccess ||
-        hipMemsetAsync(panels, 0, panel_bytes, s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
-        hipEventCreate(&e1) != hipSuccess)
-    {
-        release();
-        return -1;  // too large to measure: keep the model's choice
-    }
-    int best_tile = -1;
-    float best_ms = 0;
+    bool ok = hipMalloc(&panels, panel_bytes) == hipSuccess && hipMalloc(&cbuf, SLOTS * cslot * 8) == hipSuccess &&
+              launch_fill_random(panels, (int64_t)(panel_bytes / eb), family != FAMILY_F64, s) == hipSuccess &&
+              hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
     std::vector<PanelTask> tasks;
-    for (size_t i = 0; i < cands.size(); i++)
+    for (size_t i = 0; ok && i < nc; i++)
     {
         const int BM = panel_tile_m(family, cands[i]), BN = panel_tile_n(family, cands[i]);
         tasks = tabs[i].tasks;
@@ -634,13 +631,20 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, const std::ve
             tasks[k].c_ld = BN;
             tasks[k].c_src = (int64_t)(k % SLOTS) * BM * BN;
         }
-        if (hipMemcpyAsync(dtasks, tasks.data(), tasks.size() * sizeof(PanelTask), hipMemcpyHostToDevice, s) !=
-                hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess)
-            break;
+        ok = hipMalloc(&dtasks[i], tasks.size() * sizeof(PanelTask)) == hipSuccess &&
+             hipMemcpyAsync(dtasks[i], tasks.data(), tasks.size() * sizeof(PanelTask), hipMemcpyHostToDevice,
+                            s) == hipSuccess &&
+             hipStreamSynchronize(s) == hipSuccess;
+    }
+    if (!ok)
+    {
+        release();
+        return -1;  // too large to measure: keep the model's choice
+    }
+    auto launch = [&](size_t i) {
         PanelLaunch L{};
-        L.tasks = (const PanelTask*)dtasks;
-        L.ntasks = (int)tasks.size();
+        L.tasks = (const PanelTask*)dtasks[i];
+        L.ntasks = (int)tabs[i].tasks.size();
         L.ws = panels;
         L.cws = (double*)cbuf;
         L.alpha = 1.0;
@@ -650,22 +654,53 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, const std::ve
         L.family = family;
         L.tile = cands[i];
         L.stream = s;
-        float ms_min = -1;
-        bool ok = launch_panel_gemm(L) == hipSuccess;
-        for (int rep = 0; ok && rep < 2; rep++)
+        return launch_panel_gemm(L) == hipSuccess;
+    };
+    // fastest of `reps` timed launches, or -1 on an error
+    auto time_of = [&](size_t i, int reps) {
+        float best = -1;
+        for (int r = 0; r < reps; r++)
         {
             float ms = 0;
-            ok = hipEventRecord(e0, s) == hipSuccess && launch_panel_gemm(L) == hipSuccess &&
-                 hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
-                 hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
-            if (ok && (ms_min < 0 || ms < ms_min)) ms_min = ms;
+            if (!(hipEventRecord(e0, s) == hipSuccess && launch(i) && hipEventRecord(e1, s) == hipSuccess &&
+                  hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess))
+                return -1.0f;
+            if (best < 0 || ms < best) best = ms;
         }
-        if (!ok) break;
-        if (best_tile < 0 || ms_min < best_ms)
+        return best;
+    };
+    // Warm the clocks up (about 10 ms of work), then time every candidate,
+    // then time the two best again so that drift cannot decide alone.
+    float t0 = time_of(0, 1);
+    for (float spent = t0; ok && t0 >= 0 && spent < 10.0f; spent += t0) ok = launch(0);
+    std::vector<float> t(nc, -1);
+    for (size_t i = 0; ok && i < nc; i++)
+    {
+        t[i] = time_of(i, 3);
+        ok = t[i] >= 0;
+    }
+    int best_tile = -1;
+    if (ok)
+    {
+        std::vector<size_t> order(nc);
+        for (size_t i = 0; i < nc; i++) order[i] = i;
+        std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return t[a] < t[b]; });
+        for (size_t r = 0; r < std::min<size_t>(2, nc); r++)
         {
-            best_ms = ms_min;
-            best_tile = cands[i];
+            const float again = time_of(order[r], 3);
+            if (again < 0) ok = false;
+            t[order[r]] = std::min(t[order[r]], again);
         }
+        size_t bi = order[0];
+        if (nc > 1 && t[order[1]] < t[bi]) bi = order[1];
+        best_tile = ok ? cands[bi] : -1;
+        if (const char* v = std::getenv("TBLIS_HIP_TUNE_VERBOSE"))
+            if (v[0] == '1')
+            {
+                std::fprintf(stderr, "tblis autotune family %d:", family);
+                for (size_t i = 0; i < nc; i++) std::fprintf(stderr, " tile %d %.4f ms", cands[i], t[i]);
+                std::fprintf(stderr, " -> tile %d\n", best_tile);
+            }
     }
     if (hipStreamSynchronize(s) != hipSuccess) best_tile = -1;
     release();

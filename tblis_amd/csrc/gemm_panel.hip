@@ -118,11 +118,69 @@ __device__ __forceinline__ void mma_group(v4d (&acc)[MB][NB], const double2 (&a)
             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].y, acc[i][j], 0, 0, 0);
 }
 
-template <int BM, int BN, int WGM, int WGN, typename TC>
+// One operand read straight from its tensor (PanelTask "direct"): the
+// lane offsets of its row blocks, and the k-group (base, stride) table.
+template <typename TC, int NBLK>
+struct DirectSide
+{
+    __amdgpu_buffer_rsrc_t rsrc;
+    const int64_t* kt;
+    uint32_t roff[NBLK];
+    bool rvalid[NBLK];
+};
+
+template <typename TC>
+__device__ __forceinline__ double direct_load(__amdgpu_buffer_rsrc_t r, uint32_t byte_off)
+{
+    if constexpr (sizeof(TC) == 8)
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
+    else
+        return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
+}
+
+template <typename TC, int NBLK>
+__device__ __forceinline__ void direct_init(DirectSide<TC, NBLK>& d, const void* X, int64_t src,
+                                            uint32_t bytes, const int64_t* __restrict__ pool,
+                                            int64_t rt, int64_t kt, int blk0, int rows0, int len,
+                                            int lr)
+{
+    d.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)((const TC*)X + src), (short)0, (int)bytes,
+                                               0x00020000);
+    d.kt = pool + kt;
+#pragma unroll
+    for (int i = 0; i < NBLK; i++)
+    {
+        const int64_t base = pool[rt + 2 * (blk0 + i)], stride = pool[rt + 2 * (blk0 + i) + 1];
+        d.roff[i] = (uint32_t)(base + lr * stride);
+        d.rvalid[i] = rows0 + i * 16 + lr < len;
+    }
+}
+
+// Two k steps of NBLK row blocks of a direct operand as the double2 a
+// panel would hold: rows lr, k = 8 kg + lk and 8 kg + 4 + lk.
+template <typename TC, int NBLK>
+__device__ __forceinline__ void direct_groups(double2 (&x)[NBLK], const DirectSide<TC, NBLK>& d,
+                                              int kg, int lk, int klen)
+{
+    const int64_t kb = d.kt[2 * kg], ks = d.kt[2 * kg + 1];
+    const int k0 = kg * 8 + lk;
+    const uint32_t o0 = (uint32_t)(kb + lk * ks), o1 = (uint32_t)(kb + (lk + 4) * ks);
+#pragma unroll
+    for (int i = 0; i < NBLK; i++)
+    {
+        const uint32_t b0 = d.rvalid[i] && k0 < klen ? (d.roff[i] + o0) * (uint32_t)sizeof(TC) : 0xffffffffu;
+        const uint32_t b1 =
+            d.rvalid[i] && k0 + 4 < klen ? (d.roff[i] + o1) * (uint32_t)sizeof(TC) : 0xffffffffu;
+        x[i].x = direct_load<TC>(d.rsrc, b0);
+        x[i].y = direct_load<TC>(d.rsrc, b1);
+    }
+}
+
+template <int BM, int BN, int WGM, int WGN, typename TC, bool DA, bool DB>
 __global__ __launch_bounds__(64 * WGM * WGN) void gemm_panel_kernel(
     const PanelTask* __restrict__ tasks, const int64_t* __restrict__ pool,
-    const double* __restrict__ ws, double* __restrict__ cws, TC* __restrict__ C, double alpha,
-    double beta, int use_k)
+    const double* __restrict__ ws, double* __restrict__ cws, const void* A, const void* B,
+    TC* __restrict__ C, double alpha, double beta, int use_k)
 {
     constexpr int WM = BM / WGM, WN = BN / WGN, MB = WM / 16, NB = WN / 16;
     const PanelTask t = tasks[blockIdx.x];
@@ -140,20 +198,37 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_panel_kernel(
     const int nk = use_k ? t.nkg : 0;  // even
     if (nk > 0)
     {
+        const int lr = lane & 15, lk = lane >> 4;
         const int64_t rs = (int64_t)t.nkg * 64;  // double2 per row block
         const double2* pa = (const double2*)(ws + t.a) + (int64_t)wm * MB * rs + lane;
         const double2* pb = (const double2*)(ws + t.b) + (int64_t)wn * NB * rs + lane;
+        DirectSide<TC, MB> da;
+        DirectSide<TC, NB> db;
+        if constexpr (DA) direct_init(da, A, t.a_src, t.a_bytes, pool, t.a_rt, t.a_kt, wm * MB, wm * WM, t.mlen, lr);
+        if constexpr (DB) direct_init(db, B, t.b_src, t.b_bytes, pool, t.b_rt, t.b_kt, wn * NB, wn * WN, t.nlen, lr);
+        auto load = [&](double2 (&a)[MB], double2 (&b)[NB], int kg) {
+            if constexpr (DA)
+                direct_groups<TC, MB>(a, da, kg, lk, t.klen);
+            else
+#pragma unroll
+                for (int i = 0; i < MB; i++) a[i] = pa[i * rs + kg * 64];
+            if constexpr (DB)
+                direct_groups<TC, NB>(b, db, kg, lk, t.klen);
+            else
+#pragma unroll
+                for (int j = 0; j < NB; j++) b[j] = pb[j * rs + kg * 64];
+        };
         double2 a0[MB], b0[NB], a1[MB], b1[NB];
-        load_groups<MB, NB>(a0, b0, pa, pb, rs, 0);
-        load_groups<MB, NB>(a1, b1, pa, pb, rs, 1);
+        load(a0, b0, 0);
+        load(a1, b1, 1);
         for (int kg = 0; kg < nk; kg += 2)
         {
             // the last iteration re-reads the final groups (in bounds, unused)
             const int nx = kg + 2 < nk ? kg + 2 : nk - 2;
             mma_group<MB, NB>(acc, a0, b0);
-            load_groups<MB, NB>(a0, b0, pa, pb, rs, nx);
+            load(a0, b0, nx);
             mma_group<MB, NB>(acc, a1, b1);
-            load_groups<MB, NB>(a1, b1, pa, pb, rs, nx + 1);
+            load(a1, b1, nx + 1);
         }
     }
 
@@ -272,7 +347,7 @@ constexpr PanelTile kTiles[] = {
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
-template <typename TC>
+template <typename TC, bool DA, bool DB>
 hipError_t launch_gemm_t(const PanelLaunch& L)
 {
     const dim3 grid(L.ntasks), block(64 * kTiles[L.tile].wgm * kTiles[L.tile].wgn);
@@ -280,9 +355,9 @@ hipError_t launch_gemm_t(const PanelLaunch& L)
     {
 #define X(id, bm, bn, wgm, wgn)                                                                       \
     case id:                                                                                          \
-        hipLaunchKernelGGL((gemm_panel_kernel<bm, bn, wgm, wgn, TC>), grid, block, 0, L.stream,       \
-                           L.tasks, L.pool, (const double*)L.ws, L.cws, (TC*)L.C, L.alpha, L.beta,    \
-                           L.use_k);                                                                  \
+        hipLaunchKernelGGL((gemm_panel_kernel<bm, bn, wgm, wgn, TC, DA, DB>), grid, block, 0,         \
+                           L.stream, L.tasks, L.pool, (const double*)L.ws, L.cws, L.A, L.B, (TC*)L.C, \
+                           L.alpha, L.beta, L.use_k);                                                 \
         break;
         TBLIS_PANEL_TILES(X)
 #undef X
@@ -290,6 +365,15 @@ hipError_t launch_gemm_t(const PanelLaunch& L)
             return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <typename TC>
+hipError_t launch_gemm_d(const PanelLaunch& L)
+{
+    if (L.direct_a && L.direct_b) return launch_gemm_t<TC, true, true>(L);
+    if (L.direct_a) return launch_gemm_t<TC, true, false>(L);
+    if (L.direct_b) return launch_gemm_t<TC, false, true>(L);
+    return launch_gemm_t<TC, false, false>(L);
 }
 
 }  // namespace
@@ -329,7 +413,7 @@ hipError_t launch_panel_gemm(const PanelLaunch& L)
     if (L.ntasks == 0) return hipSuccess;
     if (L.family != FAMILY_F64) return launch_panel_gemm_bf16(L);
     if (L.tile < 0 || L.tile >= kNumTiles) return hipErrorInvalidValue;
-    return L.dtype == 0 ? launch_gemm_t<double>(L) : launch_gemm_t<float>(L);
+    return L.dtype == 0 ? launch_gemm_d<double>(L) : launch_gemm_d<float>(L);
 }
 
 hipError_t launch_fill_random(void* p, int64_t n, bool bf16, hipStream_t s)

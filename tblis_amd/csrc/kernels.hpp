@@ -71,6 +71,7 @@ struct PanelLaunch
     int dtype;   // 0: double, 1: float
     int family;  // FAMILY_F64 or a split-BF16 family
     int tile;
+    bool direct_a, direct_b;  // FP64: operand read from the tensor (PanelTask)
     hipStream_t stream;
 };
 hipError_t launch_panel_pack(const PanelLaunch& L);

@@ -218,6 +218,12 @@ void panel_from_desc(const GemmDesc& g, std::vector<PanelGemm>& out)
     }
 }
 
+bool direct_enabled()
+{
+    const char* e = std::getenv("TBLIS_HIP_DIRECT");
+    return !(e && e[0] == '0');
+}
+
 bool panel_enabled(int family, int use_mfma)
 {
     if (!use_mfma) return false;
@@ -229,7 +235,11 @@ bool panel_enabled(int family, int use_mfma)
 namespace
 {
 
-bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T);
+bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T,
+                     const bool allow_direct[2]);
+// tables_for_tile with each operand read directly only if it can be for
+// every GEMM of the plan (one kernel variant per launch)
+bool tables_uniform(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T);
 int autotune_tile(const std::vector<PanelGemm>& gemms, int family, const std::vector<int>& cands);
 
 }  // namespace
@@ -283,7 +293,7 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family)
             !(family == FAMILY_BF16X3 && panel_bf16_tile_fast_only(t)))
             tile = t;
     }
-    if (!tables_for_tile(gemms, family, tile, T)) return false;
+    if (!tables_uniform(gemms, family, tile, T)) return false;
     T.swap = swap;
     return true;
 }
@@ -291,7 +301,32 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family)
 namespace
 {
 
-bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T)
+bool tables_uniform(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T)
+{
+    bool allow[2] = {true, true};
+    for (int attempt = 0; attempt < 3; attempt++)
+    {
+        T = PanelTables();
+        if (!tables_for_tile(gemms, family, tile, T, allow)) return false;
+        bool mixed = false;
+        for (int o = 0; o < 2; o++)
+        {
+            size_t nd = 0;
+            for (auto& t : T.tasks) nd += (o ? t.b_src : t.a_src) != PANEL_MISSING;
+            T.direct[o] = nd == T.tasks.size() && nd > 0;
+            if (nd && nd != T.tasks.size())
+            {
+                allow[o] = false;
+                mixed = true;
+            }
+        }
+        if (!mixed) return true;
+    }
+    return false;
+}
+
+bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T,
+                     const bool allow_direct[2])
 {
     const Geom G = geom_of(family);
     const int64_t BM = panel_tile_m(family, tile), BN = panel_tile_n(family, tile);
@@ -345,16 +380,68 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
         T.mfma_flops += 2 * mp * np * kp;
         if (k > 0) T.reads_ab = true;
 
+        // Operands read straight from the tensor (FP64 family): a single
+        // block whose 16-row groups are unit-stride and whose 8-k groups
+        // are affine, with non-negative offsets spanning under 2 GiB, and
+        // re-read by at most 24 tiles (beyond that the packed panel's
+        // contiguous 1 KiB groups win).
+        const std::vector<PBlock>* blocks[2] = {&g.a, &g.b};
+        const size_t nrp[2] = {NM, NN};
+        const int64_t rtot[2] = {m, n};
+        bool dir[2] = {false, false};
+        int64_t drt[2] = {0, 0}, dkt[2] = {0, 0}, dsrc[2] = {0, 0};
+        uint32_t dbytes[2] = {0, 0};
+        for (int o = 0; o < 2 && G.terms == 1 && direct_enabled() && kp > 0; o++)
+        {
+            if (!allow_direct[o] || nrp[o] != 1 || NK != 1) continue;
+            if ((o == 0 ? ceil_div(n, BN) : ceil_div(m, BM)) > 24) continue;
+            const PBlock& blk = (*blocks[o])[0];
+            if (blk.base == PANEL_MISSING) continue;
+            const int64_t rm = pool.offsets(blk.rows), km = pool.offsets(blk.k);
+            const int64_t* pd = pool.data().data();
+            int64_t lo = 0, hi = 0;
+            for (int64_t i = 0; i < rtot[o]; i++) lo = std::min(lo, pd[rm + i]), hi = std::max(hi, pd[rm + i]);
+            int64_t klo = 0, khi = 0;
+            for (int64_t i = 0; i < k; i++) klo = std::min(klo, pd[km + i]), khi = std::max(khi, pd[km + i]);
+            if (lo < 0 || klo < 0 || (hi + khi + 1) * G.elem_bytes >= ((int64_t)1 << 31)) continue;
+            const int64_t rt = pool.group_table(rm, 0, rtot[o], G.gr);
+            const int64_t kt = pool.group_table(km, 0, k, G.gk);
+            bool ok = true;
+            pd = pool.data().data();
+            for (int64_t gi = 0; gi < ceil_div(rtot[o], G.gr) && ok; gi++)
+            {
+                const int64_t st = pd[rt + 2 * gi + 1];
+                ok = st == 1 || (st == 0 && std::min<int64_t>(G.gr, rtot[o] - gi * G.gr) == 1);
+            }
+            for (int64_t gi = 0; gi < ceil_div(k, G.gk) && ok; gi++) ok = pd[kt + 2 * gi + 1] != SCATTERED;
+            if (!ok) continue;
+            // copies of both tables extended with entries for the padding
+            // groups (their lanes are never valid)
+            auto extend = [&](int64_t tab, int64_t have, int64_t want) {
+                const int64_t at = (int64_t)pool.data().size();
+                pool.data().resize(pool.data().size() + 2 * want, 0);
+                std::copy(pool.data().begin() + tab, pool.data().begin() + tab + 2 * have,
+                          pool.data().begin() + at);
+                return at;
+            };
+            const int64_t rpadded = o == 0 ? mp : np;
+            dir[o] = true;
+            drt[o] = extend(rt, ceil_div(rtot[o], G.gr), rpadded / G.gr);
+            dkt[o] = extend(kt, ceil_div(k, G.gk), nkg);
+            dsrc[o] = blk.base;
+            dbytes[o] = (uint32_t)((hi + khi + 1) * G.elem_bytes);
+        }
+
         // panels
         const int64_t oa = ws;
-        ws += G.terms * mp * kp;
+        if (!dir[0]) ws += G.terms * mp * kp;
         const int64_t ob = ws;
-        ws += G.terms * np * kp;
-        const std::vector<PBlock>* blocks[2] = {&g.a, &g.b};
+        if (!dir[1]) ws += G.terms * np * kp;
         const std::vector<int64_t>* rpre[2] = {&m0, &n0};
-        const int64_t rtot[2] = {m, n}, rpad[2] = {mp, np}, outs[2] = {oa, ob};
+        const int64_t rpad[2] = {mp, np}, outs[2] = {oa, ob};
         for (int o = 0; o < 2 && kp > 0; o++)
         {
+            if (dir[o]) continue;
             const size_t NR = (o == 0 ? NM : NN);
             for (size_t p = 0; p < NR; p++)
                 for (size_t q = 0; q < NK; q++)
@@ -436,6 +523,15 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                 const int64_t ge = (int64_t)G.terms * G.gr * G.gk;  // values per group
                 t.a = oa + tm * (BM / G.gr) * nkg * ge;
                 t.b = ob + tn * (BN / G.gr) * nkg * ge;
+                t.klen = (int32_t)k;
+                t.a_src = dir[0] ? dsrc[0] : PANEL_MISSING;
+                t.b_src = dir[1] ? dsrc[1] : PANEL_MISSING;
+                t.a_rt = drt[0] + 2 * tm * (BM / G.gr);
+                t.b_rt = drt[1] + 2 * tn * (BN / G.gr);
+                t.a_kt = dkt[0];
+                t.b_kt = dkt[1];
+                t.a_bytes = dbytes[0];
+                t.b_bytes = dbytes[1];
                 t.nkg = (int32_t)nkg;
                 t.mlen = (int32_t)std::min<int64_t>(BM, m - tm * BM);
                 t.nlen = (int32_t)std::min<int64_t>(BN, n - tn * BN);
@@ -599,7 +695,9 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, const std::ve
     const int eb = geom_of(family).elem_bytes;
     for (size_t i = 0; i < cands.size(); i++)
     {
-        if (!tables_for_tile(gemms, family, cands[i], tabs[i])) return -1;
+        // timed with both operands in panels (the tuner has no tensors)
+        const bool none[2] = {false, false};
+        if (!tables_for_tile(gemms, family, cands[i], tabs[i], none)) return -1;
         panel_bytes = std::max(panel_bytes, (size_t)tabs[i].ws_panel * eb + 256);
         cslot = std::max(cslot, (size_t)panel_tile_m(family, cands[i]) * panel_tile_n(family, cands[i]));
         ntask = std::max(ntask, tabs[i].tasks.size());
@@ -732,6 +830,8 @@ bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, i
     plan.reads_ab = T.reads_ab;
     plan.panel = true;
     plan.panel_tile = T.tile;
+    plan.direct[0] = T.direct[0];
+    plan.direct[1] = T.direct[1];
     plan.key.family = family;
     plan.key.dtype = dtype;
     plan.key.use_mfma = 1;

@@ -44,6 +44,10 @@ void panel_from_desc(const GemmDesc& g, std::vector<PanelGemm>& out);
 // (direct-gather kernels).
 bool panel_enabled(int family, int use_mfma);
 
+// FP64 operands whose layout the GEMM can read directly skip packing;
+// TBLIS_HIP_DIRECT=0 packs them anyway.
+bool direct_enabled();
+
 // Host tables of a panel plan.
 struct PanelTables
 {
@@ -57,6 +61,7 @@ struct PanelTables
     int family = 0;
     bool swap = false;  // GEMMs were transposed: operand 0 is the B tensor
     bool reads_ab = false;
+    bool direct[2] = {false, false};  // operand read from the tensor by every task
     int64_t ws_panel = 0;  // panel values (double for FP64, BF16 for split-BF16)
     int64_t ws_c = 0;      // C panel doubles
     int64_t useful_flops = 0, mfma_flops = 0;

@@ -616,6 +616,8 @@ void run_plan(const std::shared_ptr<Plan>& pp, const void* A, const void* B, voi
         P.ws = ws;
         P.cws = (double*)(ws + p.ws_c_offset);
         P.family = p.key.family;
+        P.direct_a = p.direct[0];
+        P.direct_b = p.direct[1];
         P.alpha = alpha;
         P.beta = beta;
         P.use_k = L.use_k;

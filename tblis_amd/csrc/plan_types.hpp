@@ -106,16 +106,29 @@ struct PanelChunk
     int32_t job, rb0, nrb, kg0, nkg, pad;
 };
 
-// One C tile of one panel GEMM.
+// One C tile of one panel GEMM.  An operand is either read from its panel
+// (a, b) or, when its source block is already MFMA-friendly (unit-stride
+// 16-row groups, affine 8-k groups), straight from the tensor ("direct"):
+// through a buffer resource at tensor + x_src spanning x_bytes bytes, lane
+// offsets built from the row-group and k-group (base, stride) tables;
+// lanes past the valid rows or k read offset 0xffffffff, which the buffer
+// bounds check returns as zero.
 struct PanelTask
 {
-    int64_t a, b;        // workspace offsets (doubles) of the tile's first row block of A', B'
+    int64_t a, b;        // workspace offsets (values) of the tile's first row block of A', B'
     int64_t c_src;       // C: element offset of the C block; C panel: workspace offset
     int64_t c_rowmap;    // pool index of the tile's first row offset (direct epilogue)
     int64_t c_colmap;    // pool index of the tile's first col offset
     int32_t nkg;         // k groups (even)
     int32_t mlen, nlen;  // valid rows / cols of the tile
     int32_t c_ld;        // 0: write C directly; else leading dimension of the C panel
+    // direct operands (FP64 family)
+    int64_t a_src, b_src;  // element offset of the block in its tensor
+    int64_t a_rt, b_rt;    // pool index of the tile's first row-group (base, stride)
+    int64_t a_kt, b_kt;    // pool index of k group 0's (base, stride)
+    uint32_t a_bytes, b_bytes;  // extent of the block from x_src
+    int32_t klen;          // valid k
+    int32_t pad_;
 };
 
 // C panel -> C block: C[dst + rowmap[i] + colmap[j]] = alpha * P[i * ld + j] + beta * C[...]

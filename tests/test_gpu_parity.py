@@ -310,13 +310,17 @@ def test_every_tile_variant(tile, monkeypatch):
         T.lib().tblis_hip_clear_plan_cache()
 
 
+@pytest.mark.parametrize("direct", ["1", "0"])
 @pytest.mark.parametrize("tile", range(6))
-def test_every_panel_tile(tile, monkeypatch):
-    """Each tile of the FP64 panel GEMM (forced through TBLIS_HIP_PANEL_TILE):
-    dense contractions with ragged edges and scattered k, FP32 data through
-    FP64 MFMA, the 8-irrep DPD structure at small extents (merged irrep
-    GEMMs with a C panel), a DPD problem with a batch label, and a batch."""
+def test_every_panel_tile(tile, direct, monkeypatch):
+    """Each tile of the FP64 panel GEMM (forced through TBLIS_HIP_PANEL_TILE),
+    with operands read straight from the tensors where their layout allows
+    (direct) and with every operand packed: dense contractions with ragged
+    edges and scattered k, FP32 data through FP64 MFMA, the 8-irrep DPD
+    structure at small extents (merged irrep GEMMs with a C panel), a DPD
+    problem with a batch label, and a batch."""
     monkeypatch.setenv("TBLIS_HIP_PANEL_TILE", str(tile))
+    monkeypatch.setenv("TBLIS_HIP_DIRECT", direct)
     T.lib().tblis_hip_clear_plan_cache()
     try:
         rng = np.random.default_rng(31000 + tile)
@@ -333,8 +337,12 @@ def test_every_panel_tile(tile, monkeypatch):
         ext = {c: [int(rng.integers(1, 4)) for _ in range(4)] for c in "abez"}
         _dpd_case(rng, 4, "aez", "ebz", "abz", ext, 1, 2, 5, alpha=1.5, beta=0.5, mode=2)
         test_batch_uneven()
+        # permuted workload shape at reduced size: A's rows and k are affine
+        _dense_case(rng, "aebf", "fjei", "aibj", dict(a=32, b=5, e=16, f=3, i=7, j=9),
+                    ["col"] * 3, alpha=0.5, beta=1.0)
     finally:
         monkeypatch.delenv("TBLIS_HIP_PANEL_TILE")
+        monkeypatch.delenv("TBLIS_HIP_DIRECT")
         T.lib().tblis_hip_clear_plan_cache()
 
 

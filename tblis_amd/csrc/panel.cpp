@@ -27,6 +27,8 @@ namespace
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+constexpr int64_t kDirectRowFast = 24;  // re-read limit (tiles) of rows-fast direct operands
+
 void hip_check(hipError_t e, const char* what)
 {
     if (e != hipSuccess) fail(std::string(what) + ": " + hipGetErrorString(e));
@@ -218,6 +220,15 @@ void panel_from_desc(const GemmDesc& g, std::vector<PanelGemm>& out)
     }
 }
 
+int direct_kfast_rereads()
+{
+    static const int v = [] {
+        const char* e = std::getenv("TBLIS_HIP_DIRECT_KFAST");
+        return e && e[0] >= '0' && e[0] <= '9' ? std::atoi(e) : 32;
+    }();
+    return v;
+}
+
 bool direct_enabled()
 {
     const char* e = std::getenv("TBLIS_HIP_DIRECT");
@@ -394,7 +405,8 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
         for (int o = 0; o < 2 && G.terms == 1 && direct_enabled() && kp > 0; o++)
         {
             if (!allow_direct[o] || nrp[o] != 1 || NK != 1) continue;
-            if ((o == 0 ? ceil_div(n, BN) : ceil_div(m, BM)) > 24) continue;
+            const int64_t rereads = o == 0 ? ceil_div(n, BN) : ceil_div(m, BM);
+            if (rereads > std::max<int64_t>(kDirectRowFast, direct_kfast_rereads())) continue;
             const PBlock& blk = (*blocks[o])[0];
             if (blk.base == PANEL_MISSING) continue;
             const int64_t rm = pool.offsets(blk.rows), km = pool.offsets(blk.k);
@@ -406,14 +418,31 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
             if (lo < 0 || klo < 0 || (hi + khi + 1) * G.elem_bytes >= ((int64_t)1 << 31)) continue;
             const int64_t rt = pool.group_table(rm, 0, rtot[o], G.gr);
             const int64_t kt = pool.group_table(km, 0, k, G.gk);
-            bool ok = true;
+            // rows-fast (unit-stride row groups, any affine k groups; re-read
+            // by <= kDirectRowFast tiles) or k-fast (affine row groups,
+            // unit-stride k groups; re-read by <= the k-fast limit, <= 4 MiB)
+            bool rows_unit = true, rows_affine = true, k_unit = true, k_affine = true;
             pd = pool.data().data();
-            for (int64_t gi = 0; gi < ceil_div(rtot[o], G.gr) && ok; gi++)
+            for (int64_t gi = 0; gi < ceil_div(rtot[o], G.gr); gi++)
             {
                 const int64_t st = pd[rt + 2 * gi + 1];
-                ok = st == 1 || (st == 0 && std::min<int64_t>(G.gr, rtot[o] - gi * G.gr) == 1);
+                const bool single = std::min<int64_t>(G.gr, rtot[o] - gi * G.gr) == 1;
+                rows_affine &= st != SCATTERED;
+                rows_unit &= st == 1 || (st == 0 && single);
             }
-            for (int64_t gi = 0; gi < ceil_div(k, G.gk) && ok; gi++) ok = pd[kt + 2 * gi + 1] != SCATTERED;
+            for (int64_t gi = 0; gi < ceil_div(k, G.gk); gi++)
+            {
+                const int64_t st = pd[kt + 2 * gi + 1];
+                const bool single = std::min<int64_t>(G.gk, k - gi * G.gk) == 1;
+                k_affine &= st != SCATTERED;
+                k_unit &= st == 1 || (st == 0 && single);
+            }
+            // k-fast reads use 32-byte pieces of each row: only worth it
+            // while the re-reads hit in L2 (4 MiB per XCD)
+            const int64_t span = (hi + khi + 1) * G.elem_bytes;
+            const bool ok = (rows_unit && k_affine && rereads <= kDirectRowFast) ||
+                            (rows_affine && k_unit && rereads <= direct_kfast_rereads() &&
+                             span <= ((int64_t)4 << 20));
             if (!ok) continue;
             // copies of both tables extended with entries for the padding
             // groups (their lanes are never valid)

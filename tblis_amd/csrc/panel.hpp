@@ -47,6 +47,9 @@ bool panel_enabled(int family, int use_mfma);
 // FP64 operands whose layout the GEMM can read directly skip packing;
 // TBLIS_HIP_DIRECT=0 packs them anyway.
 bool direct_enabled();
+// Re-read limit (tiles) under which a k-fast operand of at most 4 MiB is
+// read directly (TBLIS_HIP_DIRECT_KFAST, default 32).
+int direct_kfast_rereads();
 
 // Host tables of a panel plan.
 struct PanelTables

@@ -48,6 +48,66 @@ __global__ __launch_bounds__(256) void pack_panel_kernel(const PanelJob* __restr
     const bool zero = j.src == PANEL_MISSING;
     const int ng = ch.nrb * ch.nkg;
     const int rbf = j.r0 / 16, kgf = j.k0 / 8;  // first groups of the job's tables
+    if (j.kfast && !zero)
+    {
+        // k-fast source: per pair of k groups, four loads of four rows x 16
+        // consecutive k (whole 128-byte lines), transposed through a per-wave
+        // LDS tile into the fragment order.  The k tables are unit-stride.
+        __shared__ double tile[4][16][17];
+        const int npair = (ch.nkg + 1) / 2;
+        const int kk = lane & 15;
+        for (int g = wave; g < ch.nrb * npair; g += 4)
+        {
+            const int rb = ch.rb0 + g / npair;
+            const int kg = ch.kg0 + 2 * (g % npair);
+            const bool two = kg + 1 < ch.kg0 + ch.nkg;
+            const int64_t rbase = pool[j.rowtab + 2 * (rb - rbf)];
+            const int64_t rstr = pool[j.rowtab + 2 * (rb - rbf) + 1];
+            const int h = kk >> 3;
+            const int64_t kb = pool[j.ktab + 2 * (kg + (two ? h : 0) - kgf)];
+            const int pk = kg * 8 + kk - j.k0;  // job-relative k
+            const bool kin = pk >= 0 && pk < j.nk && (h == 0 || two);
+            const int64_t koff = kb + (kk & 7);
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++)
+            {
+                const int r = (lane >> 4) + 4 * jj;
+                const int prow = rb * 16 + r - j.r0;
+                double v = 0.0;
+                if (kin && prow >= 0 && prow < j.nr)
+                {
+                    const int64_t roff = rstr != SCATTERED ? rbase + r * rstr : pool[j.rowmap + prow];
+                    v = (double)X[j.src + roff + koff];
+                }
+                tile[wave][r][kk] = v;
+            }
+            __builtin_amdgcn_wave_barrier();
+            const int row = rb * 16 + lr - j.r0;
+            const bool rin = row >= 0 && row < j.nr;
+            for (int hh = 0; hh < (two ? 2 : 1); hh++)
+            {
+                const int k0 = (kg + hh) * 8 + lk - j.k0, k1 = k0 + 4;
+                const bool in0 = rin && k0 >= 0 && k0 < j.nk;
+                const bool in1 = rin && k1 >= 0 && k1 < j.nk;
+                const double v0 = tile[wave][lr][8 * hh + lk], v1 = tile[wave][lr][8 * hh + 4 + lk];
+                double* out = ws + j.out + ((int64_t)rb * j.pnkg + kg + hh) * 128 + lane * 2;
+                if (in0 && in1)
+                {
+                    double2 v;
+                    v.x = v0;
+                    v.y = v1;
+                    *(double2*)out = v;
+                }
+                else
+                {
+                    if (in0) out[0] = v0;
+                    if (in1) out[1] = v1;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        return;
+    }
     for (int g = wave; g < ng; g += 4)
     {
         const int rb = ch.rb0 + g / ch.nkg;
@@ -234,6 +294,42 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_panel_kernel(
 
     // D register r of lane l holds D[l / 16 + 4 r][l % 16] of each block
     const int cl = lane & 15, rl = lane >> 4;
+    if (t.c_ld < 0)
+    {
+        // C over several blocks: each element finds its block
+        const int ns = -t.c_ld;
+        int64_t cs[NB], cc[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+        {
+            const int col = wn * WN + j * 16 + cl;
+            const int64_t code = col < t.nlen ? pool[t.c_colmap + col] : 0;
+            cs[j] = code >> 32;
+            cc[j] = code & 0xffffffff;
+        }
+#pragma unroll
+        for (int i = 0; i < MB; i++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+            {
+                const int row = wm * WM + i * 16 + rl + 4 * r;
+                if (row >= t.mlen) continue;
+                const int64_t code = pool[t.c_rowmap + row];
+                const int64_t rp = code >> 32, ri = code & 0xffffffff;
+#pragma unroll
+                for (int j = 0; j < NB; j++)
+                {
+                    const int col = wn * WN + j * 16 + cl;
+                    if (col >= t.nlen) continue;
+                    const int64_t* blk = pool + t.c_src + 3 * (rp * ns + cs[j]);
+                    TC* p = C + blk[0] + pool[blk[1] + ri] + pool[blk[2] + cc[j]];
+                    double v = alpha * acc[i][j][r];
+                    if (beta != 0.0) v += beta * (double)*p;
+                    *p = (TC)v;
+                }
+            }
+        return;
+    }
     if (t.c_ld)
     {
 #pragma unroll

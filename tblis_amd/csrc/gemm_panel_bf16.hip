@@ -54,6 +54,82 @@ __global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PanelJob* __
     const bool zero = j.src == PANEL_MISSING;
     const int ng = ch.nrb * ch.nkg;
     const int rbf = j.r0 / 32, ksf = j.k0 / 16;  // first groups of the job's tables
+    if (j.kfast && !zero)
+    {
+        // k-fast source: per pair of k groups, 16 loads of two rows x 32
+        // consecutive k (whole 128-byte lines), transposed through a per-wave
+        // LDS tile into the fragment order.  The k tables are unit-stride.
+        __shared__ float tile[4][32][33];
+        const int npair = (ch.nkg + 1) / 2;
+        const int kk = lane & 31;
+        for (int g = wave; g < ch.nrb * npair; g += 4)
+        {
+            const int rb = ch.rb0 + g / npair;
+            const int ks = ch.kg0 + 2 * (g % npair);
+            const bool two = ks + 1 < ch.kg0 + ch.nkg;
+            const int64_t rbase = pool[j.rowtab + 2 * (rb - rbf)];
+            const int64_t rstr = pool[j.rowtab + 2 * (rb - rbf) + 1];
+            const int h = kk >> 4;
+            const int64_t kb = pool[j.ktab + 2 * (ks + (two ? h : 0) - ksf)];
+            const int pk = ks * 16 + kk - j.k0;  // job-relative k
+            const bool kin = pk >= 0 && pk < j.nk && (h == 0 || two);
+            const int64_t koff = kb + (kk & 15);
+#pragma unroll
+            for (int jj = 0; jj < 16; jj++)
+            {
+                const int r = (lane >> 5) + 2 * jj;
+                const int prow = rb * 32 + r - j.r0;
+                float v = 0.0f;
+                if (kin && prow >= 0 && prow < j.nr)
+                {
+                    const int64_t roff = rstr != SCATTERED ? rbase + r * rstr : pool[j.rowmap + prow];
+                    v = X[j.src + roff + koff];
+                }
+                tile[wave][r][kk] = v;
+            }
+            __builtin_amdgcn_wave_barrier();
+            const int row = rb * 32 + lr - j.r0;
+            const bool rin = row >= 0 && row < j.nr;
+            for (int hh = 0; hh < (two ? 2 : 1); hh++)
+            {
+                const int kb0 = (ks + hh) * 16 + lh * 8 - j.k0;
+                bf16x8 hv, mv, lv;
+                unsigned inmask = 0;
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                {
+                    const int k = kb0 + e;
+                    const bool in = rin && k >= 0 && k < j.nk;
+                    __bf16 a, b, c;
+                    split3(tile[wave][lr][16 * hh + 8 * lh + e], a, b, c);
+                    hv[e] = a;
+                    mv[e] = b;
+                    lv[e] = c;
+                    inmask |= (unsigned)in << e;
+                }
+                __bf16* out = ws + j.out + ((int64_t)rb * j.pnkg + ks + hh) * 1536 + lane * 8;
+                if (inmask == 0xffu)
+                {
+                    *(bf16x8*)out = hv;
+                    *(bf16x8*)(out + 512) = mv;
+                    *(bf16x8*)(out + 1024) = lv;
+                }
+                else if (inmask)
+                {
+#pragma unroll
+                    for (int e = 0; e < 8; e++)
+                        if (inmask >> e & 1u)
+                        {
+                            out[e] = hv[e];
+                            out[512 + e] = mv[e];
+                            out[1024 + e] = lv[e];
+                        }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        return;
+    }
     for (int g = wave; g < ng; g += 4)
     {
         const int rb = ch.rb0 + g / ch.nkg;
@@ -211,14 +287,27 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_bf16_panel_kernel(
 
     // D register r of lane l holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]
     const int cl = lane & 31, rh = lane >> 5;
-    int64_t coff[NB];
-    if (!t.c_ld)
+    int64_t coff[NB], cs[NB];
 #pragma unroll
-        for (int jb = 0; jb < NB; jb++)
+    for (int jb = 0; jb < NB; jb++)
+    {
+        const int col = wn * WN + jb * 32 + cl;
+        coff[jb] = 0;
+        cs[jb] = 0;
+        if (t.c_ld <= 0 && col < t.nlen)
         {
-            const int col = wn * WN + jb * 32 + cl;
-            coff[jb] = col < t.nlen ? pool[t.c_colmap + col] : 0;
+            const int64_t v = pool[t.c_colmap + col];
+            if (t.c_ld < 0)
+            {
+                cs[jb] = v >> 32;  // column patch; index in coff
+                coff[jb] = v & 0xffffffff;
+            }
+            else
+            {
+                coff[jb] = v;
+            }
         }
+    }
 #pragma unroll
     for (int i = 0; i < MB; i++)
 #pragma unroll
@@ -226,7 +315,15 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_bf16_panel_kernel(
         {
             const int row = wm * WM + i * 32 + 8 * (r / 4) + 4 * rh + r % 4;
             if (row >= t.mlen) continue;
-            const int64_t roff = t.c_ld ? 0 : t.c_src + pool[t.c_rowmap + row];
+            int64_t roff = 0, rp = 0;
+            if (t.c_ld == 0)
+                roff = t.c_src + pool[t.c_rowmap + row];
+            else if (t.c_ld < 0)
+            {
+                const int64_t code = pool[t.c_rowmap + row];
+                rp = code >> 32;
+                roff = code & 0xffffffff;  // index in the row patch
+            }
 #pragma unroll
             for (int jb = 0; jb < NB; jb++)
             {
@@ -234,17 +331,24 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_bf16_panel_kernel(
                 if (col >= t.nlen) continue;
                 const float s = CHUNKED ? tot[i % (CHUNKED ? MB : 1)][jb % (CHUNKED ? NB : 1)][r]
                                         : acc[i][jb][r];
-                if (t.c_ld)
+                if (t.c_ld > 0)
                 {
                     cws[t.c_src + (int64_t)row * t.c_ld + col] = (double)s;
+                    continue;
+                }
+                float* p;
+                if (t.c_ld < 0)
+                {
+                    const int64_t* blk = pool + t.c_src + 3 * (rp * (-t.c_ld) + cs[jb]);
+                    p = C + blk[0] + pool[blk[1] + roff] + pool[blk[2] + coff[jb]];
                 }
                 else
                 {
-                    float* p = C + roff + coff[jb];
-                    double v = alpha * (double)s;
-                    if (beta != 0.0) v += beta * (double)*p;
-                    *p = (float)v;
+                    p = C + roff + coff[jb];
                 }
+                double v = alpha * (double)s;
+                if (beta != 0.0) v += beta * (double)*p;
+                *p = (float)v;
             }
         }
 }

@@ -85,7 +85,9 @@ class Pool
         {
             const int64_t lo = std::max<int64_t>(g * G - r0, 0), hi = std::min<int64_t>(g * G - r0 + G, n);
             const int64_t* off = data_.data() + map;
-            int64_t stride = hi - lo > 1 ? off[lo + 1] - off[lo] : 0;
+            // a group with one element gets stride 1 (consistent with any
+            // element index), so unit-stride checks accept it
+            int64_t stride = hi - lo > 1 ? off[lo + 1] - off[lo] : 1;
             for (int64_t i = lo + 1; i + 1 < hi; i++)
                 if (off[i + 1] - off[i] != stride) stride = SCATTERED;
             int64_t base = 0;
@@ -93,6 +95,14 @@ class Pool
             data_[at + 2 * (g - g0)] = base;
             data_[at + 2 * (g - g0) + 1] = stride;
         }
+        return at;
+    }
+    // append raw values; returns their pool index
+    int64_t push(const std::vector<int64_t>& v)
+    {
+        const int64_t at = (int64_t)data_.size();
+        data_.insert(data_.end(), v.begin(), v.end());
+        if (v.empty()) data_.push_back(0);
         return at;
     }
     std::vector<int64_t>& data() { return data_; }
@@ -117,6 +127,29 @@ Geom geom_of(int family)
 {
     if (family == FAMILY_F64) return Geom{16, 8, 1, 16, 8};
     return Geom{32, 16, 3, 32, 2};
+}
+
+// A job whose k groups are unit-stride and whose row groups are not: the
+// pack kernels read it row by row (whole 128-byte lines) and transpose
+// through LDS.
+bool kfast_job(Pool& pool, const PanelJob& j, const Geom& G)
+{
+    const int64_t* pd = pool.data().data();
+    const int64_t nrg = (j.r0 + j.nr + G.gr - 1) / G.gr - j.r0 / G.gr;
+    const int64_t nkg = (j.k0 + j.nk + G.gk - 1) / G.gk - j.k0 / G.gk;
+    // stride 0 marks a group with a single element
+    bool k_unit = true, rows_unit = true;
+    for (int64_t g = 0; g < nkg; g++)
+    {
+        const int64_t st = pd[j.ktab + 2 * g + 1];
+        k_unit &= st == 1 || st == 0;
+    }
+    for (int64_t g = 0; g < nrg; g++)
+    {
+        const int64_t st = pd[j.rowtab + 2 * g + 1];
+        rows_unit &= st == 1 || st == 0;
+    }
+    return k_unit && !rows_unit;
 }
 
 struct TileModel
@@ -218,6 +251,12 @@ void panel_from_desc(const GemmDesc& g, std::vector<PanelGemm>& out)
         p.c.push_back(c);
         out.push_back(std::move(p));
     }
+}
+
+bool c_unpack_forced()
+{
+    const char* e = std::getenv("TBLIS_HIP_C_UNPACK");
+    return e && e[0] == '1';
 }
 
 int direct_kfast_rereads()
@@ -492,6 +531,7 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                         j.kmap = pool.offsets(blk.k);
                         j.rowtab = pool.group_table(j.rowmap, j.r0, j.nr, G.gr);
                         j.ktab = pool.group_table(j.kmap, j.k0, j.nk, G.gk);
+                        j.kfast = kfast_job(pool, j, G) ? 1 : 0;
                     }
                     add_job(j);
                 }
@@ -515,11 +555,37 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
 
         // C
         const bool direct = g.c.size() == 1;
-        int64_t oc = 0, crow = 0, ccol = 0;
+        // split-BF16 keeps the C panel (its 64x64-per-wave epilogue pays
+        // more for per-element block lookups than the unpack pass costs)
+        const bool blocked = !direct && !c_unpack_forced() && family == FAMILY_F64;
+        int64_t oc = 0, crow = 0, ccol = 0, cblk = 0;
         if (direct)
         {
             crow = pool.offsets(g.c[0].rows);
             ccol = pool.offsets(g.c[0].cols);
+        }
+        else if (blocked)
+        {
+            // (patch << 32 | index in patch) per row and per column, and a
+            // (base, row offsets, column offsets) triple per C block
+            std::vector<int64_t> codes;
+            for (size_t p = 0; p < NM; p++)
+                for (int64_t i = 0; i < g.mlen[p]; i++) codes.push_back(((int64_t)p << 32) | i);
+            crow = pool.push(codes);
+            codes.clear();
+            for (size_t q = 0; q < NN; q++)
+                for (int64_t i = 0; i < g.nlen[q]; i++) codes.push_back(((int64_t)q << 32) | i);
+            ccol = pool.push(codes);
+            std::vector<int64_t> tab;
+            for (size_t p = 0; p < NM; p++)
+                for (size_t q = 0; q < NN; q++)
+                {
+                    const CBlock& cb = g.c[p * NN + q];
+                    tab.push_back(cb.base);
+                    tab.push_back(pool.offsets(cb.rows));
+                    tab.push_back(pool.offsets(cb.cols));
+                }
+            cblk = pool.push(tab);
         }
         else
         {
@@ -570,6 +636,13 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                     t.c_rowmap = crow + tm * BM;
                     t.c_colmap = ccol + tn * BN;
                     t.c_ld = 0;
+                }
+                else if (blocked)
+                {
+                    t.c_src = cblk;
+                    t.c_rowmap = crow + tm * BM;
+                    t.c_colmap = ccol + tn * BN;
+                    t.c_ld = -(int32_t)NN;
                 }
                 else
                 {

@@ -47,6 +47,10 @@ bool panel_enabled(int family, int use_mfma);
 // FP64 operands whose layout the GEMM can read directly skip packing;
 // TBLIS_HIP_DIRECT=0 packs them anyway.
 bool direct_enabled();
+// GEMMs over several C blocks write them from the GEMM epilogue through
+// per-row/column block codes; TBLIS_HIP_C_UNPACK=1 stores a C panel and
+// scatters it with the unpack kernel instead.
+bool c_unpack_forced();
 // Re-read limit (tiles) under which a k-fast operand of at most 4 MiB is
 // read directly (TBLIS_HIP_DIRECT_KFAST, default 32).
 int direct_kfast_rereads();

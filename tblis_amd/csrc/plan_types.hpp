@@ -97,6 +97,8 @@ struct PanelJob
     int32_t r0, nr, k0, nk;
     int32_t pnkg;          // k groups of the whole panel
     int32_t operand;       // 0: A pointer, 1: B pointer (after orientation)
+    int32_t kfast;         // k groups unit-stride, rows not: read rows, transpose in LDS
+    int32_t pad_;
 };
 
 // Rectangle of fragment groups of one job: rows blocks [rb0, rb0 + nrb),
@@ -121,7 +123,12 @@ struct PanelTask
     int64_t c_colmap;    // pool index of the tile's first col offset
     int32_t nkg;         // k groups (even)
     int32_t mlen, nlen;  // valid rows / cols of the tile
-    int32_t c_ld;        // 0: write C directly; else leading dimension of the C panel
+    // 0: write C directly through c_rowmap / c_colmap offsets; > 0: leading
+    // dimension of a C panel at c_src; < 0: C spans -c_ld column patches
+    // (per-row and per-column codes (patch << 32 | index) at c_rowmap /
+    // c_colmap, a (base, row offsets, column offsets) triple per block at
+    // pool[c_src], row-patch major)
+    int32_t c_ld;
     // direct operands (FP64 family)
     int64_t a_src, b_src;  // element offset of the block in its tensor
     int64_t a_rt, b_rt;    // pool index of the tile's first row-group (base, stride)

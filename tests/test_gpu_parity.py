@@ -358,3 +358,24 @@ def test_direct_path_dpd_and_batch(monkeypatch):
     finally:
         monkeypatch.delenv("TBLIS_HIP_PANEL")
         T.lib().tblis_hip_clear_plan_cache()
+
+
+@pytest.mark.parametrize("unpack", ["0", "1"])
+def test_dpd_c_epilogue_modes(unpack, monkeypatch):
+    """DPD irrep GEMMs span several C blocks: their C tiles are written from
+    the GEMM epilogue through per-row/column block codes (default) or stored
+    as C panels and scattered by the unpack kernel (TBLIS_HIP_C_UNPACK=1);
+    both agree with the oracle, FP64 and split-BF16, with beta != 0."""
+    monkeypatch.setenv("TBLIS_HIP_C_UNPACK", unpack)
+    T.lib().tblis_hip_clear_plan_cache()
+    try:
+        rng = np.random.default_rng(34000)
+        ext = {c: [5, 4, 3, 2, 6, 3, 2, 1] for c in "abefij"}
+        _dpd_case(rng, 8, "abef", "efij", "abij", ext, 0, 0, 5, alpha=0.5, beta=-1.0)
+        ext = {c: [3, 0, 2, 4] for c in "aebfij"}
+        _dpd_case(rng, 4, "aebf", "fjei", "aibj", ext, 1, 3, 2, alpha=2.0, beta=0.5)
+        _dpd_case(rng, 4, "aebf", "fjei", "aibj", ext, 2, 1, 0, alpha=1.0, beta=1.0,
+                  dtype=np.float32, tol=1e-6)
+    finally:
+        monkeypatch.delenv("TBLIS_HIP_C_UNPACK")
+        T.lib().tblis_hip_clear_plan_cache()

@@ -74,6 +74,18 @@ typedef struct tblis_tensor
     stride_type* stride;
 } tblis_tensor;
 
+/* 72 bytes: type@0 conj@4 scalar@8 data@32 m@40 n@48 rs@56 cs@64
+ * (src/util/basic_types.h:352-428). */
+typedef struct tblis_matrix
+{
+    type_t type;
+    int conj;
+    tblis_scalar scalar;
+    void* data;
+    len_type m, n;
+    stride_type rs, cs;
+} tblis_matrix;
+
 /* tci_comm, 24 bytes (src/external/tci/tci/communicator.h:8-15). */
 typedef struct tblis_comm
 {
@@ -132,6 +144,34 @@ void tblis_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
                        const tblis_tensor* A, const label_type* idx_A,
                        const tblis_tensor* B, const label_type* idx_B,
                        tblis_tensor* C, const label_type* idx_C);
+
+/* Matrix descriptors (src/util/basic_types.h:430-465,
+ * src/util/basic_types.cxx:86-184). */
+void tblis_init_matrix_scaled_s(tblis_matrix* mat, float scalar, len_type m, len_type n, float* data,
+                                stride_type rs, stride_type cs);
+void tblis_init_matrix_scaled_d(tblis_matrix* mat, double scalar, len_type m, len_type n, double* data,
+                                stride_type rs, stride_type cs);
+void tblis_init_matrix_scaled_c(tblis_matrix* mat, scomplex scalar, len_type m, len_type n,
+                                scomplex* data, stride_type rs, stride_type cs);
+void tblis_init_matrix_scaled_z(tblis_matrix* mat, dcomplex scalar, len_type m, len_type n,
+                                dcomplex* data, stride_type rs, stride_type cs);
+void tblis_init_matrix_s(tblis_matrix* mat, len_type m, len_type n, float* data, stride_type rs,
+                         stride_type cs);
+void tblis_init_matrix_d(tblis_matrix* mat, len_type m, len_type n, double* data, stride_type rs,
+                         stride_type cs);
+void tblis_init_matrix_c(tblis_matrix* mat, len_type m, len_type n, scomplex* data, stride_type rs,
+                         stride_type cs);
+void tblis_init_matrix_z(tblis_matrix* mat, len_type m, len_type n, dcomplex* data, stride_type rs,
+                         stride_type cs);
+
+/* C (m x n) <- alpha_A*alpha_B * A (m x k) . B (k x n) + alpha_C * C
+ * (src/iface/3m/mult.h:17-19, src/iface/3m/mult.cxx:19-191): the same panel
+ * GEMM as tblis_tensor_mult with rs/cs as the two dimensions' strides.
+ * Shapes must agree (A.m == C.m, B.n == C.n, A.n == B.m) and conj must be 0;
+ * alpha == 0 or k == 0 scales C by beta (beta == 0 writes zeros); an empty
+ * C is left alone; afterwards C->scalar = 1 and C->conj = 0. */
+void tblis_matrix_mult(const tblis_comm* comm, const tblis_config* cfg, const tblis_matrix* A,
+                       const tblis_matrix* B, tblis_matrix* C);
 
 /* Thread-count controls (src/util/thread.h:21-23).  The GPU path does not
  * use host threads; the value is stored and returned for compatibility. */

@@ -31,6 +31,15 @@ struct varray_view
     std::vector<stride_type> stride;
 };
 
+// Strided matrix view (device memory), like MArray's matrix_view.
+template <typename T>
+struct matrix_view
+{
+    T* data;
+    len_type m, n;
+    stride_type rs, cs;
+};
+
 // DPD tensor view: irrep, nirrep, per-irrep lengths [dim][irrep], layout.
 template <typename T>
 struct dpd_varray_view
@@ -103,6 +112,24 @@ void mult(T alpha, const varray_view<T>& A, const label_type* idx_A, const varra
     tblis_tensor b = detail::make_tensor(T(1), B);
     tblis_tensor c = detail::make_tensor(beta, C);
     tblis_tensor_mult(nullptr, cfg, &a, idx_A, &b, idx_B, &c, idx_C);
+}
+
+// C <- alpha * A B + beta * C (src/iface/3m/mult.h:27-38).
+template <typename T>
+void mult(T alpha, const matrix_view<T>& A, const matrix_view<T>& B, T beta, const matrix_view<T>& C,
+          const tblis_config* cfg = nullptr)
+{
+    tblis_matrix a, b, c;
+    auto init = [](tblis_matrix& x, T s, const matrix_view<T>& v) {
+        if (std::is_same<T, float>::value)
+            tblis_init_matrix_scaled_s(&x, (float)s, v.m, v.n, (float*)v.data, v.rs, v.cs);
+        else
+            tblis_init_matrix_scaled_d(&x, (double)s, v.m, v.n, (double*)v.data, v.rs, v.cs);
+    };
+    init(a, alpha, A);
+    init(b, T(1), B);
+    init(c, beta, C);
+    tblis_matrix_mult(nullptr, cfg, &a, &b, &c);
 }
 
 // DPD contraction (src/iface/3t/mult.h:53-72).

@@ -10,7 +10,7 @@ fallback: a missing library is an error.
 import ctypes as C
 import os
 
-__all__ = ["lib", "mult", "dpd_mult", "mult_batch", "prepare_mult", "prepare_dpd_mult",
+__all__ = ["lib", "mult", "matrix_mult", "dpd_mult", "mult_batch", "prepare_mult", "prepare_dpd_mult",
            "prepare_batch", "Tensor", "DpdTensor", "get_config",
            "set_stream", "set_async", "last_error", "last_algorithm", "DPD_LAYOUTS", "TblisError"]
 
@@ -40,13 +40,18 @@ class Tensor(C.Structure):
                 ("stride", C.POINTER(C.c_ssize_t))]
 
 
+class Matrix(C.Structure):
+    _fields_ = [("type", C.c_int), ("conj", C.c_int), ("scalar", Scalar), ("data", C.c_void_p),
+                ("m", C.c_ssize_t), ("n", C.c_ssize_t), ("rs", C.c_ssize_t), ("cs", C.c_ssize_t)]
+
+
 class DpdTensor(C.Structure):
     _fields_ = [("type", C.c_int), ("conj", C.c_int), ("scalar", Scalar), ("data", C.c_void_p),
                 ("ndim", C.c_uint), ("irrep", C.c_uint), ("nirrep", C.c_uint), ("layout", C.c_int),
                 ("len", C.POINTER(C.c_ssize_t))]
 
 
-assert C.sizeof(Scalar) == 24 and C.sizeof(Tensor) == 64
+assert C.sizeof(Scalar) == 24 and C.sizeof(Tensor) == 64 and C.sizeof(Matrix) == 72
 
 _LIB = None
 
@@ -56,6 +61,9 @@ EXPORTED = [
     "tblis_init_scalar_z", "tblis_init_tensor_scaled_s", "tblis_init_tensor_scaled_d",
     "tblis_init_tensor_scaled_c", "tblis_init_tensor_scaled_z", "tblis_init_tensor_s",
     "tblis_init_tensor_d", "tblis_init_tensor_c", "tblis_init_tensor_z", "tblis_tensor_mult",
+    "tblis_init_matrix_scaled_s", "tblis_init_matrix_scaled_d", "tblis_init_matrix_scaled_c",
+    "tblis_init_matrix_scaled_z", "tblis_init_matrix_s", "tblis_init_matrix_d", "tblis_init_matrix_c",
+    "tblis_init_matrix_z", "tblis_matrix_mult",
     "tblis_get_num_threads", "tblis_set_num_threads", "tblis_get_config",
     "tblis_init_dpd_tensor_d", "tblis_init_dpd_tensor_s", "tblis_dpd_tensor_size",
     "tblis_dpd_tensor_mult", "tblis_tensor_mult_batch", "tblis_hip_set_stream",
@@ -80,6 +88,8 @@ def lib():
                                         C.c_char_p]
         L.tblis_dpd_tensor_mult.argtypes = [C.c_void_p, C.c_void_p, D, C.c_char_p, D, C.c_char_p, D,
                                             C.c_char_p]
+        M = C.POINTER(Matrix)
+        L.tblis_matrix_mult.argtypes = [C.c_void_p, C.c_void_p, M, M, M]
         L.tblis_tensor_mult_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_ssize_t, P(T),
                                               P(C.c_char_p), P(T), P(C.c_char_p), P(T),
                                               P(C.c_char_p)]
@@ -238,6 +248,31 @@ def mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None):
         prepare_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config)()
     except TblisError as e:
         raise TblisError("tblis_tensor_mult: %s" % e) from None
+    return C_
+
+
+def _matrix(t, scalar):
+    if t.dim() != 2:
+        raise TblisError("matrix_mult: operands must be 2-D")
+    d = Matrix()
+    d.type = _type_code(t.dtype)
+    d.conj = 0
+    _set_scalar(d.scalar, scalar, d.type)
+    d.data = C.c_void_p(t.data_ptr())
+    d.m, d.n = t.shape
+    d.rs, d.cs = t.stride()
+    return d
+
+
+def matrix_mult(alpha, A, B, beta, C_, config=None):
+    """C_ <- alpha * A @ B + beta * C_ for 2-D torch tensors with any strides
+    (tblis::mult over matrix views, src/iface/3m/mult.h:27-38)."""
+    a, b, c = _matrix(A, alpha), _matrix(B, 1.0), _matrix(C_, beta)
+    L = lib()
+    L.tblis_hip_set_error_mode(1)
+    L.tblis_matrix_mult(None, get_config(config) if config else None, C.byref(a), C.byref(b),
+                        C.byref(c))
+    _check("tblis_matrix_mult")
     return C_
 
 

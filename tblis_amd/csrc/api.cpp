@@ -76,6 +76,22 @@ void init_tensor(tblis_tensor* t, type_t type, T scalar, unsigned ndim, len_type
     t->stride = stride;
 }
 
+template <typename T>
+void init_matrix(tblis_matrix* mat, type_t type, T scalar, len_type m, len_type n, void* data,
+                 stride_type rs, stride_type cs)
+{
+    std::memset(&mat->scalar, 0, sizeof(mat->scalar));
+    mat->type = type;
+    mat->conj = 0;
+    std::memcpy(&mat->scalar.data, &scalar, sizeof(T));
+    mat->scalar.type = type;
+    mat->data = data;
+    mat->m = m;
+    mat->n = n;
+    mat->rs = rs;
+    mat->cs = cs;
+}
+
 }  // namespace
 
 extern "C"
@@ -147,6 +163,54 @@ void tblis_init_tensor_z(tblis_tensor* t, unsigned ndim, len_type* len, dcomplex
                          stride_type* stride)
 {
     init_tensor(t, TYPE_DCOMPLEX, dcomplex(1.0, 0.0), ndim, len, data, stride);
+}
+
+void tblis_init_matrix_scaled_s(tblis_matrix* mat, float scalar, len_type m, len_type n, float* data,
+                                stride_type rs, stride_type cs)
+{
+    init_matrix(mat, TYPE_SINGLE, scalar, m, n, data, rs, cs);
+}
+void tblis_init_matrix_scaled_d(tblis_matrix* mat, double scalar, len_type m, len_type n, double* data,
+                                stride_type rs, stride_type cs)
+{
+    init_matrix(mat, TYPE_DOUBLE, scalar, m, n, data, rs, cs);
+}
+void tblis_init_matrix_scaled_c(tblis_matrix* mat, scomplex scalar, len_type m, len_type n,
+                                scomplex* data, stride_type rs, stride_type cs)
+{
+    init_matrix(mat, TYPE_SCOMPLEX, scalar, m, n, data, rs, cs);
+}
+void tblis_init_matrix_scaled_z(tblis_matrix* mat, dcomplex scalar, len_type m, len_type n,
+                                dcomplex* data, stride_type rs, stride_type cs)
+{
+    init_matrix(mat, TYPE_DCOMPLEX, scalar, m, n, data, rs, cs);
+}
+void tblis_init_matrix_s(tblis_matrix* mat, len_type m, len_type n, float* data, stride_type rs,
+                         stride_type cs)
+{
+    init_matrix(mat, TYPE_SINGLE, 1.0f, m, n, data, rs, cs);
+}
+void tblis_init_matrix_d(tblis_matrix* mat, len_type m, len_type n, double* data, stride_type rs,
+                         stride_type cs)
+{
+    init_matrix(mat, TYPE_DOUBLE, 1.0, m, n, data, rs, cs);
+}
+void tblis_init_matrix_c(tblis_matrix* mat, len_type m, len_type n, scomplex* data, stride_type rs,
+                         stride_type cs)
+{
+    init_matrix(mat, TYPE_SCOMPLEX, scomplex(1.0f, 0.0f), m, n, data, rs, cs);
+}
+void tblis_init_matrix_z(tblis_matrix* mat, len_type m, len_type n, dcomplex* data, stride_type rs,
+                         stride_type cs)
+{
+    init_matrix(mat, TYPE_DCOMPLEX, dcomplex(1.0, 0.0), m, n, data, rs, cs);
+}
+
+void tblis_matrix_mult(const tblis_comm* comm, const tblis_config* cfg, const tblis_matrix* A,
+                       const tblis_matrix* B, tblis_matrix* C)
+{
+    (void)comm;
+    guarded_void([&] { matrix_mult(cfg, A, B, C); });
 }
 
 void tblis_tensor_mult(const tblis_comm* comm, const tblis_config* cfg, const tblis_tensor* A,

@@ -261,6 +261,39 @@ void tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_typ
     reset_scalar(C);
 }
 
+// Matrix multiplication (src/iface/3m/mult.cxx:19-191) is the contraction
+// C[ab] = A[ak] B[kb] with each matrix's (rs, cs) as its two strides; the
+// tensor front end then covers the reference's special cases (empty C,
+// alpha == 0 or k == 0 scaling C by beta, length-1 dimensions).
+void matrix_mult(const tblis_config* cfg, const tblis_matrix* A, const tblis_matrix* B,
+                 tblis_matrix* C)
+{
+    if (!A || !B || !C) fail("matrix mult: null descriptor");
+    if (A->m != C->m || B->n != C->n || A->n != B->m)
+        fail("matrix mult: shapes do not agree (need A.m == C.m, B.n == C.n, A.n == B.m)");
+    if (A->type != B->type || A->type != C->type) fail("A, B and C must have the same type");
+    if (A->conj || B->conj || C->conj) fail("matrix mult: conjugation is not supported");
+    if (A->m < 0 || A->n < 0 || B->n < 0) fail("matrix mult: negative length");
+    len_type la[2] = {A->m, A->n}, lb[2] = {B->m, B->n}, lc[2] = {C->m, C->n};
+    stride_type sa[2] = {A->rs, A->cs}, sb[2] = {B->rs, B->cs}, sc[2] = {C->rs, C->cs};
+    tblis_tensor ta, tb, tc;
+    auto make = [](tblis_tensor& t, const tblis_matrix* m, len_type* len, stride_type* str) {
+        t.type = m->type;
+        t.conj = 0;
+        t.scalar = m->scalar;
+        t.data = m->data;
+        t.ndim = 2;
+        t.len = len;
+        t.stride = str;
+    };
+    make(ta, A, la, sa);
+    make(tb, B, lb, sb);
+    make(tc, C, lc, sc);
+    tensor_mult(cfg, &ta, "ak", &tb, "kb", &tc, "ab");
+    C->scalar = tc.scalar;
+    C->conj = 0;
+}
+
 void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tensor* const* A,
                        const label_type* const* idx_A, const tblis_tensor* const* B,
                        const label_type* const* idx_B, tblis_tensor* const* C,

@@ -15,6 +15,10 @@ void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tenso
                        const label_type* const* idx_B, tblis_tensor* const* C,
                        const label_type* const* idx_C);
 
+// tblis_matrix_mult as a two-dimensional tensor contraction.
+void matrix_mult(const tblis_config* cfg, const tblis_matrix* A, const tblis_matrix* B,
+                 tblis_matrix* C);
+
 void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const label_type* idx_A,
                      const tblis_dpd_tensor* B, const label_type* idx_B, tblis_dpd_tensor* C,
                      const label_type* idx_C);

@@ -44,6 +44,56 @@ def test_abi_layout():
     assert T.Scalar.type.offset == 16
 
 
+def test_matrix_abi_layout_and_init():
+    """tblis_matrix: 72 bytes, type@0 conj@4 scalar@8 data@32 m@40 n@48 rs@56
+    cs@64 (src/util/basic_types.h:352-428) -- checked in ctypes and by
+    compiling the header."""
+    assert C.sizeof(T.Matrix) == 72
+    assert (T.Matrix.scalar.offset, T.Matrix.data.offset, T.Matrix.m.offset, T.Matrix.n.offset,
+            T.Matrix.rs.offset, T.Matrix.cs.offset) == (8, 32, 40, 48, 56, 64)
+    src = os.path.join(ROOT, "_build_abi_probe.c")
+    exe = os.path.join(ROOT, "_build_abi_probe")
+    try:
+        with open(src, "w") as f:
+            f.write('#include <stddef.h>\n#include <stdio.h>\n#include "tblis.h"\n'
+                    'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(tblis_matrix),'
+                    'offsetof(tblis_matrix, scalar), offsetof(tblis_matrix, data), offsetof(tblis_matrix, m),'
+                    'offsetof(tblis_matrix, cs), sizeof(tblis_tensor), offsetof(tblis_tensor, stride));'
+                    'return 0;}\n')
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+        assert [int(x) for x in out] == [72, 8, 32, 40, 64, 64, 56]
+    finally:
+        for f in (src, exe):
+            if os.path.exists(f):
+                os.remove(f)
+    L = T.lib()
+    m = T.Matrix()
+    L.tblis_init_matrix_scaled_d(C.byref(m), C.c_double(-1.5), 3, 4, None, 1, 3)
+    assert (m.type, m.conj, m.scalar.data.d, m.m, m.n, m.rs, m.cs) == (T.TYPE_DOUBLE, 0, -1.5, 3, 4, 1, 3)
+    L.tblis_init_matrix_s(C.byref(m), 5, 2, None, 2, 1)
+    assert (m.type, m.scalar.data.s, m.m, m.n, m.rs, m.cs) == (T.TYPE_SINGLE, 1.0, 5, 2, 2, 1)
+
+
+def test_matrix_mult_argument_errors():
+    """Shape and conjugation errors are reported before any device access."""
+    L = T.lib()
+    L.tblis_hip_set_error_mode(1)
+    try:
+        a, b, c = T.Matrix(), T.Matrix(), T.Matrix()
+        L.tblis_init_matrix_d(C.byref(a), 3, 4, None, 1, 3)
+        L.tblis_init_matrix_d(C.byref(b), 5, 2, None, 1, 5)
+        L.tblis_init_matrix_d(C.byref(c), 3, 2, None, 1, 3)
+        L.tblis_matrix_mult(None, None, C.byref(a), C.byref(b), C.byref(c))
+        assert "shapes do not agree" in T.last_error()
+        L.tblis_init_matrix_d(C.byref(b), 4, 2, None, 1, 4)
+        a.conj = 1
+        L.tblis_matrix_mult(None, None, C.byref(a), C.byref(b), C.byref(c))
+        assert "conjugation" in T.last_error()
+    finally:
+        L.tblis_hip_set_error_mode(0)
+
+
 def test_init_tensor_and_threads():
     L = T.lib()
     lens = (C.c_ssize_t * 2)(3, 4)

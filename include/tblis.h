@@ -173,6 +173,24 @@ void tblis_init_matrix_z(tblis_matrix* mat, len_type m, len_type n, dcomplex* da
 void tblis_matrix_mult(const tblis_comm* comm, const tblis_config* cfg, const tblis_matrix* A,
                        const tblis_matrix* B, tblis_matrix* C);
 
+/* Level-1 operations around a contraction (src/iface/1t/add.h,
+ * scale.h, set.h):
+ *   add    B <- alpha_A * A + alpha_B * B over label-matched indices; labels
+ *          only in A are summed, labels only in B are replicated (not both),
+ *          repeated labels take a diagonal; alpha_A == 0 does not read A and
+ *          alpha_B == 0 writes B without reading it
+ *          (src/iface/1t/add.cxx:24-110)
+ *   scale  A <- alpha_A * A (src/iface/1t/scale.cxx:20-55)
+ *   set    A <- *alpha on every element the labels address
+ *          (src/iface/1t/set.cxx:20-51)
+ * Afterwards the written tensor's scalar is 1 and conj is 0. */
+void tblis_tensor_add(const tblis_comm* comm, const tblis_config* cfg, const tblis_tensor* A,
+                      const label_type* idx_A, tblis_tensor* B, const label_type* idx_B);
+void tblis_tensor_scale(const tblis_comm* comm, const tblis_config* cfg, tblis_tensor* A,
+                        const label_type* idx_A);
+void tblis_tensor_set(const tblis_comm* comm, const tblis_config* cfg, const tblis_scalar* alpha,
+                      tblis_tensor* A, const label_type* idx_A);
+
 /* Thread-count controls (src/util/thread.h:21-23).  The GPU path does not
  * use host threads; the value is stored and returned for compatibility. */
 unsigned tblis_get_num_threads(void);

@@ -1590,3 +1590,88 @@ void orc_fill_uniform_d(double* x, int64_t n, uint64_t seed)
         x[k] = 2.0 * ((double)(y >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
     }
 }
+
+
+/* ------------------------------------------------------------------ */
+/* Level-1 operations                                                   */
+/* ------------------------------------------------------------------ */
+
+/* Distinct labels of the kept (length != 1) dimensions of B, then those of
+ * A not in B; B's elements are enumerated in an odometer over the first
+ * group, A's sum over the second. */
+static void level1(int mode, double alpha, int ndA, const int64_t* lenA, const int64_t* strA,
+                   const char* idxA, const double* Ad, const float* As, double beta, int ndB,
+                   const int64_t* lenB, const int64_t* strB, const char* idxB, const double* Bd_in,
+                   const float* Bs_in, double* B_out)
+{
+    char lab[64];
+    int64_t len[64];
+    int nb = 0, nt = 0;
+    for (int i = 0; i < ndB; i++) {
+        if (lenB[i] == 1) continue;
+        int seen = 0;
+        for (int j = 0; j < nb; j++) seen |= lab[j] == idxB[i];
+        if (!seen) { lab[nb] = idxB[i]; len[nb] = lenB[i]; nb++; }
+    }
+    nt = nb;
+    if (mode == 0)
+        for (int i = 0; i < ndA; i++) {
+            if (lenA[i] == 1) continue;
+            int seen = 0;
+            for (int j = 0; j < nt; j++) seen |= lab[j] == idxA[i];
+            if (!seen) { lab[nt] = idxA[i]; len[nt] = lenA[i]; nt++; }
+        }
+    int64_t nB = 1, nS = 1;
+    for (int j = 0; j < nb; j++) nB *= len[j];
+    for (int j = nb; j < nt; j++) nS *= len[j];
+    int64_t val[64];
+    for (int64_t e = 0; e < nB; e++) {
+        int64_t r = e;
+        for (int j = 0; j < nb; j++) { val[j] = r % len[j]; r /= len[j]; }
+        int64_t offB = 0;
+        for (int i = 0; i < ndB; i++) {
+            if (lenB[i] == 1) continue;
+            for (int j = 0; j < nb; j++) if (lab[j] == idxB[i]) offB += val[j] * strB[i];
+        }
+        double v;
+        if (mode == 2) {
+            v = alpha;
+        } else {
+            double old = 0.0;
+            if (beta != 0.0) old = Bd_in ? Bd_in[offB] : (double)Bs_in[offB];
+            if (mode == 1) {
+                v = beta * old;
+            } else {
+                long double sum = 0.0L;
+                for (int64_t f = 0; f < nS; f++) {
+                    int64_t q = f;
+                    for (int j = nb; j < nt; j++) { val[j] = q % len[j]; q /= len[j]; }
+                    int64_t offA = 0;
+                    for (int i = 0; i < ndA; i++) {
+                        if (lenA[i] == 1) continue;
+                        for (int j = 0; j < nt; j++) if (lab[j] == idxA[i]) offA += val[j] * strA[i];
+                    }
+                    sum += Ad ? (long double)Ad[offA] : (long double)As[offA];
+                }
+                v = (double)((long double)alpha * sum);
+                if (beta != 0.0) v += beta * old;
+            }
+        }
+        B_out[offB] = v;
+    }
+}
+
+void orc_level1_d(int mode, double alpha, int ndA, const int64_t* lenA, const int64_t* strA,
+                  const char* idxA, const double* A, double beta, int ndB, const int64_t* lenB,
+                  const int64_t* strB, const char* idxB, double* B)
+{
+    level1(mode, alpha, ndA, lenA, strA, idxA, A, NULL, beta, ndB, lenB, strB, idxB, B, NULL, B);
+}
+
+void orc_level1_s(int mode, double alpha, int ndA, const int64_t* lenA, const int64_t* strA,
+                  const char* idxA, const float* A, double beta, int ndB, const int64_t* lenB,
+                  const int64_t* strB, const char* idxB, const float* B_in, double* B_out)
+{
+    level1(mode, alpha, ndA, lenA, strA, idxA, NULL, A, beta, ndB, lenB, strB, idxB, NULL, B_in,
+           B_out);
+}

@@ -146,6 +146,25 @@ int orc_dpd_patch_scatter(const orc_dpd* dA, const char* idx_A, const orc_dpd* d
                           int64_t max_scat);
 
 /* ------------------------------------------------------------------ */
+/* Level-1 operations (src/iface/1t/add.cxx:24-110, scale.cxx:20-55,    */
+/* set.cxx:20-51, src/internal/1t/dense/add.cxx:15-160)                 */
+/* ------------------------------------------------------------------ */
+/* mode 0 (add):   B <- alpha * sum_{labels only in A} A + beta * B
+ * mode 1 (scale): B <- beta * B            (A unused)
+ * mode 2 (set):   B <- alpha               (A unused)
+ * over the distinct labels of B (a repeated label addresses a diagonal).
+ * Length-1 dimensions are dropped first, as the reference's diagonal()
+ * does, so a label of length 1 in one tensor only replicates or sums over
+ * nothing.  beta == 0 writes B without reading it.  The value type is
+ * double; the _s variant reads FP32 A / B_in and writes double B_out. */
+void orc_level1_d(int mode, double alpha, int ndA, const int64_t* lenA, const int64_t* strA,
+                  const char* idxA, const double* A, double beta, int ndB, const int64_t* lenB,
+                  const int64_t* strB, const char* idxB, double* B);
+void orc_level1_s(int mode, double alpha, int ndA, const int64_t* lenA, const int64_t* strA,
+                  const char* idxA, const float* A, double beta, int ndB, const int64_t* lenB,
+                  const int64_t* strB, const char* idxB, const float* B_in, double* B_out);
+
+/* ------------------------------------------------------------------ */
 /* Deterministic inputs                                                 */
 /* ------------------------------------------------------------------ */
 void orc_fill_uniform_d(double* x, int64_t n, uint64_t seed);

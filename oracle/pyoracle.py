@@ -292,6 +292,27 @@ def dpd_patch_scatter(dA, idxA, dB, idxB, dC, idxC, which, irrep_ab, BS_row, BS_
             "base": pb[:n].copy(), "rscat": rs[:nr].copy(), "cscat": cs[:nc].copy()}
 
 
+def level1(mode, alpha, A, lenA, strA, idxA, beta, B, lenB, strB, idxB):
+    """Level-1 add (mode 0), scale (1), set (2) on host arrays; returns the
+    new B storage (float64)."""
+    L = lib()
+    la, pla = _i64(list(lenA) or [0])
+    sa, psa = _i64(list(strA) or [0])
+    lb, plb = _i64(lenB)
+    sb, psb = _i64(strB)
+    if B.dtype == np.float64:
+        out = B.copy()
+        Ap = A.ctypes.data_as(PD) if A is not None else None
+        L.orc_level1_d(mode, C.c_double(alpha), len(lenA), pla, psa, _lab(idxA), Ap,
+                       C.c_double(beta), len(lenB), plb, psb, _lab(idxB), out.ctypes.data_as(PD))
+        return out
+    out = B.astype(np.float64)
+    Ap = A.ctypes.data_as(PF) if A is not None else None
+    L.orc_level1_s(mode, C.c_double(alpha), len(lenA), pla, psa, _lab(idxA), Ap, C.c_double(beta),
+                   len(lenB), plb, psb, _lab(idxB), B.ctypes.data_as(PF), out.ctypes.data_as(PD))
+    return out
+
+
 def fill_uniform(n, seed):
     x = np.empty(n, np.float64)
     lib().orc_fill_uniform_d(x.ctypes.data_as(PD), I64(n), C.c_uint64(seed))

@@ -63,7 +63,8 @@ EXPORTED = [
     "tblis_init_tensor_d", "tblis_init_tensor_c", "tblis_init_tensor_z", "tblis_tensor_mult",
     "tblis_init_matrix_scaled_s", "tblis_init_matrix_scaled_d", "tblis_init_matrix_scaled_c",
     "tblis_init_matrix_scaled_z", "tblis_init_matrix_s", "tblis_init_matrix_d", "tblis_init_matrix_c",
-    "tblis_init_matrix_z", "tblis_matrix_mult",
+    "tblis_init_matrix_z", "tblis_matrix_mult", "tblis_tensor_add", "tblis_tensor_scale",
+    "tblis_tensor_set",
     "tblis_get_num_threads", "tblis_set_num_threads", "tblis_get_config",
     "tblis_init_dpd_tensor_d", "tblis_init_dpd_tensor_s", "tblis_dpd_tensor_size",
     "tblis_dpd_tensor_mult", "tblis_tensor_mult_batch", "tblis_hip_set_stream",
@@ -88,6 +89,9 @@ def lib():
                                         C.c_char_p]
         L.tblis_dpd_tensor_mult.argtypes = [C.c_void_p, C.c_void_p, D, C.c_char_p, D, C.c_char_p, D,
                                             C.c_char_p]
+        L.tblis_tensor_add.argtypes = [C.c_void_p, C.c_void_p, T, C.c_char_p, T, C.c_char_p]
+        L.tblis_tensor_scale.argtypes = [C.c_void_p, C.c_void_p, T, C.c_char_p]
+        L.tblis_tensor_set.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Scalar), T, C.c_char_p]
         M = C.POINTER(Matrix)
         L.tblis_matrix_mult.argtypes = [C.c_void_p, C.c_void_p, M, M, M]
         L.tblis_tensor_mult_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_ssize_t, P(T),
@@ -249,6 +253,40 @@ def mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None):
     except TblisError as e:
         raise TblisError("tblis_tensor_mult: %s" % e) from None
     return C_
+
+
+def add(alpha, A, idx_A, beta, B, idx_B):
+    """B <- alpha * A + beta * B over labels (tblis::add): labels only in A
+    are summed, labels only in B replicated."""
+    dA, dB = _Desc(A, alpha), _Desc(B, beta)
+    L = lib()
+    L.tblis_hip_set_error_mode(1)
+    L.tblis_tensor_add(None, None, C.byref(dA.d), _labels(idx_A, A.dim(), "A"), C.byref(dB.d),
+                       _labels(idx_B, B.dim(), "B"))
+    _check("tblis_tensor_add")
+    return B
+
+
+def scale(alpha, A, idx_A):
+    """A <- alpha * A (tblis::scale)."""
+    dA = _Desc(A, alpha)
+    L = lib()
+    L.tblis_hip_set_error_mode(1)
+    L.tblis_tensor_scale(None, None, C.byref(dA.d), _labels(idx_A, A.dim(), "A"))
+    _check("tblis_tensor_scale")
+    return A
+
+
+def set(alpha, A, idx_A):
+    """A <- alpha on every element its labels address (tblis::set)."""
+    dA = _Desc(A)
+    s = Scalar()
+    _set_scalar(s, alpha, dA.d.type)
+    L = lib()
+    L.tblis_hip_set_error_mode(1)
+    L.tblis_tensor_set(None, None, C.byref(s), C.byref(dA.d), _labels(idx_A, A.dim(), "A"))
+    _check("tblis_tensor_set")
+    return A
 
 
 def _matrix(t, scalar):

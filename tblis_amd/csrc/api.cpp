@@ -9,6 +9,7 @@
 #include "contract.hpp"
 #include "dpd.hpp"
 #include "kernels.hpp"
+#include "level1.hpp"
 #include "panel.hpp"
 #include "plan.hpp"
 
@@ -211,6 +212,27 @@ void tblis_matrix_mult(const tblis_comm* comm, const tblis_config* cfg, const tb
 {
     (void)comm;
     guarded_void([&] { matrix_mult(cfg, A, B, C); });
+}
+
+void tblis_tensor_add(const tblis_comm* comm, const tblis_config* cfg, const tblis_tensor* A,
+                      const label_type* idx_A, tblis_tensor* B, const label_type* idx_B)
+{
+    (void)comm;
+    guarded_void([&] { tensor_add(cfg, A, idx_A, B, idx_B); });
+}
+
+void tblis_tensor_scale(const tblis_comm* comm, const tblis_config* cfg, tblis_tensor* A,
+                        const label_type* idx_A)
+{
+    (void)comm;
+    guarded_void([&] { tensor_scale(cfg, A, idx_A); });
+}
+
+void tblis_tensor_set(const tblis_comm* comm, const tblis_config* cfg, const tblis_scalar* alpha,
+                      tblis_tensor* A, const label_type* idx_A)
+{
+    (void)comm;
+    guarded_void([&] { tensor_set(cfg, alpha, A, idx_A); });
 }
 
 void tblis_tensor_mult(const tblis_comm* comm, const tblis_config* cfg, const tblis_tensor* A,

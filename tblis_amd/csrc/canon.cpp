@@ -162,6 +162,82 @@ void rotate_to_second(Group& g, int u) { std::rotate(g.begin() + 1, g.begin() + 
 
 }  // namespace
 
+bool canonicalize_add(const tblis_tensor& A, const label_type* idx_A, const tblis_tensor& B,
+                      const label_type* idx_B, Level1Form& out, std::string& why)
+{
+    std::vector<TDim> da, db;
+    if (!take_diagonal(A, idx_A, da, why) || !take_diagonal(B, idx_B, db, why)) return false;
+    const std::vector<char> la = sorted_labels(da), lb = sorted_labels(db);
+    const std::vector<char> ab = set_and(la, lb), ao = set_minus(la, ab), bo = set_minus(lb, ab);
+    if (!ao.empty() && !bo.empty())
+    {
+        why = "labels present only in A and labels present only in B";
+        return false;
+    }
+    out = Level1Form();
+    for (char l : ab)
+    {
+        const TDim *x = find(da, l), *y = find(db, l);
+        if (x->len != y->len)
+        {
+            why = std::string("label '") + l + "' has different lengths in A and B";
+            return false;
+        }
+        Dim d{};
+        d.label = l;
+        d.len = x->len;
+        d.s[0] = x->stride;
+        d.s[1] = y->stride;
+        out.ab.push_back(d);
+    }
+    for (char l : ao)
+    {
+        const TDim* x = find(da, l);
+        Dim d{};
+        d.label = l;
+        d.len = x->len;
+        d.s[0] = x->stride;
+        d.s[1] = 0;
+        out.a_only.push_back(d);
+    }
+    for (char l : bo)
+    {
+        const TDim* y = find(db, l);
+        Dim d{};
+        d.label = l;
+        d.len = y->len;
+        d.s[0] = 0;
+        d.s[1] = y->stride;
+        out.b_only.push_back(d);
+    }
+    // fold (src/iface/1t/add.cxx:78-80): AB by A's strides over both
+    // tensors, A-only by A's, B-only by B's
+    fold_group(out.ab, 2);
+    fold_group(out.a_only, 1);
+    for (auto& d : out.b_only) std::swap(d.s[0], d.s[1]);
+    fold_group(out.b_only, 1);
+    for (auto& d : out.b_only) std::swap(d.s[0], d.s[1]);
+    return true;
+}
+
+bool canonicalize_one(const tblis_tensor& A, const label_type* idx_A, Group& out, std::string& why)
+{
+    std::vector<TDim> da;
+    if (!take_diagonal(A, idx_A, da, why)) return false;
+    out.clear();
+    for (auto& x : da)
+    {
+        Dim d{};
+        d.label = x.label;
+        d.len = x.len;
+        d.s[0] = x.stride;
+        out.push_back(d);
+    }
+    fold_group(out, 1);
+    for (auto& d : out) std::swap(d.s[0], d.s[1]);
+    return true;
+}
+
 bool canonicalize(const tblis_tensor& A, const label_type* idx_A, const tblis_tensor& B,
                   const label_type* idx_B, const tblis_tensor& C, const label_type* idx_C,
                   Canonical& out, std::string& why)

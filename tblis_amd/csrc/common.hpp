@@ -81,6 +81,18 @@ inline Side side_of(const Group& g, int slot, bool pack_3d)
 }
 
 // canon.cpp
+// Level-1 form of B <- alpha A + beta B (src/iface/1t/add.cxx:24-110):
+// groups of labels in both tensors (s[0] = A stride, s[1] = B stride),
+// only in A (summed, s[0]) and only in B (replicated, s[1]), each folded.
+struct Level1Form
+{
+    Group ab, a_only, b_only;
+};
+bool canonicalize_add(const tblis_tensor& A, const label_type* idx_A, const tblis_tensor& B,
+                      const label_type* idx_B, Level1Form& out, std::string& why);
+// One tensor (scale, set): diagonal taken, length-1 dims dropped, folded (s[1]).
+bool canonicalize_one(const tblis_tensor& A, const label_type* idx_A, Group& out, std::string& why);
+
 bool canonicalize(const tblis_tensor& A, const label_type* idx_A, const tblis_tensor& B,
                   const label_type* idx_B, const tblis_tensor& C, const label_type* idx_C,
                   Canonical& out, std::string& why);

@@ -26,6 +26,8 @@ void tensor_key(std::string& k, const tblis_tensor& t, const label_type* idx)
     k.append(idx, t.ndim);
 }
 
+}  // namespace
+
 int elem_size(type_t t)
 {
     switch (t)
@@ -99,6 +101,9 @@ void check_tensor(const tblis_tensor* t, const label_type* idx, const char* name
         fail(std::string(name) + ": null length, stride or label array");
     if (t->ndim > 32) fail(std::string(name) + ": too many dimensions");
 }
+
+namespace
+{
 
 // Odometer offsets of every position of a (len, stride) list.
 std::vector<int64_t> all_offsets(const Group& g, int slot)

@@ -34,6 +34,16 @@ int debug_dpd_patches(const tblis_dpd_tensor* A, const label_type* idx_A,
                       int64_t* rows, int64_t* cols, int64_t* base, int64_t* rscat,
                       int64_t* cscat, int64_t max_patch, int64_t max_scat);
 
+// Descriptor helpers shared by the entry points.
+int elem_size(type_t t);
+void check_real(type_t t);                                  // complex types: error
+double scalar_value(const tblis_scalar& s, type_t t);       // as double
+double product_scalar(const tblis_scalar& a, const tblis_scalar& b, type_t t);
+void reset_scalar(tblis_tensor* C);                         // scalar = 1, conj = 0
+void tensor_extent(const tblis_tensor& t, int64_t& lo, int64_t& hi);
+void check_data(const tblis_tensor* t, const char* name);   // device memory, in bounds
+void check_tensor(const tblis_tensor* t, const label_type* idx, const char* name);
+
 // Selected algorithm for a type under a config.
 Algo resolve_algo(const tblis_config* cfg, type_t type);
 // Kernel family (plan_types.hpp FAMILY_*) of an algorithm.

@@ -1,0 +1,38 @@
+// Level-1 tensor operations (add, scale, set): host entry points and the
+// kernel descriptor.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace tblis_amd
+{
+
+constexpr int LEVEL1_MAXDIM = 16;
+constexpr int LEVEL1_ADD = 0, LEVEL1_SCALE = 1, LEVEL1_SET = 2;
+
+// B's iteration dimensions (fastest first) with B's and A's strides, and
+// the A-only dimensions summed per element.
+struct Level1Desc
+{
+    int ndb = 0, nda = 0;
+    int64_t nb = 1, na = 1;  // element counts (product of lengths)
+    int64_t len_b[LEVEL1_MAXDIM], sb[LEVEL1_MAXDIM], sa_b[LEVEL1_MAXDIM];
+    int64_t len_a[LEVEL1_MAXDIM], sa_a[LEVEL1_MAXDIM];
+};
+
+hipError_t launch_level1(int mode, int dtype, const Level1Desc& d, const void* A, void* B,
+                         double alpha, double beta, double value, hipStream_t s);
+
+// tblis_tensor_add / _scale / _set (src/iface/1t/{add,scale,set}.cxx).
+void tensor_add(const tblis_config* cfg, const tblis_tensor* A, const label_type* idx_A,
+                tblis_tensor* B, const label_type* idx_B);
+void tensor_scale(const tblis_config* cfg, tblis_tensor* A, const label_type* idx_A);
+void tensor_set(const tblis_config* cfg, const tblis_scalar* alpha, tblis_tensor* A,
+                const label_type* idx_A);
+
+}  // namespace tblis_amd

@@ -348,3 +348,29 @@ def test_dpd_patch_scatter_bit_exact(seed):
             assert got["npatch"] == want["npatch"]
             for k in ("rows", "cols", "base", "rscat", "cscat"):
                 assert (got[k] == want[k]).all(), (which, r, k)
+
+
+def test_level1_argument_errors():
+    """Level-1 label errors are reported before any device access."""
+    L = T.lib()
+    L.tblis_hip_set_error_mode(1)
+    try:
+        a, b = _HostDesc([3, 4], [1, 3]), _HostDesc([3, 5], [1, 3])
+        L.tblis_tensor_add(None, None, C.byref(a.d), b"ab", C.byref(b.d), b"ac")
+        assert "only in A and labels present only in B" in T.last_error()
+        b = _HostDesc([3, 5], [1, 3])
+        L.tblis_tensor_add(None, None, C.byref(a.d), b"ab", C.byref(b.d), b"ab")
+        assert "different lengths" in T.last_error()
+        a = _HostDesc([3, 4], [1, 3])
+        L.tblis_tensor_scale(None, None, C.byref(a.d), b"aa")
+        assert "unequal lengths" in T.last_error()
+        s = T.Scalar()
+        s.type = T.TYPE_SINGLE
+        s.data.s = 1.0
+        L.tblis_tensor_set(None, None, C.byref(s), C.byref(a.d), b"ab")
+        assert "type differs" in T.last_error()
+        a.d.type = T.TYPE_DCOMPLEX
+        L.tblis_tensor_scale(None, None, C.byref(a.d), b"ab")
+        assert "complex" in T.last_error().lower()
+    finally:
+        L.tblis_hip_set_error_mode(0)

@@ -356,3 +356,75 @@ def test_dpd_patch_scatter_addresses(seed):
             for idx in itertools.product(*[range(l) for l in ln]):
                 want.append(off + sum(i * s for i, s in zip(idx, st)))
         assert sorted(addr) == sorted(want)
+
+
+# ---------------------------------------------------------------- level-1
+def _l1(mode, alpha, A, lA, sA, iA, beta, B, lB, sB, iB):
+    return po.level1(mode, alpha, A, lA, sA, iA, beta, B, lB, sB, iB)
+
+
+def _colmaj(l):
+    s, a = [], 1
+    for x in l:
+        s.append(a)
+        a *= x
+    return s
+
+
+def test_level1_add_against_einsum():
+    """The level-1 oracle against numpy on transposes, sums over labels only
+    A has, replication over labels only B has, and beta = 0 with NaN in B."""
+    rng = np.random.default_rng(60)
+    cases = [("abc", "cab", dict(a=3, b=4, c=5)), ("abc", "ca", dict(a=3, b=4, c=5)),
+             ("ab", "abc", dict(a=3, b=4, c=2)), ("ab", "ba", dict(a=6, b=7))]
+    for iA, iB, ext in cases:
+        lA, lB = [ext[c] for c in iA], [ext[c] for c in iB]
+        A = rng.uniform(-1, 1, int(np.prod(lA)))
+        B = rng.uniform(-1, 1, int(np.prod(lB)))
+        Ar = A.reshape(lA[::-1]).transpose(range(len(lA))[::-1])  # column-major view
+        Br = B.reshape(lB[::-1]).transpose(range(len(lB))[::-1])
+        a_only = "".join(c for c in iA if c not in iB)
+        red = np.einsum(iA + "->" + "".join(c for c in iA if c in iB), Ar)
+        kept = "".join(c for c in iA if c in iB)
+        # broadcast the reduced A over B's labels
+        want = 0.5 * np.einsum(kept + "->" + kept, red)
+        want = np.broadcast_to(np.einsum(kept + "->" + "".join(c for c in iB if c in kept), want)
+                               .reshape([ext[c] if c in kept else 1 for c in iB]), lB) + 2.0 * Br
+        got = _l1(0, 0.5, A, lA, _colmaj(lA), iA, 2.0, B, lB, _colmaj(lB), iB)
+        gotr = got.reshape(lB[::-1]).transpose(range(len(lB))[::-1])
+        np.testing.assert_allclose(gotr, want, rtol=1e-13, atol=1e-13)
+    # beta == 0 overwrites (NaN in B does not propagate)
+    B = np.full(6, np.nan)
+    A = rng.uniform(-1, 1, 6)
+    got = _l1(0, 1.0, A, [2, 3], [1, 2], "ab", 0.0, B, [2, 3], [1, 2], "ab")
+    np.testing.assert_array_equal(got, A)
+
+
+def test_level1_diagonals_scale_set_and_unit_lengths():
+    rng = np.random.default_rng(61)
+    # B[aa] = A[a] + B[aa]: only the diagonal changes
+    A = rng.uniform(-1, 1, 4)
+    B = rng.uniform(-1, 1, 16)
+    got = _l1(0, 1.0, A, [4], [1], "a", 1.0, B, [4, 4], [1, 4], "aa")
+    want = B.copy()
+    want[[0, 5, 10, 15]] += A
+    np.testing.assert_allclose(got, want, rtol=1e-15)
+    # A[aab] summed to B[ab]: diagonal of A
+    A = rng.uniform(-1, 1, 3 * 3 * 2)
+    B = np.zeros(6)
+    got = _l1(0, 1.0, A, [3, 3, 2], [1, 3, 9], "aab", 0.0, B, [3, 2], [1, 3], "ab")
+    Ar = A.reshape(2, 3, 3).transpose(2, 1, 0)
+    np.testing.assert_allclose(got.reshape(2, 3).T, np.einsum("aab->ab", Ar), rtol=1e-15)
+    # a label of length 1 in A only is dropped: A[ax] with x=1 equals A[a]
+    A = rng.uniform(-1, 1, 5)
+    B = np.zeros(5)
+    got = _l1(0, 2.0, A, [5, 1], [1, 5], "ax", 0.0, B, [5], [1], "a")
+    np.testing.assert_allclose(got, 2.0 * A, rtol=1e-15)
+    # scale and set over a diagonal
+    B = rng.uniform(-1, 1, 9)
+    got = _l1(1, 0.0, None, [], [], "", -3.0, B, [3, 3], [1, 3], "aa")
+    want = B.copy()
+    want[[0, 4, 8]] *= -3.0
+    np.testing.assert_allclose(got, want, rtol=1e-15)
+    got = _l1(2, 7.0, None, [], [], "", 0.0, B, [3, 3], [1, 3], "ab")
+    np.testing.assert_array_equal(got, np.full(9, 7.0))

@@ -24,6 +24,10 @@ makes it the headline):
   batch        4096 independent C[abij] = A[abef]*B[efij], extents drawn from
                4..40 per instance, FP64 and split-BF16, one hipGraph replay
                per step
+  level1       tblis_tensor_add / _scale on FP64 96^4 tensors (679 MB each):
+               permuted add, same-layout add, sums over the slow and over
+               the fast dimensions, scale; GB/s of algorithmic bytes against
+               the 8 TB/s HBM peak
 Panel tiles are picked by the library's autotuner (on-device timing when a
 plan is first built, during warm-up) unless --no-autotune.
 """
@@ -41,6 +45,7 @@ sys.path.insert(0, ROOT)
 METRIC = "useful TFLOP/s + %MFMA peak; split-BF16 vs FP64-MFMA at stated rel error"
 PEAK_FP64 = 78.6    # TFLOP/s, MI355X dense FP64 matrix (datasheet)
 PEAK_BF16 = 2500.0  # TFLOP/s, MI355X dense BF16 matrix (datasheet)
+PEAK_HBM = 8000.0   # GB/s, MI355X HBM3E (datasheet)
 SPLIT_PRODUCTS = 6  # BF16 MFMA products per split-BF16 multiply-add
 
 
@@ -255,6 +260,38 @@ def batch_leg(T, dev, stream, steps, warmup, count=4096, seed=7):
     return out
 
 
+# ---------------------------------------------------------------- level-1
+
+def level1_leg(T, dev, stream, steps, warmup):
+    import torch
+    L = [96] * 4
+    n = 96 ** 4
+    g = torch.Generator(device=dev).manual_seed(11)
+    A = strided(torch.rand(n, dtype=torch.float64, device=dev, generator=g) * 2 - 1, L)
+    B = strided(torch.rand(n, dtype=torch.float64, device=dev, generator=g) * 2 - 1, L)
+    B2 = strided(torch.zeros(96 * 96, dtype=torch.float64, device=dev), [96, 96])
+    e = 8
+    ops = (
+        ("permuted_add", "B[cdab] = A[abcd] + B[cdab]", lambda: T.add(1.0, A, "abcd", 1.0, B, "cdab"),
+         3 * n * e),
+        ("add", "B[abcd] = A[abcd] + B[abcd]", lambda: T.add(1.0, A, "abcd", 1.0, B, "abcd"), 3 * n * e),
+        ("sum_slow", "B[ab] = sum_cd A[abcd]", lambda: T.add(1.0, A, "abcd", 0.0, B2, "ab"),
+         (n + 96 * 96) * e),
+        ("sum_fast", "B[cd] = sum_ab A[abcd]", lambda: T.add(1.0, A, "abcd", 0.0, B2, "cd"),
+         (n + 96 * 96) * e),
+        ("scale", "A[abcd] *= 0.999", lambda: T.scale(0.999, A, "abcd"), 2 * n * e),
+    )
+    out = {"workload": "FP64 extents 96 (column-major 96^4 = 679 MB per tensor): tblis_tensor_add "
+                       "and tblis_tensor_scale", "peak_gbps": PEAK_HBM}
+    for name, what, fn, nbytes in ops:
+        ms_step, ms_kern, ms_min = time_steps(fn, steps, warmup, stream)
+        gbps = nbytes / (ms_kern * 1e-3) / 1e9
+        out[name] = {"op": what, "ms": round(ms_step, 4), "call_ms_events": round(ms_kern, 4),
+                     "algorithmic_bytes": nbytes, "gbps": round(gbps, 1),
+                     "frac_hbm": round(gbps / PEAK_HBM, 4)}
+    return out
+
+
 # ---------------------------------------------------------------- traffic
 
 COPY_MIB = 256
@@ -340,7 +377,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--only", choices=["permuted", "fp32_96", "dpd", "batch"],
+    ap.add_argument("--only", choices=["permuted", "fp32_96", "dpd", "batch", "level1"],
                     help="run one leg alone (for profiling) and report it as the headline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-traffic", action="store_true",
@@ -389,6 +426,10 @@ def main():
         legs["batch"] = batch_leg(T, dev, stream, few if args.only is None else args.steps,
                                   min(args.warmup, 2))
         torch.cuda.empty_cache()
+    if args.only in (None, "level1"):
+        legs["level1"] = level1_leg(T, dev, stream, few if args.only is None else args.steps,
+                                    min(args.warmup, 2))
+        torch.cuda.empty_cache()
     T.set_async(False)
     if args.calibrate_copy:
         n = args.calibrate_copy * (1 << 20) // 8
@@ -398,6 +439,11 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
 
+    if head is None and args.only == "level1":
+        leg = legs["level1"]["permuted_add"]
+        head = {"value": leg["gbps"], "ms_per_step": leg["ms"], "dtype": "f64", "unit": "GB/s",
+                "metric": "level-1 permuted add, GB/s of algorithmic bytes (HBM-bound)",
+                "config": {"workload": legs["level1"]["workload"] + "; headline " + leg["op"]}}
     if head is None:
         leg = legs[args.only]
         first = "split_bf16" if "split_bf16" in leg else next(k for k in leg if isinstance(leg[k], dict))
@@ -405,9 +451,9 @@ def main():
                 "dtype": "f32" if first == "split_bf16" else "f64",
                 "config": {"workload": leg["workload"], "algorithm": leg[first]["algorithm"]}}
     out = {
-        "metric": METRIC,
+        "metric": head.get("metric", METRIC),
         "value": head["value"],
-        "unit": "TFLOP/s",
+        "unit": head.get("unit", "TFLOP/s"),
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,

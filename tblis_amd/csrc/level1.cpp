@@ -40,9 +40,14 @@ void run(int mode, type_t type, const Level1Desc& d, const void* A, void* B, dou
          double beta, double value)
 {
     ThreadState& ts = thread_state();
-    hip_check(launch_level1(mode, type == TYPE_DOUBLE ? 0 : 1, d, A, B, alpha, beta, value, ts.stream),
+    const size_t wsb = level1_workspace_bytes(mode, d);
+    void* ws = wsb ? workspace(wsb, ts.stream) : nullptr;
+    hip_check(launch_level1(mode, type == TYPE_DOUBLE ? 0 : 1, d, A, B, alpha, beta, value, ws, ts.stream),
               "level-1 kernel launch");
-    if (!ts.async) hip_check(hipStreamSynchronize(ts.stream), "hipStreamSynchronize");
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    hip_check(hipStreamIsCapturing(ts.stream, &cap), "hipStreamIsCapturing");
+    if (!ts.async && cap == hipStreamCaptureStatusNone)
+        hip_check(hipStreamSynchronize(ts.stream), "hipStreamSynchronize");
 }
 
 }  // namespace

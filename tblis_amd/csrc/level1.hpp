@@ -25,8 +25,10 @@ struct Level1Desc
     int64_t len_a[LEVEL1_MAXDIM], sa_a[LEVEL1_MAXDIM];
 };
 
+// Device workspace a call needs (partial sums of a split A-only sum), bytes.
+size_t level1_workspace_bytes(int mode, const Level1Desc& d);
 hipError_t launch_level1(int mode, int dtype, const Level1Desc& d, const void* A, void* B,
-                         double alpha, double beta, double value, hipStream_t s);
+                         double alpha, double beta, double value, void* ws, hipStream_t s);
 
 // tblis_tensor_add / _scale / _set (src/iface/1t/{add,scale,set}.cxx).
 void tensor_add(const tblis_config* cfg, const tblis_tensor* A, const label_type* idx_A,

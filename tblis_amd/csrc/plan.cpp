@@ -530,9 +530,6 @@ void check_device_pointer(const void* p, const char* what, int64_t lo, int64_t h
     }
 }
 
-namespace
-{
-
 // Grow-only device workspace per (device, stream), for packed operands.
 // Calls on one stream are ordered, so they can share it.  A buffer that is
 // outgrown is retired, not freed: a captured graph may still refer to it.
@@ -560,6 +557,9 @@ void* workspace(size_t bytes, hipStream_t s)
     }
     return e.ptr;
 }
+
+namespace
+{
 
 // Plans used while a stream was being captured (never released).
 void pin_plan(const std::shared_ptr<Plan>& p)

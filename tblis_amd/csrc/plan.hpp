@@ -140,6 +140,9 @@ void cache_clear();
 // During stream capture (hipGraph) the call never waits, and the plan is
 // pinned for the life of the process: a captured graph keeps referring to
 // its device tables after the cache would have evicted them.
+// Grow-only device workspace per (device, stream); calls on one stream share it.
+void* workspace(size_t bytes, hipStream_t s);
+
 void run_plan(const std::shared_ptr<Plan>& p, const void* A, const void* B, void* C, double alpha,
               double beta, bool use_k);
 

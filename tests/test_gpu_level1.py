@@ -79,6 +79,13 @@ def test_add_transpose_reduce_replicate(dtype):
     _add_case(rng, dtype, [20, 31, 17, 5], "abcd", [17, 20], "ca", 1.5, 1.0, padA=1)
     # replicate over a label only B has
     _add_case(rng, dtype, [11, 6], "ab", [11, 6, 9], "abc", -0.75, 2.0, permB=[2, 1, 0])
+    # transpose through LDS with partial tiles and an outer dimension
+    _add_case(rng, dtype, [100, 70, 3], "abc", [70, 3, 100], "bca", 1.0, 1.0, padA=1)
+    # one wave per B element over three A-only dimensions
+    _add_case(rng, dtype, [40, 6, 9, 5], "abcd", [9], "c", 0.5, -1.0, padB=2)
+    # long strided A-only sums over few B elements: split into chunks
+    _add_case(rng, dtype, [10, 9, 40, 30], "abcd", [9, 10], "ba", 1.0, 0.5)
+    _add_case(rng, dtype, [50, 3000], "ab", [50], "a", 2.0, 1.0, padA=3)
     # everything summed into a scalar
     _add_case(rng, dtype, [64, 33, 5], "abc", [], "", 1.0, 0.5)
     # a large plain copy-add (many blocks, grid stride)

@@ -53,73 +53,103 @@ __global__ __launch_bounds__(256) void pack_panel_kernel(const PanelJob* __restr
         // k-fast source: per pair of k groups, four loads of four rows x 16
         // consecutive k (whole 128-byte lines), transposed through a per-wave
         // LDS tile into the fragment order.  The k tables are unit-stride.
+        // The loads of kBatch pairs are issued before the first transpose,
+        // so a wave waits on memory once per batch rather than per pair.
+        constexpr int kBatch = 4;
         __shared__ double tile[4][16][17];
         const int npair = (ch.nkg + 1) / 2;
+        const int niter = ch.nrb * npair;
         const int kk = lane & 15;
-        for (int g = wave; g < ch.nrb * npair; g += 4)
+        for (int g0 = wave; g0 < niter; g0 += 4 * kBatch)
         {
-            const int rb = ch.rb0 + g / npair;
-            const int kg = ch.kg0 + 2 * (g % npair);
-            const bool two = kg + 1 < ch.kg0 + ch.nkg;
-            const int64_t rbase = pool[j.rowtab + 2 * (rb - rbf)];
-            const int64_t rstr = pool[j.rowtab + 2 * (rb - rbf) + 1];
-            const int h = kk >> 3;
-            const int64_t kb = pool[j.ktab + 2 * (kg + (two ? h : 0) - kgf)];
-            const int pk = kg * 8 + kk - j.k0;  // job-relative k
-            const bool kin = pk >= 0 && pk < j.nk && (h == 0 || two);
-            const int64_t koff = kb + (kk & 7);
+            double v[kBatch][4];
 #pragma unroll
-            for (int jj = 0; jj < 4; jj++)
+            for (int u = 0; u < kBatch; u++)
             {
-                const int r = (lane >> 4) + 4 * jj;
-                const int prow = rb * 16 + r - j.r0;
-                double v = 0.0;
-                if (kin && prow >= 0 && prow < j.nr)
+                const int g = g0 + 4 * u;
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++) v[u][jj] = 0.0;
+                if (g >= niter) continue;
+                const int rb = ch.rb0 + g / npair;
+                const int kg = ch.kg0 + 2 * (g % npair);
+                const bool two = kg + 1 < ch.kg0 + ch.nkg;
+                const int64_t rbase = pool[j.rowtab + 2 * (rb - rbf)];
+                const int64_t rstr = pool[j.rowtab + 2 * (rb - rbf) + 1];
+                const int h = kk >> 3;
+                const int64_t kb = pool[j.ktab + 2 * (kg + (two ? h : 0) - kgf)];
+                const int pk = kg * 8 + kk - j.k0;  // job-relative k
+                const bool kin = pk >= 0 && pk < j.nk && (h == 0 || two);
+                const int64_t koff = kb + (kk & 7);
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++)
                 {
-                    const int64_t roff = rstr != SCATTERED ? rbase + r * rstr : pool[j.rowmap + prow];
-                    v = (double)X[j.src + roff + koff];
-                }
-                tile[wave][r][kk] = v;
-            }
-            __builtin_amdgcn_wave_barrier();
-            const int row = rb * 16 + lr - j.r0;
-            const bool rin = row >= 0 && row < j.nr;
-            for (int hh = 0; hh < (two ? 2 : 1); hh++)
-            {
-                const int k0 = (kg + hh) * 8 + lk - j.k0, k1 = k0 + 4;
-                const bool in0 = rin && k0 >= 0 && k0 < j.nk;
-                const bool in1 = rin && k1 >= 0 && k1 < j.nk;
-                const double v0 = tile[wave][lr][8 * hh + lk], v1 = tile[wave][lr][8 * hh + 4 + lk];
-                double* out = ws + j.out + ((int64_t)rb * j.pnkg + kg + hh) * 128 + lane * 2;
-                if (in0 && in1)
-                {
-                    double2 v;
-                    v.x = v0;
-                    v.y = v1;
-                    *(double2*)out = v;
-                }
-                else
-                {
-                    if (in0) out[0] = v0;
-                    if (in1) out[1] = v1;
+                    const int r = (lane >> 4) + 4 * jj;
+                    const int prow = rb * 16 + r - j.r0;
+                    if (kin && prow >= 0 && prow < j.nr)
+                    {
+                        const int64_t roff = rstr != SCATTERED ? rbase + r * rstr : pool[j.rowmap + prow];
+                        v[u][jj] = (double)X[j.src + roff + koff];
+                    }
                 }
             }
-            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int u = 0; u < kBatch; u++)
+            {
+                const int g = g0 + 4 * u;
+                if (g >= niter) break;
+                const int rb = ch.rb0 + g / npair;
+                const int kg = ch.kg0 + 2 * (g % npair);
+                const bool two = kg + 1 < ch.kg0 + ch.nkg;
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++) tile[wave][(lane >> 4) + 4 * jj][kk] = v[u][jj];
+                __builtin_amdgcn_wave_barrier();
+                const int row = rb * 16 + lr - j.r0;
+                const bool rin = row >= 0 && row < j.nr;
+                for (int hh = 0; hh < (two ? 2 : 1); hh++)
+                {
+                    const int k0 = (kg + hh) * 8 + lk - j.k0, k1 = k0 + 4;
+                    const bool in0 = rin && k0 >= 0 && k0 < j.nk;
+                    const bool in1 = rin && k1 >= 0 && k1 < j.nk;
+                    const double v0 = tile[wave][lr][8 * hh + lk], v1 = tile[wave][lr][8 * hh + 4 + lk];
+                    double* out = ws + j.out + ((int64_t)rb * j.pnkg + kg + hh) * 128 + lane * 2;
+                    if (in0 && in1)
+                    {
+                        double2 w;
+                        w.x = v0;
+                        w.y = v1;
+                        *(double2*)out = w;
+                    }
+                    else
+                    {
+                        if (in0) out[0] = v0;
+                        if (in1) out[1] = v1;
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
         }
         return;
     }
-    for (int g = wave; g < ng; g += 4)
+    // general source: one fragment group per iteration, the loads of
+    // kBatch groups issued before their stores
+    constexpr int kBatch = 8;
+    for (int g0 = wave; g0 < ng; g0 += 4 * kBatch)
     {
-        const int rb = ch.rb0 + g / ch.nkg;
-        const int kg = ch.kg0 + g % ch.nkg;
-        const int row = rb * 16 + lr - j.r0;
-        const bool rin = row >= 0 && row < j.nr;
-        const int k0 = kg * 8 + lk - j.k0, k1 = k0 + 4;
-        const bool in0 = rin && k0 >= 0 && k0 < j.nk;
-        const bool in1 = rin && k1 >= 0 && k1 < j.nk;
-        double v0 = 0.0, v1 = 0.0;
-        if (!zero)
+        double v0[kBatch], v1[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++)
         {
+            const int g = g0 + 4 * u;
+            v0[u] = 0.0;
+            v1[u] = 0.0;
+            if (g >= ng || zero) continue;
+            const int rb = ch.rb0 + g / ch.nkg;
+            const int kg = ch.kg0 + g % ch.nkg;
+            const int row = rb * 16 + lr - j.r0;
+            const bool rin = row >= 0 && row < j.nr;
+            const int k0 = kg * 8 + lk - j.k0, k1 = k0 + 4;
+            const bool in0 = rin && k0 >= 0 && k0 < j.nk;
+            const bool in1 = rin && k1 >= 0 && k1 < j.nk;
             // group tables: wave-uniform (base, stride) of this row / k group
             const int64_t rbase = pool[j.rowtab + 2 * (rb - rbf)];
             const int64_t rstr = pool[j.rowtab + 2 * (rb - rbf) + 1];
@@ -127,25 +157,37 @@ __global__ __launch_bounds__(256) void pack_panel_kernel(const PanelJob* __restr
             const int64_t kstr = pool[j.ktab + 2 * (kg - kgf) + 1];
             if (rin)
             {
-                const int64_t roff =
-                    j.src + (rstr != SCATTERED ? rbase + lr * rstr : pool[j.rowmap + row]);
-                if (in0) v0 = (double)X[roff + (kstr != SCATTERED ? kbase + lk * kstr : pool[j.kmap + k0])];
+                const int64_t roff = j.src + (rstr != SCATTERED ? rbase + lr * rstr : pool[j.rowmap + row]);
+                if (in0) v0[u] = (double)X[roff + (kstr != SCATTERED ? kbase + lk * kstr : pool[j.kmap + k0])];
                 if (in1)
-                    v1 = (double)X[roff + (kstr != SCATTERED ? kbase + (lk + 4) * kstr : pool[j.kmap + k1])];
+                    v1[u] = (double)X[roff + (kstr != SCATTERED ? kbase + (lk + 4) * kstr : pool[j.kmap + k1])];
             }
         }
-        double* out = ws + j.out + ((int64_t)rb * j.pnkg + kg) * 128 + lane * 2;
-        if (in0 && in1)
+#pragma unroll
+        for (int u = 0; u < kBatch; u++)
         {
-            double2 v;
-            v.x = v0;
-            v.y = v1;
-            *(double2*)out = v;
-        }
-        else
-        {
-            if (in0) out[0] = v0;
-            if (in1) out[1] = v1;
+            const int g = g0 + 4 * u;
+            if (g >= ng) break;
+            const int rb = ch.rb0 + g / ch.nkg;
+            const int kg = ch.kg0 + g % ch.nkg;
+            const int row = rb * 16 + lr - j.r0;
+            const bool rin = row >= 0 && row < j.nr;
+            const int k0 = kg * 8 + lk - j.k0, k1 = k0 + 4;
+            const bool in0 = rin && k0 >= 0 && k0 < j.nk;
+            const bool in1 = rin && k1 >= 0 && k1 < j.nk;
+            double* out = ws + j.out + ((int64_t)rb * j.pnkg + kg) * 128 + lane * 2;
+            if (in0 && in1)
+            {
+                double2 w;
+                w.x = v0[u];
+                w.y = v1[u];
+                *(double2*)out = w;
+            }
+            else
+            {
+                if (in0) out[0] = v0[u];
+                if (in1) out[1] = v1[u];
+            }
         }
     }
 }

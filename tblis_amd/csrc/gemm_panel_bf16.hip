@@ -59,128 +59,173 @@ __global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PanelJob* __
         // k-fast source: per pair of k groups, 16 loads of two rows x 32
         // consecutive k (whole 128-byte lines), transposed through a per-wave
         // LDS tile into the fragment order.  The k tables are unit-stride.
+        // The loads of kBatch pairs are issued before the first transpose.
+        constexpr int kBatch = 2;
         __shared__ float tile[4][32][33];
         const int npair = (ch.nkg + 1) / 2;
+        const int niter = ch.nrb * npair;
         const int kk = lane & 31;
-        for (int g = wave; g < ch.nrb * npair; g += 4)
+        for (int g0 = wave; g0 < niter; g0 += 4 * kBatch)
         {
-            const int rb = ch.rb0 + g / npair;
-            const int ks = ch.kg0 + 2 * (g % npair);
-            const bool two = ks + 1 < ch.kg0 + ch.nkg;
-            const int64_t rbase = pool[j.rowtab + 2 * (rb - rbf)];
-            const int64_t rstr = pool[j.rowtab + 2 * (rb - rbf) + 1];
-            const int h = kk >> 4;
-            const int64_t kb = pool[j.ktab + 2 * (ks + (two ? h : 0) - ksf)];
-            const int pk = ks * 16 + kk - j.k0;  // job-relative k
-            const bool kin = pk >= 0 && pk < j.nk && (h == 0 || two);
-            const int64_t koff = kb + (kk & 15);
+            float v[kBatch][16];
 #pragma unroll
-            for (int jj = 0; jj < 16; jj++)
+            for (int u = 0; u < kBatch; u++)
             {
-                const int r = (lane >> 5) + 2 * jj;
-                const int prow = rb * 32 + r - j.r0;
-                float v = 0.0f;
-                if (kin && prow >= 0 && prow < j.nr)
+                const int g = g0 + 4 * u;
+#pragma unroll
+                for (int jj = 0; jj < 16; jj++) v[u][jj] = 0.0f;
+                if (g >= niter) continue;
+                const int rb = ch.rb0 + g / npair;
+                const int ks = ch.kg0 + 2 * (g % npair);
+                const bool two = ks + 1 < ch.kg0 + ch.nkg;
+                const int64_t rbase = pool[j.rowtab + 2 * (rb - rbf)];
+                const int64_t rstr = pool[j.rowtab + 2 * (rb - rbf) + 1];
+                const int h = kk >> 4;
+                const int64_t kb = pool[j.ktab + 2 * (ks + (two ? h : 0) - ksf)];
+                const int pk = ks * 16 + kk - j.k0;  // job-relative k
+                const bool kin = pk >= 0 && pk < j.nk && (h == 0 || two);
+                const int64_t koff = kb + (kk & 15);
+#pragma unroll
+                for (int jj = 0; jj < 16; jj++)
                 {
-                    const int64_t roff = rstr != SCATTERED ? rbase + r * rstr : pool[j.rowmap + prow];
-                    v = X[j.src + roff + koff];
+                    const int r = (lane >> 5) + 2 * jj;
+                    const int prow = rb * 32 + r - j.r0;
+                    if (kin && prow >= 0 && prow < j.nr)
+                    {
+                        const int64_t roff = rstr != SCATTERED ? rbase + r * rstr : pool[j.rowmap + prow];
+                        v[u][jj] = X[j.src + roff + koff];
+                    }
                 }
-                tile[wave][r][kk] = v;
             }
-            __builtin_amdgcn_wave_barrier();
-            const int row = rb * 32 + lr - j.r0;
-            const bool rin = row >= 0 && row < j.nr;
-            for (int hh = 0; hh < (two ? 2 : 1); hh++)
-            {
-                const int kb0 = (ks + hh) * 16 + lh * 8 - j.k0;
-                bf16x8 hv, mv, lv;
-                unsigned inmask = 0;
 #pragma unroll
-                for (int e = 0; e < 8; e++)
+            for (int u = 0; u < kBatch; u++)
+            {
+                const int g = g0 + 4 * u;
+                if (g >= niter) break;
+                const int rb = ch.rb0 + g / npair;
+                const int ks = ch.kg0 + 2 * (g % npair);
+                const bool two = ks + 1 < ch.kg0 + ch.nkg;
+#pragma unroll
+                for (int jj = 0; jj < 16; jj++) tile[wave][(lane >> 5) + 2 * jj][kk] = v[u][jj];
+                __builtin_amdgcn_wave_barrier();
+                const int row = rb * 32 + lr - j.r0;
+                const bool rin = row >= 0 && row < j.nr;
+                for (int hh = 0; hh < (two ? 2 : 1); hh++)
                 {
-                    const int k = kb0 + e;
-                    const bool in = rin && k >= 0 && k < j.nk;
-                    __bf16 a, b, c;
-                    split3(tile[wave][lr][16 * hh + 8 * lh + e], a, b, c);
-                    hv[e] = a;
-                    mv[e] = b;
-                    lv[e] = c;
-                    inmask |= (unsigned)in << e;
-                }
-                __bf16* out = ws + j.out + ((int64_t)rb * j.pnkg + ks + hh) * 1536 + lane * 8;
-                if (inmask == 0xffu)
-                {
-                    *(bf16x8*)out = hv;
-                    *(bf16x8*)(out + 512) = mv;
-                    *(bf16x8*)(out + 1024) = lv;
-                }
-                else if (inmask)
-                {
+                    const int kb0 = (ks + hh) * 16 + lh * 8 - j.k0;
+                    bf16x8 hv, mv, lv;
+                    unsigned inmask = 0;
 #pragma unroll
                     for (int e = 0; e < 8; e++)
-                        if (inmask >> e & 1u)
-                        {
-                            out[e] = hv[e];
-                            out[512 + e] = mv[e];
-                            out[1024 + e] = lv[e];
-                        }
+                    {
+                        const int k = kb0 + e;
+                        const bool in = rin && k >= 0 && k < j.nk;
+                        __bf16 a, b, c;
+                        split3(tile[wave][lr][16 * hh + 8 * lh + e], a, b, c);
+                        hv[e] = a;
+                        mv[e] = b;
+                        lv[e] = c;
+                        inmask |= (unsigned)in << e;
+                    }
+                    __bf16* out = ws + j.out + ((int64_t)rb * j.pnkg + ks + hh) * 1536 + lane * 8;
+                    if (inmask == 0xffu)
+                    {
+                        *(bf16x8*)out = hv;
+                        *(bf16x8*)(out + 512) = mv;
+                        *(bf16x8*)(out + 1024) = lv;
+                    }
+                    else if (inmask)
+                    {
+#pragma unroll
+                        for (int e = 0; e < 8; e++)
+                            if (inmask >> e & 1u)
+                            {
+                                out[e] = hv[e];
+                                out[512 + e] = mv[e];
+                                out[1024 + e] = lv[e];
+                            }
+                    }
                 }
+                __builtin_amdgcn_wave_barrier();
             }
-            __builtin_amdgcn_wave_barrier();
         }
         return;
     }
-    for (int g = wave; g < ng; g += 4)
+    // general source: one fragment group per iteration, the loads of
+    // kBatch groups issued before their stores
+    constexpr int kBatch = 4;
+    for (int g0 = wave; g0 < ng; g0 += 4 * kBatch)
     {
-        const int rb = ch.rb0 + g / ch.nkg;
-        const int ks = ch.kg0 + g % ch.nkg;
-        const int row = rb * 32 + lr - j.r0;
-        const bool rin = row >= 0 && row < j.nr;
-        const int kb = ks * 16 + lh * 8 - j.k0;
-        int64_t roff = 0, kbase = 0, kstr = 0;
-        if (!zero)
+        float v[kBatch][8];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++)
         {
+            const int g = g0 + 4 * u;
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[u][e] = 0.0f;
+            if (g >= ng || zero) continue;
+            const int rb = ch.rb0 + g / ch.nkg;
+            const int ks = ch.kg0 + g % ch.nkg;
+            const int row = rb * 32 + lr - j.r0;
+            const bool rin = row >= 0 && row < j.nr;
+            const int kb = ks * 16 + lh * 8 - j.k0;
             // group tables: wave-uniform (base, stride) of this row / k group
             const int64_t rbase = pool[j.rowtab + 2 * (rb - rbf)];
             const int64_t rstr = pool[j.rowtab + 2 * (rb - rbf) + 1];
-            kbase = pool[j.ktab + 2 * (ks - ksf)];
-            kstr = pool[j.ktab + 2 * (ks - ksf) + 1];
-            if (rin) roff = j.src + (rstr != SCATTERED ? rbase + lr * rstr : pool[j.rowmap + row]);
-        }
-        bf16x8 h, m, l;
-        unsigned inmask = 0;
-#pragma unroll
-        for (int e = 0; e < 8; e++)
-        {
-            const int k = kb + e;
-            const bool in = rin && k >= 0 && k < j.nk;
-            float v = 0.0f;
-            if (in && !zero)
-                v = X[roff + (kstr != SCATTERED ? kbase + (lh * 8 + e) * kstr : pool[j.kmap + k])];
-            __bf16 a, b, c;
-            split3(v, a, b, c);
-            h[e] = a;
-            m[e] = b;
-            l[e] = c;
-            inmask |= (unsigned)in << e;
-        }
-        __bf16* out = ws + j.out + ((int64_t)rb * j.pnkg + ks) * 1536 + lane * 8;
-        if (inmask == 0xffu)
-        {
-            *(bf16x8*)out = h;
-            *(bf16x8*)(out + 512) = m;
-            *(bf16x8*)(out + 1024) = l;
-        }
-        else if (inmask)
-        {
+            const int64_t kbase = pool[j.ktab + 2 * (ks - ksf)];
+            const int64_t kstr = pool[j.ktab + 2 * (ks - ksf) + 1];
+            if (!rin) continue;
+            const int64_t roff = j.src + (rstr != SCATTERED ? rbase + lr * rstr : pool[j.rowmap + row]);
 #pragma unroll
             for (int e = 0; e < 8; e++)
-                if (inmask >> e & 1u)
-                {
-                    out[e] = h[e];
-                    out[512 + e] = m[e];
-                    out[1024 + e] = l[e];
-                }
+            {
+                const int k = kb + e;
+                if (k >= 0 && k < j.nk)
+                    v[u][e] = X[roff + (kstr != SCATTERED ? kbase + (lh * 8 + e) * kstr : pool[j.kmap + k])];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; u++)
+        {
+            const int g = g0 + 4 * u;
+            if (g >= ng) break;
+            const int rb = ch.rb0 + g / ch.nkg;
+            const int ks = ch.kg0 + g % ch.nkg;
+            const int row = rb * 32 + lr - j.r0;
+            const bool rin = row >= 0 && row < j.nr;
+            const int kb = ks * 16 + lh * 8 - j.k0;
+            bf16x8 h, m, l;
+            unsigned inmask = 0;
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+            {
+                const int k = kb + e;
+                const bool in = rin && k >= 0 && k < j.nk;
+                __bf16 a, b, c;
+                split3(v[u][e], a, b, c);
+                h[e] = a;
+                m[e] = b;
+                l[e] = c;
+                inmask |= (unsigned)in << e;
+            }
+            __bf16* out = ws + j.out + ((int64_t)rb * j.pnkg + ks) * 1536 + lane * 8;
+            if (inmask == 0xffu)
+            {
+                *(bf16x8*)out = h;
+                *(bf16x8*)(out + 512) = m;
+                *(bf16x8*)(out + 1024) = l;
+            }
+            else if (inmask)
+            {
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                    if (inmask >> e & 1u)
+                    {
+                        out[e] = h[e];
+                        out[512 + e] = m[e];
+                        out[1024 + e] = l[e];
+                    }
+            }
         }
     }
 }

@@ -28,6 +28,7 @@ namespace
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 constexpr int64_t kDirectRowFast = 24;  // re-read limit (tiles) of rows-fast direct operands
+constexpr int kXcds = 8;                 // MI355X: 8 XCDs of 32 CUs, one L2 each
 
 void hip_check(hipError_t e, const char* what)
 {
@@ -132,6 +133,69 @@ Geom geom_of(int family)
 // A job whose k groups are unit-stride and whose row groups are not: the
 // pack kernels read it row by row (whole 128-byte lines) and transpose
 // through LDS.
+// Workgroup i of a launch runs on XCD i % kXcds (round-robin dispatch), and
+// each XCD has its own L2.  Tasks that share panels are grouped into regions
+// (rectangles of one GEMM's tile grid); regions are dealt to XCDs longest
+// first onto the least loaded XCD; each XCD's list runs longest task first;
+// the lists are interleaved so that position i lands on XCD i % kXcds.
+// TBLIS_HIP_XCD_ORDER=0 keeps the plain longest-first order.
+std::vector<PanelTask> xcd_order(const std::vector<PanelTask>& tasks, const std::vector<int64_t>& cost,
+                                 const std::vector<int64_t>& region, int64_t nregions)
+{
+    std::vector<int> order(tasks.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+    auto by_cost = [&](int a, int b) { return cost[a] > cost[b]; };
+    std::vector<PanelTask> out;
+    out.reserve(tasks.size());
+    static const bool off = [] {
+        const char* e = std::getenv("TBLIS_HIP_XCD_ORDER");
+        return e && e[0] == '0';
+    }();
+    if (off)
+    {
+        std::stable_sort(order.begin(), order.end(), by_cost);
+        for (int i : order) out.push_back(tasks[i]);
+        return out;
+    }
+    std::vector<std::vector<int>> members(nregions);
+    std::vector<int64_t> rcost(nregions, 0);
+    for (int i : order)
+    {
+        members[region[i]].push_back(i);
+        rcost[region[i]] += cost[i];
+    }
+    std::vector<int64_t> rord(nregions);
+    for (int64_t r = 0; r < nregions; r++) rord[r] = r;
+    std::stable_sort(rord.begin(), rord.end(), [&](int64_t a, int64_t b) { return rcost[a] > rcost[b]; });
+    std::vector<std::vector<int>> lists(kXcds);
+    std::vector<int64_t> load(kXcds, 0);
+    for (int64_t r : rord)
+    {
+        if (members[r].empty()) continue;
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        lists[x].insert(lists[x].end(), members[r].begin(), members[r].end());
+        load[x] += rcost[r];
+    }
+    for (auto& l : lists) std::stable_sort(l.begin(), l.end(), by_cost);
+    // interleave; when a list runs out its slots go to the longest remaining ones
+    std::vector<size_t> pos(kXcds, 0);
+    while (out.size() < tasks.size())
+        for (int x = 0; x < kXcds && out.size() < tasks.size(); x++)
+        {
+            int pick = x;
+            if (pos[x] >= lists[x].size())
+            {
+                pick = -1;
+                for (int y = 0; y < kXcds; y++)
+                    if (pos[y] < lists[y].size() &&
+                        (pick < 0 || lists[y].size() - pos[y] > lists[pick].size() - pos[pick]))
+                        pick = y;
+            }
+            out.push_back(tasks[lists[pick][pos[pick]++]]);
+        }
+    return out;
+}
+
 bool kfast_job(Pool& pool, const PanelJob& j, const Geom& G)
 {
     const int64_t* pd = pool.data().data();
@@ -386,6 +450,8 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
     std::vector<PanelChunk>& chunks = T.chunks;
     std::vector<PanelTask>& tasks = T.tasks;
     std::vector<int64_t> cost;
+    std::vector<int64_t> region;  // XCD region of each task
+    int64_t nregions = 0;
     std::vector<UnpackJob>& ujobs = T.ujobs;
     std::vector<UnpackChunk>& uchunks = T.uchunks;
     int64_t ws = 0;   // panel values (family type)
@@ -401,14 +467,14 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
         const int64_t rb0 = j.r0 / G.gr, rb1 = ceil_div(j.r0 + j.nr, G.gr);
         const int64_t kg0 = j.k0 / G.gk, kg1 = ceil_div(j.k0 + j.nk, G.gk);
         for (int64_t rb = rb0; rb < rb1; rb += 4)
-            for (int64_t kg = kg0; kg < kg1; kg += 16)
+            for (int64_t kg = kg0; kg < kg1; kg += 8)
             {
                 PanelChunk c{};
                 c.job = id;
                 c.rb0 = (int32_t)rb;
                 c.nrb = (int32_t)std::min<int64_t>(4, rb1 - rb);
                 c.kg0 = (int32_t)kg;
-                c.nkg = (int32_t)std::min<int64_t>(16, kg1 - kg);
+                c.nkg = (int32_t)std::min<int64_t>(8, kg1 - kg);
                 chunks.push_back(c);
             }
     };
@@ -611,9 +677,29 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                         for (int32_t c = 0; c < u.nc; c += 64) uchunks.push_back(UnpackChunk{id, r, c, 0});
                 }
         }
+        // XCD regions: the tile grid cut into RM x RN rectangles (RM * RN <= 8),
+        // the cut that loads the fewest panel rows per region
+        const int64_t MT = mp / BM, NT = np / BN;
+        int64_t RM = 1, RN = 1;
+        {
+            int64_t best = -1;
+            for (int64_t rm = 1; rm <= kXcds; rm *= 2)
+            {
+                const int64_t rn = kXcds / rm;
+                if (rm > MT || rn > NT) continue;
+                const int64_t load = mp / rm + np / rn;
+                if (best < 0 || rm * rn > RM * RN || (rm * rn == RM * RN && load < best))
+                {
+                    best = load;
+                    RM = rm;
+                    RN = rn;
+                }
+            }
+        }
         for (int64_t tm = 0; tm < mp / BM; tm++)
             for (int64_t tn = 0; tn < np / BN; tn++)
             {
+                region.push_back(nregions + (tm * RM / MT) * RN + tn * RN / NT);
                 PanelTask t{};
                 const int64_t ge = (int64_t)G.terms * G.gr * G.gk;  // values per group
                 t.a = oa + tm * (BM / G.gr) * nkg * ge;
@@ -652,22 +738,14 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                 tasks.push_back(t);
                 cost.push_back(std::max<int64_t>(nkg, 1) * t.mlen * t.nlen);
             }
+        nregions += RM * RN;
     }
     if (tasks.empty()) return false;
     if (tasks.size() > (size_t)INT32_MAX || chunks.size() > (size_t)INT32_MAX ||
         uchunks.size() > (size_t)INT32_MAX)
         fail("contraction too large: panel task list exceeds 2^31 entries");
 
-    // longest tasks first (uneven batches and DPD irreps)
-    {
-        std::vector<int> order(tasks.size());
-        for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-        std::vector<PanelTask> t2;
-        t2.reserve(tasks.size());
-        for (int i : order) t2.push_back(tasks[i]);
-        tasks.swap(t2);
-    }
+    tasks = xcd_order(tasks, cost, region, nregions);
 
     T.ws_panel = ws;
     T.ws_c = wsc;

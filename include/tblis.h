@@ -255,6 +255,57 @@ void tblis_dpd_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
                            tblis_dpd_tensor* C, const label_type* idx_C);
 
 /* ---------------------------------------------------------------------- */
+/* Indexed tensors: C entry point for what the reference offers as the     */
+/* C++ template tblis::mult<T> over MArray indexed_varray_view             */
+/* (src/iface/3t/mult.h:74-86, src/iface/3t/mult.cxx:299-402).             */
+/* ---------------------------------------------------------------------- */
+
+/* The layout of MArray's indexed_varray (src/external/marray/include/
+ * indexed_varray_base.hpp:19-45): the first dense_ndim dimensions are dense
+ * and shared by every entry (lengths len[0..dense_ndim), strides
+ * dense_stride); the others are indexed.  Entry i is the dense sub-tensor at
+ * data[i] (device memory), placed at the index values
+ * indices[i * (ndim - dense_ndim) + j] of the indexed dimensions and
+ * multiplied by factor[i] (NULL: every factor is 1).  Positions that no
+ * entry lists are zero.  Entries must list distinct index values. */
+typedef struct tblis_indexed_tensor
+{
+    type_t type;
+    int conj;
+    tblis_scalar scalar;
+    unsigned ndim;
+    unsigned dense_ndim;
+    const len_type* len;
+    const stride_type* dense_stride;
+    len_type num_indices;
+    void* const* data;
+    const len_type* indices;
+    const void* factor;
+} tblis_indexed_tensor;
+
+void tblis_init_indexed_tensor_d(tblis_indexed_tensor* t, double scalar, unsigned ndim,
+                                 unsigned dense_ndim, const len_type* len,
+                                 const stride_type* dense_stride, len_type num_indices,
+                                 double* const* data, const len_type* indices, const double* factor);
+void tblis_init_indexed_tensor_s(tblis_indexed_tensor* t, float scalar, unsigned ndim,
+                                 unsigned dense_ndim, const len_type* len,
+                                 const stride_type* dense_stride, len_type num_indices,
+                                 float* const* data, const len_type* indices, const float* factor);
+
+/* Indexed contraction C <- alpha A.B + beta C with alpha = A.scalar *
+ * B.scalar and beta = C.scalar, following the reference
+ * (src/iface/3t/mult.cxx:299-402, src/internal/3t/indexed/mult.cxx:274-318):
+ * labels are unique within a tensor; first every C entry is scaled
+ * (beta == 0: set to 0; beta != 1: data *= factor * beta, or 0 when its
+ * factor is 0); then each C entry's data gains
+ * alpha * factor_C * (A.B) at its index values, where A and B are the
+ * full tensors their entries describe. */
+void tblis_indexed_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
+                               const tblis_indexed_tensor* A, const label_type* idx_A,
+                               const tblis_indexed_tensor* B, const label_type* idx_B,
+                               tblis_indexed_tensor* C, const label_type* idx_C);
+
+/* ---------------------------------------------------------------------- */
 /* Extension: a batch of independent contractions of uneven shapes, run   */
 /* in one persistent launch.  The reference has no counterpart; its        */
 /* equivalent is a host loop of tblis_tensor_mult calls.                   */

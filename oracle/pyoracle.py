@@ -313,6 +313,49 @@ def level1(mode, alpha, A, lenA, strA, idxA, beta, B, lenB, strB, idxB):
     return out
 
 
+# ---------------------------------------------------------------- indexed
+
+def indexed_full(entries, indices, factors, dense_len, idx_len):
+    """The full tensor an indexed tensor describes (dense dimensions first),
+    as block_to_full builds it (src/internal/1t/indexed/util.hpp:14-43):
+    each entry, times its factor, written at its index values."""
+    full = np.zeros(list(dense_len) + list(idx_len))
+    nd = len(dense_len)
+    for e, ix, f in zip(entries, indices, factors):
+        full[(slice(None),) * nd + tuple(ix)] = f * np.asarray(e, np.float64)
+    return full
+
+
+def indexed_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C):
+    """Indexed contraction on host arrays, the reference's FULL algorithm
+    (src/internal/3t/indexed/mult.cxx:15-59) behind its front end
+    (src/iface/3t/mult.cxx:299-402) and scaling rule
+    (src/internal/3t/indexed/mult.cxx:288-296, src/internal/1t/indexed/
+    scale.cxx:12-30, set.cxx:12-20).  Operands are dicts with "entries"
+    (dense numpy arrays, logical shape), "indices" (tuples), "factors",
+    "dense_len", "idx_len"; returns C's new entries (float64)."""
+    out = []
+    for e, f in zip(C_["entries"], C_["factors"]):
+        e = np.asarray(e, np.float64)
+        if beta == 0.0 or (beta != 1.0 and f == 0.0):
+            out.append(np.zeros_like(e))
+        elif beta != 1.0:
+            out.append(e * (f * beta))
+        else:
+            out.append(e.copy())
+    if alpha == 0.0:
+        return out
+    full = {}
+    for name, X in (("A", A), ("B", B)):
+        full[name] = indexed_full(X["entries"], X["indices"], X["factors"], X["dense_len"],
+                                  X["idx_len"])
+    prod = alpha * np.einsum("%s,%s->%s" % (idx_A, idx_B, idx_C), full["A"], full["B"])
+    nd = len(C_["dense_len"])
+    for i, (ix, f) in enumerate(zip(C_["indices"], C_["factors"])):
+        out[i] = out[i] + f * prod[(slice(None),) * nd + tuple(ix)]
+    return out
+
+
 def fill_uniform(n, seed):
     x = np.empty(n, np.float64)
     lib().orc_fill_uniform_d(x.ctypes.data_as(PD), I64(n), C.c_uint64(seed))

@@ -10,7 +10,8 @@ fallback: a missing library is an error.
 import ctypes as C
 import os
 
-__all__ = ["lib", "mult", "matrix_mult", "dpd_mult", "mult_batch", "prepare_mult", "prepare_dpd_mult",
+__all__ = ["lib", "mult", "matrix_mult", "dpd_mult", "mult_batch", "indexed_mult", "Indexed",
+           "prepare_indexed_mult", "prepare_mult", "prepare_dpd_mult",
            "prepare_batch", "Tensor", "DpdTensor", "get_config",
            "set_stream", "set_async", "last_error", "last_algorithm", "DPD_LAYOUTS", "TblisError"]
 
@@ -51,7 +52,16 @@ class DpdTensor(C.Structure):
                 ("len", C.POINTER(C.c_ssize_t))]
 
 
+class IndexedTensor(C.Structure):
+    _fields_ = [("type", C.c_int), ("conj", C.c_int), ("scalar", Scalar), ("ndim", C.c_uint),
+                ("dense_ndim", C.c_uint), ("len", C.POINTER(C.c_ssize_t)),
+                ("dense_stride", C.POINTER(C.c_ssize_t)), ("num_indices", C.c_ssize_t),
+                ("data", C.POINTER(C.c_void_p)), ("indices", C.POINTER(C.c_ssize_t)),
+                ("factor", C.c_void_p)]
+
+
 assert C.sizeof(Scalar) == 24 and C.sizeof(Tensor) == 64 and C.sizeof(Matrix) == 72
+assert C.sizeof(IndexedTensor) == 88
 
 _LIB = None
 
@@ -67,7 +77,8 @@ EXPORTED = [
     "tblis_tensor_set",
     "tblis_get_num_threads", "tblis_set_num_threads", "tblis_get_config",
     "tblis_init_dpd_tensor_d", "tblis_init_dpd_tensor_s", "tblis_dpd_tensor_size",
-    "tblis_dpd_tensor_mult", "tblis_tensor_mult_batch", "tblis_hip_set_stream",
+    "tblis_dpd_tensor_mult", "tblis_init_indexed_tensor_d", "tblis_init_indexed_tensor_s",
+    "tblis_indexed_tensor_mult", "tblis_tensor_mult_batch", "tblis_hip_set_stream",
     "tblis_hip_get_stream", "tblis_hip_set_async", "tblis_hip_set_error_mode",
     "tblis_hip_get_last_error", "tblis_hip_clear_plan_cache", "tblis_hip_last_algorithm",
     "tblis_hip_set_autotune", "tblis_hip_autotune_runs", "tblis_hip_last_tile",
@@ -89,6 +100,9 @@ def lib():
                                         C.c_char_p]
         L.tblis_dpd_tensor_mult.argtypes = [C.c_void_p, C.c_void_p, D, C.c_char_p, D, C.c_char_p, D,
                                             C.c_char_p]
+        IX = C.POINTER(IndexedTensor)
+        L.tblis_indexed_tensor_mult.argtypes = [C.c_void_p, C.c_void_p, IX, C.c_char_p, IX, C.c_char_p,
+                                                IX, C.c_char_p]
         L.tblis_tensor_add.argtypes = [C.c_void_p, C.c_void_p, T, C.c_char_p, T, C.c_char_p]
         L.tblis_tensor_scale.argtypes = [C.c_void_p, C.c_void_p, T, C.c_char_p]
         L.tblis_tensor_set.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Scalar), T, C.c_char_p]
@@ -366,6 +380,95 @@ def dpd_mult(alpha, A, irrep_A, idx_A, B, irrep_B, idx_B, beta, C_, irrep_C, idx
                          nirrep, lens_A, lens_B, lens_C, layout, config)()
     except TblisError as e:
         raise TblisError("tblis_dpd_tensor_mult: %s" % e) from None
+    return C_
+
+
+class Indexed:
+    """An indexed tensor (MArray indexed_varray): entries are torch tensors of
+    one dense shape and strides on the device; entry i sits at the index
+    values indices[i] (one per indexed dimension, of lengths idx_len) and is
+    multiplied by factors[i] (default 1)."""
+
+    def __init__(self, entries, indices, idx_len, factors=None, dtype=None):
+        self.entries = list(entries)
+        self.indices = [tuple(int(v) for v in ix) for ix in indices]
+        self.idx_len = [int(x) for x in idx_len]
+        self.factors = None if factors is None else [float(f) for f in factors]
+        if len(self.indices) != len(self.entries):
+            raise TblisError("Indexed: one index tuple per entry")
+        if self.entries:
+            e0 = self.entries[0]
+            for e in self.entries:
+                if tuple(e.shape) != tuple(e0.shape) or tuple(e.stride()) != tuple(e0.stride()):
+                    raise TblisError("Indexed: every entry must have the same shape and strides")
+                if e.dtype != e0.dtype:
+                    raise TblisError("Indexed: every entry must have the same dtype")
+            self.dtype = e0.dtype
+            self.dense_len = list(e0.shape)
+            self.dense_stride = list(e0.stride())
+        else:
+            if dtype is None:
+                raise TblisError("Indexed: dtype is needed when there are no entries")
+            self.dtype = dtype
+            self.dense_len, self.dense_stride = [], []
+
+    def ndim(self):
+        return len(self.dense_len) + len(self.idx_len)
+
+
+class _IxDesc:
+    """tblis_indexed_tensor over an Indexed (keeps the host arrays alive)."""
+
+    def __init__(self, X, scalar=1.0):
+        n, nd, nix = len(X.entries), X.ndim(), len(X.idx_len)
+        self.len = (C.c_ssize_t * max(nd, 1))(*(X.dense_len + X.idx_len))
+        self.stride = (C.c_ssize_t * max(len(X.dense_len), 1))(*X.dense_stride)
+        self.data = (C.c_void_p * max(n, 1))(*[e.data_ptr() for e in X.entries])
+        flat = [v for ix in X.indices for v in ix]
+        if any(len(ix) != nix for ix in X.indices):
+            raise TblisError("Indexed: each index tuple needs one value per indexed dimension")
+        self.idx = (C.c_ssize_t * max(len(flat), 1))(*flat)
+        self.d = IndexedTensor()
+        self.d.type = _type_code(X.dtype)
+        self.d.conj = 0
+        _set_scalar(self.d.scalar, scalar, self.d.type)
+        self.d.ndim = nd
+        self.d.dense_ndim = len(X.dense_len)
+        self.d.len = C.cast(self.len, C.POINTER(C.c_ssize_t))
+        self.d.dense_stride = C.cast(self.stride, C.POINTER(C.c_ssize_t))
+        self.d.num_indices = n
+        self.d.data = C.cast(self.data, C.POINTER(C.c_void_p))
+        self.d.indices = C.cast(self.idx, C.POINTER(C.c_ssize_t))
+        self.fac = None
+        if X.factors is not None:
+            ft = C.c_double if self.d.type == TYPE_DOUBLE else C.c_float
+            self.fac = (ft * max(n, 1))(*X.factors)
+            self.d.factor = C.cast(self.fac, C.c_void_p)
+
+
+def prepare_indexed_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None):
+    """Build the descriptors of indexed_mult() once; returns a callable."""
+    dA, dB, dC = _IxDesc(A, alpha), _IxDesc(B), _IxDesc(C_, beta)
+    cfg = get_config(config) if config else None
+    args = (None, cfg, C.byref(dA.d), _labels(idx_A, A.ndim(), "A"), C.byref(dB.d),
+            _labels(idx_B, B.ndim(), "B"), C.byref(dC.d), _labels(idx_C, C_.ndim(), "C"))
+    fn = lib().tblis_indexed_tensor_mult
+
+    def run():
+        _set_scalar(dC.d.scalar, beta, dC.d.type)
+        fn(*args)
+
+    return _Prepared(run, (), (dA, dB, dC))
+
+
+def indexed_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None):
+    """Indexed contraction (tblis::mult over indexed_varray_view,
+    src/iface/3t/mult.h:74-86) over Indexed operands; C_'s entries are
+    updated in place."""
+    try:
+        prepare_indexed_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config)()
+    except TblisError as e:
+        raise TblisError("tblis_indexed_tensor_mult: %s" % e) from None
     return C_
 
 

@@ -9,6 +9,7 @@
 #include "contract.hpp"
 #include "dpd.hpp"
 #include "kernels.hpp"
+#include "indexed.hpp"
 #include "level1.hpp"
 #include "panel.hpp"
 #include "plan.hpp"
@@ -311,6 +312,52 @@ void tblis_dpd_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
 {
     (void)comm;
     guarded_void([&] { dpd_tensor_mult(cfg, A, idx_A, B, idx_B, C, idx_C); });
+}
+
+static void init_indexed(tblis_indexed_tensor* t, type_t type, unsigned ndim, unsigned dense_ndim,
+                         const len_type* len, const stride_type* dense_stride, len_type num_indices,
+                         void* const* data, const len_type* indices, const void* factor)
+{
+    std::memset(t, 0, sizeof(*t));
+    t->type = type;
+    t->scalar.type = type;
+    t->ndim = ndim;
+    t->dense_ndim = dense_ndim;
+    t->len = len;
+    t->dense_stride = dense_stride;
+    t->num_indices = num_indices;
+    t->data = data;
+    t->indices = indices;
+    t->factor = factor;
+}
+
+void tblis_init_indexed_tensor_d(tblis_indexed_tensor* t, double scalar, unsigned ndim,
+                                 unsigned dense_ndim, const len_type* len,
+                                 const stride_type* dense_stride, len_type num_indices,
+                                 double* const* data, const len_type* indices, const double* factor)
+{
+    init_indexed(t, TYPE_DOUBLE, ndim, dense_ndim, len, dense_stride, num_indices, (void* const*)data,
+                 indices, factor);
+    t->scalar.data.d = scalar;
+}
+
+void tblis_init_indexed_tensor_s(tblis_indexed_tensor* t, float scalar, unsigned ndim,
+                                 unsigned dense_ndim, const len_type* len,
+                                 const stride_type* dense_stride, len_type num_indices,
+                                 float* const* data, const len_type* indices, const float* factor)
+{
+    init_indexed(t, TYPE_SINGLE, ndim, dense_ndim, len, dense_stride, num_indices, (void* const*)data,
+                 indices, factor);
+    t->scalar.data.s = scalar;
+}
+
+void tblis_indexed_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
+                               const tblis_indexed_tensor* A, const label_type* idx_A,
+                               const tblis_indexed_tensor* B, const label_type* idx_B,
+                               tblis_indexed_tensor* C, const label_type* idx_C)
+{
+    (void)comm;
+    guarded_void([&] { indexed_tensor_mult(cfg, A, idx_A, B, idx_B, C, idx_C); });
 }
 
 void tblis_tensor_mult_batch(const tblis_comm* comm, const tblis_config* cfg, len_type count,

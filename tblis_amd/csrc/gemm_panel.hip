@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void pack_panel_kernel(const PanelJob* __restr
                     if (kin && prow >= 0 && prow < j.nr)
                     {
                         const int64_t roff = rstr != SCATTERED ? rbase + r * rstr : pool[j.rowmap + prow];
-                        v[u][jj] = (double)X[j.src + roff + koff];
+                        v[u][jj] = j.scale * (double)X[j.src + roff + koff];
                     }
                 }
             }
@@ -158,9 +158,11 @@ __global__ __launch_bounds__(256) void pack_panel_kernel(const PanelJob* __restr
             if (rin)
             {
                 const int64_t roff = j.src + (rstr != SCATTERED ? rbase + lr * rstr : pool[j.rowmap + row]);
-                if (in0) v0[u] = (double)X[roff + (kstr != SCATTERED ? kbase + lk * kstr : pool[j.kmap + k0])];
+                if (in0)
+                    v0[u] = j.scale * (double)X[roff + (kstr != SCATTERED ? kbase + lk * kstr : pool[j.kmap + k0])];
                 if (in1)
-                    v1[u] = (double)X[roff + (kstr != SCATTERED ? kbase + (lk + 4) * kstr : pool[j.kmap + k1])];
+                    v1[u] = j.scale *
+                            (double)X[roff + (kstr != SCATTERED ? kbase + (lk + 4) * kstr : pool[j.kmap + k1])];
             }
         }
 #pragma unroll

@@ -54,6 +54,7 @@ __global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PanelJob* __
     const bool zero = j.src == PANEL_MISSING;
     const int ng = ch.nrb * ch.nkg;
     const int rbf = j.r0 / 32, ksf = j.k0 / 16;  // first groups of the job's tables
+    const float scale = (float)j.scale;
     if (j.kfast && !zero)
     {
         // k-fast source: per pair of k groups, 16 loads of two rows x 32
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PanelJob* __
                     if (kin && prow >= 0 && prow < j.nr)
                     {
                         const int64_t roff = rstr != SCATTERED ? rbase + r * rstr : pool[j.rowmap + prow];
-                        v[u][jj] = X[j.src + roff + koff];
+                        v[u][jj] = X[j.src + roff + koff] * scale;
                     }
                 }
             }
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PanelJob* __
             {
                 const int k = kb + e;
                 if (k >= 0 && k < j.nk)
-                    v[u][e] = X[roff + (kstr != SCATTERED ? kbase + (lh * 8 + e) * kstr : pool[j.kmap + k])];
+                    v[u][e] = X[roff + (kstr != SCATTERED ? kbase + (lh * 8 + e) * kstr : pool[j.kmap + k])] * scale;
             }
         }
 #pragma unroll

@@ -1,0 +1,531 @@
+// Host side of indexed-tensor contractions.
+//
+// An indexed tensor is a list of dense sub-tensors ("entries") placed at
+// index values of its indexed dimensions, each with a factor
+// (src/external/marray/include/indexed_varray_base.hpp:19-45).  The
+// contraction C <- alpha A.B + beta C follows src/iface/3t/mult.cxx:299-402
+// and src/internal/3t/indexed/mult.cxx:274-318: every C entry is scaled
+// first, then for every triple of entries (a, b, c) whose index values agree
+// on the labels they share, c's data gains
+//     alpha * f_a * f_b * f_c * (a . b)
+// restricted to the slices that the fixed index values select in the
+// tensors where those labels are dense (the reference's "mixed" indices,
+// src/internal/1t/indexed/util.hpp:98-140,272-330).  Labels dense in every
+// tensor that has them are the GEMM dimensions (AB: k, AC: m, BC: n, ABC:
+// batch).
+//
+// On the GPU every C block (entry and slice, and batch position) is one
+// panel GEMM whose k blocks are the matching (a, b) pairs, each scaled by
+// its coefficient while it is packed; the sum over pairs is the GEMM's k
+// loop, so no two launches ever accumulate into the same C element.  The
+// beta pass over the C entries runs first, in one launch.
+#include "indexed.hpp"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <unordered_map>
+
+#include "contract.hpp"
+#include "level1.hpp"
+#include "panel.hpp"
+#include "plan.hpp"
+
+namespace tblis_amd
+{
+
+namespace
+{
+
+template <typename T>
+void put(std::string& k, const T& v)
+{
+    k.append((const char*)&v, sizeof(T));
+}
+
+void hip_ok(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) fail(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Host view of one operand.
+struct Ix
+{
+    const tblis_indexed_tensor* t = nullptr;
+    const label_type* idx = nullptr;
+    int nd = 0, ndd = 0, nix = 0;
+    int64_t n = 0;
+    std::vector<int64_t> off;  // element offset of each entry from data[0]
+    std::vector<double> fac;
+    int64_t value(int64_t e, int pos) const { return t->indices[e * nix + (pos - ndd)]; }
+    int64_t stride(int pos) const { return t->dense_stride[pos]; }
+    int pos_of(label_type l) const
+    {
+        for (int i = 0; i < nd; i++)
+            if (idx[i] == l) return i;
+        return -1;
+    }
+    bool dense(int pos) const { return pos < ndd; }
+};
+
+void load(Ix& x, const tblis_indexed_tensor* t, const label_type* idx, const char* nm, type_t type, int es)
+{
+    const std::string name(nm);
+    if (!t) fail(name + ": null descriptor");
+    if (t->type != type) fail("A, B and C must have the same type");
+    if (t->ndim > 0 && !idx) fail(name + ": null label array");
+    if (t->dense_ndim > t->ndim) fail(name + ": more dense dimensions than dimensions");
+    if (t->ndim > 0 && !t->len) fail(name + ": null length array");
+    if (t->dense_ndim > 0 && !t->dense_stride) fail(name + ": null stride array");
+    if (t->num_indices < 0) fail(name + ": negative number of entries");
+    if (t->num_indices > 0 && !t->data) fail(name + ": null data array");
+    x.t = t;
+    x.idx = idx;
+    x.nd = (int)t->ndim;
+    x.ndd = (int)t->dense_ndim;
+    x.nix = x.nd - x.ndd;
+    x.n = t->num_indices;
+    if (x.nix > 0 && x.n > 0 && !t->indices) fail(name + ": null index array");
+    if (x.nix == 0 && x.n > 1) fail(name + ": several entries without indexed dimensions");
+    for (int i = 0; i < x.nd; i++)
+    {
+        if (t->len[i] < 0) fail(name + ": negative tensor length");
+        for (int j = 0; j < i; j++)
+            if (idx[i] == idx[j]) fail(name + ": repeated label '" + std::string(1, idx[i]) + "'");
+    }
+    x.off.resize(x.n);
+    x.fac.resize(x.n);
+    for (int64_t e = 0; e < x.n; e++)
+    {
+        if (!t->data[e]) fail(name + ": null entry pointer");
+        const intptr_t d = (const char*)t->data[e] - (const char*)t->data[0];
+        if (d % es) fail(name + ": entry pointers are not aligned to the element size");
+        x.off[e] = (int64_t)(d / es);
+        x.fac[e] = !t->factor ? 1.0
+                   : type == TYPE_DOUBLE ? ((const double*)t->factor)[e]
+                                         : (double)((const float*)t->factor)[e];
+        for (int p = x.ndd; p < x.nd; p++)
+        {
+            const int64_t v = x.value(e, p);
+            if (v < 0 || v >= t->len[p]) fail(name + ": index value out of range");
+        }
+    }
+}
+
+void key_of(std::string& k, const Ix& x)
+{
+    put(k, (int32_t)x.nd);
+    put(k, (int32_t)x.ndd);
+    put(k, x.n);
+    k.append((const char*)x.t->len, sizeof(len_type) * x.nd);
+    k.append((const char*)x.t->dense_stride, sizeof(stride_type) * x.ndd);
+    k.append(x.idx, x.nd);
+    if (x.n > 0 && x.nix > 0) k.append((const char*)x.t->indices, sizeof(len_type) * x.n * x.nix);
+    k.append((const char*)x.off.data(), sizeof(int64_t) * x.n);
+    k.append((const char*)x.fac.data(), sizeof(double) * x.n);
+}
+
+struct VecHash
+{
+    size_t operator()(const std::vector<int64_t>& v) const
+    {
+        uint64_t h = 1469598103934665603ull;
+        for (int64_t x : v) h = (h ^ (uint64_t)x) * 1099511628211ull;
+        return (size_t)h;
+    }
+};
+
+// Built state of one indexed contraction (cached by shape, indices,
+// pointer offsets and factors).
+struct IndexedPlan
+{
+    std::shared_ptr<Plan> gemm;  // null: no GEMM work
+    Level1Desc cdesc;            // C's dense part, for the beta pass
+    int64_t nent = 0;            // C entries
+    void* d_tab = nullptr;       // C entry offsets (int64), then factors (double)
+    bool pinned = false;         // used while capturing: never freed
+    ~IndexedPlan()
+    {
+        if (d_tab && !pinned) (void)hipFree(d_tab);
+    }
+};
+
+std::mutex g_mu;
+std::map<std::string, std::shared_ptr<IndexedPlan>>* g_cache = new std::map<std::string, std::shared_ptr<IndexedPlan>>();
+
+std::shared_ptr<IndexedPlan> cache_find(const std::string& k)
+{
+    std::lock_guard<std::mutex> g(g_mu);
+    auto it = g_cache->find(k);
+    return it == g_cache->end() ? nullptr : it->second;
+}
+
+void cache_store(const std::string& k, const std::shared_ptr<IndexedPlan>& p)
+{
+    std::lock_guard<std::mutex> g(g_mu);
+    if (g_cache->size() >= 128)
+        for (auto it = g_cache->begin(); it != g_cache->end();)
+            if (!it->second->pinned && it->second.use_count() == 1)
+                it = g_cache->erase(it);
+            else
+                ++it;
+    (*g_cache)[k] = p;
+}
+
+// Dense dimensions of C, smallest stride first, as a level-1 descriptor.
+Level1Desc dense_desc(const Ix& c)
+{
+    std::vector<int> order(c.ndd);
+    for (int i = 0; i < c.ndd; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return std::llabs(c.stride(a)) < std::llabs(c.stride(b));
+    });
+    Level1Desc d;
+    for (int i : order)
+    {
+        if (c.t->len[i] == 1) continue;
+        if (d.ndb >= LEVEL1_MAXDIM) fail("indexed: more than 16 dense dimensions");
+        d.len_b[d.ndb] = c.t->len[i];
+        d.sb[d.ndb] = c.stride(i);
+        d.sa_b[d.ndb] = 0;
+        d.nb *= c.t->len[i];
+        d.ndb++;
+    }
+    for (int i = 0; i < c.ndd; i++)
+        if (c.t->len[i] == 0) d.nb = 0;
+    return d;
+}
+
+std::shared_ptr<IndexedPlan> build(const Ix& A, const Ix& B, const Ix& C, int es, int dtype, int family)
+{
+    auto ip = std::make_shared<IndexedPlan>();
+    ip->cdesc = dense_desc(C);
+    ip->nent = C.n;
+    if (C.n > 0)
+    {
+        std::vector<int64_t> tab(2 * C.n);
+        for (int64_t e = 0; e < C.n; e++)
+        {
+            tab[e] = C.off[e];
+            std::memcpy(&tab[C.n + e], &C.fac[e], sizeof(double));
+        }
+        hip_ok(hipMalloc(&ip->d_tab, tab.size() * sizeof(int64_t)), "hipMalloc (indexed table)");
+        hip_ok(hipMemcpy(ip->d_tab, tab.data(), tab.size() * sizeof(int64_t), hipMemcpyHostToDevice),
+               "hipMemcpy (indexed table)");
+    }
+
+    // label classes
+    struct Lab
+    {
+        label_type l;
+        int pa, pb, pc;
+    };
+    std::vector<Lab> labs;
+    auto add = [&](label_type l) {
+        for (auto& x : labs)
+            if (x.l == l) return;
+        labs.push_back(Lab{l, A.pos_of(l), B.pos_of(l), C.pos_of(l)});
+    };
+    for (int i = 0; i < A.nd; i++) add(A.idx[i]);
+    for (int i = 0; i < B.nd; i++) add(B.idx[i]);
+    for (int i = 0; i < C.nd; i++) add(C.idx[i]);
+    auto all_dense = [&](const Lab& x) {
+        return (x.pa < 0 || A.dense(x.pa)) && (x.pb < 0 || B.dense(x.pb)) && (x.pc < 0 || C.dense(x.pc));
+    };
+    Side mA, mC, nB, nC, kA, kB;
+    std::vector<int64_t> blen, bsa, bsb, bsc;  // ABC batch
+    for (const Lab& x : labs)
+    {
+        if (!all_dense(x)) continue;
+        const int64_t len = x.pa >= 0 ? A.t->len[x.pa] : B.t->len[x.pb];
+        if (len == 1) continue;
+        if (x.pa >= 0 && x.pb >= 0 && x.pc >= 0)
+        {
+            blen.push_back(len);
+            bsa.push_back(A.stride(x.pa));
+            bsb.push_back(B.stride(x.pb));
+            bsc.push_back(C.stride(x.pc));
+        }
+        else if (x.pa >= 0 && x.pb >= 0)
+        {
+            kA.len.push_back(len), kA.stride.push_back(A.stride(x.pa));
+            kB.len.push_back(len), kB.stride.push_back(B.stride(x.pb));
+        }
+        else if (x.pa >= 0)
+        {
+            mA.len.push_back(len), mA.stride.push_back(A.stride(x.pa));
+            mC.len.push_back(len), mC.stride.push_back(C.stride(x.pc));
+        }
+        else
+        {
+            nB.len.push_back(len), nB.stride.push_back(B.stride(x.pb));
+            nC.len.push_back(len), nC.stride.push_back(C.stride(x.pc));
+        }
+    }
+    auto prod = [](const std::vector<int64_t>& v) {
+        int64_t p = 1;
+        for (int64_t x : v) p *= x;
+        return p;
+    };
+    const int64_t M = prod(mA.len), N = prod(nB.len), K = prod(kA.len), NBATCH = prod(blen);
+    if (M == 0 || N == 0 || K == 0 || NBATCH == 0 || A.n == 0 || B.n == 0 || C.n == 0) return ip;
+
+    // matching keys
+    std::vector<const Lab*> kb, kc;  // indexed in A and B; indexed in C and in A or B
+    for (const Lab& x : labs)
+    {
+        const bool ia = x.pa >= 0 && !A.dense(x.pa), ib = x.pb >= 0 && !B.dense(x.pb);
+        const bool ic = x.pc >= 0 && !C.dense(x.pc);
+        if (ia && ib) kb.push_back(&x);
+        if (ic && (ia || ib)) kc.push_back(&x);
+    }
+    std::unordered_map<std::vector<int64_t>, std::vector<int64_t>, VecHash> bmap, cmap;
+    std::vector<int64_t> key;
+    for (int64_t e = 0; e < B.n; e++)
+    {
+        key.clear();
+        for (const Lab* x : kb) key.push_back(B.value(e, x->pb));
+        bmap[key].push_back(e);
+    }
+    for (int64_t e = 0; e < C.n; e++)
+    {
+        key.clear();
+        for (const Lab* x : kc) key.push_back(C.value(e, x->pc));
+        cmap[key].push_back(e);
+    }
+
+    // C blocks, in order of first appearance: (C offset) -> k blocks
+    struct KBlk
+    {
+        int64_t oa, ob;
+        double coef;
+    };
+    std::map<int64_t, size_t> where;
+    std::vector<int64_t> cbase;
+    std::vector<std::vector<KBlk>> kblocks;
+    std::vector<int64_t> val(labs.size());
+    std::vector<char> fixed(labs.size());
+    for (int64_t a = 0; a < A.n; a++)
+    {
+        if (A.fac[a] == 0.0) continue;
+        key.clear();
+        for (const Lab* x : kb) key.push_back(A.value(a, x->pa));
+        auto bit = bmap.find(key);
+        if (bit == bmap.end()) continue;
+        for (int64_t b : bit->second)
+        {
+            if (B.fac[b] == 0.0) continue;
+            for (size_t i = 0; i < labs.size(); i++)
+            {
+                const Lab& x = labs[i];
+                fixed[i] = 0;
+                if (x.pa >= 0 && !A.dense(x.pa))
+                    val[i] = A.value(a, x.pa), fixed[i] = 1;
+                else if (x.pb >= 0 && !B.dense(x.pb))
+                    val[i] = B.value(b, x.pb), fixed[i] = 1;
+            }
+            key.clear();
+            for (const Lab* x : kc) key.push_back(val[x - labs.data()]);
+            auto cit = cmap.find(key);
+            if (cit == cmap.end()) continue;
+            for (int64_t c : cit->second)
+            {
+                const double coef = A.fac[a] * B.fac[b] * C.fac[c];
+                if (coef == 0.0) continue;
+                int64_t oa = A.off[a], ob = B.off[b], oc = C.off[c];
+                for (size_t i = 0; i < labs.size(); i++)
+                {
+                    const Lab& x = labs[i];
+                    int64_t v = val[i];
+                    bool f = fixed[i];
+                    if (!f && x.pc >= 0 && !C.dense(x.pc)) v = C.value(c, x.pc), f = true;
+                    if (!f) continue;  // dense everywhere: a GEMM dimension
+                    if (x.pa >= 0 && A.dense(x.pa)) oa += v * A.stride(x.pa);
+                    if (x.pb >= 0 && B.dense(x.pb)) ob += v * B.stride(x.pb);
+                    if (x.pc >= 0 && C.dense(x.pc)) oc += v * C.stride(x.pc);
+                }
+                auto w = where.find(oc);
+                size_t gi;
+                if (w == where.end())
+                {
+                    gi = kblocks.size();
+                    where[oc] = gi;
+                    cbase.push_back(oc);
+                    kblocks.emplace_back();
+                }
+                else
+                    gi = w->second;
+                kblocks[gi].push_back(KBlk{oa, ob, coef});
+            }
+        }
+    }
+    if (kblocks.empty()) return ip;
+
+    PlanBuilder pb(es, dtype, 1, family);
+    if (!pb.panel_route()) fail("indexed contractions run on the panel path (TBLIS_HIP_PANEL=0 is set)");
+    std::vector<int64_t> bi(blen.size(), 0);
+    for (int64_t t = 0; t < NBATCH; t++)
+    {
+        int64_t da = 0, db = 0, dc = 0;
+        for (size_t i = 0; i < blen.size(); i++)
+        {
+            da += bi[i] * bsa[i];
+            db += bi[i] * bsb[i];
+            dc += bi[i] * bsc[i];
+        }
+        for (size_t gi = 0; gi < kblocks.size(); gi++)
+        {
+            PanelGemm g;
+            g.mlen = {M};
+            g.nlen = {N};
+            for (const KBlk& kbk : kblocks[gi])
+            {
+                g.klen.push_back(K);
+                g.a.push_back(PBlock{kbk.oa + da, mA, kA});
+                g.b.push_back(PBlock{kbk.ob + db, nB, kB});
+                g.kscale.push_back(kbk.coef);
+            }
+            CBlock cb;
+            cb.base = cbase[gi] + dc;
+            cb.rows = mC;
+            cb.cols = nC;
+            g.c.push_back(cb);
+            pb.add_panel(g);
+        }
+        for (size_t i = 0; i < blen.size(); i++)
+        {
+            if (++bi[i] < blen[i]) break;
+            bi[i] = 0;
+        }
+    }
+    ip->gemm = pb.build();
+    return ip;
+}
+
+// Offsets [lo, hi] of an entry's dense part.
+void dense_extent(const Ix& x, int64_t& lo, int64_t& hi)
+{
+    lo = hi = 0;
+    for (int i = 0; i < x.ndd; i++)
+    {
+        if (x.t->len[i] == 0)
+        {
+            lo = 0, hi = -1;
+            return;
+        }
+        const int64_t s = (x.t->len[i] - 1) * x.stride(i);
+        if (s < 0)
+            lo += s;
+        else
+            hi += s;
+    }
+}
+
+void check_entries(const Ix& x, const char* nm, int es)
+{
+    int64_t lo, hi;
+    dense_extent(x, lo, hi);
+    if (hi < lo) return;
+    for (int64_t e = 0; e < x.n; e++) check_device_pointer(x.t->data[e], nm, lo, hi, es);
+}
+
+}  // namespace
+
+void indexed_tensor_mult(const tblis_config* cfg, const tblis_indexed_tensor* A, const label_type* idx_A,
+                         const tblis_indexed_tensor* B, const label_type* idx_B,
+                         tblis_indexed_tensor* C, const label_type* idx_C)
+{
+    if (!A || !B || !C) fail("indexed: null descriptor");
+    const type_t type = A->type;
+    check_real(type);
+    const int es = elem_size(type);
+    Ix a, b, c;
+    load(a, A, idx_A, "A", type, es);
+    load(b, B, idx_B, "B", type, es);
+    load(c, C, idx_C, "C", type, es);
+
+    // label rules (src/iface/3t/mult.cxx:305-388)
+    auto count_in = [](const Ix& x, label_type l) { return x.pos_of(l) >= 0 ? 1 : 0; };
+    for (const Ix* x : {&a, &b, &c})
+        for (int i = 0; i < x->nd; i++)
+        {
+            const label_type l = x->idx[i];
+            if (count_in(a, l) + count_in(b, l) + count_in(c, l) < 2)
+                fail("indexed: label '" + std::string(1, l) + "' appears in only one tensor");
+            int64_t len = x->t->len[i];
+            for (const Ix* y : {&a, &b, &c})
+            {
+                const int p = y->pos_of(l);
+                if (p >= 0 && y->t->len[p] != len)
+                    fail("indexed: label '" + std::string(1, l) + "' has unequal lengths");
+            }
+        }
+
+    Algo algo = resolve_algo(cfg, type);
+    if (algo == Algo::Reference) algo = type == TYPE_DOUBLE ? Algo::Fp64Mfma : Algo::Bf16x3;
+    thread_state().last_algo = algo_name(algo);
+    thread_state().last_tile = -1;
+
+    std::string key;
+    put(key, (int32_t)algo);
+    put(key, (int32_t)type);
+    key_of(key, a);
+    key_of(key, b);
+    key_of(key, c);
+    std::shared_ptr<IndexedPlan> ip = cache_find(key);
+    if (!ip)
+    {
+        ip = build(a, b, c, es, type == TYPE_DOUBLE ? 0 : 1, algo_family(algo));
+        cache_store(key, ip);
+    }
+
+    const double alpha = product_scalar(A->scalar, B->scalar, type);
+    const double beta = scalar_value(C->scalar, type);
+    ThreadState& ts = thread_state();
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    hip_ok(hipStreamIsCapturing(ts.stream, &cap), "hipStreamIsCapturing");
+    const bool capturing = cap != hipStreamCaptureStatusNone;
+    if (capturing) ip->pinned = true;
+
+    const bool gemm = alpha != 0.0 && ip->gemm && ip->gemm->ntasks > 0;
+    const bool scale = beta != 1.0 && ip->nent > 0 && ip->cdesc.nb > 0;
+    if (scale || gemm)
+    {
+        const void* a0 = a.n ? A->data[0] : nullptr;
+        const void* b0 = b.n ? B->data[0] : nullptr;
+        void* c0 = C->data[0];
+        Plan* vp = ip->gemm ? ip->gemm.get() : nullptr;
+        if (!vp || !vp->already_verified(a0, b0, c0, gemm))
+        {
+            check_entries(c, "C", es);
+            if (gemm)
+            {
+                check_entries(a, "A", es);
+                check_entries(b, "B", es);
+            }
+            if (vp) vp->mark_verified(a0, b0, c0, gemm);
+        }
+        if (scale)
+        {
+            const int64_t* offs = (const int64_t*)ip->d_tab;
+            const double* fac = (const double*)(offs + ip->nent);
+            hip_ok(launch_level1_entries(type == TYPE_DOUBLE ? 0 : 1, ip->cdesc, offs, fac, ip->nent, beta, c0,
+                                         ts.stream),
+                   "indexed beta pass");
+        }
+        if (gemm)
+            run_plan(ip->gemm, a0, b0, c0, alpha, 1.0, true);
+        else if (!ts.async && !capturing)
+            hip_ok(hipStreamSynchronize(ts.stream), "hipStreamSynchronize");
+    }
+    if (type == TYPE_SINGLE)
+        C->scalar.data.s = 1.0f;
+    else
+        C->scalar.data.d = 1.0;
+    C->conj = 0;
+}
+
+}  // namespace tblis_amd

@@ -1,0 +1,15 @@
+// Indexed-tensor contraction (MArray indexed_varray operands).
+#pragma once
+
+#include "common.hpp"
+
+namespace tblis_amd
+{
+
+// tblis::mult over indexed_varray_view (src/iface/3t/mult.cxx:299-402,
+// src/internal/3t/indexed/mult.cxx:63-318).
+void indexed_tensor_mult(const tblis_config* cfg, const tblis_indexed_tensor* A, const label_type* idx_A,
+                         const tblis_indexed_tensor* B, const label_type* idx_B,
+                         tblis_indexed_tensor* C, const label_type* idx_C);
+
+}  // namespace tblis_amd

@@ -25,6 +25,10 @@ struct Level1Desc
     int64_t len_a[LEVEL1_MAXDIM], sa_a[LEVEL1_MAXDIM];
 };
 
+// Beta pass over the entries of an indexed tensor (same dense shape d,
+// entry offsets and factors in device memory).
+hipError_t launch_level1_entries(int dtype, const Level1Desc& d, const int64_t* offs, const double* fac,
+                                 int64_t nent, double beta, void* B, hipStream_t s);
 // Device workspace a call needs (partial sums of a split A-only sum), bytes.
 size_t level1_workspace_bytes(int mode, const Level1Desc& d);
 hipError_t launch_level1(int mode, int dtype, const Level1Desc& d, const void* A, void* B,

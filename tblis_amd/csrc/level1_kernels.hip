@@ -344,6 +344,28 @@ __global__ __launch_bounds__(kThreads) void level1_split_kernel(Level1Desc d, Sp
     }
 }
 
+// Beta pass of an indexed contraction over every C entry (same dense
+// shape, entry offsets offs[e], factors fac[e]): beta == 0 or a zero factor
+// writes 0 without reading; otherwise data *= fac[e] * beta
+// (src/internal/1t/indexed/scale.cxx:12-30, set.cxx:12-20).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void level1_entries_kernel(Level1Desc d, const int64_t* __restrict__ offs,
+                                                                  const double* __restrict__ fac, int64_t nent,
+                                                                  double beta, T* __restrict__ B)
+{
+    const int64_t total = nent * d.nb;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += stride)
+    {
+        const int64_t e = x / d.nb, i = x - e * d.nb;
+        int64_t ob, oa;
+        row_offsets(d, i, 0, -1, ob, oa);
+        T* p = B + offs[e] + ob;
+        const double f = fac[e];
+        *p = (beta == 0.0 || f == 0.0) ? (T)0 : (T)(f * beta * (double)*p);
+    }
+}
+
 unsigned grid_for(int64_t work)
 {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(work, kMaxBlocks));
@@ -468,6 +490,20 @@ hipError_t launch_t(int mode, const Level1Desc& d, const void* A, void* B, doubl
 }
 
 }  // namespace
+
+hipError_t launch_level1_entries(int dtype, const Level1Desc& d, const int64_t* offs, const double* fac,
+                                int64_t nent, double beta, void* B, hipStream_t s)
+{
+    if (nent == 0 || d.nb == 0) return hipSuccess;
+    const unsigned grid = grid_for((nent * d.nb + kThreads - 1) / kThreads);
+    if (dtype == 0)
+        hipLaunchKernelGGL((level1_entries_kernel<double>), dim3(grid), dim3(kThreads), 0, s, d, offs, fac, nent,
+                           beta, (double*)B);
+    else
+        hipLaunchKernelGGL((level1_entries_kernel<float>), dim3(grid), dim3(kThreads), 0, s, d, offs, fac, nent,
+                           beta, (float*)B);
+    return hipGetLastError();
+}
 
 size_t level1_workspace_bytes(int mode, const Level1Desc& d)
 {

@@ -269,6 +269,7 @@ PanelGemm transpose_gemm(const PanelGemm& g)
     t.klen = g.klen;
     t.a = g.b;
     t.b = g.a;
+    t.kscale = g.kscale;
     const size_t nm = g.mlen.size(), nn = g.nlen.size();
     t.c.resize(g.c.size());
     for (size_t p = 0; p < nm; p++)
@@ -510,6 +511,7 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
         for (int o = 0; o < 2 && G.terms == 1 && direct_enabled() && kp > 0; o++)
         {
             if (!allow_direct[o] || nrp[o] != 1 || NK != 1) continue;
+            if (o == 0 && !g.kscale.empty() && g.kscale[0] != 1.0) continue;  // scaled while packed
             const int64_t rereads = o == 0 ? ceil_div(n, BN) : ceil_div(m, BM);
             if (rereads > std::max<int64_t>(kDirectRowFast, direct_kfast_rereads())) continue;
             const PBlock& blk = (*blocks[o])[0];
@@ -590,6 +592,7 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                     j.out = outs[o];
                     j.pnkg = (int32_t)nkg;
                     j.operand = o;
+                    if (o == 0 && !g.kscale.empty()) j.scale = g.kscale[q];
                     if (j.nr == 0 || j.nk == 0) continue;
                     if (blk.base != PANEL_MISSING)
                     {

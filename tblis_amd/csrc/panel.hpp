@@ -31,6 +31,7 @@ struct PanelGemm
     std::vector<int64_t> mlen, nlen, klen;
     std::vector<PBlock> a, b;
     std::vector<CBlock> c;
+    std::vector<double> kscale;  // per k block, applied to A while packing (empty: all 1)
 };
 
 // C^T = B^T A^T.

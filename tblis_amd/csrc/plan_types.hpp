@@ -99,6 +99,7 @@ struct PanelJob
     int32_t operand;       // 0: A pointer, 1: B pointer (after orientation)
     int32_t kfast;         // k groups unit-stride, rows not: read rows, transpose in LDS
     int32_t pad_;
+    double scale = 1.0;    // every value is multiplied by this while packed
 };
 
 // Rectangle of fragment groups of one job: rows blocks [rb0, rb0 + nrb),

@@ -72,3 +72,66 @@ def storage_size(lens, strides):
     if any(l == 0 for l in lens):
         return 1
     return 1 + sum((l - 1) * s for l, s in zip(lens, strides))
+
+
+def random_indexed(rng, labels, ext, dense_ndim, fill=0.5, factors=(1.0, 0.5, 0.0)):
+    """Host description of a random indexed tensor over `labels` (the first
+    dense_ndim are dense), like the reference's random_tensor for
+    indexed_varray (test/test.cxx:294-306): a random subset of the index
+    values (each kept with probability `fill`, at least one), entries
+    uniform in [-1, 1], factors drawn from `factors`."""
+    import itertools
+    dense_len = [ext[c] for c in labels[:dense_ndim]]
+    idx_len = [ext[c] for c in labels[dense_ndim:]]
+    every = list(itertools.product(*[range(n) for n in idx_len]))
+    pick = [ix for ix in every if rng.random() < fill]
+    if not pick and every:
+        pick = [every[int(rng.integers(len(every)))]]
+    entries = [rng.uniform(-1, 1, dense_len) for _ in pick]
+    facs = [float(rng.choice(factors)) for _ in pick]
+    return dict(entries=entries, indices=pick, factors=facs, dense_len=dense_len, idx_len=idx_len)
+
+
+def indexed_brute_force(alpha, A, idx_A, B, idx_B, beta, C_, idx_C):
+    """The indexed contraction by its definition, one entry triple and one
+    assignment of every label at a time (pure Python, tiny sizes only):
+    C entry c gains alpha f_a f_b f_c a[..] b[..] wherever the labels' values
+    agree with the three entries' index values; C entries are first scaled
+    by the reference's rule (beta == 0: 0; beta != 1: f_c beta)."""
+    import itertools
+    labels = sorted(set(idx_A) | set(idx_B) | set(idx_C))
+    ext = {}
+    for X, idx in ((A, idx_A), (B, idx_B), (C_, idx_C)):
+        for c, n in zip(idx, list(X["dense_len"]) + list(X["idx_len"])):
+            ext[c] = n
+    out = []
+    for e, f in zip(C_["entries"], C_["factors"]):
+        e = np.array(e, np.float64)
+        if beta == 0.0 or (beta != 1.0 and f == 0.0):
+            e[...] = 0.0
+        elif beta != 1.0:
+            e *= f * beta
+        out.append(e)
+    if alpha == 0.0:
+        return out
+
+    def split(X, idx, vals):
+        nd = len(X["dense_len"])
+        return tuple(vals[c] for c in idx[:nd]), tuple(vals[c] for c in idx[nd:])
+
+    for vs in itertools.product(*[range(ext[c]) for c in labels]):
+        vals = dict(zip(labels, vs))
+        da, ia = split(A, idx_A, vals)
+        db, ib = split(B, idx_B, vals)
+        dc, ic = split(C_, idx_C, vals)
+        for ea, xa, fa in zip(A["entries"], A["indices"], A["factors"]):
+            if tuple(xa) != ia:
+                continue
+            for eb, xb, fb in zip(B["entries"], B["indices"], B["factors"]):
+                if tuple(xb) != ib:
+                    continue
+                for k, (xc, fc) in enumerate(zip(C_["indices"], C_["factors"])):
+                    if tuple(xc) != ic:
+                        continue
+                    out[k][dc] += alpha * fa * fb * fc * ea[da] * eb[db]
+    return out

@@ -374,3 +374,74 @@ def test_level1_argument_errors():
         assert "complex" in T.last_error().lower()
     finally:
         L.tblis_hip_set_error_mode(0)
+
+
+def test_indexed_abi_layout_and_init():
+    """tblis_indexed_tensor: 88 bytes; the ctypes mirror and the compiled
+    header agree on every offset, and the init calls fill it."""
+    import tempfile
+    fields = ["type", "conj", "scalar", "ndim", "dense_ndim", "len", "dense_stride", "num_indices",
+              "data", "indices", "factor"]
+    assert C.sizeof(T.IndexedTensor) == 88
+    want = [C.sizeof(T.IndexedTensor)] + [getattr(T.IndexedTensor, f).offset for f in fields]
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "probe.c"), os.path.join(d, "probe")
+        with open(src, "w") as f:
+            f.write('#include <stddef.h>\n#include <stdio.h>\n#include "tblis.h"\nint main(void){'
+                    'printf("%zu", sizeof(tblis_indexed_tensor));' +
+                    "".join('printf(" %%zu", offsetof(tblis_indexed_tensor, %s));' % x for x in fields) +
+                    'printf("\\n"); return 0;}\n')
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    assert [int(x) for x in out] == want
+    L = T.lib()
+    lens = (C.c_ssize_t * 3)(4, 5, 3)
+    strides = (C.c_ssize_t * 2)(1, 4)
+    idx = (C.c_ssize_t * 2)(0, 2)
+    data = (C.c_void_p * 2)(0x1000, 0x2000)
+    fac = (C.c_double * 2)(1.0, 0.5)
+    t = T.IndexedTensor()
+    L.tblis_init_indexed_tensor_d(C.byref(t), C.c_double(2.0), 3, 2, lens, strides, 2, data, idx, fac)
+    assert (t.type, t.scalar.data.d, t.ndim, t.dense_ndim, t.num_indices) == (T.TYPE_DOUBLE, 2.0, 3, 2, 2)
+    assert t.len[2] == 3 and t.dense_stride[1] == 4 and t.indices[1] == 2 and t.data[1] == 0x2000
+
+
+def _ix(lens, dense_ndim, strides, n, indices=None, dtype=T.TYPE_DOUBLE):
+    d = T.IndexedTensor()
+    keep = [(C.c_ssize_t * max(len(lens), 1))(*lens), (C.c_ssize_t * max(len(strides), 1))(*strides),
+            (C.c_void_p * max(n, 1))(*([0x10000 + 0x1000 * i for i in range(n)])),
+            (C.c_ssize_t * max(len(indices or []), 1))(*(indices or []))]
+    d.type = dtype
+    d.scalar.type = dtype
+    d.scalar.data.d = 1.0
+    d.ndim, d.dense_ndim, d.num_indices = len(lens), dense_ndim, n
+    d.len = C.cast(keep[0], C.POINTER(C.c_ssize_t))
+    d.dense_stride = C.cast(keep[1], C.POINTER(C.c_ssize_t))
+    d.data = C.cast(keep[2], C.POINTER(C.c_void_p))
+    d.indices = C.cast(keep[3], C.POINTER(C.c_ssize_t))
+    return d, keep
+
+
+def test_indexed_argument_errors():
+    """Label and index errors are reported before any device access."""
+    L = T.lib()
+    L.tblis_hip_set_error_mode(1)
+    try:
+        b, kb = _ix([4, 5], 2, [1, 4], 1)
+        c, kc = _ix([3, 5], 2, [1, 3], 1)
+        a, ka = _ix([3, 4], 1, [1], 1, [7])
+        L.tblis_indexed_tensor_mult(None, None, C.byref(a), b"ae", C.byref(b), b"ej", C.byref(c), b"aj")
+        assert "index value out of range" in T.last_error()
+        a, ka = _ix([3, 4], 1, [1], 1, [2])
+        L.tblis_indexed_tensor_mult(None, None, C.byref(a), b"aa", C.byref(b), b"ej", C.byref(c), b"aj")
+        assert "repeated label" in T.last_error()
+        L.tblis_indexed_tensor_mult(None, None, C.byref(a), b"ax", C.byref(b), b"ej", C.byref(c), b"aj")
+        assert "only one tensor" in T.last_error()
+        b2, kb2 = _ix([5, 5], 2, [1, 5], 1)
+        L.tblis_indexed_tensor_mult(None, None, C.byref(a), b"ae", C.byref(b2), b"ej", C.byref(c), b"aj")
+        assert "unequal lengths" in T.last_error()
+        a3, ka3 = _ix([3, 4], 1, [1], 2, [0, 1], dtype=T.TYPE_SINGLE)
+        L.tblis_indexed_tensor_mult(None, None, C.byref(a3), b"ae", C.byref(b), b"ej", C.byref(c), b"aj")
+        assert "same type" in T.last_error()
+    finally:
+        L.tblis_hip_set_error_mode(0)

@@ -428,3 +428,59 @@ def test_level1_diagonals_scale_set_and_unit_lengths():
     np.testing.assert_allclose(got, want, rtol=1e-15)
     got = _l1(2, 7.0, None, [], [], "", 0.0, B, [3, 3], [1, 3], "ab")
     np.testing.assert_array_equal(got, np.full(9, 7.0))
+
+
+# ---------------------------------------------------------------- indexed
+INDEXED_CASES = [
+    # (idx_A, dense_A, idx_B, dense_B, idx_C, dense_C)
+    ("abef", 4, "efij", 4, "abij", 4),   # all dense: one entry each
+    ("abef", 3, "eijf", 3, "abij", 4),   # f indexed in A and in B
+    ("abef", 3, "efij", 4, "abij", 4),   # f indexed in A, dense in B (B sliced)
+    ("abef", 4, "efij", 4, "abij", 3),   # j indexed in C only (A, B dense)
+    ("aek", 3, "ebk", 3, "abk", 3),      # k dense everywhere: a batch dimension
+    ("aek", 2, "ebk", 2, "abk", 2),      # k indexed everywhere
+    ("aebk", 2, "ekb", 1, "abk", 1),     # mixed: b dense in A, indexed in B and C; k indexed in all
+]
+
+
+@pytest.mark.parametrize("case", range(len(INDEXED_CASES)))
+def test_indexed_oracle_against_brute_force(case):
+    """The oracle's FULL algorithm against the definition, entry triple by
+    entry triple (the reference's own indexed test compares its BLOCKED and
+    FULL algorithms, test/3t/mult.cxx:143-174)."""
+    from tests.helpers import indexed_brute_force, random_indexed
+    iA, nA, iB, nB, iC, nC = INDEXED_CASES[case]
+    rng = np.random.default_rng(62000 + case)
+    for trial in range(3):
+        ext = {c: int(rng.integers(1, 4)) for c in set(iA + iB + iC)}
+        A = random_indexed(rng, iA, ext, nA)
+        B = random_indexed(rng, iB, ext, nB)
+        C0 = random_indexed(rng, iC, ext, nC)
+        alpha, beta = [(0.75, 0.5), (1.0, 0.0), (-1.5, 1.0)][trial]
+        got = po.indexed_mult(alpha, A, iA, B, iB, beta, C0, iC)
+        want = indexed_brute_force(alpha, A, iA, B, iB, beta, C0, iC)
+        assert len(got) == len(want)
+        for g, w in zip(got, want):
+            np.testing.assert_allclose(g, w, rtol=1e-12, atol=1e-12)
+
+
+def test_indexed_oracle_fully_populated_is_dense_contraction():
+    """Every index value present with factor 1: the indexed contraction is
+    the dense one (checked against the dense oracle, oracle.c)."""
+    from tests.helpers import random_indexed
+    rng = np.random.default_rng(62100)
+    ext = dict(a=3, b=2, e=4, f=3, i=2, j=3)
+    A = random_indexed(rng, "abef", ext, 2, fill=1.0, factors=(1.0,))
+    B = random_indexed(rng, "efij", ext, 3, fill=1.0, factors=(1.0,))
+    C0 = random_indexed(rng, "abij", ext, 1, fill=1.0, factors=(1.0,))
+    got = po.indexed_mult(0.5, A, "abef", B, "efij", -1.0, C0, "abij")
+    full = {k: po.indexed_full(X["entries"], X["indices"], X["factors"], X["dense_len"], X["idx_len"])
+            for k, X in (("A", A), ("B", B), ("C", C0))}
+    lens = {k: list(v.shape) for k, v in full.items()}
+    store = {k: np.asfortranarray(v).ravel(order="F") for k, v in full.items()}
+    cm = lambda l: [int(np.prod(l[:i])) for i in range(len(l))]
+    want = po.tensor_mult(0.5, store["A"], lens["A"], cm(lens["A"]), "abef", store["B"], lens["B"],
+                          cm(lens["B"]), "efij", -1.0, store["C"].copy(), lens["C"], cm(lens["C"]), "abij")
+    want = np.asarray(want).reshape(lens["C"], order="F")
+    for e, ix in zip(got, C0["indices"]):
+        np.testing.assert_allclose(e, want[(slice(None),) + tuple(ix)], rtol=1e-13, atol=1e-13)

@@ -24,6 +24,10 @@ makes it the headline):
   batch        4096 independent C[abij] = A[abef]*B[efij], extents drawn from
                4..40 per instance, FP64 and split-BF16, one hipGraph replay
                per step
+  indexed      indexed contraction C[abij] += A[abef]*B[efij], a=b=e=i=j=48
+               dense, f indexed (32 values; A and B each list 24, seeded), FP64
+               and split-BF16; useful TFLOP/s over the matching f values, and
+               the same kernels on the equivalent dense contraction
   level1       tblis_tensor_add / _scale on FP64 96^4 tensors (679 MB each):
                permuted add, same-layout add, sums over the slow and over
                the fast dimensions, scale; GB/s of algorithmic bytes against
@@ -260,6 +264,46 @@ def batch_leg(T, dev, stream, steps, warmup, count=4096, seed=7):
     return out
 
 
+# ---------------------------------------------------------------- indexed
+
+def indexed_leg(T, dev, stream, steps, warmup, ext=48, nf=32, listed=24, seed=5):
+    import torch
+    rng = np.random.default_rng(seed)
+    fa = sorted(rng.choice(nf, listed, replace=False).tolist())
+    fb = sorted(rng.choice(nf, listed, replace=False).tolist())
+    match = sorted(set(fa) & set(fb))
+    dl = [ext] * 3
+    flops = 2.0 * ext ** 5 * len(match)
+    out = {"workload": "indexed C[abij] += A[abef]*B[efij], a=b=e=i=j=%d dense, f indexed (%d values, "
+                       "A and B list %d each, %d match; numpy default_rng(%d)), C dense" %
+                       (ext, nf, listed, len(match), seed), "useful_flops": flops}
+    g = torch.Generator(device=dev).manual_seed(seed)
+    for name, dt in (("fp64", torch.float64), ("split_bf16", torch.float32)):
+        sa = torch.rand(listed * ext ** 3, dtype=dt, device=dev, generator=g) * 2 - 1
+        sb = torch.rand(listed * ext ** 3, dtype=dt, device=dev, generator=g) * 2 - 1
+        cs = torch.zeros(ext ** 4, dtype=dt, device=dev)
+        st = colmaj(dl)
+        A = T.Indexed([torch.as_strided(sa, dl, st, i * ext ** 3) for i in range(listed)],
+                      [(f,) for f in fa], [nf])
+        B = T.Indexed([torch.as_strided(sb, dl, st, i * ext ** 3) for i in range(listed)],
+                      [(f,) for f in fb], [nf])
+        Cx = T.Indexed([strided(cs, [ext] * 4)], [()], [])
+        fn = T.prepare_indexed_mult(1.0, A, "abef", B, "eijf", 0.0, Cx, "abij")
+        ms_step, ms_kern, _ = time_steps(fn, steps, warmup, stream)
+        out[name] = {"algorithm": T.last_algorithm(), "tile": T.last_tile(), "ms": round(ms_step, 4),
+                     "useful_tflops": tflops(flops, ms_step)}
+        # the same kernels on the dense contraction over the matching f values
+        n = len(match)
+        da = torch.rand(ext ** 3 * n, dtype=dt, device=dev, generator=g) * 2 - 1
+        db = torch.rand(ext ** 3 * n, dtype=dt, device=dev, generator=g) * 2 - 1
+        fn = T.prepare_mult(1.0, strided(da, dl + [n]), "abef", strided(db, dl + [n]), "eijf", 0.0,
+                            strided(cs, [ext] * 4), "abij")
+        ms_d, _, _ = time_steps(fn, steps, warmup, stream)
+        out[name]["dense_equivalent_tflops"] = tflops(flops, ms_d)
+        out[name]["frac_of_dense"] = round(ms_d / ms_step, 3)
+    return out
+
+
 # ---------------------------------------------------------------- level-1
 
 def level1_leg(T, dev, stream, steps, warmup):
@@ -377,7 +421,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--only", choices=["permuted", "fp32_96", "dpd", "batch", "level1"],
+    ap.add_argument("--only", choices=["permuted", "fp32_96", "dpd", "batch", "indexed", "level1"],
                     help="run one leg alone (for profiling) and report it as the headline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-traffic", action="store_true",
@@ -425,6 +469,10 @@ def main():
     if args.only in (None, "batch"):
         legs["batch"] = batch_leg(T, dev, stream, few if args.only is None else args.steps,
                                   min(args.warmup, 2))
+        torch.cuda.empty_cache()
+    if args.only in (None, "indexed"):
+        legs["indexed"] = indexed_leg(T, dev, stream, few if args.only is None else args.steps,
+                                      min(args.warmup, 2))
         torch.cuda.empty_cache()
     if args.only in (None, "level1"):
         legs["level1"] = level1_leg(T, dev, stream, few if args.only is None else args.steps,

@@ -1,6 +1,6 @@
 /*
  * tblis.hpp -- C++ front end of libtblis_amd, mirroring the reference's
- * tblis::mult<T> templates (src/iface/3t/mult.h:28-72).
+ * tblis::mult<T> templates (src/iface/3t/mult.h:28-86).
  *
  * The reference's templates take MArray views (varray_view / dpd_varray_view)
  * and build tblis_tensor descriptors for the C call; these do the same over
@@ -49,6 +49,19 @@ struct dpd_varray_view
     unsigned nirrep;
     std::vector<std::vector<len_type>> len;
     int layout = TBLIS_DPD_PREFIX_ROW_MAJOR;
+};
+
+// Indexed tensor view (device memory), like MArray's indexed_varray_view:
+// entries with one dense shape, placed at index values, with factors.
+template <typename T>
+struct indexed_varray_view
+{
+    std::vector<T*> data;                  // one pointer per entry
+    std::vector<len_type> dense_len;       // dense dimensions (first)
+    std::vector<stride_type> dense_stride;
+    std::vector<len_type> idx_len;         // indexed dimensions (last)
+    std::vector<len_type> indices;         // entries x idx_len.size(), row major
+    std::vector<T> factor;                 // empty: all 1
 };
 
 namespace detail
@@ -100,6 +113,24 @@ tblis_dpd_tensor make_dpd(T scalar, const dpd_varray_view<T>& v, std::vector<len
     return t;
 }
 
+template <typename T>
+tblis_indexed_tensor make_indexed(T scalar, const indexed_varray_view<T>& v, std::vector<len_type>& len)
+{
+    len = v.dense_len;
+    len.insert(len.end(), v.idx_len.begin(), v.idx_len.end());
+    tblis_indexed_tensor t;
+    const len_type n = (len_type)v.data.size();
+    if (std::is_same<T, float>::value)
+        tblis_init_indexed_tensor_s(&t, (float)scalar, (unsigned)len.size(), (unsigned)v.dense_len.size(),
+                                    len.data(), v.dense_stride.data(), n, (float* const*)v.data.data(),
+                                    v.indices.data(), v.factor.empty() ? nullptr : (const float*)v.factor.data());
+    else
+        tblis_init_indexed_tensor_d(&t, (double)scalar, (unsigned)len.size(), (unsigned)v.dense_len.size(),
+                                    len.data(), v.dense_stride.data(), n, (double* const*)v.data.data(),
+                                    v.indices.data(), v.factor.empty() ? nullptr : (const double*)v.factor.data());
+    return t;
+}
+
 }  // namespace detail
 
 // C <- alpha * A.B + beta * C (src/iface/3t/mult.h:28-37).
@@ -143,6 +174,19 @@ void mult(T alpha, const dpd_varray_view<T>& A, const label_type* idx_A, const d
     tblis_dpd_tensor b = detail::make_dpd(T(1), B, fb);
     tblis_dpd_tensor c = detail::make_dpd(beta, C, fc);
     tblis_dpd_tensor_mult(nullptr, cfg, &a, idx_A, &b, idx_B, &c, idx_C);
+}
+
+// Indexed contraction (src/iface/3t/mult.h:74-86).
+template <typename T>
+void mult(T alpha, const indexed_varray_view<T>& A, const label_type* idx_A, const indexed_varray_view<T>& B,
+          const label_type* idx_B, T beta, const indexed_varray_view<T>& C, const label_type* idx_C,
+          const tblis_config* cfg = nullptr)
+{
+    std::vector<len_type> la, lb, lc;
+    tblis_indexed_tensor a = detail::make_indexed(alpha, A, la);
+    tblis_indexed_tensor b = detail::make_indexed(T(1), B, lb);
+    tblis_indexed_tensor c = detail::make_indexed(beta, C, lc);
+    tblis_indexed_tensor_mult(nullptr, cfg, &a, idx_A, &b, idx_B, &c, idx_C);
 }
 
 }  // namespace tblis

@@ -145,7 +145,10 @@ struct IndexedPlan
     std::shared_ptr<Plan> gemm;  // null: no GEMM work
     Level1Desc cdesc;            // C's dense part, for the beta pass
     int64_t nent = 0;            // C entries
-    void* d_tab = nullptr;       // C entry offsets (int64), then factors (double)
+    int64_t nunc = 0;            // C entries no GEMM writes whole (beta == 0 zeroes only these)
+    bool whole = false;          // every GEMM writes whole C entries: beta == 0 goes to the epilogue
+    void* d_tab = nullptr;       // C entry offsets (int64), factors (double), then the same for
+                                 // the uncovered entries
     bool pinned = false;         // used while capturing: never freed
     ~IndexedPlan()
     {
@@ -187,11 +190,17 @@ Level1Desc dense_desc(const Ix& c)
     for (int i : order)
     {
         if (c.t->len[i] == 1) continue;
+        d.nb *= c.t->len[i];
+        // fold into the previous dimension when contiguous with it
+        if (d.ndb > 0 && c.stride(i) == d.sb[d.ndb - 1] * d.len_b[d.ndb - 1])
+        {
+            d.len_b[d.ndb - 1] *= c.t->len[i];
+            continue;
+        }
         if (d.ndb >= LEVEL1_MAXDIM) fail("indexed: more than 16 dense dimensions");
         d.len_b[d.ndb] = c.t->len[i];
         d.sb[d.ndb] = c.stride(i);
         d.sa_b[d.ndb] = 0;
-        d.nb *= c.t->len[i];
         d.ndb++;
     }
     for (int i = 0; i < c.ndd; i++)
@@ -204,18 +213,32 @@ std::shared_ptr<IndexedPlan> build(const Ix& A, const Ix& B, const Ix& C, int es
     auto ip = std::make_shared<IndexedPlan>();
     ip->cdesc = dense_desc(C);
     ip->nent = C.n;
-    if (C.n > 0)
-    {
+    std::vector<char> covered(C.n, 0);
+    // device table of the C entries (and of those no GEMM covers whole)
+    auto upload = [&]() {
+        if (C.n == 0) return;
         std::vector<int64_t> tab(2 * C.n);
         for (int64_t e = 0; e < C.n; e++)
         {
             tab[e] = C.off[e];
             std::memcpy(&tab[C.n + e], &C.fac[e], sizeof(double));
         }
+        std::vector<int64_t> unc_off;
+        std::vector<double> unc_fac;
+        for (int64_t e = 0; e < C.n; e++)
+            if (!covered[e]) unc_off.push_back(C.off[e]), unc_fac.push_back(C.fac[e]);
+        ip->nunc = (int64_t)unc_off.size();
+        tab.insert(tab.end(), unc_off.begin(), unc_off.end());
+        for (double f : unc_fac)
+        {
+            int64_t bits;
+            std::memcpy(&bits, &f, sizeof(double));
+            tab.push_back(bits);
+        }
         hip_ok(hipMalloc(&ip->d_tab, tab.size() * sizeof(int64_t)), "hipMalloc (indexed table)");
         hip_ok(hipMemcpy(ip->d_tab, tab.data(), tab.size() * sizeof(int64_t), hipMemcpyHostToDevice),
                "hipMemcpy (indexed table)");
-    }
+    };
 
     // label classes
     struct Lab
@@ -271,7 +294,16 @@ std::shared_ptr<IndexedPlan> build(const Ix& A, const Ix& B, const Ix& C, int es
         return p;
     };
     const int64_t M = prod(mA.len), N = prod(nB.len), K = prod(kA.len), NBATCH = prod(blen);
-    if (M == 0 || N == 0 || K == 0 || NBATCH == 0 || A.n == 0 || B.n == 0 || C.n == 0) return ip;
+    if (M == 0 || N == 0 || K == 0 || NBATCH == 0 || A.n == 0 || B.n == 0 || C.n == 0)
+    {
+        upload();
+        return ip;
+    }
+    // C is sliced when a label dense in C is fixed by an A or B entry; then
+    // a GEMM writes part of a C entry, otherwise all of it
+    bool sliced = false;
+    for (const Lab& x : labs)
+        if (x.pc >= 0 && C.dense(x.pc) && !all_dense(x)) sliced = true;
 
     // matching keys
     std::vector<const Lab*> kb, kc;  // indexed in A and B; indexed in C and in A or B
@@ -362,6 +394,10 @@ std::shared_ptr<IndexedPlan> build(const Ix& A, const Ix& B, const Ix& C, int es
             }
         }
     }
+    if (!sliced)
+        for (int64_t e = 0; e < C.n; e++) covered[e] = where.count(C.off[e]) ? 1 : 0;
+    ip->whole = !sliced;
+    upload();
     if (kblocks.empty()) return ip;
 
     PlanBuilder pb(es, dtype, 1, family);
@@ -491,7 +527,11 @@ void indexed_tensor_mult(const tblis_config* cfg, const tblis_indexed_tensor* A,
     if (capturing) ip->pinned = true;
 
     const bool gemm = alpha != 0.0 && ip->gemm && ip->gemm->ntasks > 0;
-    const bool scale = beta != 1.0 && ip->nent > 0 && ip->cdesc.nb > 0;
+    // beta == 0 with GEMMs that write whole C entries: the GEMM epilogue
+    // overwrites those entries, the pass zeroes only the others
+    const bool epilogue_zero = gemm && beta == 0.0 && ip->whole;
+    const int64_t npass = epilogue_zero ? ip->nunc : ip->nent;
+    const bool scale = beta != 1.0 && npass > 0 && ip->cdesc.nb > 0;
     if (scale || gemm)
     {
         const void* a0 = a.n ? A->data[0] : nullptr;
@@ -510,14 +550,14 @@ void indexed_tensor_mult(const tblis_config* cfg, const tblis_indexed_tensor* A,
         }
         if (scale)
         {
-            const int64_t* offs = (const int64_t*)ip->d_tab;
-            const double* fac = (const double*)(offs + ip->nent);
-            hip_ok(launch_level1_entries(type == TYPE_DOUBLE ? 0 : 1, ip->cdesc, offs, fac, ip->nent, beta, c0,
+            const int64_t* offs = (const int64_t*)ip->d_tab + (epilogue_zero ? 2 * ip->nent : 0);
+            const double* fac = (const double*)(offs + npass);
+            hip_ok(launch_level1_entries(type == TYPE_DOUBLE ? 0 : 1, ip->cdesc, offs, fac, npass, beta, c0,
                                          ts.stream),
                    "indexed beta pass");
         }
         if (gemm)
-            run_plan(ip->gemm, a0, b0, c0, alpha, 1.0, true);
+            run_plan(ip->gemm, a0, b0, c0, alpha, epilogue_zero ? 0.0 : 1.0, true);
         else if (!ts.async && !capturing)
             hip_ok(hipStreamSynchronize(ts.stream), "hipStreamSynchronize");
     }

@@ -344,25 +344,38 @@ __global__ __launch_bounds__(kThreads) void level1_split_kernel(Level1Desc d, Sp
     }
 }
 
-// Beta pass of an indexed contraction over every C entry (same dense
-// shape, entry offsets offs[e], factors fac[e]): beta == 0 or a zero factor
-// writes 0 without reading; otherwise data *= fac[e] * beta
-// (src/internal/1t/indexed/scale.cxx:12-30, set.cxx:12-20).
+// Beta pass of an indexed contraction over C entries (same dense shape,
+// entry offsets offs[e], factors fac[e]): beta == 0 or a zero factor writes
+// 0 without reading; otherwise data *= fac[e] * beta
+// (src/internal/1t/indexed/scale.cxx:12-30, set.cxx:12-20).  A block covers
+// kSeg elements of one row (dimension 0) of one entry, like the long-rows
+// kernel.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void level1_entries_kernel(Level1Desc d, const int64_t* __restrict__ offs,
                                                                   const double* __restrict__ fac, int64_t nent,
                                                                   double beta, T* __restrict__ B)
 {
-    const int64_t total = nent * d.nb;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += stride)
+    const int64_t l0 = d.ndb > 0 ? d.len_b[0] : 1, sb0 = d.ndb > 0 ? d.sb[0] : 0;
+    const int64_t nrows = d.nb / l0, nseg = (l0 + kSeg - 1) / kSeg;
+    const int64_t per = nrows * nseg, nblocks = nent * per;
+    for (int64_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x)
     {
-        const int64_t e = x / d.nb, i = x - e * d.nb;
+        const int64_t e = blk / per, r = blk - e * per;
+        const int64_t row = r / nseg, seg = r - row * nseg;
         int64_t ob, oa;
-        row_offsets(d, i, 0, -1, ob, oa);
-        T* p = B + offs[e] + ob;
+        row_offsets(d, row, 1, -1, ob, oa);
+        T* base = B + offs[e] + ob;
         const double f = fac[e];
-        *p = (beta == 0.0 || f == 0.0) ? (T)0 : (T)(f * beta * (double)*p);
+        const bool zero = beta == 0.0 || f == 0.0;
+        const double c = f * beta;
+#pragma unroll
+        for (int j = 0; j < kPerThread; j++)
+        {
+            const int64_t col = seg * kSeg + threadIdx.x + j * kThreads;
+            if (col >= l0) break;
+            T* p = base + col * sb0;
+            *p = zero ? (T)0 : (T)(c * (double)*p);
+        }
     }
 }
 
@@ -495,7 +508,8 @@ hipError_t launch_level1_entries(int dtype, const Level1Desc& d, const int64_t* 
                                 int64_t nent, double beta, void* B, hipStream_t s)
 {
     if (nent == 0 || d.nb == 0) return hipSuccess;
-    const unsigned grid = grid_for((nent * d.nb + kThreads - 1) / kThreads);
+    const int64_t l0 = d.ndb > 0 ? d.len_b[0] : 1;
+    const unsigned grid = grid_for(nent * (d.nb / l0) * ((l0 + kSeg - 1) / kSeg));
     if (dtype == 0)
         hipLaunchKernelGGL((level1_entries_kernel<double>), dim3(grid), dim3(kThreads), 0, s, d, offs, fac, nent,
                            beta, (double*)B);

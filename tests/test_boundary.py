@@ -445,3 +445,39 @@ def test_indexed_argument_errors():
         assert "same type" in T.last_error()
     finally:
         L.tblis_hip_set_error_mode(0)
+
+
+def test_cpp_header_compiles_links_and_reaches_library():
+    """include/tblis.hpp: every tblis::mult overload instantiates for float
+    and double, links against the library, and an invalid call made through
+    it comes back with the library's message (no device access)."""
+    import tempfile
+    prog = r'''
+#include <cstdio>
+#include "tblis.hpp"
+template <typename T> void all()
+{
+    tblis::varray_view<T> t{nullptr, {2, 3}, {1, 2}};
+    tblis::matrix_view<T> m{nullptr, 2, 2, 1, 2};
+    tblis::dpd_varray_view<T> d{nullptr, 0, 1, {{2}, {2}}};
+    tblis::indexed_varray_view<T> x{{nullptr}, {2}, {1}, {2}, {0}, {}};
+    if (false)
+    {
+        tblis::mult<T>(1, t, "ab", t, "bc", 0, t, "ac");
+        tblis::mult<T>(1, m, m, 0, m);
+        tblis::mult<T>(1, d, "ab", d, "bc", 0, d, "ac");
+    }
+    tblis::mult<T>(1, x, "az", x, "bz", 0, x, "ab");
+    std::printf("%s\n", tblis_hip_get_last_error());
+}
+int main() { tblis_hip_set_error_mode(1); all<float>(); all<double>(); return 0; }
+'''
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "probe.cpp"), os.path.join(d, "probe")
+        with open(src, "w") as f:
+            f.write(prog)
+        libdir = os.path.join(ROOT, "tblis_amd")
+        subprocess.run(["g++", "-std=c++14", "-I", os.path.join(ROOT, "include"), src, "-o", exe,
+                        "-L", libdir, "-ltblis_amd", "-Wl,-rpath," + libdir], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.splitlines()
+    assert out == ["A: null entry pointer"] * 2, out

@@ -313,6 +313,46 @@ def level1(mode, alpha, A, lenA, strA, idxA, beta, B, lenB, strB, idxB):
     return out
 
 
+# ---------------------------------------------------------------- complex
+
+def complex_tensor_mult(alpha, A, lenA, strA, idxA, B, lenB, strB, idxB, beta, C_, lenC, strC, idxC,
+                        conj_A=False, conj_B=False):
+    """Complex contraction on interleaved host storage (complex128 arrays,
+    strides in complex elements) from four real contractions of the oracle
+    (oracle.c) over the real and imaginary views (strides doubled, offset 0
+    or 1), combined by complex arithmetic: P = op(A) op(B),
+    C <- alpha P + beta C (the reference's dense call for its complex types,
+    src/iface/3t/mult.cxx:24-165; op conjugates when conj_X is set)."""
+    ra = np.ascontiguousarray(A).view(np.float64)
+    rb = np.ascontiguousarray(B).view(np.float64)
+    s2 = lambda st: [2 * x for x in st]
+    sa = -1.0 if conj_A else 1.0
+    sb = -1.0 if conj_B else 1.0
+    nc = int(np.asarray(C_).size)
+
+    def real_prod(pa, pb):
+        out = np.zeros(2 * nc)
+        return np.asarray(tensor_mult(1.0, ra[pa:], lenA, s2(strA), idxA, rb[pb:], lenB, s2(strB), idxB,
+                                      0.0, out, lenC, s2(strC), idxC))
+
+    rr = real_prod(0, 0)
+    ii = real_prod(1, 1)
+    ri = real_prod(0, 1)
+    ir = real_prod(1, 0)
+    # the real contractions write C's layout at even offsets (stride 2)
+    P = (rr - sa * sb * ii)[0::2] + 1j * (sb * ri + sa * ir)[0::2]
+    c = np.array(C_, np.complex128)
+    # elements the view of C addresses (the rest of the storage is untouched)
+    off = np.zeros(1, np.int64)
+    for n, st in zip(lenC, strC):
+        off = (off[:, None] + np.arange(n, dtype=np.int64)[None, :] * st).ravel()
+    if any(n == 0 for n in lenC):
+        return c
+    hit = np.unique(off)
+    c[hit] = complex(alpha) * P[hit] + (complex(beta) * c[hit] if beta != 0 else 0)
+    return c
+
+
 # ---------------------------------------------------------------- indexed
 
 def indexed_full(entries, indices, factors, dense_len, idx_len):

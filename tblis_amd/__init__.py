@@ -185,14 +185,22 @@ def _type_code(dtype):
         return TYPE_SINGLE
     if dtype == torch.float64:
         return TYPE_DOUBLE
+    if dtype == torch.complex64:
+        return TYPE_SCOMPLEX
+    if dtype == torch.complex128:
+        return TYPE_DCOMPLEX
     raise TblisError("unsupported dtype %s" % dtype)
 
 
 def _set_scalar(s, value, type_code):
     if type_code == TYPE_SINGLE:
         s.data.s = value
-    else:
+    elif type_code == TYPE_DOUBLE:
         s.data.d = value
+    elif type_code == TYPE_SCOMPLEX:
+        s.data.c[0], s.data.c[1] = complex(value).real, complex(value).imag
+    else:
+        s.data.z[0], s.data.z[1] = complex(value).real, complex(value).imag
     s.type = type_code
 
 
@@ -242,9 +250,10 @@ class _Prepared:
             raise TblisError(err)
 
 
-def prepare_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None):
+def prepare_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None, conj_A=False, conj_B=False):
     """Build the descriptors of mult() once; returns a callable."""
     dA, dB, dC = _Desc(A, alpha), _Desc(B), _Desc(C_, beta)
+    dA.d.conj, dB.d.conj = int(bool(conj_A)), int(bool(conj_B))
     cfg = get_config(config) if config else None
     args = (None, cfg, C.byref(dA.d), _labels(idx_A, A.dim(), "A"), C.byref(dB.d),
             _labels(idx_B, B.dim(), "B"), C.byref(dC.d), _labels(idx_C, C_.dim(), "C"))
@@ -259,11 +268,12 @@ def prepare_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None):
     return _Prepared(run, (), (dA, dB, dC))
 
 
-def mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None):
-    """C_ <- alpha * A.B + beta * C_ (tblis::mult).  Tensors: torch float32
-    or float64 on the current HIP device, any strides."""
+def mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None, conj_A=False, conj_B=False):
+    """C_ <- alpha * A.B + beta * C_ (tblis::mult).  Tensors: torch float32,
+    float64, complex64 or complex128 on the current HIP device, any strides;
+    conj_A / conj_B conjugate a complex operand."""
     try:
-        prepare_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config)()
+        prepare_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config, conj_A, conj_B)()
     except TblisError as e:
         raise TblisError("tblis_tensor_mult: %s" % e) from None
     return C_

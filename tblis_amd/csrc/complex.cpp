@@ -1,0 +1,259 @@
+// Complex contractions (scomplex / dcomplex) on the real panel path.
+//
+// An interleaved complex tensor with strides s (complex elements) is two
+// real tensors: the real parts at real offset 0 with strides 2s, the
+// imaginary parts at offset 1.  With P = op(A) op(B) (op: conjugation when
+// the descriptor's conj flag is set, sa / sb = -1),
+//     Re P = Ar Br - sa sb Ai Bi,    Im P = sb Ar Bi + sa Ai Br,
+// and C <- alpha P + beta C splits into two real panel GEMMs, one writing
+// Re C and one Im C, whose k blocks are the terms above, each scaled while
+// packed (a complex alpha is folded into those scales: 4 terms per part).
+// A real beta goes to the GEMM epilogue; a complex beta is applied to C by
+// a pass before the GEMMs, which then accumulate with beta = 1.  The
+// canonical form, the reference's special cases (alpha == 0, k == 0, empty
+// C) and the C <- scalar = 1 side effect are those of the real call
+// (src/iface/3t/mult.cxx:24-165); the reference's mult_blis asserts that no
+// operand is conjugated (src/internal/3t/dense/mult.cxx:212), which this
+// library relaxes.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+
+#include "contract.hpp"
+#include "level1.hpp"
+#include "panel.hpp"
+#include "plan.hpp"
+
+namespace tblis_amd
+{
+
+namespace
+{
+
+template <typename T>
+void put(std::string& k, const T& v)
+{
+    k.append((const char*)&v, sizeof(T));
+}
+
+struct Cplx
+{
+    double re = 0, im = 0;
+};
+
+Cplx scalar_of(const tblis_scalar& s, type_t t)
+{
+    if (t == TYPE_SCOMPLEX) return Cplx{s.data.c[0], s.data.c[1]};
+    return Cplx{s.data.z[0], s.data.z[1]};
+}
+
+// alpha = alpha_A * alpha_B, rounded to the tensor's precision
+Cplx product(const tblis_scalar& a, const tblis_scalar& b, type_t t)
+{
+    if (t == TYPE_SCOMPLEX)
+    {
+        const float ar = a.data.c[0], ai = a.data.c[1], br = b.data.c[0], bi = b.data.c[1];
+        return Cplx{(double)(ar * br - ai * bi), (double)(ar * bi + ai * br)};
+    }
+    const double ar = a.data.z[0], ai = a.data.z[1], br = b.data.z[0], bi = b.data.z[1];
+    return Cplx{ar * br - ai * bi, ar * bi + ai * br};
+}
+
+Side twice(Side s)
+{
+    for (auto& x : s.stride) x *= 2;
+    return s;
+}
+
+PBlock part(const PBlock& b, int im)
+{
+    PBlock r;
+    r.base = b.base == PANEL_MISSING ? PANEL_MISSING : 2 * b.base + im;
+    r.rows = twice(b.rows);
+    r.k = twice(b.k);
+    return r;
+}
+
+struct Term
+{
+    int a_im, b_im;  // 0: real parts, 1: imaginary parts
+    double scale;
+};
+
+// One complex GEMM as the two real GEMMs of Re C and Im C.
+void split_gemm(const PanelGemm& g, const std::vector<Term> (&terms)[2], std::vector<PanelGemm>& out)
+{
+    const size_t NM = g.mlen.size(), NN = g.nlen.size(), NK = g.klen.size();
+    for (int ci = 0; ci < 2; ci++)
+    {
+        const std::vector<Term>& ts = terms[ci];
+        PanelGemm r;
+        r.mlen = g.mlen;
+        r.nlen = g.nlen;
+        const size_t NK2 = ts.size() * NK;
+        for (const Term& t : ts)
+            for (size_t q = 0; q < NK; q++)
+            {
+                r.klen.push_back(g.klen[q]);
+                r.kscale.push_back(t.scale);
+            }
+        r.a.resize(NM * NK2);
+        r.b.resize(NN * NK2);
+        for (size_t ti = 0; ti < ts.size(); ti++)
+            for (size_t q = 0; q < NK; q++)
+            {
+                for (size_t p = 0; p < NM; p++) r.a[p * NK2 + ti * NK + q] = part(g.a[p * NK + q], ts[ti].a_im);
+                for (size_t s = 0; s < NN; s++) r.b[s * NK2 + ti * NK + q] = part(g.b[s * NK + q], ts[ti].b_im);
+            }
+        for (const CBlock& c : g.c)
+        {
+            CBlock d;
+            d.base = 2 * c.base + ci;
+            d.rows = twice(c.rows);
+            d.cols = twice(c.cols);
+            r.c.push_back(d);
+        }
+        out.push_back(std::move(r));
+    }
+}
+
+// Complex scale of C over its dimensions (smallest stride first, contiguous
+// dimensions folded), as a level-1 descriptor in complex elements.
+Level1Desc c_desc(const tblis_tensor& C)
+{
+    std::vector<int> order(C.ndim);
+    for (unsigned i = 0; i < C.ndim; i++) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return std::llabs(C.stride[a]) < std::llabs(C.stride[b]); });
+    Level1Desc d;
+    for (int i : order)
+    {
+        d.nb *= C.len[i];
+        if (C.len[i] == 1) continue;
+        if (d.ndb > 0 && C.stride[i] == d.sb[d.ndb - 1] * d.len_b[d.ndb - 1])
+        {
+            d.len_b[d.ndb - 1] *= C.len[i];
+            continue;
+        }
+        if (d.ndb >= LEVEL1_MAXDIM) fail("complex: more than 16 dimensions of C after folding");
+        d.len_b[d.ndb] = C.len[i];
+        d.sb[d.ndb] = C.stride[i];
+        d.sa_b[d.ndb] = 0;
+        d.ndb++;
+    }
+    return d;
+}
+
+}  // namespace
+
+void complex_tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_type* idx_A,
+                         const tblis_tensor* B, const label_type* idx_B, tblis_tensor* C,
+                         const label_type* idx_C)
+{
+    const type_t type = A->type;
+    const type_t rtype = type == TYPE_SCOMPLEX ? TYPE_SINGLE : TYPE_DOUBLE;
+    Algo algo = resolve_algo(cfg, rtype);
+    if (algo == Algo::Reference) algo = rtype == TYPE_DOUBLE ? Algo::Fp64Mfma : Algo::Bf16x3;
+    thread_state().last_algo = algo_name(algo);
+
+    const Cplx alpha = product(A->scalar, B->scalar, type);
+    const Cplx beta = scalar_of(C->scalar, type);
+    const double sa = A->conj ? -1.0 : 1.0, sb = B->conj ? -1.0 : 1.0;
+    const bool fold_alpha = alpha.im != 0.0;
+
+    std::string key;
+    put(key, (int32_t)algo);
+    put(key, (int32_t)type);
+    for (auto [t, idx] : {std::pair<const tblis_tensor*, const label_type*>{A, idx_A}, {B, idx_B}, {C, idx_C}})
+    {
+        put(key, (uint32_t)t->ndim);
+        key.append((const char*)t->len, sizeof(len_type) * t->ndim);
+        key.append((const char*)t->stride, sizeof(stride_type) * t->ndim);
+        key.append(idx, t->ndim);
+    }
+    put(key, sa);
+    put(key, sb);
+    if (fold_alpha)
+    {
+        put(key, alpha.re);
+        put(key, alpha.im);
+    }
+
+    std::shared_ptr<Plan> plan = cache_get(key);
+    if (!plan)
+    {
+        Canonical c;
+        std::string why;
+        if (!canonicalize(*A, idx_A, *B, idx_B, *C, idx_C, c, why)) fail("invalid contraction: " + why);
+        GemmDesc g;
+        if (dense_gemm(c, g, 0, 0, 0))
+        {
+            std::vector<Term> terms[2];
+            if (!fold_alpha)
+            {
+                terms[0] = {{0, 0, 1.0}, {1, 1, -sa * sb}};
+                terms[1] = {{0, 1, sb}, {1, 0, sa}};
+            }
+            else
+            {
+                const double ar = alpha.re, ai = alpha.im;
+                terms[0] = {{0, 0, ar}, {1, 1, -ar * sa * sb}, {0, 1, -ai * sb}, {1, 0, -ai * sa}};
+                terms[1] = {{0, 1, ar * sb}, {1, 0, ar * sa}, {0, 0, ai}, {1, 1, -ai * sa * sb}};
+            }
+            for (auto& ts : terms)
+            {
+                std::vector<Term> keep;
+                for (const Term& t : ts)
+                    if (t.scale != 0.0) keep.push_back(t);
+                ts.swap(keep);
+            }
+            std::vector<PanelGemm> pg, cg;
+            panel_from_desc(g, pg);
+            for (const PanelGemm& p : pg) split_gemm(p, terms, cg);
+            PlanBuilder pb(rtype == TYPE_DOUBLE ? 8 : 4, rtype == TYPE_DOUBLE ? 0 : 1, 1, algo_family(algo));
+            if (!pb.panel_route()) fail("complex contractions run on the panel path (TBLIS_HIP_PANEL=0 is set)");
+            for (const PanelGemm& p : cg) pb.add_panel(p);
+            plan = pb.build();
+        }
+        if (!plan) plan = std::make_shared<Plan>();
+        cache_put(key, plan);
+    }
+
+    const bool use_k = alpha.re != 0.0 || alpha.im != 0.0;
+    const bool one = beta.re == 1.0 && beta.im == 0.0;
+    // a complex beta (or alpha == 0 with beta != 1) is applied by the pass
+    const bool pass = !one && (beta.im != 0.0 || !use_k);
+    if (plan->ntasks > 0 && (use_k || !one))
+    {
+        const bool k = use_k && plan->reads_ab;
+        if (!plan->already_verified(A->data, B->data, C->data, k))
+        {
+            check_data(C, "C");
+            if (k)
+            {
+                check_data(A, "A");
+                check_data(B, "B");
+            }
+            plan->mark_verified(A->data, B->data, C->data, k);
+        }
+        ThreadState& ts = thread_state();
+        if (pass)
+        {
+            const hipError_t e = launch_level1_cscale(rtype == TYPE_DOUBLE ? 0 : 1, c_desc(*C), beta.re, beta.im,
+                                                      C->data, ts.stream);
+            if (e != hipSuccess) fail(std::string("complex beta pass: ") + hipGetErrorString(e));
+        }
+        if (use_k)
+            run_plan(plan, A->data, B->data, C->data, fold_alpha ? 1.0 : alpha.re, pass ? 1.0 : beta.re, true);
+        else
+        {
+            hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(ts.stream, &cap) != hipSuccess) fail("hipStreamIsCapturing failed");
+            if (!ts.async && cap == hipStreamCaptureStatusNone && hipStreamSynchronize(ts.stream) != hipSuccess)
+                fail("hipStreamSynchronize failed");
+        }
+    }
+    reset_scalar(C);
+}
+
+}  // namespace tblis_amd

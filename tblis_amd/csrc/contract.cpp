@@ -62,8 +62,12 @@ void reset_scalar(tblis_tensor* C)
 {
     if (C->type == TYPE_SINGLE)
         C->scalar.data.s = 1.0f;
-    else
+    else if (C->type == TYPE_DOUBLE)
         C->scalar.data.d = 1.0;
+    else if (C->type == TYPE_SCOMPLEX)
+        C->scalar.data.c[0] = 1.0f, C->scalar.data.c[1] = 0.0f;
+    else
+        C->scalar.data.z[0] = 1.0, C->scalar.data.z[1] = 0.0;
     C->conj = 0;
 }
 
@@ -132,6 +136,10 @@ std::vector<int64_t> all_offsets(const Group& g, int slot)
 
 // The GEMM of one canonical dense contraction (m = AC, n = BC, k = AB,
 // batch = ABC), with bases added for batched calls.
+int dtype_code(type_t t) { return t == TYPE_DOUBLE ? 0 : 1; }
+
+}  // namespace
+
 bool dense_gemm(const Canonical& c, GemmDesc& g, int64_t base_a, int64_t base_b, int64_t base_c)
 {
     MatrixViews v = blis_views(c);
@@ -163,10 +171,6 @@ bool dense_gemm(const Canonical& c, GemmDesc& g, int64_t base_a, int64_t base_b,
     }
     return true;
 }
-
-int dtype_code(type_t t) { return t == TYPE_DOUBLE ? 0 : 1; }
-
-}  // namespace
 
 const char* algo_name(Algo a)
 {
@@ -214,6 +218,8 @@ void tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_typ
     check_tensor(B, idx_B, "B");
     check_tensor(C, idx_C, "C");
     if (A->type != B->type || A->type != C->type) fail("A, B and C must have the same type");
+    if (A->type == TYPE_SCOMPLEX || A->type == TYPE_DCOMPLEX)
+        return complex_tensor_mult(cfg, A, idx_A, B, idx_B, C, idx_C);
     check_real(A->type);
     const type_t type = A->type;
     Algo algo = resolve_algo(cfg, type);

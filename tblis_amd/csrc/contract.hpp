@@ -6,6 +6,8 @@
 namespace tblis_amd
 {
 
+struct GemmDesc;
+
 void tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_type* idx_A,
                  const tblis_tensor* B, const label_type* idx_B, tblis_tensor* C,
                  const label_type* idx_C);
@@ -33,6 +35,16 @@ int debug_dpd_patches(const tblis_dpd_tensor* A, const label_type* idx_A,
                       unsigned irrep_ab, int64_t BS_row, int64_t BS_col, int* npatch,
                       int64_t* rows, int64_t* cols, int64_t* base, int64_t* rscat,
                       int64_t* cscat, int64_t max_patch, int64_t max_scat);
+
+// Complex (scomplex / dcomplex) contraction: two real panel GEMMs per
+// complex GEMM (complex.cpp).
+void complex_tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_type* idx_A,
+                         const tblis_tensor* B, const label_type* idx_B, tblis_tensor* C,
+                         const label_type* idx_C);
+
+// The batched GEMM of a canonical contraction (m = AC, n = BC, k = AB,
+// batch = ABC) at the given element bases; false when C is empty.
+bool dense_gemm(const Canonical& c, GemmDesc& g, int64_t base_a, int64_t base_b, int64_t base_c);
 
 // Descriptor helpers shared by the entry points.
 int elem_size(type_t t);

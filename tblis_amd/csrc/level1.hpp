@@ -29,6 +29,9 @@ struct Level1Desc
 // entry offsets and factors in device memory).
 hipError_t launch_level1_entries(int dtype, const Level1Desc& d, const int64_t* offs, const double* fac,
                                  int64_t nent, double beta, void* B, hipStream_t s);
+// C <- (br + i bi) C over an interleaved complex tensor of real type dtype
+// (d in complex elements).
+hipError_t launch_level1_cscale(int dtype, const Level1Desc& d, double br, double bi, void* B, hipStream_t s);
 // Device workspace a call needs (partial sums of a split A-only sum), bytes.
 size_t level1_workspace_bytes(int mode, const Level1Desc& d);
 hipError_t launch_level1(int mode, int dtype, const Level1Desc& d, const void* A, void* B,

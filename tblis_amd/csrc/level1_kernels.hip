@@ -379,6 +379,41 @@ __global__ __launch_bounds__(kThreads) void level1_entries_kernel(Level1Desc d, 
     }
 }
 
+// C <- beta C for an interleaved complex tensor (d in complex elements):
+// beta == 0 writes zeros without reading.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void level1_cscale_kernel(Level1Desc d, double br, double bi,
+                                                                 T* __restrict__ B)
+{
+    const int64_t l0 = d.ndb > 0 ? d.len_b[0] : 1, sb0 = d.ndb > 0 ? d.sb[0] : 0;
+    const int64_t nrows = d.nb / l0, nseg = (l0 + kSeg - 1) / kSeg;
+    const bool zero = br == 0.0 && bi == 0.0;
+    for (int64_t blk = blockIdx.x; blk < nrows * nseg; blk += gridDim.x)
+    {
+        const int64_t row = blk / nseg, seg = blk - row * nseg;
+        int64_t ob, oa;
+        row_offsets(d, row, 1, -1, ob, oa);
+#pragma unroll
+        for (int j = 0; j < kPerThread; j++)
+        {
+            const int64_t col = seg * kSeg + threadIdx.x + j * kThreads;
+            if (col >= l0) break;
+            T* p = B + 2 * (ob + col * sb0);
+            if (zero)
+            {
+                p[0] = (T)0;
+                p[1] = (T)0;
+            }
+            else
+            {
+                const double re = (double)p[0], im = (double)p[1];
+                p[0] = (T)(br * re - bi * im);
+                p[1] = (T)(br * im + bi * re);
+            }
+        }
+    }
+}
+
 unsigned grid_for(int64_t work)
 {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(work, kMaxBlocks));
@@ -516,6 +551,18 @@ hipError_t launch_level1_entries(int dtype, const Level1Desc& d, const int64_t* 
     else
         hipLaunchKernelGGL((level1_entries_kernel<float>), dim3(grid), dim3(kThreads), 0, s, d, offs, fac, nent,
                            beta, (float*)B);
+    return hipGetLastError();
+}
+
+hipError_t launch_level1_cscale(int dtype, const Level1Desc& d, double br, double bi, void* B, hipStream_t s)
+{
+    if (d.nb == 0) return hipSuccess;
+    const int64_t l0 = d.ndb > 0 ? d.len_b[0] : 1;
+    const unsigned grid = grid_for((d.nb / l0) * ((l0 + kSeg - 1) / kSeg));
+    if (dtype == 0)
+        hipLaunchKernelGGL((level1_cscale_kernel<double>), dim3(grid), dim3(kThreads), 0, s, d, br, bi, (double*)B);
+    else
+        hipLaunchKernelGGL((level1_cscale_kernel<float>), dim3(grid), dim3(kThreads), 0, s, d, br, bi, (float*)B);
     return hipGetLastError();
 }
 

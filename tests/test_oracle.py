@@ -484,3 +484,25 @@ def test_indexed_oracle_fully_populated_is_dense_contraction():
     want = np.asarray(want).reshape(lens["C"], order="F")
     for e, ix in zip(got, C0["indices"]):
         np.testing.assert_allclose(e, want[(slice(None),) + tuple(ix)], rtol=1e-13, atol=1e-13)
+
+
+# ---------------------------------------------------------------- complex
+@pytest.mark.parametrize("seed", range(6))
+def test_complex_oracle_against_numpy(seed):
+    """The complex restatement (four real oracle contractions combined)
+    against numpy's complex einsum, with conjugation and complex scalars."""
+    from tests.helpers import layout_for, random_contraction, storage_size
+    rng = np.random.default_rng(64000 + seed)
+    iA, iB, iC, ext = random_contraction(rng, allow_diag=False)
+    (lA, sA), (lB, sB), (lC, sC) = [layout_for(i, ext, rng, "random") for i in (iA, iB, iC)]
+    mk = lambda l, s: rng.uniform(-1, 1, storage_size(l, s)) + 1j * rng.uniform(-1, 1, storage_size(l, s))
+    A, B, C0 = mk(lA, sA), mk(lB, sB), mk(lC, sC)
+    alpha, beta = [(0.5 - 0.25j, 1.5 + 0.5j), (2.0, 0.0), (1j, -1.0)][seed % 3]
+    cA, cB = bool(seed & 1), bool(seed & 2)
+    got = po.complex_tensor_mult(alpha, A, lA, sA, iA, B, lB, sB, iB, beta, C0, lC, sC, iC, cA, cB)
+    view = lambda X, l, s: np.lib.stride_tricks.as_strided(X, l, [16 * x for x in s])
+    a, b = view(A, lA, sA), view(B, lB, sB)
+    a = np.conj(a) if cA else a
+    b = np.conj(b) if cB else b
+    want_v = alpha * np.einsum("%s,%s->%s" % (iA, iB, iC), a, b) + beta * view(C0.copy(), lC, sC)
+    np.testing.assert_allclose(view(got, lC, sC), want_v, rtol=1e-12, atol=1e-12)

@@ -20,10 +20,13 @@ makes it the headline):
   dpd          8-irrep DPD C[abij] = A[abef]*B[efij], per-irrep extents
                [12,9,7,5,11,8,6,4]: FP64 and split-BF16, and the same kernels
                on the dense 62^4 contraction (dense-equivalent throughput),
-               each call replayed from a hipGraph
+               as back-to-back asynchronous calls and replayed from a hipGraph
   batch        4096 independent C[abij] = A[abef]*B[efij], extents drawn from
                4..40 per instance, FP64 and split-BF16, one hipGraph replay
                per step
+  complex      the headline contraction in complex128 (FP64 MFMA) and
+               complex64 (split-BF16): 8*m*n*k real flop per call, against
+               the real FP64 rate on the same shape
   indexed      indexed contraction C[abij] += A[abef]*B[efij], a=b=e=i=j=48
                dense, f indexed (32 values; A and B each list 24, seeded), FP64
                and split-BF16; useful TFLOP/s over the matching f values, and
@@ -191,10 +194,13 @@ def dpd_leg(T, dev, stream, steps, warmup):
     for name, a, b, c in legs:
         fn = T.prepare_dpd_mult(1.0, a, 0, "abef", b, 0, "efij", 0.0, c, 0, "abij", nirrep,
                                 lens, lens, lens)
+        # back-to-back asynchronous calls, then the same call replayed from a hipGraph
+        ms_eager, _, _ = time_steps(fn, steps, warmup, stream)
         graph = capture(T, fn, stream)
-        ms_step, ms_kern, _ = time_steps(graph.replay, steps, warmup, torch.cuda.current_stream())
-        out[name] = {"algorithm": T.last_algorithm(), "tile": T.last_tile(), "ms": round(ms_step, 4),
-                     "useful_tflops": tflops(flops, ms_step), "launch": "one hipGraph replay"}
+        ms_graph, _, _ = time_steps(graph.replay, steps, warmup, torch.cuda.current_stream())
+        out[name] = {"algorithm": T.last_algorithm(), "tile": T.last_tile(), "ms": round(ms_eager, 4),
+                     "useful_tflops": tflops(flops, ms_eager), "launch": "asynchronous calls",
+                     "graph_ms": round(ms_graph, 4), "graph_useful_tflops": tflops(flops, ms_graph)}
     out["split_bf16"]["rel_err_vs_fp64"] = float("%.3e" % rel_fro(Cf, Cd))
     # dense-equivalent: the same kernels on C[abij] = A[abef]*B[efij], extents 62
     L = [62] * 4
@@ -204,11 +210,13 @@ def dpd_leg(T, dev, stream, steps, warmup):
         X = [torch.rand(62 ** 4, dtype=dt, device=dev, generator=gd) * 2 - 1 for _ in range(3)]
         fn = T.prepare_mult(1.0, strided(X[0], L), "abef", strided(X[1], L), "efij", 0.0,
                             strided(X[2], L), "abij")
+        ms_eager, _, _ = time_steps(fn, steps, warmup, stream)
         graph = capture(T, fn, stream)
-        ms_step, _, _ = time_steps(graph.replay, steps, warmup, torch.cuda.current_stream())
-        dense = tflops(dflops, ms_step)
+        ms_graph, _, _ = time_steps(graph.replay, steps, warmup, torch.cuda.current_stream())
+        dense, dense_g = tflops(dflops, ms_eager), tflops(dflops, ms_graph)
         out[name]["dense62_useful_tflops"] = dense
         out[name]["frac_of_dense"] = round(out[name]["useful_tflops"] / dense, 3)
+        out[name]["graph_frac_of_dense"] = round(out[name]["graph_useful_tflops"] / dense_g, 3)
     return out
 
 
@@ -261,6 +269,31 @@ def batch_leg(T, dev, stream, steps, warmup, count=4096, seed=7):
     got = torch.cat([c.reshape(-1) for c in insts["split_bf16"][2]])
     want = torch.cat([c.reshape(-1) for c in insts["fp64"][2]])
     out["split_bf16"]["rel_err_vs_fp64"] = float("%.3e" % rel_fro(got, want))
+    return out
+
+
+# ---------------------------------------------------------------- complex
+
+def complex_leg(T, dev, stream, steps, warmup):
+    import torch
+    ext = dict(a=64, b=64, e=64, f=64, i=32, j=32)
+    lens = lambda s: [ext[c] for c in s]
+    lA, lB, lC = lens("aebf"), lens("fjei"), lens("aibj")
+    flops = 8.0 * (64 * 64) * (32 * 32) * (64 * 64)
+    out = {"workload": "C[aibj] += (0.5+0.25i)*A[aebf]*B[fjei], a=b=e=f=64, i=j=32, column-major, "
+                       "complex", "useful_flops_real": flops}
+    g = torch.Generator(device=dev).manual_seed(9)
+    for name, dt in (("complex128", torch.complex128), ("complex64", torch.complex64)):
+        rt = torch.float64 if dt == torch.complex128 else torch.float32
+        mk = lambda n: torch.complex(torch.rand(n, dtype=rt, device=dev, generator=g) * 2 - 1,
+                                     torch.rand(n, dtype=rt, device=dev, generator=g) * 2 - 1)
+        A, B, C = mk(int(np.prod(lA))), mk(int(np.prod(lB))), mk(int(np.prod(lC)))
+        fn = T.prepare_mult(0.5 + 0.25j, strided(A, lA), "aebf", strided(B, lB), "fjei", 1.0,
+                            strided(C, lC), "aibj")
+        ms_step, _, _ = time_steps(fn, steps, warmup, stream)
+        out[name] = {"algorithm": T.last_algorithm(), "tile": T.last_tile(), "ms": round(ms_step, 4),
+                     "useful_tflops": tflops(flops, ms_step)}
+        del A, B, C
     return out
 
 
@@ -421,7 +454,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--only", choices=["permuted", "fp32_96", "dpd", "batch", "indexed", "level1"],
+    ap.add_argument("--only", choices=["permuted", "fp32_96", "dpd", "batch", "complex", "indexed",
+                                       "level1"],
                     help="run one leg alone (for profiling) and report it as the headline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-traffic", action="store_true",
@@ -469,6 +503,10 @@ def main():
     if args.only in (None, "batch"):
         legs["batch"] = batch_leg(T, dev, stream, few if args.only is None else args.steps,
                                   min(args.warmup, 2))
+        torch.cuda.empty_cache()
+    if args.only in (None, "complex"):
+        legs["complex"] = complex_leg(T, dev, stream, few if args.only is None else args.steps,
+                                      min(args.warmup, 2))
         torch.cuda.empty_cache()
     if args.only in (None, "indexed"):
         legs["indexed"] = indexed_leg(T, dev, stream, few if args.only is None else args.steps,

@@ -6,10 +6,12 @@
 // the descriptor's conj flag is set, sa / sb = -1),
 //     Re P = Ar Br - sa sb Ai Bi,    Im P = sb Ar Bi + sa Ai Br,
 // and C <- alpha P + beta C splits into two real panel GEMMs, one writing
-// Re C and one Im C, whose k blocks are the terms above, each scaled while
-// packed (a complex alpha is folded into those scales: 4 terms per part).
-// A real beta goes to the GEMM epilogue; a complex beta is applied to C by
-// a pass before the GEMMs, which then accumulate with beta = 1.  The
+// Re C and one Im C, whose k blocks are the terms above, each scaled by its
+// sign while packed.  A real alpha and beta go to the GEMM epilogue.  A
+// complex beta is applied to C by a pass before the GEMMs, which then
+// accumulate with beta = 1; a complex alpha by C <- (beta / alpha) C before
+// and C <- alpha C after GEMMs that accumulate P (two passes over C instead
+// of twice the MFMA work of folding alpha into the terms).  The
 // canonical form, the reference's special cases (alpha == 0, k == 0, empty
 // C) and the C <- scalar = 1 side effect are those of the real call
 // (src/iface/3t/mult.cxx:24-165); the reference's mult_blis asserts that no
@@ -159,7 +161,6 @@ void complex_tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const l
     const Cplx alpha = product(A->scalar, B->scalar, type);
     const Cplx beta = scalar_of(C->scalar, type);
     const double sa = A->conj ? -1.0 : 1.0, sb = B->conj ? -1.0 : 1.0;
-    const bool fold_alpha = alpha.im != 0.0;
 
     std::string key;
     put(key, (int32_t)algo);
@@ -173,11 +174,6 @@ void complex_tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const l
     }
     put(key, sa);
     put(key, sb);
-    if (fold_alpha)
-    {
-        put(key, alpha.re);
-        put(key, alpha.im);
-    }
 
     std::shared_ptr<Plan> plan = cache_get(key);
     if (!plan)
@@ -189,24 +185,8 @@ void complex_tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const l
         if (dense_gemm(c, g, 0, 0, 0))
         {
             std::vector<Term> terms[2];
-            if (!fold_alpha)
-            {
-                terms[0] = {{0, 0, 1.0}, {1, 1, -sa * sb}};
-                terms[1] = {{0, 1, sb}, {1, 0, sa}};
-            }
-            else
-            {
-                const double ar = alpha.re, ai = alpha.im;
-                terms[0] = {{0, 0, ar}, {1, 1, -ar * sa * sb}, {0, 1, -ai * sb}, {1, 0, -ai * sa}};
-                terms[1] = {{0, 1, ar * sb}, {1, 0, ar * sa}, {0, 0, ai}, {1, 1, -ai * sa * sb}};
-            }
-            for (auto& ts : terms)
-            {
-                std::vector<Term> keep;
-                for (const Term& t : ts)
-                    if (t.scale != 0.0) keep.push_back(t);
-                ts.swap(keep);
-            }
+            terms[0] = {{0, 0, 1.0}, {1, 1, -sa * sb}};
+            terms[1] = {{0, 1, sb}, {1, 0, sa}};
             std::vector<PanelGemm> pg, cg;
             panel_from_desc(g, pg);
             for (const PanelGemm& p : pg) split_gemm(p, terms, cg);
@@ -221,8 +201,17 @@ void complex_tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const l
 
     const bool use_k = alpha.re != 0.0 || alpha.im != 0.0;
     const bool one = beta.re == 1.0 && beta.im == 0.0;
-    // a complex beta (or alpha == 0 with beta != 1) is applied by the pass
-    const bool pass = !one && (beta.im != 0.0 || !use_k);
+    const bool bzero = beta.re == 0.0 && beta.im == 0.0;
+    const bool calpha = alpha.im != 0.0;
+    // C <- pre C before the GEMMs (pre: beta, or beta / alpha for a complex
+    // alpha), C <- alpha C after them for a complex alpha
+    Cplx pre = beta;
+    if (calpha && !bzero)
+    {
+        const double d = alpha.re * alpha.re + alpha.im * alpha.im;
+        pre = Cplx{(beta.re * alpha.re + beta.im * alpha.im) / d, (beta.im * alpha.re - beta.re * alpha.im) / d};
+    }
+    const bool pass = use_k ? (calpha ? !bzero : beta.im != 0.0) : !one;
     if (plan->ntasks > 0 && (use_k || !one))
     {
         const bool k = use_k && plan->reads_ab;
@@ -237,14 +226,37 @@ void complex_tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const l
             plan->mark_verified(A->data, B->data, C->data, k);
         }
         ThreadState& ts = thread_state();
+        const Level1Desc cd = c_desc(*C);
+        const int dt = rtype == TYPE_DOUBLE ? 0 : 1;
         if (pass)
         {
-            const hipError_t e = launch_level1_cscale(rtype == TYPE_DOUBLE ? 0 : 1, c_desc(*C), beta.re, beta.im,
-                                                      C->data, ts.stream);
+            const hipError_t e = launch_level1_cscale(dt, cd, pre.re, pre.im, C->data, ts.stream);
             if (e != hipSuccess) fail(std::string("complex beta pass: ") + hipGetErrorString(e));
         }
-        if (use_k)
-            run_plan(plan, A->data, B->data, C->data, fold_alpha ? 1.0 : alpha.re, pass ? 1.0 : beta.re, true);
+        if (use_k && calpha)
+        {
+            // GEMMs accumulate P (over C scaled by beta / alpha, or over
+            // nothing when beta == 0), then C <- alpha C; the synchronising
+            // variant of run_plan must not return before the last pass
+            {
+                struct Restore
+                {
+                    ThreadState& t;
+                    bool async;
+                    ~Restore() { t.async = async; }
+                } restore{ts, ts.async};
+                ts.async = true;
+                run_plan(plan, A->data, B->data, C->data, 1.0, bzero ? 0.0 : 1.0, true);
+            }
+            const hipError_t e = launch_level1_cscale(dt, cd, alpha.re, alpha.im, C->data, ts.stream);
+            if (e != hipSuccess) fail(std::string("complex alpha pass: ") + hipGetErrorString(e));
+            hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(ts.stream, &cap) != hipSuccess) fail("hipStreamIsCapturing failed");
+            if (!ts.async && cap == hipStreamCaptureStatusNone && hipStreamSynchronize(ts.stream) != hipSuccess)
+                fail("hipStreamSynchronize failed");
+        }
+        else if (use_k)
+            run_plan(plan, A->data, B->data, C->data, alpha.re, pass ? 1.0 : beta.re, true);
         else
         {
             hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;

@@ -322,17 +322,25 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_panel_kernel(
 #pragma unroll
                 for (int j = 0; j < NB; j++) b[j] = pb[j * rs + kg * 64];
         };
-        double2 a0[MB], b0[NB], a1[MB], b1[NB];
-        load(a0, b0, 0);
-        load(a1, b1, 1);
-        for (int kg = 0; kg < nk; kg += 2)
+        // k groups in flight: four for waves of at most 32x32 (their MFMA
+        // work per group is short), three for 64x64 waves (as many as fit
+        // in the registers left next to the accumulators at two waves per
+        // SIMD); measured 1-18% faster than two
+        constexpr int PF = MB * NB <= 4 ? 4 : 3;
+        double2 a[PF][MB], b[PF][NB];
+#pragma unroll
+        for (int d = 0; d < PF; d++) load(a[d], b[d], d < nk ? d : nk - 1);
+        for (int kg = 0; kg < nk; kg += PF)
         {
-            // the last iteration re-reads the final groups (in bounds, unused)
-            const int nx = kg + 2 < nk ? kg + 2 : nk - 2;
-            mma_group<MB, NB>(acc, a0, b0);
-            load(a0, b0, nx);
-            mma_group<MB, NB>(acc, a1, b1);
-            load(a1, b1, nx + 1);
+#pragma unroll
+            for (int d = 0; d < PF; d++)
+            {
+                if (kg + d >= nk) break;  // the last pass may be partial
+                mma_group<MB, NB>(acc, a[d], b[d]);
+                // past the end: re-read the final group (in bounds, unused)
+                const int nx = kg + d + PF < nk ? kg + d + PF : nk - 1;
+                load(a[d], b[d], nx);
+            }
         }
     }
 

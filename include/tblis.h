@@ -305,6 +305,43 @@ void tblis_indexed_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
                                const tblis_indexed_tensor* B, const label_type* idx_B,
                                tblis_indexed_tensor* C, const label_type* idx_C);
 
+/* Indexed DPD tensors (MArray indexed_dpd_varray,
+ * src/external/marray/include/indexed_dpd_varray_base.hpp:21-213): the
+ * first dense_ndim dimensions form a DPD tensor of irrep
+ * irrep ^ idx_irrep[0] ^ ... in the given layout (lengths len[d*nirrep + r]);
+ * indexed dimension j has the fixed irrep idx_irrep[j] and its index values
+ * count within that irrep.  Entry i is the DPD storage at data[i] (device
+ * memory), at indices[i * (ndim - dense_ndim) + j], times factor[i]. */
+typedef struct tblis_indexed_dpd_tensor
+{
+    type_t type;
+    int conj;
+    tblis_scalar scalar;
+    unsigned ndim;
+    unsigned dense_ndim;
+    unsigned irrep;
+    unsigned nirrep;
+    int layout;                 /* tblis_dpd_layout */
+    const len_type* len;        /* ndim x nirrep, row major */
+    const unsigned* idx_irrep;  /* ndim - dense_ndim */
+    len_type num_indices;
+    void* const* data;
+    const len_type* indices;
+    const void* factor;
+} tblis_indexed_dpd_tensor;
+
+/* Indexed DPD contraction (src/iface/3t/mult.h:95-114,
+ * src/iface/3t/mult.cxx:413-540): the same rules as
+ * tblis_indexed_tensor_mult, block by block.  Computed as the reference's
+ * FULL algorithm (src/internal/3t/indexed_dpd/mult.cxx:68-118): the
+ * operands' DPD parts are expanded to dense entries on the device, the
+ * indexed contraction runs on them, and C's symmetry-allowed blocks are
+ * written back. */
+void tblis_indexed_dpd_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
+                                   const tblis_indexed_dpd_tensor* A, const label_type* idx_A,
+                                   const tblis_indexed_dpd_tensor* B, const label_type* idx_B,
+                                   tblis_indexed_dpd_tensor* C, const label_type* idx_C);
+
 /* ---------------------------------------------------------------------- */
 /* Extension: a batch of independent contractions of uneven shapes, run   */
 /* in one persistent launch.  The reference has no counterpart; its        */

@@ -396,6 +396,75 @@ def indexed_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C):
     return out
 
 
+# ---------------------------------------------------------------- indexed DPD
+
+def _irrep_blocks(nirrep, ndim, irrep):
+    """Every irrep combination of ndim dimensions whose XOR is irrep."""
+    import itertools
+    return [c for c in itertools.product(range(nirrep), repeat=ndim)
+            if (np.bitwise_xor.reduce(c) if c else 0) == irrep]
+
+
+def indexed_dpd_expand(X):
+    """An indexed DPD operand as a plain indexed one over full extents
+    (block_to_full, src/internal/1t/indexed_dpd/util.hpp): each entry's DPD
+    storage spread into a dense array of the summed per-irrep lengths,
+    index values shifted by the offset of their irrep."""
+    lens, nir, nd = X["lens"], X["nirrep"], X["dense_ndim"]
+    full = [int(sum(l)) for l in lens]
+    ioff = [np.concatenate([[0], np.cumsum(l)[:-1]]).astype(int) for l in lens]
+    dense_irrep = X["irrep"]
+    for r in X["idx_irrep"]:
+        dense_irrep ^= r
+    d = dpd_init(dense_irrep, nir, [lens[i] for i in range(nd)], X["layout"]) if nd else None
+    entries = []
+    for e in X["entries"]:
+        out = np.zeros(full[:nd])
+        if nd:
+            for irr in _irrep_blocks(nir, nd, dense_irrep):
+                bl, off, st = dpd_get_block(d, list(irr))
+                if 0 in bl:
+                    continue
+                src = np.lib.stride_tricks.as_strided(np.asarray(e, np.float64)[off:], bl,
+                                                      [8 * x for x in st])
+                out[tuple(slice(ioff[i][irr[i]], ioff[i][irr[i]] + bl[i]) for i in range(nd))] = src
+        else:
+            out = np.asarray(e, np.float64).reshape(())
+        entries.append(out)
+    idx = [tuple(ioff[nd + j][X["idx_irrep"][j]] + v for j, v in enumerate(ix)) for ix in X["indices"]]
+    return dict(entries=entries, indices=idx, factors=list(X["factors"]), dense_len=full[:nd],
+                idx_len=full[nd:]), d, dense_irrep, ioff
+
+
+def indexed_dpd_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C):
+    """Indexed DPD contraction on host storage by the reference's FULL
+    algorithm (src/internal/3t/indexed_dpd/mult.cxx:68-118): A, B and C
+    expanded to full indexed tensors, the indexed oracle above, and C's
+    symmetry-allowed blocks read back.  Operands are dicts with "entries"
+    (1-D DPD storage), "indices" (irrep-local), "factors", "irrep",
+    "nirrep", "lens" (ndim x nirrep), "dense_ndim", "idx_irrep", "layout";
+    returns C's new entries (float64 storage)."""
+    ea, _, _, _ = indexed_dpd_expand(A)
+    eb, _, _, _ = indexed_dpd_expand(B)
+    ec, dc, cirr, ioff = indexed_dpd_expand(C_)
+    res = indexed_mult(alpha, ea, idx_A, eb, idx_B, beta, ec, idx_C)
+    nd, nir = C_["dense_ndim"], C_["nirrep"]
+    out = []
+    for e0, full in zip(C_["entries"], res):
+        e = np.array(e0, np.float64)
+        if nd:
+            for irr in _irrep_blocks(nir, nd, cirr):
+                bl, off, st = dpd_get_block(dc, list(irr))
+                if 0 in bl:
+                    continue
+                dst = np.lib.stride_tricks.as_strided(e[off:], bl, [8 * x for x in st])
+                dst[...] = full[tuple(slice(ioff[i][irr[i]], ioff[i][irr[i]] + bl[i]) for i in range(nd))]
+        else:
+            e[...] = full
+        out.append(e)
+    return out
+
+
 def fill_uniform(n, seed):
     x = np.empty(n, np.float64)
     lib().orc_fill_uniform_d(x.ctypes.data_as(PD), I64(n), C.c_uint64(seed))

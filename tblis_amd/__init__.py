@@ -11,6 +11,7 @@ import ctypes as C
 import os
 
 __all__ = ["lib", "mult", "matrix_mult", "dpd_mult", "mult_batch", "indexed_mult", "Indexed",
+           "indexed_dpd_mult", "IndexedDpd",
            "prepare_indexed_mult", "prepare_mult", "prepare_dpd_mult",
            "prepare_batch", "Tensor", "DpdTensor", "get_config",
            "set_stream", "set_async", "last_error", "last_algorithm", "DPD_LAYOUTS", "TblisError"]
@@ -60,8 +61,16 @@ class IndexedTensor(C.Structure):
                 ("factor", C.c_void_p)]
 
 
+class IndexedDpdTensor(C.Structure):
+    _fields_ = [("type", C.c_int), ("conj", C.c_int), ("scalar", Scalar), ("ndim", C.c_uint),
+                ("dense_ndim", C.c_uint), ("irrep", C.c_uint), ("nirrep", C.c_uint), ("layout", C.c_int),
+                ("len", C.POINTER(C.c_ssize_t)), ("idx_irrep", C.POINTER(C.c_uint)),
+                ("num_indices", C.c_ssize_t), ("data", C.POINTER(C.c_void_p)),
+                ("indices", C.POINTER(C.c_ssize_t)), ("factor", C.c_void_p)]
+
+
 assert C.sizeof(Scalar) == 24 and C.sizeof(Tensor) == 64 and C.sizeof(Matrix) == 72
-assert C.sizeof(IndexedTensor) == 88
+assert C.sizeof(IndexedTensor) == 88 and C.sizeof(IndexedDpdTensor) == 104
 
 _LIB = None
 
@@ -78,7 +87,8 @@ EXPORTED = [
     "tblis_get_num_threads", "tblis_set_num_threads", "tblis_get_config",
     "tblis_init_dpd_tensor_d", "tblis_init_dpd_tensor_s", "tblis_dpd_tensor_size",
     "tblis_dpd_tensor_mult", "tblis_init_indexed_tensor_d", "tblis_init_indexed_tensor_s",
-    "tblis_indexed_tensor_mult", "tblis_tensor_mult_batch", "tblis_hip_set_stream",
+    "tblis_indexed_tensor_mult", "tblis_indexed_dpd_tensor_mult", "tblis_tensor_mult_batch",
+    "tblis_hip_set_stream",
     "tblis_hip_get_stream", "tblis_hip_set_async", "tblis_hip_set_error_mode",
     "tblis_hip_get_last_error", "tblis_hip_clear_plan_cache", "tblis_hip_last_algorithm",
     "tblis_hip_set_autotune", "tblis_hip_autotune_runs", "tblis_hip_last_tile",
@@ -103,6 +113,9 @@ def lib():
         IX = C.POINTER(IndexedTensor)
         L.tblis_indexed_tensor_mult.argtypes = [C.c_void_p, C.c_void_p, IX, C.c_char_p, IX, C.c_char_p,
                                                 IX, C.c_char_p]
+        IXD = C.POINTER(IndexedDpdTensor)
+        L.tblis_indexed_dpd_tensor_mult.argtypes = [C.c_void_p, C.c_void_p, IXD, C.c_char_p, IXD,
+                                                    C.c_char_p, IXD, C.c_char_p]
         L.tblis_tensor_add.argtypes = [C.c_void_p, C.c_void_p, T, C.c_char_p, T, C.c_char_p]
         L.tblis_tensor_scale.argtypes = [C.c_void_p, C.c_void_p, T, C.c_char_p]
         L.tblis_tensor_set.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Scalar), T, C.c_char_p]
@@ -479,6 +492,71 @@ def indexed_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None):
         prepare_indexed_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config)()
     except TblisError as e:
         raise TblisError("tblis_indexed_tensor_mult: %s" % e) from None
+    return C_
+
+
+class IndexedDpd:
+    """An indexed DPD tensor (MArray indexed_dpd_varray): entries are 1-D
+    torch tensors holding the DPD storage of the dense part (the first
+    dense_ndim dimensions, irrep irrep ^ idx_irrep[0] ^ ...); lens is
+    ndim x nirrep; indexed dimension j has irrep idx_irrep[j] and entry i
+    sits at the irrep-local values indices[i]."""
+
+    def __init__(self, entries, indices, idx_irrep, irrep, nirrep, lens, dense_ndim, factors=None,
+                 layout="PREFIX_ROW_MAJOR", dtype=None):
+        self.entries = list(entries)
+        self.indices = [tuple(int(v) for v in ix) for ix in indices]
+        self.idx_irrep = [int(r) for r in idx_irrep]
+        self.irrep, self.nirrep, self.dense_ndim = int(irrep), int(nirrep), int(dense_ndim)
+        self.lens = [[int(x) for x in row] for row in lens]
+        self.factors = None if factors is None else [float(f) for f in factors]
+        self.layout = DPD_LAYOUTS[layout] if isinstance(layout, str) else int(layout)
+        if len(self.indices) != len(self.entries):
+            raise TblisError("IndexedDpd: one index tuple per entry")
+        self.dtype = self.entries[0].dtype if self.entries else dtype
+        if self.dtype is None:
+            raise TblisError("IndexedDpd: dtype is needed when there are no entries")
+
+    def ndim(self):
+        return len(self.lens)
+
+
+class _IxDpdDesc:
+    def __init__(self, X, scalar=1.0):
+        n, nd = len(X.entries), X.ndim()
+        flat = [v for row in X.lens for v in row]
+        self.len = (C.c_ssize_t * max(len(flat), 1))(*flat)
+        self.irr = (C.c_uint * max(len(X.idx_irrep), 1))(*X.idx_irrep)
+        self.data = (C.c_void_p * max(n, 1))(*[e.data_ptr() for e in X.entries])
+        ix = [v for t in X.indices for v in t]
+        self.idx = (C.c_ssize_t * max(len(ix), 1))(*ix)
+        self.d = IndexedDpdTensor()
+        self.d.type = _type_code(X.dtype)
+        _set_scalar(self.d.scalar, scalar, self.d.type)
+        self.d.ndim, self.d.dense_ndim = nd, X.dense_ndim
+        self.d.irrep, self.d.nirrep, self.d.layout = X.irrep, X.nirrep, X.layout
+        self.d.len = C.cast(self.len, C.POINTER(C.c_ssize_t))
+        self.d.idx_irrep = C.cast(self.irr, C.POINTER(C.c_uint))
+        self.d.num_indices = n
+        self.d.data = C.cast(self.data, C.POINTER(C.c_void_p))
+        self.d.indices = C.cast(self.idx, C.POINTER(C.c_ssize_t))
+        self.fac = None
+        if X.factors is not None:
+            ft = C.c_double if self.d.type == TYPE_DOUBLE else C.c_float
+            self.fac = (ft * max(n, 1))(*X.factors)
+            self.d.factor = C.cast(self.fac, C.c_void_p)
+
+
+def indexed_dpd_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None):
+    """Indexed DPD contraction (tblis::mult over indexed_dpd_varray_view,
+    src/iface/3t/mult.h:95-114) over IndexedDpd operands."""
+    dA, dB, dC = _IxDpdDesc(A, alpha), _IxDpdDesc(B), _IxDpdDesc(C_, beta)
+    cfg = get_config(config) if config else None
+    L = lib()
+    L.tblis_hip_set_error_mode(1)
+    L.tblis_indexed_dpd_tensor_mult(None, cfg, C.byref(dA.d), _labels(idx_A, A.ndim(), "A"), C.byref(dB.d),
+                                    _labels(idx_B, B.ndim(), "B"), C.byref(dC.d), _labels(idx_C, C_.ndim(), "C"))
+    _check("tblis_indexed_dpd_tensor_mult")
     return C_
 
 

@@ -360,6 +360,15 @@ void tblis_indexed_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
     guarded_void([&] { indexed_tensor_mult(cfg, A, idx_A, B, idx_B, C, idx_C); });
 }
 
+void tblis_indexed_dpd_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
+                                   const tblis_indexed_dpd_tensor* A, const label_type* idx_A,
+                                   const tblis_indexed_dpd_tensor* B, const label_type* idx_B,
+                                   tblis_indexed_dpd_tensor* C, const label_type* idx_C)
+{
+    (void)comm;
+    guarded_void([&] { indexed_dpd_tensor_mult(cfg, A, idx_A, B, idx_B, C, idx_C); });
+}
+
 void tblis_tensor_mult_batch(const tblis_comm* comm, const tblis_config* cfg, len_type count,
                              const tblis_tensor* const* A, const label_type* const* idx_A,
                              const tblis_tensor* const* B, const label_type* const* idx_B,

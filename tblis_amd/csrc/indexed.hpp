@@ -12,4 +12,10 @@ void indexed_tensor_mult(const tblis_config* cfg, const tblis_indexed_tensor* A,
                          const tblis_indexed_tensor* B, const label_type* idx_B,
                          tblis_indexed_tensor* C, const label_type* idx_C);
 
+// tblis::mult over indexed_dpd_varray_view (src/iface/3t/mult.cxx:413-540)
+// by the FULL algorithm (src/internal/3t/indexed_dpd/mult.cxx:68-118).
+void indexed_dpd_tensor_mult(const tblis_config* cfg, const tblis_indexed_dpd_tensor* A,
+                             const label_type* idx_A, const tblis_indexed_dpd_tensor* B,
+                             const label_type* idx_B, tblis_indexed_dpd_tensor* C, const label_type* idx_C);
+
 }  // namespace tblis_amd

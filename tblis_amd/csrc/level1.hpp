@@ -32,6 +32,17 @@ hipError_t launch_level1_entries(int dtype, const Level1Desc& d, const int64_t* 
 // C <- (br + i bi) C over an interleaved complex tensor of real type dtype
 // (d in complex elements).
 hipError_t launch_level1_cscale(int dtype, const Level1Desc& d, double br, double bi, void* B, hipStream_t s);
+// One strided block copied between two layouts (up to 8 dimensions):
+// dst[dst + sum_d i_d ds[d]] = src[src + sum_d i_d ss[d]].
+constexpr int BLOCK_COPY_MAXDIM = 8;
+struct BlockCopy
+{
+    int64_t src, dst, n;  // element offsets, element count
+    int32_t nd, pad_;
+    int64_t len[BLOCK_COPY_MAXDIM], ss[BLOCK_COPY_MAXDIM], ds[BLOCK_COPY_MAXDIM];
+};
+hipError_t launch_block_copy(int dtype, const BlockCopy* tasks, int64_t ntasks, const void* src, void* dst,
+                             hipStream_t s);
 // Device workspace a call needs (partial sums of a split A-only sum), bytes.
 size_t level1_workspace_bytes(int mode, const Level1Desc& d);
 hipError_t launch_level1(int mode, int dtype, const Level1Desc& d, const void* A, void* B,

@@ -414,6 +414,31 @@ __global__ __launch_bounds__(kThreads) void level1_cscale_kernel(Level1Desc d, d
     }
 }
 
+// Block copies between layouts (indexed DPD expansion and write-back): one
+// workgroup per block, the elements decomposed over the block's dimensions.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void block_copy_kernel(const BlockCopy* __restrict__ tasks, int64_t ntasks,
+                                                              const T* __restrict__ src, T* __restrict__ dst)
+{
+    for (int64_t t = blockIdx.x; t < ntasks; t += gridDim.x)
+    {
+        const BlockCopy& b = tasks[t];
+        const int nd = b.nd;
+        for (int64_t e = threadIdx.x; e < b.n; e += kThreads)
+        {
+            int64_t r = e, so = b.src, dof = b.dst;
+            for (int d = 0; d < nd; d++)
+            {
+                const int64_t q = r % b.len[d];
+                r /= b.len[d];
+                so += q * b.ss[d];
+                dof += q * b.ds[d];
+            }
+            dst[dof] = src[so];
+        }
+    }
+}
+
 unsigned grid_for(int64_t work)
 {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(work, kMaxBlocks));
@@ -563,6 +588,20 @@ hipError_t launch_level1_cscale(int dtype, const Level1Desc& d, double br, doubl
         hipLaunchKernelGGL((level1_cscale_kernel<double>), dim3(grid), dim3(kThreads), 0, s, d, br, bi, (double*)B);
     else
         hipLaunchKernelGGL((level1_cscale_kernel<float>), dim3(grid), dim3(kThreads), 0, s, d, br, bi, (float*)B);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_copy(int dtype, const BlockCopy* tasks, int64_t ntasks, const void* src, void* dst,
+                             hipStream_t s)
+{
+    if (ntasks == 0) return hipSuccess;
+    const unsigned grid = grid_for(ntasks);
+    if (dtype == 0)
+        hipLaunchKernelGGL((block_copy_kernel<double>), dim3(grid), dim3(kThreads), 0, s, tasks, ntasks,
+                           (const double*)src, (double*)dst);
+    else
+        hipLaunchKernelGGL((block_copy_kernel<float>), dim3(grid), dim3(kThreads), 0, s, tasks, ntasks,
+                           (const float*)src, (float*)dst);
     return hipGetLastError();
 }
 

@@ -135,3 +135,29 @@ def indexed_brute_force(alpha, A, idx_A, B, idx_B, beta, C_, idx_C):
                         continue
                     out[k][dc] += alpha * fa * fb * fc * ea[da] * eb[db]
     return out
+
+
+def random_indexed_dpd(rng, labels, lens, dense_ndim, irrep, nirrep, layout=5, fill=0.8,
+                       factors=(1.0, 0.5, -0.75, 2.0)):
+    """Host description of a random indexed DPD tensor (the reference's
+    random_tensor for indexed_dpd_varray, test/test.cxx:312-351): lens maps a
+    label to its nirrep per-irrep lengths; each indexed dimension gets a
+    random irrep; entries are DPD storage of the dense part."""
+    import itertools
+    from oracle import pyoracle as po
+    nd = dense_ndim
+    L = [list(lens[c]) for c in labels]
+    idx_irrep = [int(rng.integers(nirrep)) for _ in labels[nd:]]
+    dirrep = irrep
+    for r in idx_irrep:
+        dirrep ^= r
+    size = po.dpd_size(po.dpd_init(dirrep, nirrep, L[:nd], layout)) if nd else 1
+    ranges = [range(L[nd + j][r]) for j, r in enumerate(idx_irrep)]
+    every = list(itertools.product(*ranges))
+    pick = [ix for ix in every if rng.random() < fill]
+    if not pick and every:
+        pick = [every[int(rng.integers(len(every)))]]
+    entries = [rng.uniform(-1, 1, max(size, 1)) for _ in pick]
+    facs = [float(rng.choice(factors)) for _ in pick]
+    return dict(entries=entries, indices=pick, factors=facs, irrep=irrep, nirrep=nirrep, lens=L,
+                dense_ndim=nd, idx_irrep=idx_irrep, layout=layout)

@@ -481,3 +481,22 @@ int main() { tblis_hip_set_error_mode(1); all<float>(); all<double>(); return 0;
                         "-L", libdir, "-ltblis_amd", "-Wl,-rpath," + libdir], check=True)
         out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.splitlines()
     assert out == ["A: null entry pointer"] * 2, out
+
+
+def test_indexed_dpd_abi_layout():
+    """tblis_indexed_dpd_tensor: 104 bytes; ctypes mirror and compiled header agree."""
+    import tempfile
+    fields = ["type", "conj", "scalar", "ndim", "dense_ndim", "irrep", "nirrep", "layout", "len",
+              "idx_irrep", "num_indices", "data", "indices", "factor"]
+    assert C.sizeof(T.IndexedDpdTensor) == 104
+    want = [C.sizeof(T.IndexedDpdTensor)] + [getattr(T.IndexedDpdTensor, f).offset for f in fields]
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "probe.c"), os.path.join(d, "probe")
+        with open(src, "w") as f:
+            f.write('#include <stddef.h>\n#include <stdio.h>\n#include "tblis.h"\nint main(void){'
+                    'printf("%zu", sizeof(tblis_indexed_dpd_tensor));' +
+                    "".join('printf(" %%zu", offsetof(tblis_indexed_dpd_tensor, %s));' % x for x in fields) +
+                    'printf("\\n"); return 0;}\n')
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    assert [int(x) for x in out] == want

@@ -506,3 +506,46 @@ def test_complex_oracle_against_numpy(seed):
     b = np.conj(b) if cB else b
     want_v = alpha * np.einsum("%s,%s->%s" % (iA, iB, iC), a, b) + beta * view(C0.copy(), lC, sC)
     np.testing.assert_allclose(view(got, lC, sC), want_v, rtol=1e-12, atol=1e-12)
+
+
+# ---------------------------------------------------------------- indexed DPD
+def test_indexed_dpd_oracle_one_irrep_is_indexed():
+    """With one irrep the DPD parts are plain dense arrays (row-major for the
+    PREFIX_ROW_MAJOR layout): the indexed DPD oracle equals the indexed one."""
+    from tests.helpers import random_indexed_dpd
+    rng = np.random.default_rng(66000)
+    for trial in range(4):
+        ext = {c: [int(rng.integers(1, 4))] for c in "abefij"}
+        A = random_indexed_dpd(rng, "abef", ext, 3, 0, 1)
+        B = random_indexed_dpd(rng, "eijf", ext, 3, 0, 1)
+        C0 = random_indexed_dpd(rng, "abij", ext, 2, 0, 1)
+        got = po.indexed_dpd_mult(0.75, A, "abef", B, "eijf", 0.5, C0, "abij")
+
+        def plain(X):
+            nd = X["dense_ndim"]
+            dl = [l[0] for l in X["lens"][:nd]]
+            return dict(entries=[np.asarray(e).reshape(dl) for e in X["entries"]], indices=X["indices"],
+                        factors=X["factors"], dense_len=dl, idx_len=[l[0] for l in X["lens"][nd:]])
+        want = po.indexed_mult(0.75, plain(A), "abef", plain(B), "eijf", 0.5, plain(C0), "abij")
+        for g, w in zip(got, want):
+            np.testing.assert_allclose(g, w.ravel(), rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("layout", [0, 3, 5])
+def test_indexed_dpd_oracle_no_indices_is_dpd(layout):
+    """Without indexed dimensions (one entry, factor 1) the indexed DPD oracle
+    equals the DPD oracle (oracle.c, pinned by the MArray layout table)."""
+    from tests.helpers import random_indexed_dpd
+    rng = np.random.default_rng(66100 + layout)
+    nir = 4
+    ext = {c: [int(rng.integers(0, 4)) for _ in range(nir)] for c in "abef"}
+    A = random_indexed_dpd(rng, "abe", ext, 3, 1, nir, layout, factors=(1.0,))
+    B = random_indexed_dpd(rng, "ef", ext, 2, 2, nir, layout, factors=(1.0,))
+    C0 = random_indexed_dpd(rng, "abf", ext, 3, 3, nir, layout, factors=(1.0,))
+    got = po.indexed_dpd_mult(1.5, A, "abe", B, "ef", -0.5, C0, "abf")
+    dA = po.dpd_init(1, nir, A["lens"], layout)
+    dB = po.dpd_init(2, nir, B["lens"], layout)
+    dC = po.dpd_init(3, nir, C0["lens"], layout)
+    want = po.dpd_mult(1.5, dA, "abe", A["entries"][0], dB, "ef", B["entries"][0], -0.5, dC, "abf",
+                       np.array(C0["entries"][0]))
+    np.testing.assert_allclose(got[0], want, rtol=1e-13, atol=1e-13)

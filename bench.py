@@ -31,6 +31,10 @@ makes it the headline):
                dense, f indexed (32 values; A and B each list 24, seeded), FP64
                and split-BF16; useful TFLOP/s over the matching f values, and
                the same kernels on the equivalent dense contraction
+  indexed_dpd  indexed DPD C[abij] += A[abef]*B[efij] (4 irreps, per-irrep
+               extents [12,10,8,6], f indexed at irrep 0 with 16 values, A and
+               B list 12, seeded), FP64, by the FULL algorithm: block-sparse
+               useful TFLOP/s and the time of the expanded dense contraction
   level1       tblis_tensor_add / _scale on FP64 96^4 tensors (679 MB each):
                permuted add, same-layout add, sums over the slow and over
                the fast dimensions, scale; GB/s of algorithmic bytes against
@@ -337,6 +341,46 @@ def indexed_leg(T, dev, stream, steps, warmup, ext=48, nf=32, listed=24, seed=5)
     return out
 
 
+# ---------------------------------------------------------------- indexed DPD
+
+def indexed_dpd_leg(T, dev, stream, steps, warmup, seed=13):
+    import torch
+    from itertools import product
+    nir, ext = 4, [12, 10, 8, 6]
+    rng = np.random.default_rng(seed)
+    nf, listed = 16, 12
+    fa = sorted(rng.choice(nf, listed, replace=False).tolist())
+    fb = sorted(rng.choice(nf, listed, replace=False).tolist())
+    match = sorted(set(fa) & set(fb))
+    lens_a = [ext, ext, ext, [nf, 0, 0, 0]]   # a b e | f
+    lens_b = [ext, ext, ext, [nf, 0, 0, 0]]   # e i j | f
+    lens_c = [ext] * 4                        # a b i j
+    size3 = T.dpd_size(0, nir, [ext] * 3)
+    size4 = T.dpd_size(0, nir, lens_c)
+    # block-sparse flops over the matching f values
+    blk = sum(ext[ia] * ext[ib] * ext[ia ^ ib] * ext[ii] * ext[ia ^ ib ^ ii]
+              for ia, ib, ii in product(range(nir), repeat=3))
+    flops = 2.0 * blk * len(match)
+    g = torch.Generator(device=dev).manual_seed(seed)
+    sa = torch.rand(listed * size3, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    sb = torch.rand(listed * size3, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    sc = torch.zeros(size4, dtype=torch.float64, device=dev)
+    A = T.IndexedDpd([sa[i * size3:(i + 1) * size3] for i in range(listed)], [(f,) for f in fa], [0], 0, nir,
+                     lens_a, 3)
+    B = T.IndexedDpd([sb[i * size3:(i + 1) * size3] for i in range(listed)], [(f,) for f in fb], [0], 0, nir,
+                     lens_b, 3)
+    Cx = T.IndexedDpd([sc], [()], [], 0, nir, lens_c, 4)
+    fn = lambda: T.indexed_dpd_mult(1.0, A, "abef", B, "eijf", 0.0, Cx, "abij")
+    ms_step, _, _ = time_steps(fn, steps, warmup, stream)
+    full = 36
+    dense = 2.0 * full ** 5 * len(match)
+    return {"workload": "indexed DPD C[abij] += A[abef]*B[efij], 4 irreps, extents %s per irrep, f indexed at "
+                        "irrep 0 (%d values, A and B list %d, %d match; numpy default_rng(%d)), FP64, FULL "
+                        "algorithm" % (ext, nf, listed, len(match), seed),
+            "useful_flops": flops, "ms": round(ms_step, 4), "useful_tflops": tflops(flops, ms_step),
+            "expanded_dense_flops": dense, "expanded_tflops": tflops(dense, ms_step)}
+
+
 # ---------------------------------------------------------------- level-1
 
 def level1_leg(T, dev, stream, steps, warmup):
@@ -455,7 +499,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--only", choices=["permuted", "fp32_96", "dpd", "batch", "complex", "indexed",
-                                       "level1"],
+                                       "indexed_dpd", "level1"],
                     help="run one leg alone (for profiling) and report it as the headline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-traffic", action="store_true",
@@ -512,6 +556,10 @@ def main():
         legs["indexed"] = indexed_leg(T, dev, stream, few if args.only is None else args.steps,
                                       min(args.warmup, 2))
         torch.cuda.empty_cache()
+    if args.only in (None, "indexed_dpd"):
+        legs["indexed_dpd"] = indexed_dpd_leg(T, dev, stream, few if args.only is None else args.steps,
+                                              min(args.warmup, 2))
+        torch.cuda.empty_cache()
     if args.only in (None, "level1"):
         legs["level1"] = level1_leg(T, dev, stream, few if args.only is None else args.steps,
                                     min(args.warmup, 2))
@@ -525,6 +573,10 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
 
+    if head is None and args.only == "indexed_dpd":
+        leg = legs["indexed_dpd"]
+        head = {"value": leg["useful_tflops"], "ms_per_step": leg["ms"], "dtype": "f64",
+                "config": {"workload": leg["workload"]}}
     if head is None and args.only == "level1":
         leg = legs["level1"]["permuted_add"]
         head = {"value": leg["gbps"], "ms_per_step": leg["ms"], "dtype": "f64", "unit": "GB/s",

@@ -114,6 +114,8 @@ void load(Op& x, const tblis_indexed_dpd_tensor* t, const label_type* idx, const
 // The allowed blocks of an operand's dense part: copies between each
 // entry's DPD storage (offset from data[0]) and its dense scratch slot
 // (offset base + e * dense_elems, column-major over the summed extents).
+constexpr int64_t kPiece = 2048;  // elements per block-copy workgroup
+
 void block_tasks(const Op& x, int64_t base, bool to_dense, std::vector<BlockCopy>& out)
 {
     if (x.ndd == 0)
@@ -121,6 +123,7 @@ void block_tasks(const Op& x, int64_t base, bool to_dense, std::vector<BlockCopy
         for (int64_t e = 0; e < x.n; e++)
         {
             BlockCopy b{};
+            b.e0 = 0;
             b.n = 1;
             b.src = to_dense ? x.off[e] : base + e;
             b.dst = to_dense ? base + e : x.off[e];
@@ -160,7 +163,6 @@ void block_tasks(const Op& x, int64_t base, bool to_dense, std::vector<BlockCopy
         for (int64_t e = 0; e < x.n; e++)
         {
             BlockCopy b{};
-            b.n = n;
             b.nd = x.ndd;
             for (int d = 0; d < x.ndd; d++)
             {
@@ -170,7 +172,13 @@ void block_tasks(const Op& x, int64_t base, bool to_dense, std::vector<BlockCopy
             }
             b.src = to_dense ? x.off[e] + boff : base + e * x.dense_elems + doff;
             b.dst = to_dense ? base + e * x.dense_elems + doff : x.off[e] + boff;
-            out.push_back(b);
+            // pieces of kPiece elements, one workgroup each
+            for (int64_t e0 = 0; e0 < n; e0 += kPiece)
+            {
+                b.e0 = e0;
+                b.n = std::min<int64_t>(kPiece, n - e0);
+                out.push_back(b);
+            }
         }
     }
 }

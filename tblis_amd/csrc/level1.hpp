@@ -37,7 +37,8 @@ hipError_t launch_level1_cscale(int dtype, const Level1Desc& d, double br, doubl
 constexpr int BLOCK_COPY_MAXDIM = 8;
 struct BlockCopy
 {
-    int64_t src, dst, n;  // element offsets, element count
+    int64_t src, dst;     // element offsets of the block in each layout
+    int64_t e0, n;        // this piece: elements [e0, e0 + n) of the block
     int32_t nd, pad_;
     int64_t len[BLOCK_COPY_MAXDIM], ss[BLOCK_COPY_MAXDIM], ds[BLOCK_COPY_MAXDIM];
 };

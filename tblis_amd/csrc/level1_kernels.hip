@@ -415,7 +415,8 @@ __global__ __launch_bounds__(kThreads) void level1_cscale_kernel(Level1Desc d, d
 }
 
 // Block copies between layouts (indexed DPD expansion and write-back): one
-// workgroup per block, the elements decomposed over the block's dimensions.
+// workgroup per piece of a block, the elements decomposed over the block's
+// dimensions.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void block_copy_kernel(const BlockCopy* __restrict__ tasks, int64_t ntasks,
                                                               const T* __restrict__ src, T* __restrict__ dst)
@@ -424,7 +425,7 @@ __global__ __launch_bounds__(kThreads) void block_copy_kernel(const BlockCopy* _
     {
         const BlockCopy& b = tasks[t];
         const int nd = b.nd;
-        for (int64_t e = threadIdx.x; e < b.n; e += kThreads)
+        for (int64_t e = b.e0 + threadIdx.x; e < b.e0 + b.n; e += kThreads)
         {
             int64_t r = e, so = b.src, dof = b.dst;
             for (int d = 0; d < nd; d++)

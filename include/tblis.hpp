@@ -64,6 +64,22 @@ struct indexed_varray_view
     std::vector<T> factor;                 // empty: all 1
 };
 
+// Indexed DPD tensor view (device memory), like MArray's
+// indexed_dpd_varray_view: the dense part is DPD storage per entry.
+template <typename T>
+struct indexed_dpd_varray_view
+{
+    std::vector<T*> data;
+    unsigned irrep;
+    unsigned nirrep;
+    std::vector<std::vector<len_type>> len;  // [dim][irrep], dense dims first
+    unsigned dense_ndim;
+    std::vector<unsigned> idx_irrep;         // one per indexed dimension
+    std::vector<len_type> indices;           // entries x idx_irrep.size(), irrep-local
+    std::vector<T> factor;                   // empty: all 1
+    int layout = TBLIS_DPD_PREFIX_ROW_MAJOR;
+};
+
 namespace detail
 {
 
@@ -131,6 +147,33 @@ tblis_indexed_tensor make_indexed(T scalar, const indexed_varray_view<T>& v, std
     return t;
 }
 
+template <typename T>
+tblis_indexed_dpd_tensor make_indexed_dpd(T scalar, const indexed_dpd_varray_view<T>& v, std::vector<len_type>& flat)
+{
+    flat.clear();
+    for (auto& row : v.len) flat.insert(flat.end(), row.begin(), row.end());
+    tblis_indexed_dpd_tensor t;
+    t.type = type_tag<T>::value;
+    t.conj = 0;
+    t.scalar.type = t.type;
+    if (std::is_same<T, float>::value)
+        t.scalar.data.s = (float)scalar;
+    else
+        t.scalar.data.d = (double)scalar;
+    t.ndim = (unsigned)v.len.size();
+    t.dense_ndim = v.dense_ndim;
+    t.irrep = v.irrep;
+    t.nirrep = v.nirrep;
+    t.layout = v.layout;
+    t.len = flat.data();
+    t.idx_irrep = v.idx_irrep.data();
+    t.num_indices = (len_type)v.data.size();
+    t.data = (void* const*)v.data.data();
+    t.indices = v.indices.data();
+    t.factor = v.factor.empty() ? nullptr : (const void*)v.factor.data();
+    return t;
+}
+
 }  // namespace detail
 
 // C <- alpha * A.B + beta * C (src/iface/3t/mult.h:28-37).
@@ -187,6 +230,19 @@ void mult(T alpha, const indexed_varray_view<T>& A, const label_type* idx_A, con
     tblis_indexed_tensor b = detail::make_indexed(T(1), B, lb);
     tblis_indexed_tensor c = detail::make_indexed(beta, C, lc);
     tblis_indexed_tensor_mult(nullptr, cfg, &a, idx_A, &b, idx_B, &c, idx_C);
+}
+
+// Indexed DPD contraction (src/iface/3t/mult.h:95-114).
+template <typename T>
+void mult(T alpha, const indexed_dpd_varray_view<T>& A, const label_type* idx_A,
+          const indexed_dpd_varray_view<T>& B, const label_type* idx_B, T beta,
+          const indexed_dpd_varray_view<T>& C, const label_type* idx_C, const tblis_config* cfg = nullptr)
+{
+    std::vector<len_type> la, lb, lc;
+    tblis_indexed_dpd_tensor a = detail::make_indexed_dpd(alpha, A, la);
+    tblis_indexed_dpd_tensor b = detail::make_indexed_dpd(T(1), B, lb);
+    tblis_indexed_dpd_tensor c = detail::make_indexed_dpd(beta, C, lc);
+    tblis_indexed_dpd_tensor_mult(nullptr, cfg, &a, idx_A, &b, idx_B, &c, idx_C);
 }
 
 }  // namespace tblis

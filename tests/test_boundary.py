@@ -461,8 +461,10 @@ template <typename T> void all()
     tblis::matrix_view<T> m{nullptr, 2, 2, 1, 2};
     tblis::dpd_varray_view<T> d{nullptr, 0, 1, {{2}, {2}}};
     tblis::indexed_varray_view<T> x{{nullptr}, {2}, {1}, {2}, {0}, {}};
+    tblis::indexed_dpd_varray_view<T> y{{nullptr}, 0, 1, {{2}, {2}}, 1, {0}, {0}, {}};
     if (false)
     {
+        tblis::mult<T>(1, y, "az", y, "bz", 0, y, "ab");
         tblis::mult<T>(1, t, "ab", t, "bc", 0, t, "ac");
         tblis::mult<T>(1, m, m, 0, m);
         tblis::mult<T>(1, d, "ab", d, "bc", 0, d, "ac");

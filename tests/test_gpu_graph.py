@@ -1,5 +1,6 @@
 """hipGraph capture of library calls: a prepared dense, packed split-BF16,
-DPD and batched contraction is captured once and replayed several times,
+DPD, batched, complex and indexed contraction is captured once and replayed
+several times,
 with the inputs changed by eager work between replays; every replay must
 match an eager call on the same inputs (FP64: 1e-12, split-BF16: 1e-6
 relative Frobenius).  The plan cache is cleared after capture to show that
@@ -134,3 +135,72 @@ def test_graph_batch(dtype):
     fn = T.prepare_batch(1.0, As, ["abef"] * n, Bs, ["efij"] * n, 0.0, Cs, ["abij"] * n)
     tol = 1e-12 if dtype == torch.float64 else 1e-6
     _replay_check(_capture(fn), As + Bs, Cs, eager, tol)
+
+
+def test_graph_complex_alpha_and_beta():
+    """Complex contraction with a complex alpha and beta: the C passes before
+    and after the GEMMs are kernels, captured with them."""
+    g = torch.Generator(device=DEV).manual_seed(11)
+    ext = dict(a=20, b=12, e=16, f=6, i=9, j=14)
+    L = lambda s: [ext[c] for c in s]
+    mk = lambda s: torch.complex(torch.rand(int(np.prod(L(s))), dtype=torch.float64, device=DEV, generator=g),
+                                 torch.rand(int(np.prod(L(s))), dtype=torch.float64, device=DEV, generator=g))
+    A, B, C0 = mk("aebf"), mk("fjei"), mk("aibj")
+    C = C0.clone()
+    v = lambda x, s: torch.as_strided(x, L(s), colmaj(L(s)))
+    alpha, beta = 0.5 - 0.25j, 0.75 + 0.5j
+
+    def eager():
+        Ce = C0.clone()
+        T.mult(alpha, v(A, "aebf"), "aebf", v(B, "fjei"), "fjei", beta, v(Ce, "aibj"), "aibj")
+        return [Ce]
+
+    fn = T.prepare_mult(alpha, v(A, "aebf"), "aebf", v(B, "fjei"), "fjei", beta, v(C, "aibj"), "aibj")
+    graph = _capture(fn)
+    # each replay starts from the same C (beta != 0 reads it)
+    T.lib().tblis_hip_clear_plan_cache()
+    for rep in range(3):
+        A.mul_(1.5 if rep % 2 == 0 else -0.75)
+        C.copy_(C0)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        got = C.clone()
+        want = eager()[0]
+        torch.cuda.synchronize()
+        assert rel(torch.view_as_real(got), torch.view_as_real(want)) <= 1e-12, rep
+
+
+def test_graph_indexed_with_beta_pass():
+    """Indexed contraction with beta = 0.5 (beta pass kernel, then GEMMs)."""
+    g = torch.Generator(device=DEV).manual_seed(12)
+    n = 10 * 9 * 8
+    sa = torch.rand(3 * n, dtype=torch.float64, device=DEV, generator=g)
+    sb = torch.rand(3 * 8 * 7 * 6, dtype=torch.float64, device=DEV, generator=g)
+    sc0 = torch.rand(10 * 9 * 7 * 6, dtype=torch.float64, device=DEV, generator=g)
+    sc = sc0.clone()
+    A = T.Indexed([torch.as_strided(sa, [10, 9, 8], colmaj([10, 9, 8]), i * n) for i in range(3)],
+                  [(0,), (2,), (3,)], [4], [1.0, -0.5, 2.0])
+    B = T.Indexed([torch.as_strided(sb, [8, 7, 6], colmaj([8, 7, 6]), i * 336) for i in range(3)],
+                  [(0,), (1,), (3,)], [4], [0.5, 1.0, 1.0])
+    Cx = T.Indexed([torch.as_strided(sc, [10, 9, 7, 6], colmaj([10, 9, 7, 6]))], [()], [], [0.75])
+
+    def eager():
+        ce = sc0.clone()
+        Ce = T.Indexed([torch.as_strided(ce, [10, 9, 7, 6], colmaj([10, 9, 7, 6]))], [()], [], [0.75])
+        T.indexed_mult(1.25, A, "abef", B, "eijf", 0.5, Ce, "abij")
+        return [ce]
+
+    fn = T.prepare_indexed_mult(1.25, A, "abef", B, "eijf", 0.5, Cx, "abij")
+    graph = _capture(fn)
+    T.lib().tblis_hip_clear_plan_cache()
+    for rep in range(3):
+        sa.mul_(1.5 if rep % 2 == 0 else -0.75)
+        sc.copy_(sc0)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        got = sc.clone()
+        want = eager()[0]
+        torch.cuda.synchronize()
+        assert rel(got, want) <= 1e-12, rep

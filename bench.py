@@ -14,6 +14,8 @@ resident in HBM.
 
 Further legs (fields of the same line; --only runs one of them alone and
 makes it the headline):
+  dense24      C[abij] = A[abef]*B[efij], extents 24, FP64, column-major,
+               alpha 1, beta 0 (a small contraction: launch-bound)
   fp32_96      C[abcd] = A[aebf]*B[ecfd], extents 96, FP32 normal(0,1) with
                1% of entries x1e4: split-BF16 MFMA vs FP64 MFMA, relative
                Frobenius error of each against a float64 torch product
@@ -131,6 +133,24 @@ def permuted_leg(T, dev, stream, steps, warmup):
                      "kernel_ms_mean": round(ms_kern, 4), "kernel_ms_min": round(ms_min, 4),
                      "algorithmic_flops": flops, "algorithmic_bytes": nbytes},
     }, (lA, lB, lC)
+
+
+# ---------------------------------------------------------------- dense 24^4
+
+def dense24_leg(T, dev, stream, steps, warmup):
+    import torch
+    L = [24] * 4
+    g = torch.Generator(device=dev).manual_seed(42)
+    A, B, C = (torch.rand(24 ** 4, dtype=torch.float64, device=dev, generator=g) * 2 - 1 for _ in range(3))
+    flops = 2.0 * 24 ** 6
+    fn = T.prepare_mult(1.0, strided(A, L), "abef", strided(B, L), "efij", 0.0, strided(C, L), "abij")
+    ms_step, ms_kern, _ = time_steps(fn, steps, warmup, stream)
+    graph = capture(T, fn, stream)
+    ms_graph, _, _ = time_steps(graph.replay, steps, warmup, torch.cuda.current_stream())
+    return {"workload": "C[abij] = A[abef]*B[efij], extents 24, FP64, column-major, alpha 1, beta 0",
+            "useful_flops": flops, "algorithm": T.last_algorithm(), "tile": T.last_tile(),
+            "ms": round(ms_step, 4), "useful_tflops": tflops(flops, ms_step), "call_ms_events": round(ms_kern, 4),
+            "graph_ms": round(ms_graph, 4), "graph_useful_tflops": tflops(flops, ms_graph)}
 
 
 # ---------------------------------------------------------------- fp32 96^4
@@ -498,7 +518,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--only", choices=["permuted", "fp32_96", "dpd", "batch", "complex", "indexed",
+    ap.add_argument("--only", choices=["permuted", "dense24", "fp32_96", "dpd", "batch", "complex", "indexed",
                                        "indexed_dpd", "level1"],
                     help="run one leg alone (for profiling) and report it as the headline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -538,6 +558,8 @@ def main():
     if args.only in (None, "permuted"):
         head, lens = permuted_leg(T, dev, stream, args.steps, args.warmup)
     legs = {}
+    if args.only in (None, "dense24"):
+        legs["dense24"] = dense24_leg(T, dev, stream, args.steps, args.warmup)
     if args.only in (None, "fp32_96"):
         legs["fp32_96"] = fp32_leg(T, dev, stream, few if args.only is None else args.steps,
                                    min(args.warmup, 2))
@@ -573,15 +595,15 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
 
-    if head is None and args.only == "indexed_dpd":
-        leg = legs["indexed_dpd"]
-        head = {"value": leg["useful_tflops"], "ms_per_step": leg["ms"], "dtype": "f64",
-                "config": {"workload": leg["workload"]}}
     if head is None and args.only == "level1":
         leg = legs["level1"]["permuted_add"]
         head = {"value": leg["gbps"], "ms_per_step": leg["ms"], "dtype": "f64", "unit": "GB/s",
                 "metric": "level-1 permuted add, GB/s of algorithmic bytes (HBM-bound)",
                 "config": {"workload": legs["level1"]["workload"] + "; headline " + leg["op"]}}
+    if head is None and "useful_tflops" in legs.get(args.only, {}):
+        leg = legs[args.only]
+        head = {"value": leg["useful_tflops"], "ms_per_step": leg["ms"], "dtype": "f64",
+                "config": {"workload": leg["workload"], "algorithm": leg.get("algorithm")}}
     if head is None:
         leg = legs[args.only]
         first = "split_bf16" if "split_bf16" in leg else next(k for k in leg if isinstance(leg[k], dict))

@@ -416,28 +416,30 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family)
 namespace
 {
 
+// Tasks grouped by which operands they read straight from the tensors
+// (GEMMs of one batch may differ): the groups become separate launches of
+// the matching kernel variant.
 bool tables_uniform(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T)
 {
-    bool allow[2] = {true, true};
-    for (int attempt = 0; attempt < 3; attempt++)
-    {
-        T = PanelTables();
-        if (!tables_for_tile(gemms, family, tile, T, allow)) return false;
-        bool mixed = false;
-        for (int o = 0; o < 2; o++)
-        {
-            size_t nd = 0;
-            for (auto& t : T.tasks) nd += (o ? t.b_src : t.a_src) != PANEL_MISSING;
-            T.direct[o] = nd == T.tasks.size() && nd > 0;
-            if (nd && nd != T.tasks.size())
+    const bool allow[2] = {true, true};
+    T = PanelTables();
+    if (!tables_for_tile(gemms, family, tile, T, allow)) return false;
+    auto group = [](const PanelTask& t) {
+        return (t.a_src != PANEL_MISSING ? 1 : 0) + (t.b_src != PANEL_MISSING ? 2 : 0);
+    };
+    std::vector<PanelTask> sorted;
+    sorted.reserve(T.tasks.size());
+    for (int g = 0; g < 4; g++)
+        for (const PanelTask& t : T.tasks)
+            if (group(t) == g)
             {
-                allow[o] = false;
-                mixed = true;
+                sorted.push_back(t);
+                T.group_n[g]++;
             }
-        }
-        if (!mixed) return true;
-    }
-    return false;
+    T.tasks.swap(sorted);
+    for (int o = 0; o < 2; o++)
+        T.direct[o] = T.group_n[o == 0 ? 1 : 2] + T.group_n[3] == (int64_t)T.tasks.size() && !T.tasks.empty();
+    return true;
 }
 
 bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T,
@@ -1015,6 +1017,7 @@ bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, i
     plan.panel_tile = T.tile;
     plan.direct[0] = T.direct[0];
     plan.direct[1] = T.direct[1];
+    for (int g = 0; g < 4; g++) plan.group_n[g] = T.group_n[g];
     plan.key.family = family;
     plan.key.dtype = dtype;
     plan.key.use_mfma = 1;

@@ -70,6 +70,9 @@ struct PanelTables
     bool swap = false;  // GEMMs were transposed: operand 0 is the B tensor
     bool reads_ab = false;
     bool direct[2] = {false, false};  // operand read from the tensor by every task
+    // tasks come in four consecutive groups by which operands they read from
+    // the tensors (none, A, B, both); one GEMM launch per non-empty group
+    int64_t group_n[4] = {0, 0, 0, 0};
     int64_t ws_panel = 0;  // panel values (double for FP64, BF16 for split-BF16)
     int64_t ws_c = 0;      // C panel doubles
     int64_t useful_flops = 0, mfma_flops = 0;

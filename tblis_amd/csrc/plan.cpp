@@ -625,7 +625,19 @@ void run_plan(const std::shared_ptr<Plan>& pp, const void* A, const void* B, voi
         P.tile = p.panel_tile;
         P.stream = ts.stream;
         if (P.use_k) hip_check(launch_panel_pack(P), "pack launch");
-        hip_check(launch_panel_gemm(P), "kernel launch");
+        // one GEMM launch per group of tasks with the same direct operands
+        int64_t at = 0;
+        for (int g = 0; g < 4; g++)
+        {
+            if (p.group_n[g] == 0) continue;
+            PanelLaunch G = P;
+            G.tasks = p.d_ptasks + at;
+            G.ntasks = (int)p.group_n[g];
+            G.direct_a = (g & 1) != 0;
+            G.direct_b = (g & 2) != 0;
+            hip_check(launch_panel_gemm(G), "kernel launch");
+            at += p.group_n[g];
+        }
         hip_check(launch_panel_unpack(P), "unpack launch");
     }
     else if (p.packed && L.use_k)

@@ -55,7 +55,8 @@ struct Plan
     bool panel = false;
     int panel_tile = 0;
     size_t ws_c_offset = 0;  // byte offset of the C panels in the workspace
-    bool direct[2] = {false, false};  // FP64 operands read from the tensors
+    bool direct[2] = {false, false};  // FP64 operands read from the tensors (every task)
+    int64_t group_n[4] = {0, 0, 0, 0};  // panel tasks reading none / A / B / both from the tensors
     int p_nchunks = 0, p_nuchunks = 0;
     PanelJob* d_pjobs = nullptr;
     PanelChunk* d_pchunks = nullptr;

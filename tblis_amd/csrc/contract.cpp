@@ -3,6 +3,7 @@
 
 #include <cstring>
 
+#include "panel.hpp"
 #include "plan.hpp"
 
 namespace tblis_amd
@@ -206,6 +207,13 @@ int family_of(Algo a)
 
 std::shared_ptr<Plan> empty_plan() { return std::make_shared<Plan>(); }
 
+bool gemm_reads(const GemmDesc& g)
+{
+    for (const GemmSeg& s : g.segs)
+        if (s.k > 0) return true;
+    return false;
+}
+
 }  // namespace
 
 int algo_family(Algo a) { return family_of(a); }
@@ -231,6 +239,9 @@ void tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_typ
     tensor_key(key, *B, idx_B);
     tensor_key(key, *C, idx_C);
 
+    const double alpha = product_scalar(A->scalar, B->scalar, type);
+    const double beta = scalar_value(C->scalar, type);
+
     std::shared_ptr<Plan> plan = cache_get(key);
     if (!plan)
     {
@@ -240,6 +251,15 @@ void tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_typ
         GemmDesc g;
         if (dense_gemm(c, g, 0, 0, 0))
         {
+            // the autotuner may time its candidates on A and B (read only):
+            // check them first
+            const bool reads = alpha != 0.0 && gemm_reads(g);
+            if (reads)
+            {
+                check_data(A, "A");
+                check_data(B, "B");
+            }
+            TuneOperandsScope ops(reads ? A->data : nullptr, reads ? B->data : nullptr);
             PlanBuilder pb(elem_size(type), dtype_code(type), algo != Algo::Reference, family_of(algo));
             pb.add(g);
             plan = pb.build();
@@ -248,8 +268,6 @@ void tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_typ
         cache_put(key, plan);
     }
 
-    const double alpha = product_scalar(A->scalar, B->scalar, type);
-    const double beta = scalar_value(C->scalar, type);
     if (plan->ntasks > 0)
     {
         const bool use_k = alpha != 0.0;
@@ -360,6 +378,7 @@ void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tenso
     if (!plan)
     {
         PlanBuilder pb(es, dtype_code(type), algo != Algo::Reference, family_of(algo));
+        bool reads = false;
         for (int64_t i = 0; i < count; i++)
         {
             Canonical c;
@@ -369,8 +388,21 @@ void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tenso
             GemmDesc g;
             if (dense_gemm(c, g, delta(A[i]->data, A[0]->data), delta(B[i]->data, B[0]->data),
                            delta(C[i]->data, C[0]->data)))
+            {
+                reads |= gemm_reads(g);
                 pb.add(g);
+            }
         }
+        // the autotuner may time its candidates on the A and B instances
+        // (read only): check them first
+        reads &= alpha != 0.0;
+        if (reads)
+            for (int64_t i = 0; i < count; i++)
+            {
+                check_data(A[i], "A");
+                check_data(B[i], "B");
+            }
+        TuneOperandsScope ops(reads ? A[0]->data : nullptr, reads ? B[0]->data : nullptr);
         plan = pb.build();
         if (!plan) plan = empty_plan();
         cache_put(key, plan);

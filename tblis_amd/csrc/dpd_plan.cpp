@@ -498,6 +498,19 @@ void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const l
     dpd_key(key, *B, idx_B);
     dpd_key(key, *C, idx_C);
 
+    double alpha = type == TYPE_SINGLE ? (double)(A->scalar.data.s * B->scalar.data.s)
+                                       : A->scalar.data.d * B->scalar.data.d;
+    const double beta = scalar_of(C->scalar, type);
+    const int es = type == TYPE_DOUBLE ? 8 : 4;
+    auto total = [&](const tblis_dpd_tensor* t) {
+        DpdLayout L;
+        L.init(t->irrep, t->nirrep, t->ndim, t->len, t->layout);
+        return L.total();
+    };
+    auto check = [&](const tblis_dpd_tensor* t, const char* nm) {
+        check_device_pointer(t->data, nm, 0, total(t) - 1, es);
+    };
+
     std::shared_ptr<Plan> plan = cache_get(key);
     if (!plan)
     {
@@ -505,15 +518,20 @@ void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const l
         P.init(A, idx_A, B, idx_B, C, idx_C);
         PlanBuilder pb(type == TYPE_DOUBLE ? 8 : 4, dtype_code(type), algo != Algo::Reference,
                        algo_family(algo));
+        // the autotuner may time its candidates on A and B (read only):
+        // check them first
+        const bool reads = alpha != 0.0 && total(A) > 0 && total(B) > 0;
+        if (reads)
+        {
+            check(A, "A");
+            check(B, "B");
+        }
+        TuneOperandsScope ops(reads ? A->data : nullptr, reads ? B->data : nullptr);
         if (!pb.panel_route() || !build_panel_gemms(P, pb)) build_gemms(P, pb);
         plan = pb.build();
         if (!plan) plan = std::make_shared<Plan>();
         cache_put(key, plan);
     }
-
-    double alpha = type == TYPE_SINGLE ? (double)(A->scalar.data.s * B->scalar.data.s)
-                                       : A->scalar.data.d * B->scalar.data.d;
-    const double beta = scalar_of(C->scalar, type);
 
     // Optional reproduction of the reference front end's irrep test, which
     // parses as A ^ (B != C) (src/iface/3t/mult.cxx:270): when it fires the
@@ -534,12 +552,6 @@ void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const l
         const bool use_k = alpha != 0.0;
         if (use_k || beta != 1.0)
         {
-            const int es = type == TYPE_DOUBLE ? 8 : 4;
-            auto check = [&](const tblis_dpd_tensor* t, const char* nm) {
-                DpdLayout L;
-                L.init(t->irrep, t->nirrep, t->ndim, t->len, t->layout);
-                check_device_pointer(t->data, nm, 0, L.total() - 1, es);
-            };
             const bool k = use_k && plan->reads_ab;
             if (!plan->already_verified(A->data, B->data, C->data, k))
             {

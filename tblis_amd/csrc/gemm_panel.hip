@@ -31,23 +31,23 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------ pack
 
+// Every address of a chunk follows from its PackChunk record (job fields
+// and the chunk's group tables), so a wave goes from the record straight
+// to its data loads.
 template <typename T>
-__global__ __launch_bounds__(256) void pack_panel_kernel(const PanelJob* __restrict__ jobs,
-                                                         const PanelChunk* __restrict__ chunks,
+__global__ __launch_bounds__(256) void pack_panel_kernel(const PackChunk* __restrict__ chunks,
                                                          const int64_t* __restrict__ pool,
                                                          const T* __restrict__ A,
                                                          const T* __restrict__ B,
                                                          double* __restrict__ ws)
 {
-    const PanelChunk ch = chunks[blockIdx.x];
-    const PanelJob j = jobs[ch.job];
+    const PackChunk& j = chunks[blockIdx.x];
     const T* __restrict__ X = j.operand ? B : A;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lr = lane & 15, lk = lane >> 4;
     const bool zero = j.src == PANEL_MISSING;
-    const int ng = ch.nrb * ch.nkg;
-    const int rbf = j.r0 / 16, kgf = j.k0 / 8;  // first groups of the job's tables
+    const int ng = j.nrb * j.nkg;
     if (j.kfast && !zero)
     {
         // k-fast source: per pair of k groups, four loads of four rows x 16
@@ -57,8 +57,8 @@ __global__ __launch_bounds__(256) void pack_panel_kernel(const PanelJob* __restr
         // so a wave waits on memory once per batch rather than per pair.
         constexpr int kBatch = 4;
         __shared__ double tile[4][16][17];
-        const int npair = (ch.nkg + 1) / 2;
-        const int niter = ch.nrb * npair;
+        const int npair = (j.nkg + 1) / 2;
+        const int niter = j.nrb * npair;
         const int kk = lane & 15;
         for (int g0 = wave; g0 < niter; g0 += 4 * kBatch)
         {
@@ -70,13 +70,14 @@ __global__ __launch_bounds__(256) void pack_panel_kernel(const PanelJob* __restr
 #pragma unroll
                 for (int jj = 0; jj < 4; jj++) v[u][jj] = 0.0;
                 if (g >= niter) continue;
-                const int rb = ch.rb0 + g / npair;
-                const int kg = ch.kg0 + 2 * (g % npair);
-                const bool two = kg + 1 < ch.kg0 + ch.nkg;
-                const int64_t rbase = pool[j.rowtab + 2 * (rb - rbf)];
-                const int64_t rstr = pool[j.rowtab + 2 * (rb - rbf) + 1];
+                const int ri = g / npair, ki = 2 * (g % npair);  // groups within the chunk
+                const int rb = j.rb0 + ri;
+                const int kg = j.kg0 + ki;
+                const bool two = ki + 1 < j.nkg;
+                const int64_t rbase = j.rt[ri][0];
+                const int64_t rstr = j.rt[ri][1];
                 const int h = kk >> 3;
-                const int64_t kb = pool[j.ktab + 2 * (kg + (two ? h : 0) - kgf)];
+                const int64_t kb = j.kt[ki + (two ? h : 0)][0];
                 const int pk = kg * 8 + kk - j.k0;  // job-relative k
                 const bool kin = pk >= 0 && pk < j.nk && (h == 0 || two);
                 const int64_t koff = kb + (kk & 7);
@@ -97,9 +98,9 @@ __global__ __launch_bounds__(256) void pack_panel_kernel(const PanelJob* __restr
             {
                 const int g = g0 + 4 * u;
                 if (g >= niter) break;
-                const int rb = ch.rb0 + g / npair;
-                const int kg = ch.kg0 + 2 * (g % npair);
-                const bool two = kg + 1 < ch.kg0 + ch.nkg;
+                const int rb = j.rb0 + g / npair;
+                const int kg = j.kg0 + 2 * (g % npair);
+                const bool two = kg + 1 < j.kg0 + j.nkg;
 #pragma unroll
                 for (int jj = 0; jj < 4; jj++) tile[wave][(lane >> 4) + 4 * jj][kk] = v[u][jj];
                 __builtin_amdgcn_wave_barrier();
@@ -143,18 +144,19 @@ __global__ __launch_bounds__(256) void pack_panel_kernel(const PanelJob* __restr
             v0[u] = 0.0;
             v1[u] = 0.0;
             if (g >= ng || zero) continue;
-            const int rb = ch.rb0 + g / ch.nkg;
-            const int kg = ch.kg0 + g % ch.nkg;
+            const int ri = g / j.nkg, ki = g % j.nkg;  // groups within the chunk
+            const int rb = j.rb0 + ri;
+            const int kg = j.kg0 + ki;
             const int row = rb * 16 + lr - j.r0;
             const bool rin = row >= 0 && row < j.nr;
             const int k0 = kg * 8 + lk - j.k0, k1 = k0 + 4;
             const bool in0 = rin && k0 >= 0 && k0 < j.nk;
             const bool in1 = rin && k1 >= 0 && k1 < j.nk;
             // group tables: wave-uniform (base, stride) of this row / k group
-            const int64_t rbase = pool[j.rowtab + 2 * (rb - rbf)];
-            const int64_t rstr = pool[j.rowtab + 2 * (rb - rbf) + 1];
-            const int64_t kbase = pool[j.ktab + 2 * (kg - kgf)];
-            const int64_t kstr = pool[j.ktab + 2 * (kg - kgf) + 1];
+            const int64_t rbase = j.rt[ri][0];
+            const int64_t rstr = j.rt[ri][1];
+            const int64_t kbase = j.kt[ki][0];
+            const int64_t kstr = j.kt[ki][1];
             if (rin)
             {
                 const int64_t roff = j.src + (rstr != SCATTERED ? rbase + lr * rstr : pool[j.rowmap + row]);
@@ -170,8 +172,8 @@ __global__ __launch_bounds__(256) void pack_panel_kernel(const PanelJob* __restr
         {
             const int g = g0 + 4 * u;
             if (g >= ng) break;
-            const int rb = ch.rb0 + g / ch.nkg;
-            const int kg = ch.kg0 + g % ch.nkg;
+            const int rb = j.rb0 + g / j.nkg;
+            const int kg = j.kg0 + g % j.nkg;
             const int row = rb * 16 + lr - j.r0;
             const bool rin = row >= 0 && row < j.nr;
             const int k0 = kg * 8 + lk - j.k0, k1 = k0 + 4;
@@ -548,11 +550,11 @@ hipError_t launch_panel_pack(const PanelLaunch& L)
     if (L.nchunks == 0) return hipSuccess;
     if (L.family != FAMILY_F64) return launch_panel_pack_bf16(L);
     if (L.dtype == 0)
-        hipLaunchKernelGGL(pack_panel_kernel<double>, dim3(L.nchunks), dim3(256), 0, L.stream, L.jobs,
-                           L.chunks, L.pool, (const double*)L.A, (const double*)L.B, (double*)L.ws);
+        hipLaunchKernelGGL(pack_panel_kernel<double>, dim3(L.nchunks), dim3(256), 0, L.stream, L.chunks,
+                           L.pool, (const double*)L.A, (const double*)L.B, (double*)L.ws);
     else
-        hipLaunchKernelGGL(pack_panel_kernel<float>, dim3(L.nchunks), dim3(256), 0, L.stream, L.jobs,
-                           L.chunks, L.pool, (const float*)L.A, (const float*)L.B, (double*)L.ws);
+        hipLaunchKernelGGL(pack_panel_kernel<float>, dim3(L.nchunks), dim3(256), 0, L.stream, L.chunks,
+                           L.pool, (const float*)L.A, (const float*)L.B, (double*)L.ws);
     return hipGetLastError();
 }
 

@@ -38,22 +38,20 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
 
 // ------------------------------------------------------------------ pack
 
-__global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PanelJob* __restrict__ jobs,
-                                                              const PanelChunk* __restrict__ chunks,
+// As the FP64 pack: every address of a chunk follows from its PackChunk record.
+__global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PackChunk* __restrict__ chunks,
                                                               const int64_t* __restrict__ pool,
                                                               const float* __restrict__ A,
                                                               const float* __restrict__ B,
                                                               __bf16* __restrict__ ws)
 {
-    const PanelChunk ch = chunks[blockIdx.x];
-    const PanelJob j = jobs[ch.job];
+    const PackChunk& j = chunks[blockIdx.x];
     const float* __restrict__ X = j.operand ? B : A;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lr = lane & 31, lh = lane >> 5;
     const bool zero = j.src == PANEL_MISSING;
-    const int ng = ch.nrb * ch.nkg;
-    const int rbf = j.r0 / 32, ksf = j.k0 / 16;  // first groups of the job's tables
+    const int ng = j.nrb * j.nkg;
     const float scale = (float)j.scale;
     if (j.kfast && !zero)
     {
@@ -63,8 +61,8 @@ __global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PanelJob* __
         // The loads of kBatch pairs are issued before the first transpose.
         constexpr int kBatch = 2;
         __shared__ float tile[4][32][33];
-        const int npair = (ch.nkg + 1) / 2;
-        const int niter = ch.nrb * npair;
+        const int npair = (j.nkg + 1) / 2;
+        const int niter = j.nrb * npair;
         const int kk = lane & 31;
         for (int g0 = wave; g0 < niter; g0 += 4 * kBatch)
         {
@@ -76,13 +74,14 @@ __global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PanelJob* __
 #pragma unroll
                 for (int jj = 0; jj < 16; jj++) v[u][jj] = 0.0f;
                 if (g >= niter) continue;
-                const int rb = ch.rb0 + g / npair;
-                const int ks = ch.kg0 + 2 * (g % npair);
-                const bool two = ks + 1 < ch.kg0 + ch.nkg;
-                const int64_t rbase = pool[j.rowtab + 2 * (rb - rbf)];
-                const int64_t rstr = pool[j.rowtab + 2 * (rb - rbf) + 1];
+                const int ri = g / npair, ki = 2 * (g % npair);  // groups within the chunk
+                const int rb = j.rb0 + ri;
+                const int ks = j.kg0 + ki;
+                const bool two = ki + 1 < j.nkg;
+                const int64_t rbase = j.rt[ri][0];
+                const int64_t rstr = j.rt[ri][1];
                 const int h = kk >> 4;
-                const int64_t kb = pool[j.ktab + 2 * (ks + (two ? h : 0) - ksf)];
+                const int64_t kb = j.kt[ki + (two ? h : 0)][0];
                 const int pk = ks * 16 + kk - j.k0;  // job-relative k
                 const bool kin = pk >= 0 && pk < j.nk && (h == 0 || two);
                 const int64_t koff = kb + (kk & 15);
@@ -103,9 +102,9 @@ __global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PanelJob* __
             {
                 const int g = g0 + 4 * u;
                 if (g >= niter) break;
-                const int rb = ch.rb0 + g / npair;
-                const int ks = ch.kg0 + 2 * (g % npair);
-                const bool two = ks + 1 < ch.kg0 + ch.nkg;
+                const int rb = j.rb0 + g / npair;
+                const int ks = j.kg0 + 2 * (g % npair);
+                const bool two = ks + 1 < j.kg0 + j.nkg;
 #pragma unroll
                 for (int jj = 0; jj < 16; jj++) tile[wave][(lane >> 5) + 2 * jj][kk] = v[u][jj];
                 __builtin_amdgcn_wave_barrier();
@@ -165,16 +164,17 @@ __global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PanelJob* __
 #pragma unroll
             for (int e = 0; e < 8; e++) v[u][e] = 0.0f;
             if (g >= ng || zero) continue;
-            const int rb = ch.rb0 + g / ch.nkg;
-            const int ks = ch.kg0 + g % ch.nkg;
+            const int ri = g / j.nkg, ki = g % j.nkg;  // groups within the chunk
+            const int rb = j.rb0 + ri;
+            const int ks = j.kg0 + ki;
             const int row = rb * 32 + lr - j.r0;
             const bool rin = row >= 0 && row < j.nr;
             const int kb = ks * 16 + lh * 8 - j.k0;
             // group tables: wave-uniform (base, stride) of this row / k group
-            const int64_t rbase = pool[j.rowtab + 2 * (rb - rbf)];
-            const int64_t rstr = pool[j.rowtab + 2 * (rb - rbf) + 1];
-            const int64_t kbase = pool[j.ktab + 2 * (ks - ksf)];
-            const int64_t kstr = pool[j.ktab + 2 * (ks - ksf) + 1];
+            const int64_t rbase = j.rt[ri][0];
+            const int64_t rstr = j.rt[ri][1];
+            const int64_t kbase = j.kt[ki][0];
+            const int64_t kstr = j.kt[ki][1];
             if (!rin) continue;
             const int64_t roff = j.src + (rstr != SCATTERED ? rbase + lr * rstr : pool[j.rowmap + row]);
 #pragma unroll
@@ -190,8 +190,8 @@ __global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PanelJob* __
         {
             const int g = g0 + 4 * u;
             if (g >= ng) break;
-            const int rb = ch.rb0 + g / ch.nkg;
-            const int ks = ch.kg0 + g % ch.nkg;
+            const int rb = j.rb0 + g / j.nkg;
+            const int ks = j.kg0 + g % j.nkg;
             const int row = rb * 32 + lr - j.r0;
             const bool rin = row >= 0 && row < j.nr;
             const int kb = ks * 16 + lh * 8 - j.k0;
@@ -435,8 +435,8 @@ hipError_t launch_panel_pack_bf16(const PanelLaunch& L)
 {
     if (L.nchunks == 0) return hipSuccess;
     if (L.dtype != 1) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(pack_bf16_panel_kernel, dim3(L.nchunks), dim3(256), 0, L.stream, L.jobs,
-                       L.chunks, L.pool, (const float*)L.A, (const float*)L.B, (__bf16*)L.ws);
+    hipLaunchKernelGGL(pack_bf16_panel_kernel, dim3(L.nchunks), dim3(256), 0, L.stream, L.chunks, L.pool,
+                       (const float*)L.A, (const float*)L.B, (__bf16*)L.ws);
     return hipGetLastError();
 }
 

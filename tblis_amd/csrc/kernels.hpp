@@ -52,8 +52,7 @@ int panel_tile_n(int family, int tile);
 int panel_tile_waves(int family, int tile);
 struct PanelLaunch
 {
-    const PanelJob* jobs;
-    const PanelChunk* chunks;
+    const PackChunk* chunks;
     int nchunks;
     const PanelTask* tasks;
     int ntasks;

@@ -355,11 +355,22 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
 // tables_for_tile with each operand read directly only if it can be for
 // every GEMM of the plan (one kernel variant per launch)
 bool tables_uniform(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T);
-int autotune_tile(const std::vector<PanelGemm>& gemms, int family, const std::vector<int>& cands);
+int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bool swap,
+                  const std::vector<int>& cands);
+
+struct TuneOperands
+{
+    const void* A = nullptr;
+    const void* B = nullptr;
+};
+thread_local TuneOperands t_tune_ops;
 
 }  // namespace
 
-bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family)
+TuneOperandsScope::TuneOperandsScope(const void* A, const void* B) { t_tune_ops = TuneOperands{A, B}; }
+TuneOperandsScope::~TuneOperandsScope() { t_tune_ops = TuneOperands{}; }
+
+bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family, int dtype)
 {
     // drop GEMMs without rows or columns
     gemms.erase(std::remove_if(gemms.begin(), gemms.end(),
@@ -399,7 +410,7 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family)
             tile = tm.tile;
         }
     }
-    const int tuned = autotune_tile(gemms, family, cands);
+    const int tuned = autotune_tile(gemms, family, dtype, swap, cands);
     if (tuned >= 0) tile = tuned;
     if (const char* e = std::getenv("TBLIS_HIP_PANEL_TILE"))
     {
@@ -463,6 +474,8 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
     T.mfma_flops = 0;
     T.reads_ab = false;
 
+    int32_t cur_gemm = 0;  // index of the GEMM being laid out (GEMMs with tiles only)
+    T.need.clear();
     auto add_job = [&](const PanelJob& j) {
         if (j.nr <= 0 || j.nk <= 0) return;
         const int32_t id = (int32_t)jobs.size();
@@ -478,7 +491,9 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                 c.nrb = (int32_t)std::min<int64_t>(4, rb1 - rb);
                 c.kg0 = (int32_t)kg;
                 c.nkg = (int32_t)std::min<int64_t>(8, kg1 - kg);
+                c.gemm = cur_gemm;
                 chunks.push_back(c);
+                T.need[cur_gemm]++;
             }
     };
 
@@ -491,6 +506,8 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
         for (size_t i = 0; i < NK; i++) k0[i + 1] = k0[i] + g.klen[i];
         const int64_t m = m0[NM], n = n0[NN], k = k0[NK];
         if (m == 0 || n == 0) continue;
+        cur_gemm = (int32_t)T.need.size();
+        T.need.push_back(0);
         const int64_t mp = ceil_div(m, BM) * BM, np = ceil_div(n, BN) * BN;
         const int64_t kp = ceil_div(k, G.kpad) * G.kpad;
         const int64_t nkg = kp / G.gk;
@@ -721,6 +738,7 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                 t.nkg = (int32_t)nkg;
                 t.mlen = (int32_t)std::min<int64_t>(BM, m - tm * BM);
                 t.nlen = (int32_t)std::min<int64_t>(BN, n - tn * BN);
+                t.gemm = cur_gemm;
                 if (direct)
                 {
                     t.c_src = g.c[0].base;
@@ -858,7 +876,25 @@ void save_tune(int family, uint64_t h, int tile)
     std::fclose(f);
 }
 
-int autotune_tile(const std::vector<PanelGemm>& gemms, int family, const std::vector<int>& cands)
+// Stride signature of the single-block operands: with tensors to read, the
+// groups of directly read operands (and so the timing) depend on it.
+std::string layout_sig(const std::vector<PanelGemm>& gemms)
+{
+    std::string sig;
+    for (const PanelGemm& g : gemms)
+        for (const std::vector<PBlock>* bl : {&g.a, &g.b})
+        {
+            sig += bl->size() == 1 ? "|" : "|m";
+            if (bl->size() != 1) continue;
+            for (int64_t x : (*bl)[0].rows.stride) sig += std::to_string(x) + ",";
+            sig += "/";
+            for (int64_t x : (*bl)[0].k.stride) sig += std::to_string(x) + ",";
+        }
+    return sig;
+}
+
+int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bool swap,
+                  const std::vector<int>& cands)
 {
     if (!autotune_on() || cands.size() < 2) return -1;
     hipStream_t s = thread_state().stream;
@@ -868,7 +904,13 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, const std::ve
         (void)hipGetLastError();
         return -1;  // no allocation or timing inside a graph capture
     }
-    const uint64_t h = shape_hash(gemms, family);
+    // time candidates as they will run when the tensors can be read (FP64)
+    const TuneOperands ops = t_tune_ops;
+    const bool real = family == FAMILY_F64 && ops.A && ops.B && direct_enabled();
+    const void* opA = swap ? ops.B : ops.A;
+    const void* opB = swap ? ops.A : ops.B;
+    uint64_t h = shape_hash(gemms, family);
+    if (real) h = fnv1a(std::to_string(h) + "r" + layout_sig(gemms));
     TuneDb& db = tune_db();
     std::lock_guard<std::mutex> lock(db.mu);
     if (!db.loaded) load_tune_file(db);
@@ -880,9 +922,12 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, const std::ve
     const int eb = geom_of(family).elem_bytes;
     for (size_t i = 0; i < cands.size(); i++)
     {
-        // timed with both operands in panels (the tuner has no tensors)
+        // with tensors: grouped as the plan will be; without: both operands
+        // in panels
         const bool none[2] = {false, false};
-        if (!tables_for_tile(gemms, family, cands[i], tabs[i], none)) return -1;
+        if (real ? !tables_uniform(gemms, family, cands[i], tabs[i])
+                 : !tables_for_tile(gemms, family, cands[i], tabs[i], none))
+            return -1;
         panel_bytes = std::max(panel_bytes, (size_t)tabs[i].ws_panel * eb + 256);
         cslot = std::max(cslot, (size_t)panel_tile_m(family, cands[i]) * panel_tile_n(family, cands[i]));
         ntask = std::max(ntask, tabs[i].tasks.size());
@@ -890,13 +935,14 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, const std::ve
     constexpr size_t SLOTS = 1024;
     const size_t nc = cands.size();
     void *panels = nullptr, *cbuf = nullptr;
-    std::vector<void*> dtasks(nc, nullptr);
+    std::vector<void*> dtasks(nc, nullptr), dpool(nc, nullptr), dchunks(nc, nullptr);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     auto release = [&]() {
         if (panels) (void)hipFree(panels);
         if (cbuf) (void)hipFree(cbuf);
-        for (void* p : dtasks)
-            if (p) (void)hipFree(p);
+        for (auto* v : {&dtasks, &dpool, &dchunks})
+            for (void* p : *v)
+                if (p) (void)hipFree(p);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         (void)hipGetLastError();
@@ -916,8 +962,19 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, const std::ve
         }
         ok = hipMalloc(&dtasks[i], tasks.size() * sizeof(PanelTask)) == hipSuccess &&
              hipMemcpyAsync(dtasks[i], tasks.data(), tasks.size() * sizeof(PanelTask), hipMemcpyHostToDevice,
-                            s) == hipSuccess &&
-             hipStreamSynchronize(s) == hipSuccess;
+                            s) == hipSuccess;
+        if (ok && real)
+        {
+            // the pack and direct reads use the plan's own tables
+            const std::vector<PackChunk> pc = pack_records(tabs[i], family);
+            const std::vector<int64_t>& pool = tabs[i].pool;
+            ok = hipMalloc(&dpool[i], std::max<size_t>(pool.size(), 1) * 8) == hipSuccess &&
+                 hipMemcpyAsync(dpool[i], pool.data(), pool.size() * 8, hipMemcpyHostToDevice, s) == hipSuccess &&
+                 hipMalloc(&dchunks[i], std::max<size_t>(pc.size(), 1) * sizeof(PackChunk)) == hipSuccess &&
+                 hipMemcpyAsync(dchunks[i], pc.data(), pc.size() * sizeof(PackChunk), hipMemcpyHostToDevice,
+                                s) == hipSuccess;
+        }
+        ok = ok && hipStreamSynchronize(s) == hipSuccess;
     }
     if (!ok)
     {
@@ -933,11 +990,31 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, const std::ve
         L.alpha = 1.0;
         L.beta = 0.0;
         L.use_k = 1;
-        L.dtype = family == FAMILY_F64 ? 0 : 1;
+        L.dtype = real ? dtype : (family == FAMILY_F64 ? 0 : 1);
         L.family = family;
         L.tile = cands[i];
         L.stream = s;
-        return launch_panel_gemm(L) == hipSuccess;
+        if (!real) return launch_panel_gemm(L) == hipSuccess;
+        // pack into the scratch panels, then one GEMM launch per group
+        L.chunks = (const PackChunk*)dchunks[i];
+        L.nchunks = (int)tabs[i].chunks.size();
+        L.pool = (const int64_t*)dpool[i];
+        L.A = opA;
+        L.B = opB;
+        if (launch_panel_pack(L) != hipSuccess) return false;
+        int64_t at = 0;
+        for (int g = 0; g < 4; g++)
+        {
+            if (tabs[i].group_n[g] == 0) continue;
+            PanelLaunch G = L;
+            G.tasks = L.tasks + at;
+            G.ntasks = (int)tabs[i].group_n[g];
+            G.direct_a = (g & 1) != 0;
+            G.direct_b = (g & 2) != 0;
+            if (launch_panel_gemm(G) != hipSuccess) return false;
+            at += tabs[i].group_n[g];
+        }
+        return true;
     };
     // fastest of `reps` timed launches, or -1 on an error
     auto time_of = [&](size_t i, int reps) {
@@ -1005,10 +1082,48 @@ int autotune_runs()
     return db.runs;
 }
 
+std::vector<PackChunk> pack_records(const PanelTables& T, int family)
+{
+    const Geom G = geom_of(family);
+    std::vector<PackChunk> pc(T.chunks.size());
+    for (size_t i = 0; i < T.chunks.size(); i++)
+    {
+        const PanelChunk& c = T.chunks[i];
+        const PanelJob& j = T.jobs[c.job];
+        PackChunk& o = pc[i];
+        o = PackChunk{};
+        o.src = j.src;
+        o.out = j.out;
+        o.rowmap = j.rowmap;
+        o.kmap = j.kmap;
+        o.scale = j.scale;
+        o.r0 = j.r0;
+        o.nr = j.nr;
+        o.k0 = j.k0;
+        o.nk = j.nk;
+        o.pnkg = j.pnkg;
+        o.operand = j.operand;
+        o.kfast = j.kfast;
+        o.gemm = c.gemm;
+        o.rb0 = c.rb0;
+        o.nrb = c.nrb;
+        o.kg0 = c.kg0;
+        o.nkg = c.nkg;
+        if (c.nrb > 4 || c.nkg > 8) fail("internal: pack chunk larger than its record");
+        if (j.src == PANEL_MISSING) continue;
+        const int64_t rbf = j.r0 / G.gr, kgf = j.k0 / G.gk;
+        for (int r = 0; r < c.nrb; r++)
+            for (int h = 0; h < 2; h++) o.rt[r][h] = T.pool[j.rowtab + 2 * (c.rb0 + r - rbf) + h];
+        for (int q = 0; q < c.nkg; q++)
+            for (int h = 0; h < 2; h++) o.kt[q][h] = T.pool[j.ktab + 2 * (c.kg0 + q - kgf) + h];
+    }
+    return pc;
+}
+
 bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, int dtype, int family)
 {
     PanelTables T;
-    if (!plan_panel_tables(std::move(gemms), T, family)) return false;
+    if (!plan_panel_tables(std::move(gemms), T, family, dtype)) return false;
     plan.swap = T.swap;
     plan.useful_flops = T.useful_flops;
     plan.mfma_flops = T.mfma_flops;
@@ -1031,19 +1146,20 @@ bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, i
     plan.ws_c_offset = (pbytes + 255) / 256 * 256;
     plan.ws_bytes = std::max<size_t>(plan.ws_c_offset + (size_t)T.ws_c * sizeof(double), 256);
 
+    const std::vector<PackChunk> pc = pack_records(T, family);
+
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
-    const size_t s_jobs = T.jobs.size() * sizeof(PanelJob), s_chunks = T.chunks.size() * sizeof(PanelChunk);
+    const size_t s_chunks = pc.size() * sizeof(PackChunk);
     const size_t s_tasks = T.tasks.size() * sizeof(PanelTask), s_uj = T.ujobs.size() * sizeof(UnpackJob);
     const size_t s_uc = T.uchunks.size() * sizeof(UnpackChunk);
     const size_t s_pool = std::max<size_t>(T.pool.size(), 1) * 8;
-    const size_t o_chunks = al(s_jobs), o_tasks = o_chunks + al(s_chunks), o_uj = o_tasks + al(s_tasks);
+    const size_t o_chunks = 0, o_tasks = o_chunks + al(s_chunks), o_uj = o_tasks + al(s_tasks);
     const size_t o_uc = o_uj + al(s_uj), o_pool = o_uc + al(s_uc);
     plan.bytes = o_pool + s_pool;
     hip_check(hipGetDevice(&plan.device), "hipGetDevice");
     hip_check(hipMalloc(&plan.dev, plan.bytes), "hipMalloc (plan)");
     char* base = (char*)plan.dev;
-    plan.d_pjobs = (PanelJob*)base;
-    plan.d_pchunks = (PanelChunk*)(base + o_chunks);
+    plan.d_pchunks = (PackChunk*)(base + o_chunks);
     plan.d_ptasks = (PanelTask*)(base + o_tasks);
     plan.d_ujobs = (UnpackJob*)(base + o_uj);
     plan.d_uchunks = (UnpackChunk*)(base + o_uc);
@@ -1051,8 +1167,7 @@ bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, i
     auto up = [&](void* dst, const void* src, size_t n) {
         if (n) hip_check(hipMemcpy(dst, src, n, hipMemcpyHostToDevice), "hipMemcpy (plan)");
     };
-    up(plan.d_pjobs, T.jobs.data(), s_jobs);
-    up(plan.d_pchunks, T.chunks.data(), s_chunks);
+    up(plan.d_pchunks, pc.data(), s_chunks);
     up(plan.d_ptasks, T.tasks.data(), s_tasks);
     up(plan.d_ujobs, T.ujobs.data(), s_uj);
     up(plan.d_uchunks, T.uchunks.data(), s_uc);

@@ -65,6 +65,7 @@ struct PanelTables
     std::vector<UnpackJob> ujobs;
     std::vector<UnpackChunk> uchunks;
     std::vector<int64_t> pool;
+    std::vector<int32_t> need;  // pack chunks of each GEMM (chunk and task .gemm index it)
     int tile = 0;
     int family = 0;
     bool swap = false;  // GEMMs were transposed: operand 0 is the B tensor
@@ -79,8 +80,24 @@ struct PanelTables
 };
 
 // Orientation, tile, pack jobs, tasks and C unpack jobs (host only).
-// Returns false when there is nothing to launch.
-bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family);
+// Returns false when there is nothing to launch.  dtype: 0 double, 1 float.
+bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family, int dtype);
+
+// The pack kernels' chunk records (job fields and group tables per chunk).
+std::vector<PackChunk> pack_records(const PanelTables& T, int family);
+
+// Tensors the autotuner may read, never write, while it times candidate
+// tiles: those of the call being planned, already checked by the caller
+// (set for the life of a TuneOperandsScope on this thread).  With them an
+// FP64 candidate is timed as it will run -- pack, then one GEMM launch per
+// group of tasks with the same operands read straight from the tensors --
+// since which operands can be read directly depends on the tile.  Without
+// them every candidate is timed on scratch panels.
+struct TuneOperandsScope
+{
+    TuneOperandsScope(const void* A, const void* B);
+    ~TuneOperandsScope();
+};
 
 // Autotuner switch (initially TBLIS_HIP_AUTOTUNE) and the number of tuning
 // measurements made by this process.

@@ -600,7 +600,6 @@ void run_plan(const std::shared_ptr<Plan>& pp, const void* A, const void* B, voi
     if (p.panel)
     {
         PanelLaunch P{};
-        P.jobs = p.d_pjobs;
         P.chunks = p.d_pchunks;
         P.nchunks = p.p_nchunks;
         P.tasks = p.d_ptasks;

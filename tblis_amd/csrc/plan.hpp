@@ -58,8 +58,7 @@ struct Plan
     bool direct[2] = {false, false};  // FP64 operands read from the tensors (every task)
     int64_t group_n[4] = {0, 0, 0, 0};  // panel tasks reading none / A / B / both from the tensors
     int p_nchunks = 0, p_nuchunks = 0;
-    PanelJob* d_pjobs = nullptr;
-    PanelChunk* d_pchunks = nullptr;
+    PackChunk* d_pchunks = nullptr;
     PanelTask* d_ptasks = nullptr;
     UnpackJob* d_ujobs = nullptr;
     UnpackChunk* d_uchunks = nullptr;

@@ -17,8 +17,9 @@ makes it the headline):
   dense24      C[abij] = A[abef]*B[efij], extents 24, FP64, column-major,
                alpha 1, beta 0 (a small contraction: launch-bound)
   fp32_96      C[abcd] = A[aebf]*B[ecfd], extents 96, FP32 normal(0,1) with
-               1% of entries x1e4: split-BF16 MFMA vs FP64 MFMA, relative
-               Frobenius error of each against a float64 torch product
+               1% of entries x1e4: split-BF16 MFMA vs FP64 MFMA, and the
+               FP64-accurate int8 digit mode on the same values held as FP64;
+               relative Frobenius error of each against a float64 torch product
   dpd          8-irrep DPD C[abij] = A[abef]*B[efij], per-irrep extents
                [12,9,7,5,11,8,6,4]: FP64 and split-BF16, and the same kernels
                on the dense 62^4 contraction (dense-equivalent throughput),
@@ -60,6 +61,12 @@ PEAK_FP64 = 78.6    # TFLOP/s, MI355X dense FP64 matrix (datasheet)
 PEAK_BF16 = 2500.0  # TFLOP/s, MI355X dense BF16 matrix (datasheet)
 PEAK_HBM = 8000.0   # GB/s, MI355X HBM3E (datasheet)
 SPLIT_PRODUCTS = 6  # BF16 MFMA products per split-BF16 multiply-add
+PEAK_INT8 = 5000.0  # TOP/s, MI355X dense INT8 matrix (datasheet)
+# digits per value of the FP64-accurate mode (the library's default 8 unless
+# TBLIS_HIP_OZAKI_SLICES picks 6 or 7); digit pairs = d (d + 1) / 2
+OZAKI_DIGITS = int(os.environ.get("TBLIS_HIP_OZAKI_SLICES", "8"))
+if OZAKI_DIGITS not in (6, 7, 8):
+    OZAKI_DIGITS = 8
 
 
 def colmaj(lens):
@@ -186,7 +193,22 @@ def fp32_leg(T, dev, stream, steps, warmup):
     del A2, B2
     for name, Cx in (("split_bf16", Cs), ("fp64_mfma", Cd)):
         out[name]["rel_err"] = float("%.3e" % rel_fro(strided(Cx, L), ref))
+    # FP64-accurate mode (int8 digit planes) on the same values held as FP64
+    Ad, Bd = A.double(), B.double()
+    Co = torch.zeros(n4, dtype=torch.float64, device=dev)
+    fn = T.prepare_mult(1.0, strided(Ad, L), "aebf", strided(Bd, L), "ecfd", 0.0, strided(Co, L), "abcd",
+                        config="ozaki")
+    ms_o, ms_ok, _ = time_steps(fn, steps, warmup, stream)
+    pairs = OZAKI_DIGITS * (OZAKI_DIGITS + 1) // 2
+    out["ozaki"] = {"algorithm": T.last_algorithm(), "digits": OZAKI_DIGITS, "ms": round(ms_o, 4),
+                    "useful_tflops": tflops(flops, ms_o), "call_ms_events": round(ms_ok, 4),
+                    "rel_err": float("%.3e" % rel_fro(strided(Co, L), ref)),
+                    "int8_tops": tflops(pairs * flops, ms_o),
+                    "mfma_frac": round(pairs * flops / (ms_o * 1e-3) / 1e12 / PEAK_INT8, 4),
+                    "inputs": "the FP32 values held as FP64"}
+    del Ad, Bd, Co
     ms_s, ms_d = out["split_bf16"]["ms"], out["fp64_mfma"]["ms"]
+    out["speedup_ozaki_vs_fp64"] = round(ms_d / ms_o, 3)
     out["speedup_split_vs_fp64"] = round(ms_d / ms_s, 3)
     out["split_bf16"]["mfma_tflops"] = tflops(SPLIT_PRODUCTS * flops, ms_s)
     out["split_bf16"]["mfma_frac"] = round(SPLIT_PRODUCTS * flops / (ms_s * 1e-3) / 1e12 / PEAK_BF16, 4)

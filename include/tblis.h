@@ -138,8 +138,8 @@ void tblis_init_tensor_z(tblis_tensor* t, unsigned ndim, len_type* len, dcomplex
  * overwrites C without reading it; alpha == 0 reads neither A nor B.  After
  * the call C->scalar is 1 and C->conj is 0.  comm may be NULL or any
  * communicator (the GPU grid replaces the thread team).  cfg NULL selects
- * the default algorithm for the type (FP64: FP64 MFMA; FP32: 3-term
- * split-BF16 MFMA). */
+ * the default algorithm for the type (FP64 and complex double: FP64 MFMA;
+ * FP32 and complex float: 3-term split-BF16 MFMA). */
 void tblis_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
                        const tblis_tensor* A, const label_type* idx_A,
                        const tblis_tensor* B, const label_type* idx_B,
@@ -202,6 +202,11 @@ void tblis_set_num_threads(unsigned num_threads);
  *   "bf16x3"        3-term split-BF16 (6 products), FP32 accumulation
  *   "bf16x3_fast"   3-term split-BF16 without the compensated k-blocking
  *   "reference"     plain FMA kernel (no MFMA), for cross-checks
+ *   "ozaki"         FP64-accurate for double and float: each row of A and B
+ *                   scaled by a power of two and cut into 8 signed 7-bit
+ *                   digits (6 or 7 with TBLIS_HIP_OZAKI_SLICES), digit-pair
+ *                   products on int8 MFMA with exact int32 sums, combined in
+ *                   FP64 (no counterpart in the reference; finite inputs)
  * Unknown names abort with a message, as in the reference. */
 const tblis_config* tblis_get_config(const char* name);
 

@@ -26,7 +26,7 @@ unsigned g_num_threads = 0;
 const tblis_config_s g_configs[] = {
     {Algo::Default, "mi355x"},       {Algo::Fp64Mfma, "fp64_mfma"},
     {Algo::Bf16x3, "bf16x3"},        {Algo::Bf16x3Fast, "bf16x3_fast"},
-    {Algo::Reference, "reference"},
+    {Algo::Reference, "reference"},  {Algo::Ozaki, "ozaki"},
 };
 
 // Run f; on an argument/runtime error abort with a message (the

@@ -111,6 +111,7 @@ enum class Algo : int
     Bf16x3 = 2,
     Bf16x3Fast = 3,
     Reference = 4,
+    Ozaki = 5,  // FP64-accurate products from int8 MFMA digit planes
 };
 
 const char* algo_name(Algo a);

@@ -182,6 +182,7 @@ const char* algo_name(Algo a)
         case Algo::Bf16x3: return "bf16x3";
         case Algo::Bf16x3Fast: return "bf16x3_fast";
         case Algo::Reference: return "reference";
+        case Algo::Ozaki: return "ozaki";
     }
     return "?";
 }
@@ -189,7 +190,7 @@ const char* algo_name(Algo a)
 Algo resolve_algo(const tblis_config* cfg, type_t type)
 {
     Algo a = cfg ? cfg->algo : Algo::Default;
-    if (a == Algo::Reference) return a;
+    if (a == Algo::Reference || a == Algo::Ozaki) return a;
     if (type == TYPE_DOUBLE) return Algo::Fp64Mfma;
     if (a == Algo::Default) return Algo::Bf16x3;
     return a;
@@ -202,6 +203,7 @@ int family_of(Algo a)
 {
     if (a == Algo::Bf16x3) return FAMILY_BF16X3;
     if (a == Algo::Bf16x3Fast) return FAMILY_BF16X3_FAST;
+    if (a == Algo::Ozaki) return FAMILY_OZAKI;
     return FAMILY_F64;
 }
 

@@ -528,20 +528,25 @@ hipError_t launch_gemm_d(const PanelLaunch& L)
 
 }  // namespace
 
+// The Ozaki family has one tile: 64 x 64 per workgroup, 2 x 2 waves.
 int panel_num_tiles(int family)
 {
+    if (family == FAMILY_OZAKI) return 1;
     return family == FAMILY_F64 ? kNumTiles : panel_bf16_num_tiles();
 }
 int panel_tile_m(int family, int t)
 {
+    if (family == FAMILY_OZAKI) return 64;
     return family == FAMILY_F64 ? kTiles[t].bm : panel_bf16_tile_m(t);
 }
 int panel_tile_n(int family, int t)
 {
+    if (family == FAMILY_OZAKI) return 64;
     return family == FAMILY_F64 ? kTiles[t].bn : panel_bf16_tile_n(t);
 }
 int panel_tile_waves(int family, int t)
 {
+    if (family == FAMILY_OZAKI) return 4;
     return family == FAMILY_F64 ? kTiles[t].wgm * kTiles[t].wgn : panel_bf16_tile_waves(t);
 }
 

@@ -85,6 +85,10 @@ int panel_bf16_tile_n(int tile);
 int panel_bf16_tile_waves(int tile);
 bool panel_bf16_tile_fast_only(int tile);
 hipError_t launch_panel_pack_bf16(const PanelLaunch& L);
+// Ozaki family (gemm_ozaki.hip): row maxima, digit planes and the int8 GEMM
+// in one call; rmax holds nrows panel-row maxima in the workspace.
+hipError_t launch_panel_ozaki(const PanelLaunch& L, int slices, unsigned long long* rmax, int64_t nrows);
+int ozaki_slices();
 hipError_t launch_panel_gemm_bf16(const PanelLaunch& L);
 
 // family-generic helpers

@@ -127,6 +127,7 @@ struct Geom
 Geom geom_of(int family)
 {
     if (family == FAMILY_F64) return Geom{16, 8, 1, 16, 8};
+    if (family == FAMILY_OZAKI) return Geom{32, 32, ozaki_slices(), 32, 1};
     return Geom{32, 16, 3, 32, 2};
 }
 
@@ -228,6 +229,7 @@ struct TileModel
 const TileModel kModelsF64[] = {{0, 0.85, 2}, {2, 0.80, 2}, {3, 0.82, 2}, {4, 0.82, 2},
                                 {1, 0.50, 6}, {5, 0.45, 6}};
 const TileModel kModelsBf16[] = {{0, 0.55, 1}, {1, 0.55, 1}, {2, 0.45, 2}};
+const TileModel kModelsOzaki[] = {{0, 0.30, 1}};
 
 // Estimated run time of the GEMM tasks for a tile.
 double tile_cost(const std::vector<PanelGemm>& gemms, const TileModel& tm, int cus, int family)
@@ -339,9 +341,20 @@ bool direct_enabled()
     return !(e && e[0] == '0');
 }
 
+int ozaki_slices()
+{
+    static const int s = [] {
+        const char* e = std::getenv("TBLIS_HIP_OZAKI_SLICES");
+        const int v = e ? std::atoi(e) : 8;
+        return v >= 6 && v <= 8 ? v : 8;
+    }();
+    return s;
+}
+
 bool panel_enabled(int family, int use_mfma)
 {
     if (!use_mfma) return false;
+    if (family == FAMILY_OZAKI) return true;  // digit planes exist only on the panel path
     if (const char* e = std::getenv("TBLIS_HIP_PANEL"))
         if (e[0] == '0') return false;
     return true;
@@ -395,8 +408,8 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family,
     const int cus = device_cu_count();
     int tile = 0;
     double best = -1;
-    const TileModel* models = family == FAMILY_F64 ? kModelsF64 : kModelsBf16;
-    const int nmodels = family == FAMILY_F64 ? 6 : 3;
+    const TileModel* models = family == FAMILY_F64 ? kModelsF64 : family == FAMILY_OZAKI ? kModelsOzaki : kModelsBf16;
+    const int nmodels = family == FAMILY_F64 ? 6 : family == FAMILY_OZAKI ? 1 : 3;
     std::vector<int> cands;
     for (int mi = 0; mi < nmodels; mi++)
     {
@@ -475,6 +488,8 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
     T.reads_ab = false;
 
     int32_t cur_gemm = 0;  // index of the GEMM being laid out (GEMMs with tiles only)
+    int64_t rows = 0;      // Ozaki family: panel rows (row-maximum array entries)
+    const bool ozaki = family == FAMILY_OZAKI;
     T.need.clear();
     auto add_job = [&](const PanelJob& j) {
         if (j.nr <= 0 || j.nk <= 0) return;
@@ -592,6 +607,8 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
         if (!dir[0]) ws += G.terms * mp * kp;
         const int64_t ob = ws;
         if (!dir[1]) ws += G.terms * np * kp;
+        const int64_t ea = rows, eb = rows + mp;
+        if (ozaki) rows += mp + np;
         const std::vector<int64_t>* rpre[2] = {&m0, &n0};
         const int64_t rpad[2] = {mp, np}, outs[2] = {oa, ob};
         for (int o = 0; o < 2 && kp > 0; o++)
@@ -611,6 +628,7 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                     j.out = outs[o];
                     j.pnkg = (int32_t)nkg;
                     j.operand = o;
+                    j.eoff = o == 0 ? ea : eb;
                     if (o == 0 && !g.kscale.empty()) j.scale = g.kscale[q];
                     if (j.nr == 0 || j.nk == 0) continue;
                     if (blk.base != PANEL_MISSING)
@@ -629,6 +647,7 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
             z.out = outs[o];
             z.pnkg = (int32_t)nkg;
             z.operand = o;
+            z.eoff = o == 0 ? ea : eb;
             z.r0 = (int32_t)rtot[o];
             z.nr = (int32_t)(rpad[o] - rtot[o]);
             z.k0 = 0;
@@ -729,8 +748,8 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                 t.klen = (int32_t)k;
                 t.a_src = dir[0] ? dsrc[0] : PANEL_MISSING;
                 t.b_src = dir[1] ? dsrc[1] : PANEL_MISSING;
-                t.a_rt = drt[0] + 2 * tm * (BM / G.gr);
-                t.b_rt = drt[1] + 2 * tn * (BN / G.gr);
+                t.a_rt = ozaki ? ea + tm * BM : drt[0] + 2 * tm * (BM / G.gr);
+                t.b_rt = ozaki ? eb + tn * BN : drt[1] + 2 * tn * (BN / G.gr);
                 t.a_kt = dkt[0];
                 t.b_kt = dkt[1];
                 t.a_bytes = dbytes[0];
@@ -771,6 +790,7 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
     tasks = xcd_order(tasks, cost, region, nregions);
 
     T.ws_panel = ws;
+    T.ws_rows = rows;
     T.ws_c = wsc;
     T.family = family;
     T.tile = tile;
@@ -1109,6 +1129,7 @@ std::vector<PackChunk> pack_records(const PanelTables& T, int family)
         o.nrb = c.nrb;
         o.kg0 = c.kg0;
         o.nkg = c.nkg;
+        o.eoff = j.eoff;
         if (c.nrb > 4 || c.nkg > 8) fail("internal: pack chunk larger than its record");
         if (j.src == PANEL_MISSING) continue;
         const int64_t rbf = j.r0 / G.gr, kgf = j.k0 / G.gk;
@@ -1143,7 +1164,10 @@ bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, i
     plan.p_nchunks = (int)T.chunks.size();
     plan.p_nuchunks = (int)T.uchunks.size();
     const size_t pbytes = (size_t)T.ws_panel * geom_of(family).elem_bytes;
-    plan.ws_c_offset = (pbytes + 255) / 256 * 256;
+    // workspace: panels | Ozaki row maxima | C panels
+    plan.ws_rmax_offset = (pbytes + 255) / 256 * 256;
+    plan.oz_rows = T.ws_rows;
+    plan.ws_c_offset = plan.ws_rmax_offset + ((size_t)T.ws_rows * 8 + 255) / 256 * 256;
     plan.ws_bytes = std::max<size_t>(plan.ws_c_offset + (size_t)T.ws_c * sizeof(double), 256);
 
     const std::vector<PackChunk> pc = pack_records(T, family);

@@ -74,7 +74,8 @@ struct PanelTables
     // tasks come in four consecutive groups by which operands they read from
     // the tensors (none, A, B, both); one GEMM launch per non-empty group
     int64_t group_n[4] = {0, 0, 0, 0};
-    int64_t ws_panel = 0;  // panel values (double for FP64, BF16 for split-BF16)
+    int64_t ws_panel = 0;  // panel values (double for FP64, BF16 for split-BF16, bytes for Ozaki)
+    int64_t ws_rows = 0;   // Ozaki: row-maximum entries (one per panel row)
     int64_t ws_c = 0;      // C panel doubles
     int64_t useful_flops = 0, mfma_flops = 0;
 };

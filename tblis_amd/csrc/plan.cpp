@@ -623,6 +623,16 @@ void run_plan(const std::shared_ptr<Plan>& pp, const void* A, const void* B, voi
         P.dtype = p.key.dtype;
         P.tile = p.panel_tile;
         P.stream = ts.stream;
+        if (P.family == FAMILY_OZAKI)
+        {
+            // row maxima, digit planes, int8 GEMM; then C panels if any
+            hip_check(launch_panel_ozaki(P, ozaki_slices(), (unsigned long long*)(ws + p.ws_rmax_offset),
+                                         P.use_k ? p.oz_rows : 0),
+                      "kernel launch");
+            hip_check(launch_panel_unpack(P), "unpack launch");
+            if (!ts.async && !capturing) hip_check(hipStreamSynchronize(ts.stream), "hipStreamSynchronize");
+            return;
+        }
         if (P.use_k) hip_check(launch_panel_pack(P), "pack launch");
         // one GEMM launch per group of tasks with the same direct operands
         int64_t at = 0;

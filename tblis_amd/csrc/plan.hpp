@@ -55,6 +55,8 @@ struct Plan
     bool panel = false;
     int panel_tile = 0;
     size_t ws_c_offset = 0;  // byte offset of the C panels in the workspace
+    size_t ws_rmax_offset = 0;  // Ozaki: byte offset of the row-maximum array
+    int64_t oz_rows = 0;        // Ozaki: its entries
     bool direct[2] = {false, false};  // FP64 operands read from the tensors (every task)
     int64_t group_n[4] = {0, 0, 0, 0};  // panel tasks reading none / A / B / both from the tensors
     int p_nchunks = 0, p_nuchunks = 0;

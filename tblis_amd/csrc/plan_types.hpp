@@ -100,6 +100,7 @@ struct PanelJob
     int32_t kfast;         // k groups unit-stride, rows not: read rows, transpose in LDS
     int32_t pad_;
     double scale = 1.0;    // every value is multiplied by this while packed
+    int64_t eoff = 0;      // Ozaki family: row-maximum array index of panel row 0
 };
 
 // Rectangle of fragment groups of one job: rows blocks [rb0, rb0 + nrb),
@@ -124,6 +125,7 @@ struct PackChunk
     int32_t rb0, nrb, kg0, nkg;
     int64_t rt[4][2];
     int64_t kt[8][2];
+    int64_t eoff;  // Ozaki family: row-maximum array index of the panel's row 0
 };
 
 // One C tile of one panel GEMM.  An operand is either read from its panel
@@ -149,7 +151,9 @@ struct PanelTask
     int32_t c_ld;
     // direct operands (FP64 family)
     int64_t a_src, b_src;  // element offset of the block in its tensor
-    int64_t a_rt, b_rt;    // pool index of the tile's first row-group (base, stride)
+    // pool index of the tile's first row-group (base, stride); in the Ozaki
+    // family the row-maximum array index of the tile's first row of A' / B'
+    int64_t a_rt, b_rt;
     int64_t a_kt, b_kt;    // pool index of k group 0's (base, stride)
     uint32_t a_bytes, b_bytes;  // extent of the block from x_src
     int32_t klen;          // valid k
@@ -172,6 +176,7 @@ struct UnpackChunk
 constexpr int FAMILY_F64 = 0;          // FP64 MFMA (FP32 data widened)
 constexpr int FAMILY_BF16X3 = 1;       // FP32 as 3-term split BF16, chunked accumulation
 constexpr int FAMILY_BF16X3_FAST = 2;  // same, single accumulator
+constexpr int FAMILY_OZAKI = 3;        // FP64-accurate: int8 digit planes (gemm_ozaki.hip)
 
 // Kernel variant selector.
 struct KernelKey

@@ -218,7 +218,7 @@ def test_abort_mode_and_unknown_config():
             "C.byref(c.d),b'c')\n")
     r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True)
     assert r.returncode != 0 and "tblis: invalid contraction" in r.stderr
-    for name in ("mi355x", "fp64_mfma", "bf16x3", "bf16x3_fast", "reference"):
+    for name in ("mi355x", "fp64_mfma", "bf16x3", "bf16x3_fast", "reference", "ozaki"):
         assert T.get_config(name)
 
 

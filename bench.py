@@ -561,9 +561,17 @@ def main():
         sys.exit(2)
 
     # counter passes first: they run this script as rocprofv3 children, before
-    # this process initialises the GPU
+    # this process initialises the GPU.  The children and this process share
+    # one autotuner file, so the tile the counters see is the tile timed here
+    # (the first child tunes, the others read its choice).
     traffic = None
+    tune_tmp = None
     if args.only in (None, "permuted") and not args.no_traffic:
+        if not os.environ.get("TBLIS_HIP_TUNE_FILE"):
+            import tempfile
+            fd, tune_tmp = tempfile.mkstemp(prefix="tblis_tune_", suffix=".jsonl")
+            os.close(fd)
+            os.environ["TBLIS_HIP_TUNE_FILE"] = tune_tmp
         traffic = measure_traffic()
 
     import torch
@@ -653,8 +661,8 @@ def main():
             out["roofline"]["traffic"] = traffic["gemm"]
             out["roofline"]["traffic_note"] = (
                 "HBM bytes per GEMM launch: rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a %d MiB "
-                "copy (factors %s); pack kernel %s bytes per launch" %
-                (COPY_MIB, traffic["scale"], traffic.get("pack")))
+                "copy (factors %s), in child runs that use this run's tile (shared autotuner file); "
+                "pack kernel %s bytes per launch" % (COPY_MIB, traffic["scale"], traffic.get("pack")))
     out.update(legs)
     if not args.no_cpu and lens is not None:
         threads = min(os.cpu_count() or 1, 16)
@@ -666,6 +674,8 @@ def main():
                                          "scatter vectors on %d host threads" %
                                          (args.cpu_cols, fl, runs, dt, threads)}
     print(json.dumps(out))
+    if tune_tmp:
+        os.remove(tune_tmp)
 
 
 if __name__ == "__main__":

@@ -341,6 +341,18 @@ bool direct_enabled()
     return !(e && e[0] == '0');
 }
 
+// Rows of tiles per group in the task order (TBLIS_HIP_GROUP_M, default 8;
+// 1 is a row-major walk).
+int tile_group_rows()
+{
+    static const int g = [] {
+        const char* e = std::getenv("TBLIS_HIP_GROUP_M");
+        const int v = e ? std::atoi(e) : 8;
+        return v >= 1 && v <= 64 ? v : 8;
+    }();
+    return g;
+}
+
 int ozaki_slices()
 {
     static const int s = [] {
@@ -737,8 +749,16 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                 }
             }
         }
-        for (int64_t tm = 0; tm < mp / BM; tm++)
-            for (int64_t tn = 0; tn < np / BN; tn++)
+        // Tiles in groups of G rows, columns walked with the rows fastest:
+        // the tiles an XCD runs at once then share G row panels of A' and
+        // a few column panels of B' in its L2, where a row-major walk would
+        // share one A' panel and stream a new B' panel per tile.
+        const int64_t GR = std::max<int64_t>(1, tile_group_rows());
+        for (int64_t pid = 0; pid < MT * NT; pid++)
+        {
+            const int64_t first = pid / (GR * NT) * GR, gm = std::min(GR, MT - first);
+            const int64_t local = pid % (GR * NT);  // < gm * NT
+            const int64_t tm = first + local % gm, tn = local / gm;
             {
                 region.push_back(nregions + (tm * RM / MT) * RN + tn * RN / NT);
                 PanelTask t{};
@@ -780,6 +800,7 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                 tasks.push_back(t);
                 cost.push_back(std::max<int64_t>(nkg, 1) * t.mlen * t.nlen);
             }
+        }
         nregions += RM * RN;
     }
     if (tasks.empty()) return false;

@@ -469,3 +469,57 @@ def fill_uniform(n, seed):
     x = np.empty(n, np.float64)
     lib().orc_fill_uniform_d(x.ctypes.data_as(PD), I64(n), C.c_uint64(seed))
     return x
+
+
+# ------------------------------------------------------------ FP64-accurate mode
+# No counterpart in the reference.  Restates the Ozaki scheme as the library
+# runs it (Ozaki, Ogita, Oishi and Rump, Numer. Algorithms 59 (2012); integer
+# form after Ootomo, Ozaki and Yokota, IJHPCA 2024), with the same operation
+# order, so that the library's result can be compared bit for bit.
+
+def ozaki_exponents(X):
+    """Per row of X: the smallest e with max|x| < 2^e, from the bit pattern of
+    the row maximum (0 for an all-zero row, -1022 for a subnormal maximum)."""
+    m = np.max(np.abs(X), axis=1) if X.shape[1] else np.zeros(X.shape[0])
+    biased = (m.view(np.uint64) >> np.uint64(52)).astype(np.int64)
+    e = np.where(biased == 0, -1022, biased - 1022)
+    return np.where(m == 0, 0, e).astype(np.int64)
+
+
+def ozaki_digits(X, S):
+    """S signed 7-bit digit planes of the row-scaled X: X[i] 2^-e_i =
+    sum_t D[t][i] 2^(-7 (t + 1)) + r, each step exact in float64."""
+    e = ozaki_exponents(X)
+    v = np.ldexp(X.astype(np.float64), -e[:, None])
+    D = np.zeros((S,) + X.shape, dtype=np.int64)
+    for t in range(S):
+        v = v * 128.0
+        d = np.trunc(v)
+        v = v - d
+        D[t] = d.astype(np.int64)
+    return D, e
+
+
+def ozaki_gemm(A, B, S=8, flush_k=8192):
+    """C = A B (m x k times k x n) as the FP64-accurate mode forms it: digit
+    pairs t + u <= S + 1 (1-based), exact integer sums per diagonal over
+    blocks of flush_k, added to an FP64 sum in diagonal order per block,
+    then scaled by 2^(e_i + f_j)."""
+    A = np.asarray(A, np.float64)
+    B = np.asarray(B, np.float64)
+    DA, ea = ozaki_digits(A, S)
+    DB, eb = ozaki_digits(B.T, S)
+    m, k = A.shape
+    n = B.shape[1]
+    tot = np.zeros((m, n))
+    for k0 in range(0, max(k, 1), flush_k):
+        ks = slice(k0, min(k0 + flush_k, k))
+        acc = [np.zeros((m, n), dtype=np.int64) for _ in range(S)]
+        for t in range(S):
+            for u in range(S - t):
+                # integer digits, |sum| <= 127^2 * flush_k < 2^53: a float64
+                # product is exact
+                acc[t + u] += (DA[t][:, ks].astype(np.float64) @ DB[u][:, ks].T.astype(np.float64)).astype(np.int64)
+        for d in range(S):
+            tot = tot + np.ldexp(1.0, -7 * (d + 2)) * acc[d].astype(np.float64)
+    return np.ldexp(tot, ea[:, None] + eb[None, :])

@@ -23,6 +23,8 @@
 //           combined per row with a 64-bit atomic max on the bit pattern
 //           (non-negative doubles order like their bits);
 //   pack    per pack chunk, the S digit planes in the MFMA operand order;
+//           x is the job's scale (complex signs, indexed entry factors)
+//           times the source value, in both passes;
 //   gemm    one 64 x 64 C tile per workgroup, 32 x 32 per wave, S int32
 //           accumulators (one per digit diagonal), FP64 epilogue with the
 //           row and column scales, alpha and beta.
@@ -101,7 +103,7 @@ __global__ __launch_bounds__(256) void oz_rowmax_kernel(const PackChunk* __restr
         for (int e = 0; e < 16; e++)
         {
             const int64_t off = oz_offset(j, pool, ri, ki, lr, 16 * lh + e);
-            if (off >= 0) m = fmax(m, oz_abs((double)X[off]));
+            if (off >= 0) m = fmax(m, oz_abs(j.scale * (double)X[off]));
         }
         m = fmax(m, __shfl_xor(m, 32));
         const int prow = (j.rb0 + ri) * 32 + lr;  // panel row
@@ -143,7 +145,7 @@ __global__ __launch_bounds__(256) void oz_pack_kernel(const PackChunk* __restric
             if (in && !zero)
             {
                 const int64_t off = oz_offset(j, pool, ri, ki, lr, 16 * lh + q);
-                v = ldexp((double)X[off], -e);  // |v| < 1
+                v = ldexp(j.scale * (double)X[off], -e);  // |v| < 1
             }
 #pragma unroll
             for (int t = 0; t < S; t++)

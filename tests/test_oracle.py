@@ -549,3 +549,52 @@ def test_indexed_dpd_oracle_no_indices_is_dpd(layout):
     want = po.dpd_mult(1.5, dA, "abe", A["entries"][0], dB, "ef", B["entries"][0], -0.5, dC, "abf",
                        np.array(C0["entries"][0]))
     np.testing.assert_allclose(got[0], want, rtol=1e-13, atol=1e-13)
+
+
+# ------------------------------------------------------------ FP64-accurate mode
+def test_ozaki_digits_reconstruct():
+    """Digits rebuild each value to within 2^-7S of its row scale, exactly
+    when the value needs no more bits than the digits hold."""
+    rng = np.random.default_rng(900)
+    X = rng.uniform(-1, 1, (20, 50)) * 10.0 ** rng.uniform(-5, 5, (20, 1))
+    for S in (6, 7, 8):
+        D, e = po.ozaki_digits(X, S)
+        assert np.all(np.abs(D) <= 127)
+        w = np.array([2.0 ** (-7 * (t + 1)) for t in range(S)])
+        rec = np.ldexp(np.tensordot(w, D.astype(np.float64), axes=1), e[:, None])
+        scale = np.ldexp(1.0, e)[:, None]
+        assert np.all(np.abs(rec - X) < 2.0 ** (-7 * S) * scale)
+        assert np.all(np.abs(rec) <= np.abs(X))  # truncation toward zero
+    # 10-bit integers need two digits below a 2^10 row scale: exact
+    Y = rng.integers(-1023, 1024, (8, 30)).astype(np.float64)
+    D, e = po.ozaki_digits(Y, 6)
+    rec = np.ldexp(np.tensordot([2.0 ** (-7 * (t + 1)) for t in range(6)], D.astype(np.float64), axes=1), e[:, None])
+    assert np.array_equal(rec, Y)
+    assert list(po.ozaki_exponents(np.array([[0.0, 0.0], [0.5, -1.0], [3.0, 1.0]]))) == [0, 1, 2]
+
+
+def test_ozaki_gemm_exact_on_small_integers():
+    """Integer matrices whose digits are exact and whose pairs all survive the
+    truncation: the scheme returns the exact product."""
+    rng = np.random.default_rng(901)
+    A = rng.integers(-100, 101, (13, 40)).astype(np.float64)
+    B = rng.integers(-100, 101, (40, 9)).astype(np.float64)
+    exact = np.array([[sum(int(A[i, k]) * int(B[k, j]) for k in range(40)) for j in range(9)]
+                      for i in range(13)], dtype=np.float64)
+    for S in (6, 7, 8):
+        assert np.array_equal(po.ozaki_gemm(A, B, S), exact)
+
+
+def test_ozaki_gemm_error_and_flush():
+    """Within the scheme's bound of float64 matmul on uniform data, and the
+    blocked integer sums (flush_k) change nothing when no block overflows."""
+    rng = np.random.default_rng(902)
+    A = rng.uniform(-1, 1, (30, 700))
+    B = rng.uniform(-1, 1, (700, 25))
+    ref = A @ B
+    err = np.linalg.norm(po.ozaki_gemm(A, B, 8) - ref) / np.linalg.norm(ref)
+    assert err < 1e-14
+    assert np.linalg.norm(po.ozaki_gemm(A, B, 6) - ref) / np.linalg.norm(ref) < 1e-11
+    a = po.ozaki_gemm(A, B, 7, flush_k=128)
+    b = po.ozaki_gemm(A, B, 7, flush_k=8192)
+    assert np.linalg.norm(a - b) <= 1e-15 * np.linalg.norm(ref)

@@ -500,7 +500,7 @@ def ozaki_digits(X, S):
     return D, e
 
 
-def ozaki_gemm(A, B, S=8, flush_k=8192):
+def ozaki_gemm(A, B, S=8, flush_k=8160):
     """C = A B (m x k times k x n) as the FP64-accurate mode forms it: digit
     pairs t + u <= S + 1 (1-based), exact integer sums per diagonal over
     blocks of flush_k, added to an FP64 sum in diagonal order per block,

@@ -13,8 +13,8 @@
 // applications", Numer. Algorithms 59 (2012); the int8 form follows Ootomo,
 // Ozaki and Yokota, "DGEMM on integer matrix multiplication unit", IJHPCA
 // 2024).  Pairs with the same t + u share one int32 accumulator:
-// v_mfma_i32_32x32x32_i8 sums them exactly (at most 8 pairs * 127^2 * 8192 k
-// < 2^31 between flushes), and every 8192 k the accumulators are added to
+// v_mfma_i32_32x32x32_i8 sums them exactly (at most 8 pairs * 127^2 * 8160 k
+// < 2^31 between flushes), and every 8160 k the accumulators are added to
 // an FP64 sum with their weights 2^(-7 (t + u)).
 //
 // Kernels, in launch order:
@@ -26,8 +26,9 @@
 //           x is the job's scale (complex signs, indexed entry factors)
 //           times the source value, in both passes;
 //   gemm    one 64 x 64 C tile per workgroup, 32 x 32 per wave, S int32
-//           accumulators (one per digit diagonal), FP64 epilogue with the
-//           row and column scales, alpha and beta.
+//           accumulators (one per digit diagonal), five k groups of digit
+//           planes in flight, FP64 epilogue with the row and column scales,
+//           alpha and beta.
 //
 // Panels (plan_types.hpp jobs and tasks): 32-row by 32-k fragment groups
 // of S planes of 1 KiB, group (rb, kg) plane t at
@@ -212,7 +213,11 @@ __global__ __launch_bounds__(256) void oz_gemm_kernel(const PanelTask* __restric
                                                       double* __restrict__ cws, TC* __restrict__ C,
                                                       double alpha, double beta, int use_k)
 {
-    constexpr int kFlush = 256;  // k groups (8192 k) between FP64 flushes
+    constexpr int PF = 5;
+    // k groups between FP64 flushes: a multiple of PF (a pass never straddles
+    // a flush) and at most 256 (8 pairs * 127^2 * 8192 k < 2^31): 255 groups,
+    // 8160 k
+    constexpr int kFlush = 256 / PF * PF;
     const PanelTask t = tasks[blockIdx.x];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -235,8 +240,9 @@ __global__ __launch_bounds__(256) void oz_gemm_kernel(const PanelTask* __restric
         const v4i* pa = (const v4i*)(ws + t.a) + wm * rs + lane;
         const v4i* pb = (const v4i*)(ws + t.b) + wn * rs + lane;
         // PF k groups in flight: one group's MFMA work (S (S + 1) / 2
-        // instructions) is shorter than a load's latency at one wave per SIMD
-        constexpr int PF = 4;
+        // instructions) is shorter than a load's latency at one wave per SIMD.
+        // Five fit the register file at S = 8 (492 of 512); measured on 96^4:
+        // one group 38 ms, four 22.1 ms, five 21.8 ms.
         v4i a[PF][S], b[PF][S];
 #pragma unroll
         for (int d = 0; d < PF; d++) oz_load<S>(a[d], b[d], pa, pb, d < nk ? d : nk - 1);

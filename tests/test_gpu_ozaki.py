@@ -84,7 +84,7 @@ def test_ozaki_fp32_inputs_fp64_accurate():
 
 
 def test_ozaki_long_k_flush():
-    """k = 9000 > 8192: the int32 digit accumulators are flushed mid-sum."""
+    """k = 9000 > 8160: the int32 digit accumulators are flushed mid-sum."""
     rng = np.random.default_rng(71400)
     m, n, k = 40, 72, 9000
     A = rng.uniform(-1, 1, (m, k))
@@ -180,11 +180,11 @@ def test_ozaki_digit_truncation_is_the_int8_path():
     assert C.item() == x
 
 
-@pytest.mark.parametrize("shape", [(37, 45, 1000), (64, 64, 64), (70, 9, 9000), (5, 130, 33)])
+@pytest.mark.parametrize("shape", [(37, 45, 1000), (64, 64, 64), (70, 9, 9000), (5, 130, 33), (16, 20, 20000)])
 def test_ozaki_bit_exact_vs_restatement(shape):
     """Bit for bit against the oracle's restatement of the scheme (same
     digits, exact integer diagonal sums, the same FP64 order; k = 9000 spans
-    two accumulator flushes), with rows and columns of very different
+    two accumulator flushes of 8160 k), with rows and columns of very different
     magnitude and an all-zero row."""
     import os
     S = int(os.environ.get("TBLIS_HIP_OZAKI_SLICES", "8"))

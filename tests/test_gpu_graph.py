@@ -1,5 +1,6 @@
 """hipGraph capture of library calls: a prepared dense, packed split-BF16,
-DPD, batched, complex and indexed contraction is captured once and replayed
+DPD, batched, complex, indexed and FP64-accurate (int8 digit) contraction is
+captured once and replayed
 several times,
 with the inputs changed by eager work between replays; every replay must
 match an eager call on the same inputs (FP64: 1e-12, split-BF16: 1e-6
@@ -92,6 +93,36 @@ def test_graph_packed_split_bf16():
     ext = dict(a=24, b=16, e=20, f=16, i=24, j=16)
     fn, ins, outs, eager = _dense(torch.float32, ext, "aebf", "fjei", "aibj", seed=1)
     _replay_check(_capture(fn), ins, outs, eager, 1e-6)
+
+
+def test_graph_ozaki():
+    """The FP64-accurate mode (row maxima cleared by a fill kernel, digit
+    planes, int8 GEMM): replays equal eager calls exactly (the scheme is
+    deterministic), dense and DPD (C panels and the unpack pass)."""
+    fn, ins, outs, eager = _dense(torch.float64, dict(a=24, b=20, e=16, f=18, i=12, j=10),
+                                  "aebf", "fjei", "aibj", config="ozaki", seed=4)
+    _replay_check(_capture(fn), ins, outs, eager, 0.0)
+    nirrep, ext = 4, [5, 3, 4, 2]
+    lens = [ext] * 4
+    n = T.dpd_size(0, nirrep, lens)
+    g = torch.Generator(device=DEV).manual_seed(5)
+    A = torch.rand(n, dtype=torch.float64, device=DEV, generator=g) * 2 - 1
+    B = torch.rand(n, dtype=torch.float64, device=DEV, generator=g) * 2 - 1
+    C = torch.rand(n, dtype=torch.float64, device=DEV, generator=g) * 2 - 1
+    C0 = C.clone()
+
+    def eager_dpd():
+        Ce = C0.clone()
+        T.dpd_mult(1.0, A, 0, "abef", B, 0, "efij", 0.5, Ce, 0, "abij", nirrep, lens, lens, lens,
+                   config="ozaki")
+        return [Ce]
+
+    def fn_dpd():
+        C.copy_(C0)
+        T.dpd_mult(1.0, A, 0, "abef", B, 0, "efij", 0.5, C, 0, "abij", nirrep, lens, lens, lens,
+                   config="ozaki")
+
+    _replay_check(_capture(fn_dpd), [A, B], [C], eager_dpd, 0.0)
 
 
 def test_graph_dpd():

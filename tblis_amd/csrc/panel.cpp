@@ -499,10 +499,8 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
     T.mfma_flops = 0;
     T.reads_ab = false;
 
-    int32_t cur_gemm = 0;  // index of the GEMM being laid out (GEMMs with tiles only)
     int64_t rows = 0;      // Ozaki family: panel rows (row-maximum array entries)
     const bool ozaki = family == FAMILY_OZAKI;
-    T.need.clear();
     auto add_job = [&](const PanelJob& j) {
         if (j.nr <= 0 || j.nk <= 0) return;
         const int32_t id = (int32_t)jobs.size();
@@ -518,9 +516,7 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                 c.nrb = (int32_t)std::min<int64_t>(4, rb1 - rb);
                 c.kg0 = (int32_t)kg;
                 c.nkg = (int32_t)std::min<int64_t>(8, kg1 - kg);
-                c.gemm = cur_gemm;
                 chunks.push_back(c);
-                T.need[cur_gemm]++;
             }
     };
 
@@ -533,8 +529,6 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
         for (size_t i = 0; i < NK; i++) k0[i + 1] = k0[i] + g.klen[i];
         const int64_t m = m0[NM], n = n0[NN], k = k0[NK];
         if (m == 0 || n == 0) continue;
-        cur_gemm = (int32_t)T.need.size();
-        T.need.push_back(0);
         const int64_t mp = ceil_div(m, BM) * BM, np = ceil_div(n, BN) * BN;
         const int64_t kp = ceil_div(k, G.kpad) * G.kpad;
         const int64_t nkg = kp / G.gk;
@@ -777,7 +771,6 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                 t.nkg = (int32_t)nkg;
                 t.mlen = (int32_t)std::min<int64_t>(BM, m - tm * BM);
                 t.nlen = (int32_t)std::min<int64_t>(BN, n - tn * BN);
-                t.gemm = cur_gemm;
                 if (direct)
                 {
                     t.c_src = g.c[0].base;
@@ -1145,7 +1138,6 @@ std::vector<PackChunk> pack_records(const PanelTables& T, int family)
         o.pnkg = j.pnkg;
         o.operand = j.operand;
         o.kfast = j.kfast;
-        o.gemm = c.gemm;
         o.rb0 = c.rb0;
         o.nrb = c.nrb;
         o.kg0 = c.kg0;

@@ -65,7 +65,6 @@ struct PanelTables
     std::vector<UnpackJob> ujobs;
     std::vector<UnpackChunk> uchunks;
     std::vector<int64_t> pool;
-    std::vector<int32_t> need;  // pack chunks of each GEMM (chunk and task .gemm index it)
     int tile = 0;
     int family = 0;
     bool swap = false;  // GEMMs were transposed: operand 0 is the B tensor

@@ -104,10 +104,10 @@ struct PanelJob
 };
 
 // Rectangle of fragment groups of one job: rows blocks [rb0, rb0 + nrb),
-// k groups [kg0, kg0 + nkg); gemm is the index of the GEMM the job feeds.
+// k groups [kg0, kg0 + nkg).
 struct PanelChunk
 {
-    int32_t job, rb0, nrb, kg0, nkg, gemm;
+    int32_t job, rb0, nrb, kg0, nkg, pad;
 };
 
 // A pack chunk as the pack kernels read it: its job's fields and the
@@ -121,7 +121,7 @@ struct PackChunk
     int64_t rowmap, kmap;  // pool indices of the per-element maps (scattered groups)
     double scale;          // applied to every value
     int32_t r0, nr, k0, nk;
-    int32_t pnkg, operand, kfast, gemm;
+    int32_t pnkg, operand, kfast, pad;
     int32_t rb0, nrb, kg0, nkg;
     int64_t rt[4][2];
     int64_t kt[8][2];
@@ -157,7 +157,7 @@ struct PanelTask
     int64_t a_kt, b_kt;    // pool index of k group 0's (base, stride)
     uint32_t a_bytes, b_bytes;  // extent of the block from x_src
     int32_t klen;          // valid k
-    int32_t gemm;          // index of the tile's GEMM (its pack chunks' gemm)
+    int32_t pad_;
 };
 
 // C panel -> C block: C[dst + rowmap[i] + colmap[j]] = alpha * P[i * ld + j] + beta * C[...]

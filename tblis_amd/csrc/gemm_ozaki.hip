@@ -80,9 +80,10 @@ __global__ __launch_bounds__(256) void oz_zero_kernel(unsigned long long* __rest
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = 0;
 }
 
-// One 32 x 32 group per wave and iteration: lane l reads row l % 32, k
-// 16 (l / 32) .. + 15; the two halves of a row meet through one shuffle and
-// lanes 0..31 publish the row maxima.
+// Rows-fast source: one 32 x 32 group per wave and iteration, lane l reads
+// row l % 32, k 16 (l / 32) .. + 15 (consecutive lanes, consecutive rows);
+// the two halves of a row meet through one shuffle and lanes 0..31 publish
+// the row maxima.  K-fast source: lanes along k instead.
 template <typename T>
 __global__ __launch_bounds__(256) void oz_rowmax_kernel(const PackChunk* __restrict__ chunks,
                                                         const int64_t* __restrict__ pool,
@@ -94,6 +95,28 @@ __global__ __launch_bounds__(256) void oz_rowmax_kernel(const PackChunk* __restr
     const T* __restrict__ X = j.operand ? B : A;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (j.kfast)
+    {
+        // k-fast source: one row per wave and iteration, lanes along k
+        // (whole cache lines), the row maximum by a shuffle reduction
+        const int nrows = j.nrb * 32, kspan = j.nkg * 32;
+        for (int r = wave; r < nrows; r += 4)
+        {
+            const int ri = r >> 5, lr = r & 31;
+            double m = 0.0;
+            for (int kk = lane; kk < kspan; kk += 64)
+            {
+                const int64_t off = oz_offset(j, pool, ri, kk >> 5, lr, kk & 31);
+                if (off >= 0) m = fmax(m, oz_abs(j.scale * (double)X[off]));
+            }
+#pragma unroll
+            for (int w = 32; w >= 1; w >>= 1) m = fmax(m, __shfl_xor(m, w));
+            const int prow = j.rb0 * 32 + r;  // panel row
+            if (lane == 0 && prow >= j.r0 && prow < j.r0 + j.nr && m > 0.0)
+                atomicMax(rmax + j.eoff + prow, (unsigned long long)__double_as_longlong(m));
+        }
+        return;
+    }
     const int lr = lane & 31, lh = lane >> 5;
     const int ng = j.nrb * j.nkg;
     for (int g = wave; g < ng; g += 4)

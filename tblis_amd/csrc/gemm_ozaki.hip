@@ -206,8 +206,9 @@ __global__ __launch_bounds__(256) void oz_pack_kernel(const PackChunk* __restric
     }
 }
 
-// Digit pairs (t, u), t + u <= S + 1 (1-based), ordered so that
-// consecutive MFMAs update different diagonals.
+// Digit pairs (t, u), t + u <= S + 1 (1-based), row by row.  Consecutive
+// MFMAs into one diagonal's accumulator do not stall (an order that spaced
+// them at least two apart measured the same).
 template <int S>
 __device__ __forceinline__ void oz_mma(v16i (&acc)[S], const v4i (&a)[S], const v4i (&b)[S])
 {

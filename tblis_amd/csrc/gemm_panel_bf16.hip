@@ -39,6 +39,10 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
 // ------------------------------------------------------------------ pack
 
 // As the FP64 pack: every address of a chunk follows from its PackChunk record.
+// Loads per batch: one pair of k groups (k-fast sources) or two groups
+// (others) -- 66 registers, 7 waves per SIMD; twice that batch took 96
+// registers (5 waves per SIMD) and was 22% slower on the 4096-instance batch
+// and on 96^4.
 __global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PackChunk* __restrict__ chunks,
                                                               const int64_t* __restrict__ pool,
                                                               const float* __restrict__ A,
@@ -59,7 +63,7 @@ __global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PackChunk* _
         // consecutive k (whole 128-byte lines), transposed through a per-wave
         // LDS tile into the fragment order.  The k tables are unit-stride.
         // The loads of kBatch pairs are issued before the first transpose.
-        constexpr int kBatch = 2;
+        constexpr int kBatch = 1;
         __shared__ float tile[4][32][33];
         const int npair = (j.nkg + 1) / 2;
         const int niter = j.nrb * npair;
@@ -153,7 +157,7 @@ __global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PackChunk* _
     }
     // general source: one fragment group per iteration, the loads of
     // kBatch groups issued before their stores
-    constexpr int kBatch = 4;
+    constexpr int kBatch = 2;
     for (int g0 = wave; g0 < ng; g0 += 4 * kBatch)
     {
         float v[kBatch][8];

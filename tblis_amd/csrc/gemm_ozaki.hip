@@ -157,7 +157,11 @@ __global__ __launch_bounds__(256) void oz_pack_kernel(const PackChunk* __restric
         const int prow = rb * 32 + lr;
         const bool rin = prow >= j.r0 && prow < j.r0 + j.nr;
         const int e = rin && !zero ? oz_exponent(rmax[j.eoff + prow]) : 0;
-        int8_t dig[S][16];
+        uint32_t w[S][4];  // digit planes, four digits per word (byte q % 4 of word q / 4)
+#pragma unroll
+        for (int t = 0; t < S; t++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) w[t][c] = 0;
         unsigned inmask = 0;
 #pragma unroll
         for (int q = 0; q < 16; q++)
@@ -177,7 +181,7 @@ __global__ __launch_bounds__(256) void oz_pack_kernel(const PackChunk* __restric
                 v *= 128.0;
                 const double d = __builtin_trunc(v);
                 v -= d;
-                dig[t][q] = (int8_t)(int)d;
+                w[t][q >> 2] |= (uint32_t)(uint8_t)(int8_t)(int)d << (8 * (q & 3));
             }
         }
         int8_t* out = ws + j.out + ((int64_t)rb * j.pnkg + kg) * S * 1024 + lane * 16;
@@ -186,13 +190,10 @@ __global__ __launch_bounds__(256) void oz_pack_kernel(const PackChunk* __restric
 #pragma unroll
             for (int t = 0; t < S; t++)
             {
-                v4i w;
+                v4i x;
 #pragma unroll
-                for (int c = 0; c < 4; c++)
-                    w[c] = (int)((uint32_t)(uint8_t)dig[t][4 * c] | (uint32_t)(uint8_t)dig[t][4 * c + 1] << 8 |
-                                 (uint32_t)(uint8_t)dig[t][4 * c + 2] << 16 |
-                                 (uint32_t)(uint8_t)dig[t][4 * c + 3] << 24);
-                *(v4i*)(out + t * 1024) = w;
+                for (int c = 0; c < 4; c++) x[c] = (int)w[t][c];
+                *(v4i*)(out + t * 1024) = x;
             }
         }
         else if (inmask)
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(256) void oz_pack_kernel(const PackChunk* __restric
             for (int t = 0; t < S; t++)
 #pragma unroll
                 for (int q = 0; q < 16; q++)
-                    if (inmask >> q & 1u) out[t * 1024 + q] = dig[t][q];
+                    if (inmask >> q & 1u) out[t * 1024 + q] = (int8_t)(w[t][q >> 2] >> (8 * (q & 3)));
         }
     }
 }

@@ -324,11 +324,12 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_panel_kernel(
 #pragma unroll
                 for (int j = 0; j < NB; j++) b[j] = pb[j * rs + kg * 64];
         };
-        // k groups in flight: four for waves of at most 32x32 (their MFMA
-        // work per group is short), three for 64x64 waves (as many as fit
-        // in the registers left next to the accumulators at two waves per
-        // SIMD); measured 1-18% faster than two
-        constexpr int PF = MB * NB <= 4 ? 4 : 3;
+        // k groups in flight: five for waves of at most 32x32 (their MFMA
+        // work per group is short; 126 registers, still four waves per SIMD;
+        // 1-2% faster than four on the permuted workload), three for 64x64
+        // waves (as many as fit in the registers left next to the
+        // accumulators at two waves per SIMD); 1-18% faster than two
+        constexpr int PF = MB * NB <= 4 ? 5 : 3;
         double2 a[PF][MB], b[PF][NB];
 #pragma unroll
         for (int d = 0; d < PF; d++) load(a[d], b[d], d < nk ? d : nk - 1);

@@ -27,7 +27,17 @@ namespace
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-constexpr int64_t kDirectRowFast = 24;  // re-read limit (tiles) of rows-fast direct operands
+// Re-read limit (tiles) of rows-fast direct operands (TBLIS_HIP_DIRECT_ROWS,
+// default 24).
+int64_t direct_rows_rereads()
+{
+    static const int64_t v = [] {
+        const char* e = std::getenv("TBLIS_HIP_DIRECT_ROWS");
+        const long x = e ? std::atol(e) : 24;
+        return (int64_t)(x >= 0 ? x : 24);
+    }();
+    return v;
+}
 constexpr int kXcds = 8;                 // MI355X: 8 XCDs of 32 CUs, one L2 each
 
 void hip_check(hipError_t e, const char* what)
@@ -553,7 +563,7 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
             if (!allow_direct[o] || nrp[o] != 1 || NK != 1) continue;
             if (o == 0 && !g.kscale.empty() && g.kscale[0] != 1.0) continue;  // scaled while packed
             const int64_t rereads = o == 0 ? ceil_div(n, BN) : ceil_div(m, BM);
-            if (rereads > std::max<int64_t>(kDirectRowFast, direct_kfast_rereads())) continue;
+            if (rereads > std::max<int64_t>(direct_rows_rereads(), direct_kfast_rereads())) continue;
             const PBlock& blk = (*blocks[o])[0];
             if (blk.base == PANEL_MISSING) continue;
             const int64_t rm = pool.offsets(blk.rows), km = pool.offsets(blk.k);
@@ -566,7 +576,7 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
             const int64_t rt = pool.group_table(rm, 0, rtot[o], G.gr);
             const int64_t kt = pool.group_table(km, 0, k, G.gk);
             // rows-fast (unit-stride row groups, any affine k groups; re-read
-            // by <= kDirectRowFast tiles) or k-fast (affine row groups,
+            // by <= direct_rows_rereads() tiles) or k-fast (affine row groups,
             // unit-stride k groups; re-read by <= the k-fast limit, <= 4 MiB)
             bool rows_unit = true, rows_affine = true, k_unit = true, k_affine = true;
             pd = pool.data().data();
@@ -587,7 +597,7 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
             // k-fast reads use 32-byte pieces of each row: only worth it
             // while the re-reads hit in L2 (4 MiB per XCD)
             const int64_t span = (hi + khi + 1) * G.elem_bytes;
-            const bool ok = (rows_unit && k_affine && rereads <= kDirectRowFast) ||
+            const bool ok = (rows_unit && k_affine && rereads <= direct_rows_rereads()) ||
                             (rows_affine && k_unit && rereads <= direct_kfast_rereads() &&
                              span <= ((int64_t)4 << 20));
             if (!ok) continue;

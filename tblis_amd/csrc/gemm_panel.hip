@@ -282,8 +282,12 @@ __device__ __forceinline__ void direct_groups(double2 (&x)[NBLK], const DirectSi
     }
 }
 
-template <int BM, int BN, int WGM, int WGN, typename TC, bool DA, bool DB>
-__global__ __launch_bounds__(64 * WGM * WGN) void gemm_panel_kernel(
+// KS > 1 splits k inside the workgroup: KS sets of WGM x WGN waves take
+// interleaved k groups (group kg goes to set kg % KS) and the partial sums
+// meet in LDS before set 0 runs the epilogue.  For short k (DPD irrep
+// GEMMs, small batch instances) this doubles the waves resident per SIMD.
+template <int BM, int BN, int WGM, int WGN, typename TC, bool DA, bool DB, int KS = 1>
+__global__ __launch_bounds__(64 * WGM * WGN * KS) void gemm_panel_kernel(
     const PanelTask* __restrict__ tasks, const int64_t* __restrict__ pool,
     const double* __restrict__ ws, double* __restrict__ cws, const void* A, const void* B,
     TC* __restrict__ C, double alpha, double beta, int use_k)
@@ -291,9 +295,11 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_panel_kernel(
     constexpr int WM = BM / WGM, WN = BN / WGN, MB = WM / 16, NB = WN / 16;
     const PanelTask t = tasks[blockIdx.x];
     const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wave_all = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int kw = wave_all / (WGM * WGN), wave = wave_all % (WGM * WGN);
     const int wm = wave % WGM, wn = wave / WGM;
-    if (wm * WM >= t.mlen || wn * WN >= t.nlen) return;  // no barriers below
+    const bool active = !(wm * WM >= t.mlen || wn * WN >= t.nlen);
+    if (KS == 1 && !active) return;  // no barriers below when KS == 1
 
     v4d acc[MB][NB];
 #pragma unroll
@@ -301,7 +307,8 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_panel_kernel(
 #pragma unroll
         for (int j = 0; j < NB; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-    const int nk = use_k ? t.nkg : 0;  // even
+    // this wave's k groups: kw, kw + KS, ... (nk is even)
+    const int nk = use_k && active ? (t.nkg - kw + KS - 1) / KS : 0;
     if (nk > 0)
     {
         const int lr = lane & 15, lk = lane >> 4;
@@ -312,7 +319,8 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_panel_kernel(
         DirectSide<TC, NB> db;
         if constexpr (DA) direct_init(da, A, t.a_src, t.a_bytes, pool, t.a_rt, t.a_kt, wm * MB, wm * WM, t.mlen, lr);
         if constexpr (DB) direct_init(db, B, t.b_src, t.b_bytes, pool, t.b_rt, t.b_kt, wn * NB, wn * WN, t.nlen, lr);
-        auto load = [&](double2 (&a)[MB], double2 (&b)[NB], int kg) {
+        auto load = [&](double2 (&a)[MB], double2 (&b)[NB], int li) {
+            const int kg = kw + li * KS;
             if constexpr (DA)
                 direct_groups<TC, MB>(a, da, kg, lk, t.klen);
             else
@@ -345,6 +353,24 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_panel_kernel(
                 load(a[d], b[d], nx);
             }
         }
+    }
+    if constexpr (KS > 1)
+    {
+        // partial sums of the other k sets through LDS, then set 0 finishes
+        __shared__ v4d red[KS - 1][WGM * WGN][MB * NB][64];
+        if (kw > 0 && active)
+#pragma unroll
+            for (int i = 0; i < MB; i++)
+#pragma unroll
+                for (int j = 0; j < NB; j++) red[kw - 1][wave][i * NB + j][lane] = acc[i][j];
+        __syncthreads();
+        if (kw > 0 || !active) return;
+#pragma unroll
+        for (int q = 0; q < KS - 1; q++)
+#pragma unroll
+            for (int i = 0; i < MB; i++)
+#pragma unroll
+                for (int j = 0; j < NB; j++) acc[i][j] += red[q][wave][i * NB + j][lane];
     }
 
     // D register r of lane l holds D[l / 16 + 4 r][l % 16] of each block
@@ -480,19 +506,22 @@ __global__ __launch_bounds__(256) void fill_random_kernel(void* p, int64_t n, in
 
 struct PanelTile
 {
-    int bm, bn, wgm, wgn;
+    int bm, bn, wgm, wgn, ks;
 };
 
-#define TBLIS_PANEL_TILES(X) \
-    X(0, 128, 128, 2, 2)     \
-    X(1, 64, 64, 2, 2)       \
-    X(2, 64, 64, 1, 1)       \
-    X(3, 128, 64, 2, 1)      \
-    X(4, 64, 128, 1, 2)      \
-    X(5, 32, 32, 1, 1)
+// id, BM, BN, waves (WGM x WGN), k sets inside the workgroup
+#define TBLIS_PANEL_TILES(X)  \
+    X(0, 128, 128, 2, 2, 1)   \
+    X(1, 64, 64, 2, 2, 1)     \
+    X(2, 64, 64, 1, 1, 1)     \
+    X(3, 128, 64, 2, 1, 1)    \
+    X(4, 64, 128, 1, 2, 1)    \
+    X(5, 32, 32, 1, 1, 1)     \
+    X(6, 64, 64, 2, 2, 2)     \
+    X(7, 32, 32, 1, 1, 2)
 
 constexpr PanelTile kTiles[] = {
-#define X(id, bm, bn, wgm, wgn) {bm, bn, wgm, wgn},
+#define X(id, bm, bn, wgm, wgn, ks) {bm, bn, wgm, wgn, ks},
     TBLIS_PANEL_TILES(X)
 #undef X
 };
@@ -501,12 +530,12 @@ constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 template <typename TC, bool DA, bool DB>
 hipError_t launch_gemm_t(const PanelLaunch& L)
 {
-    const dim3 grid(L.ntasks), block(64 * kTiles[L.tile].wgm * kTiles[L.tile].wgn);
+    const dim3 grid(L.ntasks), block(64 * kTiles[L.tile].wgm * kTiles[L.tile].wgn * kTiles[L.tile].ks);
     switch (L.tile)
     {
-#define X(id, bm, bn, wgm, wgn)                                                                       \
+#define X(id, bm, bn, wgm, wgn, ks)                                                                   \
     case id:                                                                                          \
-        hipLaunchKernelGGL((gemm_panel_kernel<bm, bn, wgm, wgn, TC, DA, DB>), grid, block, 0,         \
+        hipLaunchKernelGGL((gemm_panel_kernel<bm, bn, wgm, wgn, TC, DA, DB, ks>), grid, block, 0,     \
                            L.stream, L.tasks, L.pool, (const double*)L.ws, L.cws, L.A, L.B, (TC*)L.C, \
                            L.alpha, L.beta, L.use_k);                                                 \
         break;
@@ -548,7 +577,7 @@ int panel_tile_n(int family, int t)
 int panel_tile_waves(int family, int t)
 {
     if (family == FAMILY_OZAKI) return 4;
-    return family == FAMILY_F64 ? kTiles[t].wgm * kTiles[t].wgn : panel_bf16_tile_waves(t);
+    return family == FAMILY_F64 ? kTiles[t].wgm * kTiles[t].wgn * kTiles[t].ks : panel_bf16_tile_waves(t);
 }
 
 hipError_t launch_panel_pack(const PanelLaunch& L)

@@ -237,7 +237,7 @@ struct TileModel
 // Per-wave efficiency by wave tile.  FP64: 64x64 waves are 4x4 MFMA
 // blocks, 32x32 waves 2x2.  Split-BF16: per tile of the BF16 table.
 const TileModel kModelsF64[] = {{0, 0.85, 2}, {2, 0.80, 2}, {3, 0.82, 2}, {4, 0.82, 2},
-                                {1, 0.50, 6}, {5, 0.45, 6}};
+                                {1, 0.50, 6}, {5, 0.45, 6}, {6, 0.50, 4}, {7, 0.45, 4}};
 const TileModel kModelsBf16[] = {{0, 0.55, 1}, {1, 0.55, 1}, {2, 0.45, 2}};
 const TileModel kModelsOzaki[] = {{0, 0.30, 1}};
 
@@ -431,7 +431,7 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family,
     int tile = 0;
     double best = -1;
     const TileModel* models = family == FAMILY_F64 ? kModelsF64 : family == FAMILY_OZAKI ? kModelsOzaki : kModelsBf16;
-    const int nmodels = family == FAMILY_F64 ? 6 : family == FAMILY_OZAKI ? 1 : 3;
+    const int nmodels = family == FAMILY_F64 ? 8 : family == FAMILY_OZAKI ? 1 : 3;
     std::vector<int> cands;
     for (int mi = 0; mi < nmodels; mi++)
     {

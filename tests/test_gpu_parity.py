@@ -311,9 +311,10 @@ def test_every_tile_variant(tile, monkeypatch):
 
 
 @pytest.mark.parametrize("direct", ["1", "0"])
-@pytest.mark.parametrize("tile", range(6))
+@pytest.mark.parametrize("tile", range(8))
 def test_every_panel_tile(tile, direct, monkeypatch):
-    """Each tile of the FP64 panel GEMM (forced through TBLIS_HIP_PANEL_TILE),
+    """Each tile of the FP64 panel GEMM (forced through TBLIS_HIP_PANEL_TILE;
+    tiles 6 and 7 split k between two sets of waves inside the workgroup),
     with operands read straight from the tensors where their layout allows
     (direct) and with every operand packed: dense contractions with ragged
     edges and scattered k, FP32 data through FP64 MFMA, the 8-irrep DPD

@@ -123,7 +123,7 @@ struct PackChunk
     int32_t r0, nr, k0, nk;
     int32_t pnkg, operand, kfast, pad;
     int32_t rb0, nrb, kg0, nkg;
-    int64_t rt[4][2];
+    int64_t rt[8][2];
     int64_t kt[8][2];
     int64_t eoff;  // Ozaki family: row-maximum array index of the panel's row 0
 };

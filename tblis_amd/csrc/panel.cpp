@@ -511,22 +511,27 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
 
     int64_t rows = 0;      // Ozaki family: panel rows (row-maximum array entries)
     const bool ozaki = family == FAMILY_OZAKI;
-    auto add_job = [&](const PanelJob& j) {
-        if (j.nr <= 0 || j.nk <= 0) return;
-        const int32_t id = (int32_t)jobs.size();
-        jobs.push_back(j);
+    // Pack chunks of one job.  Narrow: 4 row groups x at most 8 k groups.
+    // Wide: the job's k groups split evenly into chunks of at most 8, and as
+    // many row groups per chunk (4 to 8) as keep it within 32 groups and 16
+    // pairs of k groups (one batch of loads per FP64 pack wave), so that
+    // jobs with short k (DPD blocks) fill their chunks.  Wide chunks are used
+    // only when the narrow ones outnumber the four-wave workgroups resident
+    // at once: there they save a round (8-irrep DPD workload: 2 154 chunks
+    // instead of 2 288 for 2 048 slots, pack 22.9 us instead of 25.4); below
+    // it they only lengthen each wave's work (indexed 48 workload, split-BF16:
+    // 81 TFLOP/s against 95 with narrow chunks).  A 64-group budget (two
+    // batches per wave, 1 792 DPD chunks) took 30.7 us.
+    auto chunk_job = [&](int32_t id, bool wide) {
+        const PanelJob& j = jobs[id];
         const int64_t rb0 = j.r0 / G.gr, rb1 = ceil_div(j.r0 + j.nr, G.gr);
         const int64_t kg0 = j.k0 / G.gk, kg1 = ceil_div(j.k0 + j.nk, G.gk);
-        // The job's k groups split evenly into chunks of at most 8, and as
-        // many row groups per chunk (4 to 8) as keep it within 32 groups and
-        // 16 pairs of k groups (one batch of loads per pack wave).  Jobs with
-        // short k (DPD blocks) then fill their chunks instead of leaving
-        // rows of them empty: 2 154 chunks instead of 2 288 for the 8-irrep
-        // DPD workload, pack 22.9 us instead of 25.4.  A 64-group budget
-        // (two batches per wave, 1 792 chunks) took 30.7 us.
-        const int64_t nkc = ceil_div(kg1 - kg0, 8);
-        const int64_t per = ceil_div(kg1 - kg0, nkc);
-        const int64_t rstep = std::min<int64_t>({8, 32 / per, 16 / ((per + 1) / 2)});
+        int64_t per = 8, rstep = 4;
+        if (wide)
+        {
+            per = ceil_div(kg1 - kg0, ceil_div(kg1 - kg0, 8));
+            rstep = std::min<int64_t>({8, 32 / per, 16 / ((per + 1) / 2)});
+        }
         for (int64_t rb = rb0; rb < rb1; rb += rstep)
             for (int64_t kg = kg0; kg < kg1; kg += per)
             {
@@ -538,6 +543,12 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                 c.nkg = (int32_t)std::min<int64_t>(per, kg1 - kg);
                 chunks.push_back(c);
             }
+    };
+    auto add_job = [&](const PanelJob& j) {
+        if (j.nr <= 0 || j.nk <= 0) return;
+        const int32_t id = (int32_t)jobs.size();
+        jobs.push_back(j);
+        chunk_job(id, false);
     };
 
     for (const PanelGemm& g : gemms)
@@ -817,6 +828,11 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
         nregions += RM * RN;
     }
     if (tasks.empty()) return false;
+    if (chunks.size() > (size_t)8 * device_cu_count())
+    {
+        chunks.clear();
+        for (size_t id = 0; id < jobs.size(); id++) chunk_job((int32_t)id, true);
+    }
     if (tasks.size() > (size_t)INT32_MAX || chunks.size() > (size_t)INT32_MAX ||
         uchunks.size() > (size_t)INT32_MAX)
         fail("contraction too large: panel task list exceeds 2^31 entries");

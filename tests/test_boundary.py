@@ -332,7 +332,7 @@ def test_dpd_patch_scatter_bit_exact(seed):
     layout = seed % 6
     cases = [("abef", "efij", "abij"), ("aebf", "fjei", "aibj"), ("abe", "eij", "abij")]
     iA, iB, iC = cases[seed % 3]
-    ext = {c: [int(rng.integers(0, 4)) for _ in range(nirrep)] for c in set(iA + iB + iC)}
+    ext = {c: [int(rng.integers(0, 4)) for _ in range(nirrep)] for c in sorted(set(iA + iB + iC))}
     irA, irB = int(rng.integers(0, nirrep)), int(rng.integers(0, nirrep))
     irC = irA ^ irB
     lens = lambda idx: [ext[c] for c in idx]

@@ -73,7 +73,7 @@ def _run(case, dtype, seed, alpha, beta, ext_range=(2, 9), fill=0.8, separate=Fa
     entry triple contributed (so that a case cannot pass vacuously)."""
     iA, nA, iB, nB, iC, nC = case
     rng = np.random.default_rng(seed)
-    ext = {c: int(rng.integers(*ext_range)) for c in set(iA + iB + iC)}
+    ext = {c: int(rng.integers(*ext_range)) for c in sorted(set(iA + iB + iC))}
     # indexed dimensions short, so that every case has a few entries
     for idx, nd in ((iA, nA), (iB, nB), (iC, nC)):
         for c in idx[nd:]:

@@ -130,7 +130,7 @@ def test_ozaki_dpd(seed):
     rng = np.random.default_rng(71600 + seed)
     nirrep = [1, 2, 4, 8][seed % 4]
     iA, iB, iC = [("abef", "efij", "abij"), ("aebf", "fjei", "aibj"), ("abe", "eij", "abij")][seed % 3]
-    ext = {c: [int(rng.integers(0, 7)) for _ in range(nirrep)] for c in set(iA + iB + iC)}
+    ext = {c: [int(rng.integers(0, 7)) for _ in range(nirrep)] for c in sorted(set(iA + iB + iC))}
     irA, irB = int(rng.integers(0, nirrep)), int(rng.integers(0, nirrep))
     _dpd_case(rng, nirrep, iA, iB, iC, ext, irA, irB, seed % 6, alpha=0.5, beta=[0.0, 1.0, -2.0][seed % 3],
               config="ozaki")

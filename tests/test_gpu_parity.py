@@ -223,7 +223,7 @@ def test_dpd_random(seed):
     cases = [("abef", "efij", "abij"), ("aebf", "fjei", "aibj"), ("abe", "eij", "abij"),
              ("aeb", "ebcd", "acbd")]
     iA, iB, iC = cases[seed % 4]
-    ext = {c: [int(rng.integers(0, 7)) for _ in range(nirrep)] for c in set(iA + iB + iC)}
+    ext = {c: [int(rng.integers(0, 7)) for _ in range(nirrep)] for c in sorted(set(iA + iB + iC))}
     irA, irB = int(rng.integers(0, nirrep)), int(rng.integers(0, nirrep))
     _dpd_case(rng, nirrep, iA, iB, iC, ext, irA, irB, seed % 6, alpha=0.5,
               beta=[0.0, 1.0, -2.0][seed % 3])

@@ -452,7 +452,7 @@ def test_indexed_oracle_against_brute_force(case):
     iA, nA, iB, nB, iC, nC = INDEXED_CASES[case]
     rng = np.random.default_rng(62000 + case)
     for trial in range(3):
-        ext = {c: int(rng.integers(1, 4)) for c in set(iA + iB + iC)}
+        ext = {c: int(rng.integers(1, 4)) for c in sorted(set(iA + iB + iC))}
         A = random_indexed(rng, iA, ext, nA)
         B = random_indexed(rng, iB, ext, nB)
         C0 = random_indexed(rng, iC, ext, nC)

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
@@ -350,6 +351,18 @@ bool direct_enabled()
 {
     const char* e = std::getenv("TBLIS_HIP_DIRECT");
     return !(e && e[0] == '0');
+}
+
+// TBLIS_HIP_PACK_CHUNKS=wide or =narrow forces one pack-chunk shape (read
+// at every plan build, so that tests can switch it); unset, the planner
+// decides by the chunk count.
+int pack_chunks_forced()
+{
+    const char* e = std::getenv("TBLIS_HIP_PACK_CHUNKS");
+    if (!e) return 0;
+    if (std::strcmp(e, "wide") == 0) return 1;
+    if (std::strcmp(e, "narrow") == 0) return -1;
+    return 0;
 }
 
 // Rows of tiles per group in the task order (TBLIS_HIP_GROUP_M, default 8;
@@ -829,7 +842,8 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
         nregions += RM * RN;
     }
     if (tasks.empty()) return false;
-    if (chunks.size() > (size_t)8 * device_cu_count())
+    const int forced = pack_chunks_forced();
+    if (forced > 0 || (forced == 0 && chunks.size() > (size_t)8 * device_cu_count()))
     {
         chunks.clear();
         for (size_t id = 0; id < jobs.size(); id++) chunk_job((int32_t)id, true);

@@ -1134,16 +1134,17 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bo
         std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return t[a] < t[b]; });
         // The finalists (within 8% of the best single call, at most four)
         // run again as runs of back-to-back calls, at least ~3 ms of work
-        // per sample, samples interleaved between finalists: single calls
-        // after an idle GPU can rank tiles differently from the steady state
-        // of a caller's loop (permuted 64/32 workload: tiles 1, 2, 5, 6
-        // within 3% as single calls, tile 6 3-4% ahead back to back).
+        // per sample (one call from 3 ms up), samples interleaved between
+        // finalists: single calls after an idle GPU can rank tiles
+        // differently from the steady state of a caller's loop (permuted
+        // 64/32 workload: tiles 1, 2, 5, 6 within 3% as single calls, tile 6
+        // 3-4% ahead back to back).
         for (size_t r = 0; r < nc && fin.size() < 4; r++)
             if (t[order[r]] <= 1.08f * t[order[0]]) fin.push_back(order[r]);
         size_t bi = order[0];
         if (fin.size() > 1)
         {
-            const int n = (int)std::min(10.0f, std::max(2.0f, std::ceil(3.0f / std::max(t[order[0]], 1e-3f))));
+            const int n = (int)std::min(10.0f, std::max(1.0f, std::ceil(3.0f / std::max(t[order[0]], 1e-3f))));
             steady.assign(fin.size(), -1.0f);
             for (int rep = 0; ok && rep < 2; rep++)
                 for (size_t f = 0; ok && f < fin.size(); f++)

@@ -115,6 +115,12 @@ tblis_tensor make_tensor(T scalar, const varray_view<T>& v)
 }
 
 template <typename T>
+tblis_tensor make_tensor(T scalar, const varray_view<const T>& v)
+{
+    return make_tensor(scalar, varray_view<T>{const_cast<T*>(v.data), v.len, v.stride});
+}
+
+template <typename T>
 tblis_dpd_tensor make_dpd(T scalar, const dpd_varray_view<T>& v, std::vector<len_type>& flat)
 {
     flat.clear();
@@ -179,6 +185,19 @@ tblis_indexed_dpd_tensor make_indexed_dpd(T scalar, const indexed_dpd_varray_vie
 // C <- alpha * A.B + beta * C (src/iface/3t/mult.h:28-37).
 template <typename T>
 void mult(T alpha, const varray_view<T>& A, const label_type* idx_A, const varray_view<T>& B,
+          const label_type* idx_B, T beta, const varray_view<T>& C, const label_type* idx_C,
+          const tblis_config* cfg = nullptr)
+{
+    tblis_tensor a = detail::make_tensor(alpha, A);
+    tblis_tensor b = detail::make_tensor(T(1), B);
+    tblis_tensor c = detail::make_tensor(beta, C);
+    tblis_tensor_mult(nullptr, cfg, &a, idx_A, &b, idx_B, &c, idx_C);
+}
+
+// The same with read-only A and B views, the reference's signature
+// (varray_view<const T>, src/iface/3t/mult.h:28-37).
+template <typename T>
+void mult(T alpha, const varray_view<const T>& A, const label_type* idx_A, const varray_view<const T>& B,
           const label_type* idx_B, T beta, const varray_view<T>& C, const label_type* idx_C,
           const tblis_config* cfg = nullptr)
 {

@@ -342,12 +342,17 @@ def complex_tensor_mult(alpha, A, lenA, strA, idxA, B, lenB, strB, idxB, beta, C
     # the real contractions write C's layout at even offsets (stride 2)
     P = (rr - sa * sb * ii)[0::2] + 1j * (sb * ri + sa * ir)[0::2]
     c = np.array(C_, np.complex128)
-    # elements the view of C addresses (the rest of the storage is untouched)
-    off = np.zeros(1, np.int64)
-    for n, st in zip(lenC, strC):
-        off = (off[:, None] + np.arange(n, dtype=np.int64)[None, :] * st).ravel()
+    # elements C's labels address: a repeated label is one index over the
+    # sum of its strides (the diagonal, src/iface/3t/mult.cxx:44-48); the
+    # rest of the storage is untouched
     if any(n == 0 for n in lenC):
         return c
+    diag = {}
+    for n, st, lab in zip(lenC, strC, idxC):
+        diag[lab] = (n, diag.get(lab, (n, 0))[1] + st)
+    off = np.zeros(1, np.int64)
+    for n, st in diag.values():
+        off = (off[:, None] + np.arange(n, dtype=np.int64)[None, :] * st).ravel()
     hit = np.unique(off)
     c[hit] = complex(alpha) * P[hit] + (complex(beta) * c[hit] if beta != 0 else 0)
     return c

@@ -64,6 +64,7 @@ Cplx product(const tblis_scalar& a, const tblis_scalar& b, type_t t)
 Side twice(Side s)
 {
     for (auto& x : s.stride) x *= 2;
+    for (auto& x : s.offs) x *= 2;
     return s;
 }
 
@@ -119,27 +120,30 @@ void split_gemm(const PanelGemm& g, const std::vector<Term> (&terms)[2], std::ve
     }
 }
 
-// Complex scale of C over its dimensions (smallest stride first, contiguous
-// dimensions folded), as a level-1 descriptor in complex elements.
-Level1Desc c_desc(const tblis_tensor& C)
+// Complex scale of the elements of C the contraction writes, as a level-1
+// descriptor in complex elements: C's diagonal (a repeated label addresses
+// one element per value, src/iface/3t/mult.cxx:44-48), length-1 dimensions
+// dropped, smallest stride first, contiguous dimensions folded.
+Level1Desc c_desc(const tblis_tensor& C, const label_type* idx_C)
 {
-    std::vector<int> order(C.ndim);
-    for (unsigned i = 0; i < C.ndim; i++) order[i] = (int)i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return std::llabs(C.stride[a]) < std::llabs(C.stride[b]); });
+    Group g;
+    std::string why;
+    if (!canonicalize_one(C, idx_C, g, why)) fail("complex: invalid C: " + why);
+    std::stable_sort(g.begin(), g.end(),
+                     [](const Dim& a, const Dim& b) { return std::llabs(a.s[1]) < std::llabs(b.s[1]); });
     Level1Desc d;
-    for (int i : order)
+    for (const Dim& x : g)
     {
-        d.nb *= C.len[i];
-        if (C.len[i] == 1) continue;
-        if (d.ndb > 0 && C.stride[i] == d.sb[d.ndb - 1] * d.len_b[d.ndb - 1])
+        d.nb *= x.len;
+        if (x.len == 1) continue;
+        if (d.ndb > 0 && x.s[1] == d.sb[d.ndb - 1] * d.len_b[d.ndb - 1])
         {
-            d.len_b[d.ndb - 1] *= C.len[i];
+            d.len_b[d.ndb - 1] *= x.len;
             continue;
         }
         if (d.ndb >= LEVEL1_MAXDIM) fail("complex: more than 16 dimensions of C after folding");
-        d.len_b[d.ndb] = C.len[i];
-        d.sb[d.ndb] = C.stride[i];
+        d.len_b[d.ndb] = x.len;
+        d.sb[d.ndb] = x.s[1];
         d.sa_b[d.ndb] = 0;
         d.ndb++;
     }
@@ -226,7 +230,7 @@ void complex_tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const l
             plan->mark_verified(A->data, B->data, C->data, k);
         }
         ThreadState& ts = thread_state();
-        const Level1Desc cd = c_desc(*C);
+        const Level1Desc cd = c_desc(*C, idx_C);
         const int dt = rtype == TYPE_DOUBLE ? 0 : 1;
         if (pass)
         {

@@ -333,6 +333,9 @@ void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tenso
     if (count < 0) fail("batch: negative count");
     if (count == 0) return;
     if (!A || !B || !C || !idx_A || !idx_B || !idx_C) fail("batch: null array argument");
+    check_tensor(A[0], idx_A[0], "A");
+    check_tensor(B[0], idx_B[0], "B");
+    check_tensor(C[0], idx_C[0], "C");
     const type_t type = A[0]->type;
     check_real(type);
     const int es = elem_size(type);

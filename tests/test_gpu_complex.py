@@ -79,6 +79,19 @@ def test_complex_special_cases(dtype):
     _case(rng, "ak", "kb", "ab", ext, dtype, 1.0, 1.0)                      # beta = 1
 
 
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_complex_diagonal_c(dtype):
+    """C[aa] = A[ae] B[ea]: a repeated C label addresses C's diagonal
+    (src/iface/3t/mult.cxx:44-48), so only the diagonal changes, also when
+    complex beta and alpha are applied by passes over C; the off-diagonal
+    entries keep their values."""
+    rng = np.random.default_rng(65300)
+    ext = dict(a=6, e=5)
+    for alpha, beta in [(1.0 + 1.0j, 0.0), (0.0, 0.5 + 0.5j), (0.0, 0.5), (1.0 + 1.0j, 0.5 + 0.5j),
+                        (2.0, 0.5 + 0.5j), (0.5 - 1.0j, 1.0)]:
+        _case(rng, "ae", "ea", "aa", ext, dtype, alpha, beta, kinds="col")
+
+
 def test_complex_scalar_reset():
     """After the call C's scalar is 1 + 0i (src/iface/3t/mult.cxx:162-163)."""
     A = torch.ones(3, 4, dtype=torch.complex128, device=DEV)

@@ -184,6 +184,35 @@ def test_float_through_fp64_mfma():
                 config="fp64_mfma", tol=2e-7)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_broadcast_stride0_dims(dtype):
+    """Operands with a stride-0 (broadcast) dimension, which every element of
+    it reads at one address: k of stride 0 under rows of stride 2 (packed
+    through the general, not the k-fast, gather), broadcast rows, and a
+    broadcast B; FP64 and split-BF16."""
+    rng = np.random.default_rng(60 + (dtype == np.float32))
+    tol = FP64_TOL if dtype == np.float64 else 1e-6
+    cases = [([40, 24], [2, 0], [24, 16], [1, 24]),
+             ([40, 24], [0, 1], [24, 16], [16, 1]),
+             ([40, 24], [1, 40], [24, 16], [0, 1]),
+             ([40, 24], [24, 1], [24, 16], [1, 0]),
+             ([70, 33], [3, 0], [33, 50], [0, 33])]
+    for lA, sA, lB, sB in cases:
+        lC = [lA[0], lB[1]]
+        sC = col_major(lC)
+        A = rng.uniform(-1, 1, storage_size(lA, sA)).astype(dtype)
+        B = rng.uniform(-1, 1, storage_size(lB, sB)).astype(dtype)
+        C0 = rng.uniform(-1, 1, storage_size(lC, sC)).astype(dtype)
+        ref = po.tensor_mult(0.5, A, lA, sA, "ae", B, lB, sB, "eb", 1.0, C0.copy(), lC, sC, "ab")
+        _, a = dev_view(A, lA, sA)
+        _, b = dev_view(B, lB, sB)
+        cs, c = dev_view(C0.copy(), lC, sC)
+        T.mult(0.5, a, "ae", b, "eb", 1.0, c, "ab")
+        torch.cuda.synchronize()
+        err = rel_fro(cs.cpu().numpy(), ref)
+        assert err <= tol, (lA, sA, lB, sB, dtype, err)
+
+
 def test_plan_cache_reuse_and_host_pointer_error():
     rng = np.random.default_rng(12)
     for it in range(3):

@@ -297,6 +297,159 @@ def test_tensor_mult_special_rules():
     assert (C == 1.0).all()
 
 
+def _strided_view(X, lens, strides):
+    return np.lib.stride_tricks.as_strided(X, shape=lens, strides=[X.itemsize * s for s in strides])
+
+
+def _entry_offsets(d, ent_l, ent_m, ent_n):
+    """C element offsets of flat (l, m, n) logical indices of a canonical form
+    (first dimension of each group fastest)."""
+    off = np.zeros(len(ent_l), np.int64)
+    for arr, lens, strides in ((ent_l, d["len_abc"], d["sC_abc"]), (ent_m, d["len_ac"], d["sC_ac"]),
+                               (ent_n, d["len_bc"], d["sC_bc"])):
+        f = np.array(arr, np.int64)
+        for ln, s in zip(lens, strides):
+            off += (f % ln) * s
+            f //= ln
+    return off
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_mult_entries_matches_einsum(seed):
+    """orc_mult_entries_d (the checker of the full-size GPU tests) against a
+    textbook numpy.einsum of the same strided operands, on random canonical
+    forms with batch labels, diagonals in A, and FP32 as well as FP64 data:
+    every (l, m, n) entry, mapped to C through the canonical form's C
+    strides.  A dropped beta, a swapped stride slot or a transposed operand
+    fails here."""
+    rng = np.random.default_rng(800 + seed)
+    iA, iB, iC, ext = random_contraction(rng, max_dims=2, max_len=4, allow_diag=seed % 4 == 0)
+    (lA, sA), (lB, sB), (lC, sC) = [layout_for(i, ext, rng) for i in (iA, iB, iC)]
+    dt = np.float32 if seed % 2 else np.float64
+    A = rng.uniform(-1, 1, storage_size(lA, sA)).astype(dt)
+    B = rng.uniform(-1, 1, storage_size(lB, sB)).astype(dt)
+    C0 = rng.uniform(-1, 1, storage_size(lC, sC)).astype(dt)
+    alpha, beta = [(0.75, 0.0), (1.0, 1.0), (-0.5, 2.5), (1.5, -1.0)][seed % 4]
+    g = po.canonicalize(lA, sA, iA, lB, sB, iB, lC, sC, iC)
+    d = g.as_dict()
+    L = int(np.prod(d["len_abc"])) if d["len_abc"] else 1
+    M = int(np.prod(d["len_ac"])) if d["len_ac"] else 1
+    N = int(np.prod(d["len_bc"])) if d["len_bc"] else 1
+    el, em, en = (x.ravel() for x in np.meshgrid(np.arange(L), np.arange(M), np.arange(N), indexing="ij"))
+    got = po.mult_entries(g, alpha, A, B, beta, C0, el, em, en)
+    want_st = C0.astype(np.float64)
+    v = _strided_view(want_st, lC, sC)
+    prod = np.einsum("%s,%s->%s" % (iA, iB, iC), _strided_view(A.astype(np.float64), lA, sA),
+                     _strided_view(B.astype(np.float64), lB, sB))
+    v[...] = alpha * prod + beta * v
+    want = want_st[_entry_offsets(d, el, em, en)]
+    np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-13)
+
+
+def test_mult_entries_beta_zero_and_workload_form():
+    """beta == 0 does not read C (NaN there is not propagated), and on the
+    permuted workload's canonical form at reduced extents every entry
+    equals the einsum."""
+    rng = np.random.default_rng(870)
+    ext = dict(a=6, b=5, e=4, f=3, i=2, j=3)
+    lens = lambda s: [ext[c] for c in s]
+    lA, lB, lC = lens("aebf"), lens("fjei"), lens("aibj")
+    sA, sB, sC = col_major(lA), col_major(lB), col_major(lC)
+    A = rng.uniform(-1, 1, int(np.prod(lA)))
+    B = rng.uniform(-1, 1, int(np.prod(lB)))
+    C0 = np.full(int(np.prod(lC)), np.nan)
+    g = po.canonicalize(lA, sA, "aebf", lB, sB, "fjei", lC, sC, "aibj")
+    d = g.as_dict()
+    M, N = int(np.prod(d["len_ac"])), int(np.prod(d["len_bc"]))
+    em, en = (x.ravel() for x in np.meshgrid(np.arange(M), np.arange(N), indexing="ij"))
+    got = po.mult_entries(g, 0.5, A, B, 0.0, C0, np.zeros_like(em), em, en)
+    want_st = np.zeros(int(np.prod(lC)))
+    _strided_view(want_st, lC, sC)[...] = 0.5 * np.einsum(
+        "aebf,fjei->aibj", _strided_view(A, lA, sA), _strided_view(B, lB, sB))
+    np.testing.assert_allclose(got, want_st[_entry_offsets(d, np.zeros_like(em), em, en)],
+                               rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_blis_contract_matches_einsum(seed):
+    """orc_blis_contract_d (the CPU baseline: Goto loop nest over scatter
+    vectors) against numpy.einsum on random strided contractions, including
+    batch labels and sizes that span several MC / KC / NR blocks."""
+    rng = np.random.default_rng(900 + seed)
+    iA, iB, iC, ext = random_contraction(rng, max_dims=2, max_len=9 if seed < 12 else 4)
+    if seed >= 12:  # a long k and a wide n: several KC and MC blocks
+        iA, iB, iC = "aek", "kbe", "ab"
+        ext = dict(a=75 + seed, b=13 + 3 * seed, e=16, k=33)
+    (lA, sA), (lB, sB), (lC, sC) = [layout_for(i, ext, rng) for i in (iA, iB, iC)]
+    A = rng.uniform(-1, 1, storage_size(lA, sA))
+    B = rng.uniform(-1, 1, storage_size(lB, sB))
+    C0 = rng.uniform(-1, 1, storage_size(lC, sC))
+    alpha, beta = [(0.75, 0.0), (1.0, 1.0), (-0.5, 2.5), (0.5, 1.0)][seed % 4]
+    if beta == 0.0:
+        C0[:] = np.nan  # beta == 0 must not read C
+    want = np.where(np.isnan(C0), 0.0, C0)
+    v = _strided_view(want, lC, sC)
+    v[...] = alpha * np.einsum("%s,%s->%s" % (iA, iB, iC), _strided_view(A, lA, sA),
+                               _strided_view(B, lB, sB)) + (beta * v if beta else 0.0)
+    C = C0.copy()
+    flops = po.blis_contract(2, alpha, A, lA, sA, iA, B, lB, sB, iB, beta, C, lC, sC, iC)
+    written = np.unique(_entry_offsets_all(lC, sC))
+    np.testing.assert_allclose(C[written], want[written], rtol=1e-12, atol=1e-12)
+    assert flops == 2.0 * np.prod([ext[c] for c in set(iA + iB + iC)], dtype=np.float64)
+
+
+def _entry_offsets_all(lens, strides):
+    off = np.zeros(1, np.int64)
+    for n, s in zip(lens, strides):
+        off = (off[:, None] + np.arange(n, dtype=np.int64)[None, :] * s).ravel()
+    return off
+
+
+def test_blis_contract_column_limit():
+    """n_limit computes only the first n_limit columns of C (the bench's
+    bounded CPU sample): those match the full contraction, the rest of C
+    is untouched."""
+    rng = np.random.default_rng(950)
+    ext = dict(a=20, b=18, e=12, f=7, i=6, j=5)
+    lens = lambda s: [ext[c] for c in s]
+    lA, lB, lC = lens("aebf"), lens("fjei"), lens("aibj")
+    sA, sB, sC = col_major(lA), col_major(lB), col_major(lC)
+    A, B = rng.uniform(-1, 1, int(np.prod(lA))), rng.uniform(-1, 1, int(np.prod(lB)))
+    C0 = rng.uniform(-1, 1, int(np.prod(lC)))
+    full, part = C0.copy(), C0.copy()
+    po.blis_contract(3, 0.5, A, lA, sA, "aebf", B, lB, sB, "fjei", 1.0, full, lC, sC, "aibj")
+    po.blis_contract(3, 0.5, A, lA, sA, "aebf", B, lB, sB, "fjei", 1.0, part, lC, sC, "aibj", n_limit=7)
+    g = po.blis_reorder(po.canonicalize(lA, sA, "aebf", lB, sB, "fjei", lC, sC, "aibj")).as_dict()
+    # odometer order, first dimension fastest (the scatter vectors' order)
+    cols = np.zeros(1, np.int64)
+    for n, s in zip(g["len_n"], g["sC_n"]):
+        cols = (np.arange(n, dtype=np.int64)[:, None] * s + cols[None, :]).ravel()
+    rows = np.zeros(1, np.int64)
+    for n, s in zip(g["len_m"], g["sC_m"]):
+        rows = (np.arange(n, dtype=np.int64)[:, None] * s + rows[None, :]).ravel()
+    done = (rows[:, None] + cols[None, :7]).ravel()
+    rest = (rows[:, None] + cols[None, 7:]).ravel()
+    np.testing.assert_allclose(part[done], full[done], rtol=1e-13, atol=1e-13)
+    assert (part[rest] == C0[rest]).all()
+
+
+def test_batch_workload_generator():
+    """The batch workload's extents: std::mt19937_64 (the C++ standard's
+    known answer: the 10000th output for the default seed 5489 is
+    9981545732273789042, [rand.predef]) and libstdc++'s
+    uniform_int_distribution<int>(4, 40), whose first draws for seed 7 were
+    taken from a g++ 11 build of that call."""
+    import bench
+    g = bench.Mt19937_64(5489)
+    for _ in range(9999):
+        g()
+    assert g() == 9981545732273789042
+    e = bench.batch_extents(4096, 7)
+    assert e[:3].tolist() == [[31, 39, 8, 37, 9, 6], [34, 37, 13, 30, 31, 26], [18, 15, 34, 15, 40, 40]]
+    assert e.min() == 4 and e.max() == 40
+    assert float((2.0 * np.prod(e.astype(np.float64), axis=1)).sum()) == 934466025590.0
+
+
 # ---------------------------------------------------------------- DPD contraction
 def _dpd_problem(rng, nirrep, ndims=(2, 2, 2), maxlen=3, layout=5, irreps=None):
     iA, iB, iC = "abef", "efij", "abij"
@@ -506,6 +659,22 @@ def test_complex_oracle_against_numpy(seed):
     b = np.conj(b) if cB else b
     want_v = alpha * np.einsum("%s,%s->%s" % (iA, iB, iC), a, b) + beta * view(C0.copy(), lC, sC)
     np.testing.assert_allclose(view(got, lC, sC), want_v, rtol=1e-12, atol=1e-12)
+
+
+def test_complex_oracle_diagonal_c():
+    """C[aa]: only C's diagonal is written (alpha P + beta C there); the
+    off-diagonal entries keep their values whatever alpha and beta are."""
+    rng = np.random.default_rng(64100)
+    A = rng.uniform(-1, 1, 30) + 1j * rng.uniform(-1, 1, 30)
+    B = rng.uniform(-1, 1, 30) + 1j * rng.uniform(-1, 1, 30)
+    C0 = rng.uniform(-1, 1, 36) + 1j * rng.uniform(-1, 1, 36)
+    for alpha, beta in [(1 + 1j, 0.0), (0.0, 0.5 + 0.5j), (2.0, 0.5 - 1j)]:
+        got = po.complex_tensor_mult(alpha, A, [6, 5], [1, 6], "ae", B, [5, 6], [1, 5], "ea", beta, C0,
+                                     [6, 6], [1, 6], "aa")
+        want = C0.copy()
+        d = np.arange(6) * 7
+        want[d] = alpha * np.einsum("ae,ea->a", A.reshape(5, 6).T, B.reshape(6, 5).T) + beta * C0[d]
+        np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-13)
 
 
 # ---------------------------------------------------------------- indexed DPD

@@ -18,9 +18,15 @@
  *     of the dense workloads (hand-derived from the reference), invariance of
  *     the contraction result under relabelling and dimension permutation, and
  *     the element-offset identity (fold preserves every element's address);
- *   - orc_tensor_mult_d/_s, orc_mult_entries_d: textbook contraction
- *     (numpy.einsum) on small random shapes, including the beta==0 overwrite
- *     rule (NaN in C is not read) and alpha==0 / k==0 / empty-m rules;
+ *   - orc_tensor_mult_d/_s: textbook contraction (numpy.einsum) on small
+ *     random shapes, including the beta==0 overwrite rule (NaN in C is not
+ *     read) and alpha==0 / k==0 / empty-m rules;
+ *   - orc_mult_entries_d: every (l, m, n) entry of random canonical forms
+ *     (batch labels, diagonals, FP32 and FP64 data) against numpy.einsum
+ *     mapped through the canonical C strides, and beta==0 over NaN;
+ *   - orc_blis_contract_d: numpy.einsum on random strided contractions
+ *     (batch labels, several MC/KC/NR blocks, beta==0 over NaN) and the
+ *     column limit of the bounded CPU sample;
  *   - orc_dpd_mult_d: block-wise vs full dense expansion (mode 2), which are
  *     two different algorithms for src/internal/3t/dpd/mult.cxx:29-74.
  * tests/test_oracle.py holds each of these checks.
@@ -850,23 +856,93 @@ void orc_mult_entries_d(const orc_groups* g, double alpha, const void* A, const 
 #define BL_KC 256
 #define BL_NC 4080
 
+#ifdef __AVX2__
+#include <immintrin.h>
+#endif
+
+/* ab (MR x NR, row major) = sum_p a[p*MR + i] * b[p*NR + j]: the register
+ * tile of the reference's haswell_d6x8 configuration (MR=6, NR=8,
+ * src/configs/haswell/config.hpp:82-100), written here with AVX2 FMA
+ * intrinsics: 12 accumulators of 4 doubles, one broadcast of A and two
+ * loads of B per row step. */
 static void bl_ukr(int64_t k, const double* restrict a, const double* restrict b, double* restrict ab)
 {
+#ifdef __AVX2__
+    __m256d c[BL_MR][2];
+    for (int i = 0; i < BL_MR; i++) c[i][0] = c[i][1] = _mm256_setzero_pd();
+    for (int64_t p = 0; p < k; p++)
+    {
+        const __m256d b0 = _mm256_load_pd(b + p * BL_NR), b1 = _mm256_load_pd(b + p * BL_NR + 4);
+        for (int i = 0; i < BL_MR; i++)
+        {
+            const __m256d ai = _mm256_broadcast_sd(a + p * BL_MR + i);
+            c[i][0] = _mm256_fmadd_pd(ai, b0, c[i][0]);
+            c[i][1] = _mm256_fmadd_pd(ai, b1, c[i][1]);
+        }
+    }
+    for (int i = 0; i < BL_MR; i++)
+    {
+        _mm256_storeu_pd(ab + i * BL_NR, c[i][0]);
+        _mm256_storeu_pd(ab + i * BL_NR + 4, c[i][1]);
+    }
+#else
     double acc[BL_MR][BL_NR];
     for (int i = 0; i < BL_MR; i++)
         for (int j = 0; j < BL_NR; j++) acc[i][j] = 0;
     for (int64_t p = 0; p < k; p++)
-    {
         for (int i = 0; i < BL_MR; i++)
         {
             double ai = a[p * BL_MR + i];
             for (int j = 0; j < BL_NR; j++) acc[i][j] += ai * b[p * BL_NR + j];
         }
-    }
     for (int i = 0; i < BL_MR; i++)
         for (int j = 0; j < BL_NR; j++) ab[i * BL_NR + j] = acc[i][j];
+#endif
 }
 
+/* Thread split of the GEMM loops (make_gemm_thread_config,
+ * src/util/gemm_thread.hpp:22-57, with the default ratios of
+ * src/configs/config_builder.hpp:111-117: m counts twice n, at most 3
+ * threads on the jr loop and 1 on the ir loop): each prime factor of the
+ * thread count goes to the ic loop unless the n work per thread is at least
+ * the m work (tci_partition_2x2, src/external/tci/tci/parallel.c:280-325);
+ * a factor 3 or 2 of the n part then moves to the jr loop. */
+static void bl_threads(int nt, int64_t m, int64_t n, int* ic_nt, int* jc_nt, int* jr_nt)
+{
+    uint64_t w1 = (uint64_t)m * 2, w2 = (uint64_t)n;
+    int n1 = 1, n2 = 1;
+    if (nt < 4)
+    {
+        if (w1 >= w2) n1 = nt; else n2 = nt;
+    }
+    else
+    {
+        int x = nt;
+        for (int f = 2; x > 1; )
+        {
+            if (x % f) { f++; continue; }
+            x /= f;
+            if (w2 >= w1) { w2 /= (uint64_t)f; n2 *= f; }
+            else { w1 /= (uint64_t)f; n1 *= f; }
+        }
+    }
+    int jr = 1;
+    for (int r = 3; r > 1; r--)
+        if (n2 % r == 0) { jr = r; n2 /= r; break; }
+    *ic_nt = n1;
+    *jc_nt = n2;
+    *jr_nt = jr;
+}
+
+/* The loop nest of TensorGEMM (src/nodes/gemm.hpp:178-187): NC columns ->
+ * KC of k -> pack B (KC x NC, NR-wide panels, from the scatter vectors as
+ * pack_nb/pack_sb do, src/kernels/3m/packm.hpp:293-404) -> MC rows -> pack
+ * A (MC x KC, MR-wide panels) -> micro-kernel over the NR x MR grid, C
+ * updated through its scatter vectors (beta == 0 writes without reading,
+ * src/kernels/3m/gemm.hpp:54-58; beta becomes 1 after the first KC block,
+ * src/nodes/partm.hpp:83).  Threads are split over the jc, ic and jr loops
+ * as bl_threads gives; all of them pack the shared B panel, each packs the
+ * A blocks of its ic range. */
 double orc_blis_contract_d(int nthreads, double alpha, int ndim_A, const int64_t* len_A,
                            const int64_t* stride_A, const char* idx_A, const double* A,
                            int ndim_B, const int64_t* len_B, const int64_t* stride_B,
@@ -891,63 +967,80 @@ double orc_blis_contract_d(int nthreads, double alpha, int ndim_A, const int64_t
     orc_fill_scatter(t.nn, t.len_n, t.sB_n, BL_NR, 0, n, cB, t.pack_n_3d, 8);
     orc_fill_scatter(t.nm, t.len_m, t.sC_m, BL_MR, 0, m, rC, t.pack_m_3d, 8);
     orc_fill_scatter(t.nn, t.len_n, t.sC_n, BL_NR, 0, n, cC, t.pack_n_3d, 8);
-
+    int nt = 1;
 #ifdef _OPENMP
-    if (nthreads > 0) omp_set_num_threads(nthreads);
+    nt = nthreads > 0 ? nthreads : omp_get_max_threads();
 #endif
-    for (int64_t l = 0; l < L; l++)
+    int ic_nt, jc_nt, jr_nt;
+    bl_threads(nt, m, n, &ic_nt, &jc_nt, &jr_nt);
+    const int64_t ncp_max = ((n < BL_NC ? n : BL_NC) + BL_NR - 1) / BL_NR * BL_NR;
+    double* bp = aligned_alloc(64, sizeof(double) * (size_t)(ncp_max * BL_KC + 8));
+    const int64_t nic = (m + BL_MC - 1) / BL_MC;
+
+    for (int64_t l = 0; l < L && m > 0 && n > 0; l++)
     {
-        int64_t oA = flat_offset(l, t.len_l, t.sA_l, t.nl);
-        int64_t oB = flat_offset(l, t.len_l, t.sB_l, t.nl);
-        int64_t oC = flat_offset(l, t.len_l, t.sC_l, t.nl);
+        const int64_t oA = flat_offset(l, t.len_l, t.sA_l, t.nl);
+        const int64_t oB = flat_offset(l, t.len_l, t.sB_l, t.nl);
+        const int64_t oC = flat_offset(l, t.len_l, t.sC_l, t.nl);
         for (int64_t jc = 0; jc < n; jc += BL_NC)
         {
-            int64_t nc = n - jc < BL_NC ? n - jc : BL_NC;
-            int64_t ncp = (nc + BL_NR - 1) / BL_NR * BL_NR;
-            for (int64_t pc = 0; pc < k; pc += BL_KC)
+            const int64_t nc = n - jc < BL_NC ? n - jc : BL_NC;
+            const int64_t ncp = (nc + BL_NR - 1) / BL_NR * BL_NR;
+            const int64_t npan = ncp / BL_NR;
+            for (int64_t pc = 0; pc < k || (k == 0 && pc == 0); pc += BL_KC)
             {
-                int64_t kc = k - pc < BL_KC ? k - pc : BL_KC;
-                double beta_l = pc == 0 ? beta : 1.0;
-                double* bp = aligned_alloc(64, sizeof(double) * ncp * kc);
-                #pragma omp parallel for schedule(static)
-                for (int64_t jr = 0; jr < ncp; jr += BL_NR)
-                    for (int64_t p = 0; p < kc; p++)
-                        for (int j = 0; j < BL_NR; j++)
-                            bp[jr * kc + p * BL_NR + j] = jc + jr + j < jc + nc ?
-                                B[oB + rB[pc + p] + cB[jc + jr + j]] : 0.0;
-                int64_t nic = (m + BL_MC - 1) / BL_MC;
-                #pragma omp parallel
+                const int64_t kc = k - pc < BL_KC ? k - pc : BL_KC;
+                const double beta_l = pc == 0 ? beta : 1.0;
+                #pragma omp parallel num_threads(nt)
                 {
+                    #pragma omp for schedule(static)
+                    for (int64_t jr = 0; jr < ncp; jr += BL_NR)
+                        for (int64_t p = 0; p < kc; p++)
+                            for (int j = 0; j < BL_NR; j++)
+                                bp[jr * kc + p * BL_NR + j] = jr + j < nc ?
+                                    B[oB + rB[pc + p] + cB[jc + jr + j]] : 0.0;
+                    int tid = 0;
+#ifdef _OPENMP
+                    tid = omp_get_thread_num();
+#endif
+                    /* this thread's column range (jc and jr parts, whole
+                     * NR panels) and MC blocks (ic part) */
+                    const int wc = (tid / (ic_nt * jr_nt)) * jr_nt + tid % jr_nt, ncol = jc_nt * jr_nt;
+                    const int ig = (tid / jr_nt) % ic_nt;
+                    const int64_t p0 = npan * wc / ncol, p1 = npan * (wc + 1) / ncol;
+                    const int64_t b0 = nic * ig / ic_nt, b1 = nic * (ig + 1) / ic_nt;
                     double ap[BL_MC * BL_KC] __attribute__((aligned(64)));
-                    double ab[BL_MR * BL_NR];
-                    #pragma omp for schedule(dynamic)
-                    for (int64_t ib = 0; ib < nic; ib++)
+                    double ab[BL_MR * BL_NR] __attribute__((aligned(64)));
+                    for (int64_t ib = b0; ib < b1 && p0 < p1; ib++)
                     {
-                        int64_t ic = ib * BL_MC;
-                        int64_t mc = m - ic < BL_MC ? m - ic : BL_MC;
+                        const int64_t ic = ib * BL_MC;
+                        const int64_t mc = m - ic < BL_MC ? m - ic : BL_MC;
                         for (int64_t ir = 0; ir < mc; ir += BL_MR)
                             for (int64_t p = 0; p < kc; p++)
                                 for (int i = 0; i < BL_MR; i++)
                                     ap[ir * kc + p * BL_MR + i] = ir + i < mc ?
                                         A[oA + rA[ic + ir + i] + cA[pc + p]] : 0.0;
-                        for (int64_t jr = 0; jr < nc; jr += BL_NR)
+                        for (int64_t jr = p0 * BL_NR; jr < p1 * BL_NR && jr < nc; jr += BL_NR)
                             for (int64_t ir = 0; ir < mc; ir += BL_MR)
                             {
                                 bl_ukr(kc, ap + ir * kc, bp + jr * kc, ab);
                                 for (int i = 0; i < BL_MR && ir + i < mc; i++)
+                                {
+                                    double* crow = C + oC + rC[ic + ir + i];
                                     for (int j = 0; j < BL_NR && jr + j < nc; j++)
                                     {
-                                        double* c = &C[oC + rC[ic + ir + i] + cC[jc + jr + j]];
+                                        double* c = crow + cC[jc + jr + j];
                                         *c = beta_l == 0.0 ? alpha * ab[i * BL_NR + j]
                                                            : alpha * ab[i * BL_NR + j] + beta_l * *c;
                                     }
+                                }
                             }
                     }
                 }
-                free(bp);
             }
         }
     }
+    free(bp);
     free(rA); free(cA); free(rB); free(cB); free(rC); free(cC);
     return 2.0 * (double)m * (double)n * (double)k * (double)L;
 }

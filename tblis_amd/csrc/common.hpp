@@ -55,11 +55,30 @@ struct MatrixViews
 };
 
 // (len, stride) list of one matrix side for one tensor.
+// A side may instead list its element offsets explicitly (offs non-empty;
+// len and stride then unused): a DPD irrep GEMM's merged rows or k, whose
+// offsets are per-block bases plus the blocks' own odometer offsets.
 struct Side
 {
     std::vector<int64_t> len, stride;
     bool pack_3d = false;
+    std::vector<int64_t> offs;
 };
+
+inline int64_t side_count(const Side& s)
+{
+    if (!s.offs.empty()) return (int64_t)s.offs.size();
+    int64_t n = 1;
+    for (auto l : s.len) n *= l;
+    return n;
+}
+
+// Stride between the first two elements (0 when the side has one element).
+inline int64_t first_step(const Side& s)
+{
+    if (!s.offs.empty()) return s.offs.size() > 1 ? s.offs[1] - s.offs[0] : 0;
+    return !s.len.empty() && s.len[0] > 1 ? s.stride[0] : 0;
+}
 
 inline int64_t prod_len(const Group& g)
 {

@@ -390,6 +390,108 @@ std::vector<std::vector<unsigned>> irrep_combos(int nd, int nir, unsigned x)
     return out;
 }
 
+// Odometer offsets of a side (first dimension fastest), or its explicit list.
+std::vector<int64_t> side_offsets(const Side& s)
+{
+    if (!s.offs.empty()) return s.offs;
+    const int64_t n = side_count(s);
+    std::vector<int64_t> out(std::max<int64_t>(n, 0));
+    std::vector<int64_t> idx(s.len.size(), 0);
+    int64_t off = 0;
+    for (int64_t i = 0; i < n; i++)
+    {
+        out[i] = off;
+        for (size_t d = 0; d < s.len.size(); d++)
+        {
+            off += s.stride[d];
+            if (++idx[d] < s.len[d]) break;
+            off -= s.stride[d] * s.len[d];
+            idx[d] = 0;
+        }
+    }
+    return out;
+}
+
+struct BlockRef
+{
+    int64_t base;
+    const Side* r;
+    const Side* c;
+};
+
+// NP x NQ blocks (row-block major) as one matrix, when every address splits
+// into a row part and a column part: block (p, q) at base(p, q) with row
+// offsets that do not depend on q and column offsets that do not depend on
+// p, and base(p, q) - base(p, 0) - base(0, q) + base(0, 0) == 0.  Element
+// (i, j) of the merged matrix is then at R[i] + C[j], R = base(p, 0) + the
+// rows of block (p, 0), C = base(0, q) - base(0, 0) + the columns of block
+// (0, q), both shifted so that neither has a negative entry.
+bool merge_blocks(const std::vector<BlockRef>& b, size_t NP, size_t NQ, Side& R, Side& C)
+{
+    auto same = [](const Side& x, const Side& y) {
+        return x.len == y.len && x.stride == y.stride && x.offs == y.offs;
+    };
+    for (size_t p = 0; p < NP; p++)
+        for (size_t q = 0; q < NQ; q++)
+        {
+            const BlockRef& x = b[p * NQ + q];
+            if (!same(*x.r, *b[p * NQ].r) || !same(*x.c, *b[q].c)) return false;
+            if (x.base - b[p * NQ].base - b[q].base + b[0].base != 0) return false;
+        }
+    std::vector<int64_t> ro, co;
+    for (size_t p = 0; p < NP; p++)
+        for (int64_t x : side_offsets(*b[p * NQ].r)) ro.push_back(b[p * NQ].base + x);
+    for (size_t q = 0; q < NQ; q++)
+        for (int64_t x : side_offsets(*b[q].c)) co.push_back(b[q].base - b[0].base + x);
+    if (ro.empty() || co.empty()) return false;
+    const int64_t cmin = *std::min_element(co.begin(), co.end());
+    for (auto& x : co) x -= cmin;
+    for (auto& x : ro) x += cmin;
+    R = Side();
+    C = Side();
+    R.offs = std::move(ro);
+    C.offs = std::move(co);
+    return true;
+}
+
+// A merged irrep GEMM whose A, B and C blocks are all separable becomes one
+// block per operand with explicit offsets (the GEMM reads and writes the
+// DPD storage through them, with no pack or unpack pass); otherwise it
+// keeps its blocks.  TBLIS_HIP_DPD_MERGE=0 keeps the blocks always.
+void merge_irrep_gemm(PanelGemm& g)
+{
+    if (const char* e = std::getenv("TBLIS_HIP_DPD_MERGE"))
+        if (e[0] == '0') return;
+    const size_t NM = g.mlen.size(), NN = g.nlen.size(), NK = g.klen.size();
+    if (NM * NK != g.a.size() || NN * NK != g.b.size() || NM * NN != g.c.size()) return;
+    std::vector<BlockRef> ra, rb, rc;
+    for (const PBlock& x : g.a) ra.push_back(BlockRef{x.base, &x.rows, &x.k});
+    for (const PBlock& x : g.b) rb.push_back(BlockRef{x.base, &x.rows, &x.k});
+    for (const CBlock& x : g.c) rc.push_back(BlockRef{x.base, &x.rows, &x.cols});
+    for (auto* v : {&ra, &rb})
+        for (const BlockRef& x : *v)
+            if (x.base == PANEL_MISSING) return;
+    PBlock A, B;
+    CBlock Cm;
+    if (!merge_blocks(ra, NM, NK, A.rows, A.k) || !merge_blocks(rb, NN, NK, B.rows, B.k) ||
+        !merge_blocks(rc, NM, NN, Cm.rows, Cm.cols))
+        return;
+    A.base = B.base = Cm.base = 0;
+    int64_t m = 0, n = 0, k = 0;
+    for (auto x : g.mlen) m += x;
+    for (auto x : g.nlen) n += x;
+    for (auto x : g.klen) k += x;
+    PanelGemm out;
+    out.mlen = {m};
+    out.nlen = {n};
+    out.klen = {k};
+    out.a = {A};
+    out.b = {B};
+    out.c = {Cm};
+    if (!g.kscale.empty()) return;  // per-block scales need the blocks
+    g = std::move(out);
+}
+
 // Panel path: one GEMM per contracted irrep r (XOR of the AB irreps).  Its
 // rows are every AC irrep combination allowed with r, its columns every BC
 // combination, its k every AB combination; A, B and C blocks sit at the
@@ -469,6 +571,7 @@ bool build_panel_gemms(const DpdProblem& P, PlanBuilder& pb)
                 b.cols = DpdProblem::side(len, str, P.c_bc, false);
                 g.c.push_back(b);
             }
+        merge_irrep_gemm(g);
         pb.add_panel(g);
     }
     return true;

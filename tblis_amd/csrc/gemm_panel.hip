@@ -16,6 +16,9 @@
 //           stores a dense C panel when the GEMM spans several C blocks;
 //   unpack  (DPD) scatters C panels into the C blocks with alpha/beta.
 #include <algorithm>
+#include <cstdlib>
+#include <map>
+#include <mutex>
 
 #include <hip/hip_runtime.h>
 
@@ -225,12 +228,15 @@ __device__ __forceinline__ void mma_group(v4d (&acc)[MB][NB], const double2 (&a)
 }
 
 // One operand read straight from its tensor (PanelTask "direct"): the
-// lane offsets of its row blocks, and the k-group (base, stride) table.
+// lane's row offsets of its row blocks (taken once per task), and the k
+// tables: (base, stride) per 8-k group, per-element offsets for groups
+// that are not affine.
 template <typename TC, int NBLK>
 struct DirectSide
 {
     __amdgpu_buffer_rsrc_t rsrc;
     const int64_t* kt;
+    const int64_t* km;
     uint32_t roff[NBLK];
     bool rvalid[NBLK];
 };
@@ -244,21 +250,24 @@ __device__ __forceinline__ double direct_load(__amdgpu_buffer_rsrc_t r, uint32_t
         return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
 }
 
+// rows: pool index of the offset of the task's first row; this wave's
+// rows start at rows0 (of len valid rows).
 template <typename TC, int NBLK>
 __device__ __forceinline__ void direct_init(DirectSide<TC, NBLK>& d, const void* X, int64_t src,
                                             uint32_t bytes, const int64_t* __restrict__ pool,
-                                            int64_t rt, int64_t kt, int blk0, int rows0, int len,
+                                            int64_t rows, int64_t kt, int64_t km, int rows0, int len,
                                             int lr)
 {
     d.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)((const TC*)X + src), (short)0, (int)bytes,
                                                0x00020000);
     d.kt = pool + kt;
+    d.km = pool + km;
 #pragma unroll
     for (int i = 0; i < NBLK; i++)
     {
-        const int64_t base = pool[rt + 2 * (blk0 + i)], stride = pool[rt + 2 * (blk0 + i) + 1];
-        d.roff[i] = (uint32_t)(base + lr * stride);
-        d.rvalid[i] = rows0 + i * 16 + lr < len;
+        const int r = rows0 + i * 16 + lr;
+        d.rvalid[i] = r < len;
+        d.roff[i] = d.rvalid[i] ? (uint32_t)pool[rows + r] : 0u;
     }
 }
 
@@ -270,7 +279,17 @@ __device__ __forceinline__ void direct_groups(double2 (&x)[NBLK], const DirectSi
 {
     const int64_t kb = d.kt[2 * kg], ks = d.kt[2 * kg + 1];
     const int k0 = kg * 8 + lk;
-    const uint32_t o0 = (uint32_t)(kb + lk * ks), o1 = (uint32_t)(kb + (lk + 4) * ks);
+    uint32_t o0, o1;
+    if (ks != SCATTERED)  // wave-uniform
+    {
+        o0 = (uint32_t)(kb + lk * ks);
+        o1 = (uint32_t)(kb + (lk + 4) * ks);
+    }
+    else
+    {
+        o0 = k0 < klen ? (uint32_t)d.km[k0] : 0u;
+        o1 = k0 + 4 < klen ? (uint32_t)d.km[k0 + 4] : 0u;
+    }
 #pragma unroll
     for (int i = 0; i < NBLK; i++)
     {
@@ -282,18 +301,18 @@ __device__ __forceinline__ void direct_groups(double2 (&x)[NBLK], const DirectSi
     }
 }
 
-// KS > 1 splits k inside the workgroup: KS sets of WGM x WGN waves take
-// interleaved k groups (group kg goes to set kg % KS) and the partial sums
-// meet in LDS before set 0 runs the epilogue.  For short k (DPD irrep
-// GEMMs, small batch instances) this doubles the waves resident per SIMD.
-template <int BM, int BN, int WGM, int WGN, typename TC, bool DA, bool DB, int KS = 1>
-__global__ __launch_bounds__(64 * WGM * WGN * KS) void gemm_panel_kernel(
-    const PanelTask* __restrict__ tasks, const int64_t* __restrict__ pool,
-    const double* __restrict__ ws, double* __restrict__ cws, const void* A, const void* B,
-    TC* __restrict__ C, double alpha, double beta, int use_k)
+// One C tile.  KS > 1 splits k inside the workgroup: KS sets of WGM x WGN
+// waves take interleaved k groups (group kg goes to set kg % KS) and the
+// partial sums meet in LDS before set 0 runs the epilogue.  For short k
+// (DPD irrep GEMMs, small batch instances) this doubles the waves resident
+// per SIMD.  Every wave reaches the LDS barrier when KS > 1.
+template <int BM, int BN, int WGM, int WGN, typename TC, bool DA, bool DB, int KS>
+__device__ __forceinline__ void panel_task(const PanelTask& t, const int64_t* __restrict__ pool,
+                                           const double* __restrict__ ws, double* __restrict__ cws,
+                                           const void* A, const void* B, TC* __restrict__ C, double alpha,
+                                           double beta, int use_k)
 {
     constexpr int WM = BM / WGM, WN = BN / WGN, MB = WM / 16, NB = WN / 16;
-    const PanelTask t = tasks[blockIdx.x];
     const int lane = threadIdx.x & 63;
     const int wave_all = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int kw = wave_all / (WGM * WGN), wave = wave_all % (WGM * WGN);
@@ -317,8 +336,10 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void gemm_panel_kernel(
         const double2* pb = (const double2*)(ws + t.b) + (int64_t)wn * NB * rs + lane;
         DirectSide<TC, MB> da;
         DirectSide<TC, NB> db;
-        if constexpr (DA) direct_init(da, A, t.a_src, t.a_bytes, pool, t.a_rt, t.a_kt, wm * MB, wm * WM, t.mlen, lr);
-        if constexpr (DB) direct_init(db, B, t.b_src, t.b_bytes, pool, t.b_rt, t.b_kt, wn * NB, wn * WN, t.nlen, lr);
+        if constexpr (DA)
+            direct_init(da, A, t.a_src, t.a_bytes, pool, t.a_rt, t.a_kt, t.a_km, wm * WM, t.mlen, lr);
+        if constexpr (DB)
+            direct_init(db, B, t.b_src, t.b_bytes, pool, t.b_rt, t.b_kt, t.b_km, wn * WN, t.nlen, lr);
         auto load = [&](double2 (&a)[MB], double2 (&b)[NB], int li) {
             const int kg = kw + li * KS;
             if constexpr (DA)
@@ -457,6 +478,25 @@ __global__ __launch_bounds__(64 * WGM * WGN * KS) void gemm_panel_kernel(
         }
 }
 
+// Persistent over the task list: workgroup b runs tasks b, b + G, b + 2G,
+// ... (G = gridDim.x, a multiple of 8, so every task stays on the XCD the
+// task order planned it for: task i on XCD i % 8).  With G = ntasks it is
+// one task per workgroup.
+template <int BM, int BN, int WGM, int WGN, typename TC, bool DA, bool DB, int KS = 1>
+__global__ __launch_bounds__(64 * WGM * WGN * KS) void gemm_panel_kernel(
+    const PanelTask* __restrict__ tasks, int ntasks, const int64_t* __restrict__ pool,
+    const double* __restrict__ ws, double* __restrict__ cws, const void* A, const void* B,
+    TC* __restrict__ C, double alpha, double beta, int use_k)
+{
+    for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x)
+    {
+        const PanelTask t = tasks[ti];
+        panel_task<BM, BN, WGM, WGN, TC, DA, DB, KS>(t, pool, ws, cws, A, B, C, alpha, beta, use_k);
+        // the next task's partial sums reuse the LDS that set 0 just read
+        if constexpr (KS > 1) __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------ unpack
 
 template <typename TC>
@@ -530,15 +570,19 @@ constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 template <typename TC, bool DA, bool DB>
 hipError_t launch_gemm_t(const PanelLaunch& L)
 {
-    const dim3 grid(L.ntasks), block(64 * kTiles[L.tile].wgm * kTiles[L.tile].wgn * kTiles[L.tile].ks);
+    const int threads = 64 * kTiles[L.tile].wgm * kTiles[L.tile].wgn * kTiles[L.tile].ks;
+    const dim3 block(threads);
     switch (L.tile)
     {
-#define X(id, bm, bn, wgm, wgn, ks)                                                                   \
-    case id:                                                                                          \
-        hipLaunchKernelGGL((gemm_panel_kernel<bm, bn, wgm, wgn, TC, DA, DB, ks>), grid, block, 0,     \
-                           L.stream, L.tasks, L.pool, (const double*)L.ws, L.cws, L.A, L.B, (TC*)L.C, \
-                           L.alpha, L.beta, L.use_k);                                                 \
-        break;
+#define X(id, bm, bn, wgm, wgn, ks)                                                                        \
+    case id:                                                                                               \
+    {                                                                                                      \
+        auto* k = &gemm_panel_kernel<bm, bn, wgm, wgn, TC, DA, DB, ks>;                                   \
+        const dim3 grid(persistent_grid((const void*)k, threads, L.ntasks));                              \
+        hipLaunchKernelGGL(k, grid, block, 0, L.stream, L.tasks, L.ntasks, L.pool, (const double*)L.ws,   \
+                           L.cws, L.A, L.B, (TC*)L.C, L.alpha, L.beta, L.use_k);                          \
+        break;                                                                                             \
+    }
         TBLIS_PANEL_TILES(X)
 #undef X
         default:
@@ -557,6 +601,41 @@ hipError_t launch_gemm_d(const PanelLaunch& L)
 }
 
 }  // namespace
+
+// Workgroups of a persistent launch: the tasks when they all fit on the
+// device at once, else as many as are resident together (a multiple of the
+// 8 XCDs).  TBLIS_HIP_PERSISTENT=0 launches one workgroup per task.
+int persistent_grid(const void* kernel, int block, int ntasks)
+{
+    static const bool off = [] {
+        const char* e = std::getenv("TBLIS_HIP_PERSISTENT");
+        return e && e[0] == '0';
+    }();
+    if (off) return ntasks;
+    static std::mutex mu;
+    static std::map<const void*, int> cap;
+    int slots = 0;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = cap.find(kernel);
+        if (it != cap.end())
+            slots = it->second;
+        else
+        {
+            int per_cu = 0, dev = 0, cus = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess ||
+                hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            {
+                (void)hipGetLastError();
+                per_cu = 0;
+            }
+            slots = per_cu > 0 && cus > 0 ? per_cu * cus / 8 * 8 : 0;
+            cap[kernel] = slots;
+        }
+    }
+    return slots >= 8 && ntasks > slots ? slots : ntasks;
+}
 
 // The Ozaki family has one tile: 64 x 64 per workgroup, 2 x 2 waves.
 int panel_num_tiles(int family)

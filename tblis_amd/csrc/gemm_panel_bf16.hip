@@ -277,15 +277,131 @@ __device__ __forceinline__ void mma_frags(f32x16 (&acc)[MB][NB], const bf16x8 (&
                                                                     acc[i][jb], 0, 0, 0);
 }
 
-template <int BM, int BN, int WGM, int WGN, bool CHUNKED>
-__global__ __launch_bounds__(64 * WGM * WGN) void gemm_bf16_panel_kernel(
-    const PanelTask* __restrict__ tasks, const int64_t* __restrict__ pool,
-    const __bf16* __restrict__ ws, double* __restrict__ cws, float* __restrict__ C, double alpha,
-    double beta, int use_k)
+// An FP32 operand read straight from its tensor (PanelTask "direct"): the
+// lane's row offset per 32-row block (taken once per task) and the k
+// tables, (base, stride) per 16-k group or per-element offsets for groups
+// that are not affine.  Each 16-k step a lane loads its 8 consecutive k of
+// every block and splits them into the three BF16 terms in registers, so
+// no pack pass runs.
+template <int NBLK>
+struct DirectF32
+{
+    __amdgpu_buffer_rsrc_t rsrc;
+    const int64_t* kt;
+    const int64_t* km;
+    uint32_t roff[NBLK];
+    bool rvalid[NBLK];
+};
+
+template <int NBLK>
+__device__ __forceinline__ void direct_f32_init(DirectF32<NBLK>& d, const float* X, int64_t src,
+                                                uint32_t bytes, const int64_t* __restrict__ pool,
+                                                int64_t rows, int64_t kt, int64_t km, int rows0, int len,
+                                                int lr)
+{
+    d.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)(X + src), (short)0, (int)bytes, 0x00020000);
+    d.kt = pool + kt;
+    d.km = pool + km;
+#pragma unroll
+    for (int i = 0; i < NBLK; i++)
+    {
+        const int r = rows0 + i * 32 + lr;
+        d.rvalid[i] = r < len;
+        d.roff[i] = d.rvalid[i] ? (uint32_t)pool[rows + r] : 0u;
+    }
+}
+
+// Raw values of one 16-k step: lane (lr, lh) gets row lr of each block at
+// k = 16 ks + 8 lh + e, e = 0..7 (zero past the valid rows or k).
+template <int NBLK>
+__device__ __forceinline__ void direct_f32_load(float (&v)[NBLK][8], const DirectF32<NBLK>& d, int ks, int lh,
+                                                int klen)
+{
+    const int64_t kb = d.kt[2 * ks], kst = d.kt[2 * ks + 1];
+    const int k0 = ks * 16 + lh * 8;
+    uint32_t o[8];
+    if (kst != SCATTERED)  // wave-uniform
+    {
+#pragma unroll
+        for (int e = 0; e < 8; e++) o[e] = (uint32_t)(kb + (lh * 8 + e) * kst);
+    }
+    else
+    {
+#pragma unroll
+        for (int e = 0; e < 8; e++) o[e] = k0 + e < klen ? (uint32_t)d.km[k0 + e] : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < NBLK; i++)
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+        {
+            const uint32_t off = d.rvalid[i] && k0 + e < klen ? (d.roff[i] + o[e]) * 4u : 0xffffffffu;
+            v[i][e] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(d.rsrc, off, 0, 0));
+        }
+}
+
+template <int NBLK>
+__device__ __forceinline__ void split_frags(bf16x8 (&x)[NBLK][3], const float (&v)[NBLK][8])
+{
+#pragma unroll
+    for (int i = 0; i < NBLK; i++)
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+        {
+            __bf16 h, m, l;
+            split3(v[i][e], h, m, l);
+            x[i][0][e] = h;
+            x[i][1][e] = m;
+            x[i][2][e] = l;
+        }
+}
+
+// One operand's registers for one 16-k step: split fragments loaded from a
+// panel, or raw FP32 values loaded from the tensor.
+template <int NBLK, bool D>
+struct Stage
+{
+    bf16x8 f[NBLK][3];
+};
+template <int NBLK>
+struct Stage<NBLK, true>
+{
+    float v[NBLK][8];
+};
+
+template <int NBLK, bool D>
+__device__ __forceinline__ void stage_load(Stage<NBLK, D>& st, const uint4* p, int64_t rs, const DirectF32<NBLK>& d,
+                                           int ks, int lh, int klen)
+{
+    if constexpr (D)
+        direct_f32_load<NBLK>(st.v, d, ks, lh, klen);
+    else
+#pragma unroll
+        for (int i = 0; i < NBLK; i++)
+#pragma unroll
+            for (int t = 0; t < 3; t++) st.f[i][t] = __builtin_bit_cast(bf16x8, p[i * rs + ks * 192 + t * 64]);
+}
+
+template <int NBLK, bool D>
+__device__ __forceinline__ void stage_frags(bf16x8 (&x)[NBLK][3], const Stage<NBLK, D>& st)
+{
+    if constexpr (D)
+        split_frags<NBLK>(x, st.v);
+    else
+#pragma unroll
+        for (int i = 0; i < NBLK; i++)
+#pragma unroll
+            for (int t = 0; t < 3; t++) x[i][t] = st.f[i][t];
+}
+
+template <int BM, int BN, int WGM, int WGN, bool CHUNKED, bool DA, bool DB>
+__device__ __forceinline__ void bf16_task(const PanelTask& t, const int64_t* __restrict__ pool,
+                                          const __bf16* __restrict__ ws, double* __restrict__ cws,
+                                          const float* A, const float* B, float* __restrict__ C, double alpha,
+                                          double beta, int use_k)
 {
     constexpr int WM = BM / WGM, WN = BN / WGN, MB = WM / 32, NB = WN / 32;
     constexpr int CHUNK = 16;  // k16 steps per accumulation chunk (256 k)
-    const PanelTask t = tasks[blockIdx.x];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = wave % WGM, wn = wave / WGM;
@@ -307,20 +423,37 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_bf16_panel_kernel(
     const int nk = use_k ? t.nkg : 0;  // even
     if (nk > 0)
     {
+        const int lr = lane & 31, lh = lane >> 5;
         const int64_t rs = (int64_t)t.nkg * 192;  // uint4 per row block
         const uint4* pa = (const uint4*)(ws + t.a) + (int64_t)wm * MB * rs + lane;
         const uint4* pb = (const uint4*)(ws + t.b) + (int64_t)wn * NB * rs + lane;
-        bf16x8 a0[MB][3], b0[NB][3], a1[MB][3], b1[NB][3];
-        load_frags<MB, NB>(a0, b0, pa, pb, rs, 0);
-        load_frags<MB, NB>(a1, b1, pa, pb, rs, 1);
+        DirectF32<MB> da;
+        DirectF32<NB> db;
+        if constexpr (DA)
+            direct_f32_init<MB>(da, A, t.a_src, t.a_bytes, pool, t.a_rt, t.a_kt, t.a_km, wm * WM, t.mlen, lr);
+        if constexpr (DB)
+            direct_f32_init<NB>(db, B, t.b_src, t.b_bytes, pool, t.b_rt, t.b_kt, t.b_km, wn * WN, t.nlen, lr);
+        Stage<MB, DA> a0, a1;
+        Stage<NB, DB> b0, b1;
+        stage_load<MB, DA>(a0, pa, rs, da, 0, lh, t.klen);
+        stage_load<NB, DB>(b0, pb, rs, db, 0, lh, t.klen);
+        stage_load<MB, DA>(a1, pa, rs, da, 1, lh, t.klen);
+        stage_load<NB, DB>(b1, pb, rs, db, 1, lh, t.klen);
         for (int ks = 0; ks < nk; ks += 2)
         {
             // the last iteration re-reads the final groups (in bounds, unused)
             const int nx = ks + 2 < nk ? ks + 2 : nk - 2;
-            mma_frags<MB, NB>(acc, a0, b0);
-            load_frags<MB, NB>(a0, b0, pa, pb, rs, nx);
-            mma_frags<MB, NB>(acc, a1, b1);
-            load_frags<MB, NB>(a1, b1, pa, pb, rs, nx + 1);
+            bf16x8 fa[MB][3], fb[NB][3];
+            stage_frags<MB, DA>(fa, a0);
+            stage_frags<NB, DB>(fb, b0);
+            mma_frags<MB, NB>(acc, fa, fb);
+            stage_load<MB, DA>(a0, pa, rs, da, nx, lh, t.klen);
+            stage_load<NB, DB>(b0, pb, rs, db, nx, lh, t.klen);
+            stage_frags<MB, DA>(fa, a1);
+            stage_frags<NB, DB>(fb, b1);
+            mma_frags<MB, NB>(acc, fa, fb);
+            stage_load<MB, DA>(a1, pa, rs, da, nx + 1, lh, t.klen);
+            stage_load<NB, DB>(b1, pb, rs, db, nx + 1, lh, t.klen);
             if (CHUNKED && ((ks + 2) % CHUNK == 0 || ks + 2 >= nk))
             {
 #pragma unroll
@@ -403,29 +536,72 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_bf16_panel_kernel(
         }
 }
 
+// Persistent over the task list, as the FP64 panel GEMM.
+template <int BM, int BN, int WGM, int WGN, bool CHUNKED, bool DA, bool DB>
+__global__ __launch_bounds__(64 * WGM * WGN) void gemm_bf16_panel_kernel(
+    const PanelTask* __restrict__ tasks, int ntasks, const int64_t* __restrict__ pool,
+    const __bf16* __restrict__ ws, double* __restrict__ cws, const float* A, const float* B,
+    float* __restrict__ C, double alpha, double beta, int use_k)
+{
+    for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x)
+    {
+        const PanelTask t = tasks[ti];
+        bf16_task<BM, BN, WGM, WGN, CHUNKED, DA, DB>(t, pool, ws, cws, A, B, C, alpha, beta, use_k);
+    }
+}
+
 // ------------------------------------------------------------------ table
 
 struct Bf16Tile
 {
     int bm, bn, wgm, wgn;
     bool fast_only;  // the chunked accumulator does not fit the register file
+    bool direct;     // variants that read FP32 operands from the tensors
 };
 
-// id, BM, BN, waves (WGM x WGN), fast only.  Registers per wave (gfx950):
-// 128x64 and 64x128 wave tiles 228 (single accumulator) / 410 (chunked),
-// 64x64 174 / 230.  (A 128x128 wave tile needs all 512 registers for the
-// single accumulator and spills; it is not offered.)
-#define TBLIS_PANEL_BF16_TILES(X) \
-    X(0, 256, 128, 2, 2, false)   \
-    X(1, 128, 256, 2, 2, false)   \
-    X(2, 128, 128, 2, 2, false)
+// id, BM, BN, waves (WGM x WGN), fast only, direct variants.  Registers
+// per wave (gfx950): 128x64 and 64x128 wave tiles 228 (single accumulator)
+// / 410 (chunked), 64x64 174 / 230.  (A 128x128 wave tile needs all 512
+// registers for the single accumulator and spills; it is not offered.)
+// Direct operands hold two 16-k steps of raw FP32 per block and split them
+// in registers, so they are offered on the 64x64 and 32x32 wave tiles.
+#define TBLIS_PANEL_BF16_TILES(X)    \
+    X(0, 256, 128, 2, 2, false, false) \
+    X(1, 128, 256, 2, 2, false, false) \
+    X(2, 128, 128, 2, 2, false, true)  \
+    X(3, 64, 64, 2, 2, false, true)
 
 constexpr Bf16Tile kBTiles[] = {
-#define X(id, bm, bn, wgm, wgn, fo) {bm, bn, wgm, wgn, fo},
+#define X(id, bm, bn, wgm, wgn, fo, dir) {bm, bn, wgm, wgn, fo, dir},
     TBLIS_PANEL_BF16_TILES(X)
 #undef X
 };
 constexpr int kNumBTiles = sizeof(kBTiles) / sizeof(kBTiles[0]);
+
+template <int BM, int BN, int WGM, int WGN, bool CHUNKED, bool DA, bool DB>
+hipError_t launch_bf16_t(const PanelLaunch& L)
+{
+    auto* k = &gemm_bf16_panel_kernel<BM, BN, WGM, WGN, CHUNKED, DA, DB>;
+    const int threads = 64 * WGM * WGN;
+    const dim3 grid(persistent_grid((const void*)k, threads, L.ntasks));
+    hipLaunchKernelGGL(k, grid, dim3(threads), 0, L.stream, L.tasks, L.ntasks, L.pool, (const __bf16*)L.ws,
+                       L.cws, (const float*)L.A, (const float*)L.B, (float*)L.C, L.alpha, L.beta, L.use_k);
+    return hipGetLastError();
+}
+
+template <int BM, int BN, int WGM, int WGN, bool CHUNKED, bool DIR>
+hipError_t launch_bf16_d(const PanelLaunch& L)
+{
+    if constexpr (DIR)
+    {
+        if (L.direct_a && L.direct_b) return launch_bf16_t<BM, BN, WGM, WGN, CHUNKED, true, true>(L);
+        if (L.direct_a) return launch_bf16_t<BM, BN, WGM, WGN, CHUNKED, true, false>(L);
+        if (L.direct_b) return launch_bf16_t<BM, BN, WGM, WGN, CHUNKED, false, true>(L);
+    }
+    else if (L.direct_a || L.direct_b)
+        return hipErrorInvalidValue;
+    return launch_bf16_t<BM, BN, WGM, WGN, CHUNKED, false, false>(L);
+}
 
 }  // namespace
 
@@ -434,6 +610,7 @@ int panel_bf16_tile_m(int t) { return kBTiles[t].bm; }
 int panel_bf16_tile_n(int t) { return kBTiles[t].bn; }
 int panel_bf16_tile_waves(int t) { return kBTiles[t].wgm * kBTiles[t].wgn; }
 bool panel_bf16_tile_fast_only(int t) { return kBTiles[t].fast_only; }
+bool panel_bf16_tile_direct(int t) { return t >= 0 && t < kNumBTiles && kBTiles[t].direct; }
 
 hipError_t launch_panel_pack_bf16(const PanelLaunch& L)
 {
@@ -450,27 +627,19 @@ hipError_t launch_panel_gemm_bf16(const PanelLaunch& L)
     if (L.tile < 0 || L.tile >= kNumBTiles || L.dtype != 1) return hipErrorInvalidValue;
     const bool chunked = L.family == FAMILY_BF16X3;
     if (chunked && kBTiles[L.tile].fast_only) return hipErrorInvalidValue;
-    const dim3 grid(L.ntasks), block(64 * kBTiles[L.tile].wgm * kBTiles[L.tile].wgn);
     switch (L.tile * 2 + (chunked ? 1 : 0))
     {
-#define X(id, bm, bn, wgm, wgn, fo)                                                                   \
-    case 2 * id:                                                                                      \
-        hipLaunchKernelGGL((gemm_bf16_panel_kernel<bm, bn, wgm, wgn, false>), grid, block, 0,         \
-                           L.stream, L.tasks, L.pool, (const __bf16*)L.ws, L.cws, (float*)L.C,        \
-                           L.alpha, L.beta, L.use_k);                                                 \
-        break;                                                                                        \
-    case 2 * id + 1:                                                                                  \
-        if (!fo)                                                                                      \
-            hipLaunchKernelGGL((gemm_bf16_panel_kernel<bm, bn, wgm, wgn, !fo>), grid, block, 0,       \
-                               L.stream, L.tasks, L.pool, (const __bf16*)L.ws, L.cws, (float*)L.C,    \
-                               L.alpha, L.beta, L.use_k);                                             \
-        break;
+#define X(id, bm, bn, wgm, wgn, fo, dir)                           \
+    case 2 * id:                                                   \
+        return launch_bf16_d<bm, bn, wgm, wgn, false, dir>(L);     \
+    case 2 * id + 1:                                               \
+        if constexpr (!fo) return launch_bf16_d<bm, bn, wgm, wgn, true, dir>(L); \
+        return hipErrorInvalidValue;
         TBLIS_PANEL_BF16_TILES(X)
 #undef X
         default:
             return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 }  // namespace tblis_amd

@@ -84,6 +84,10 @@ int panel_bf16_tile_m(int tile);
 int panel_bf16_tile_n(int tile);
 int panel_bf16_tile_waves(int tile);
 bool panel_bf16_tile_fast_only(int tile);
+bool panel_bf16_tile_direct(int tile);  // has variants reading FP32 operands directly
+// Workgroups of a persistent launch of `kernel` over ntasks tasks (kernels
+// loop over tasks with a stride of the grid size).
+int persistent_grid(const void* kernel, int block, int ntasks);
 hipError_t launch_panel_pack_bf16(const PanelLaunch& L);
 // Ozaki family (gemm_ozaki.hip): row maxima, digit planes and the int8 GEMM
 // in one call; rmax holds nrows panel-row maxima in the workspace.

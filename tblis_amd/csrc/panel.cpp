@@ -47,14 +47,9 @@ void hip_check(hipError_t e, const char* what)
     if (e != hipSuccess) fail(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-bool unit_first(const Side& s) { return !s.len.empty() && s.stride[0] == 1 && s.len[0] > 1; }
+bool unit_first(const Side& s) { return first_step(s) == 1; }
 
-int64_t side_len(const Side& s)
-{
-    int64_t n = 1;
-    for (auto l : s.len) n *= l;
-    return n;
-}
+int64_t side_len(const Side& s) { return side_count(s); }
 
 // Offset vectors, deduplicated by (lengths, strides).
 class Pool
@@ -62,6 +57,16 @@ class Pool
   public:
     int64_t offsets(const Side& s)
     {
+        if (!s.offs.empty())
+        {
+            std::vector<int64_t> key(s.offs);
+            key.push_back(-1);  // explicit lists never collide with (len, stride) keys
+            auto it = index_.find(key);
+            if (it != index_.end()) return it->second;
+            const int64_t at = push(s.offs);
+            index_[key] = at;
+            return at;
+        }
         std::vector<int64_t> key(s.len);
         key.insert(key.end(), s.stride.begin(), s.stride.end());
         key.push_back((int64_t)s.len.size());
@@ -214,18 +219,12 @@ bool kfast_job(Pool& pool, const PanelJob& j, const Geom& G)
     const int64_t* pd = pool.data().data();
     const int64_t nrg = (j.r0 + j.nr + G.gr - 1) / G.gr - j.r0 / G.gr;
     const int64_t nkg = (j.k0 + j.nk + G.gk - 1) / G.gk - j.k0 / G.gk;
-    // stride 0 marks a group with a single element
+    // group_table gives a single-element group stride 1; a stride of 0 is a
+    // broadcast dimension (every element of the group at one address), which
+    // the k-fast reads (base + element index) would get wrong
     bool k_unit = true, rows_unit = true;
-    for (int64_t g = 0; g < nkg; g++)
-    {
-        const int64_t st = pd[j.ktab + 2 * g + 1];
-        k_unit &= st == 1 || st == 0;
-    }
-    for (int64_t g = 0; g < nrg; g++)
-    {
-        const int64_t st = pd[j.rowtab + 2 * g + 1];
-        rows_unit &= st == 1 || st == 0;
-    }
+    for (int64_t g = 0; g < nkg; g++) k_unit &= pd[j.ktab + 2 * g + 1] == 1;
+    for (int64_t g = 0; g < nrg; g++) rows_unit &= pd[j.rowtab + 2 * g + 1] == 1;
     return k_unit && !rows_unit;
 }
 
@@ -240,7 +239,7 @@ struct TileModel
 // blocks, 32x32 waves 2x2.  Split-BF16: per tile of the BF16 table.
 const TileModel kModelsF64[] = {{0, 0.85, 2}, {2, 0.80, 2}, {3, 0.82, 2}, {4, 0.82, 2},
                                 {1, 0.50, 6}, {5, 0.45, 6}, {6, 0.50, 4}, {7, 0.45, 4}};
-const TileModel kModelsBf16[] = {{0, 0.55, 1}, {1, 0.55, 1}, {2, 0.45, 2}};
+const TileModel kModelsBf16[] = {{0, 0.55, 1}, {1, 0.55, 1}, {2, 0.45, 2}, {3, 0.35, 4}};
 const TileModel kModelsOzaki[] = {{0, 0.30, 1}};
 
 // Estimated run time of the GEMM tasks for a tile.
@@ -399,11 +398,13 @@ bool panel_enabled(int family, int use_mfma)
 namespace
 {
 
+// src_elem: bytes per element of the tensors (the buffer extent of direct
+// operands)
 bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T,
-                     const bool allow_direct[2]);
+                     const bool allow_direct[2], int src_elem);
 // tables_for_tile with each operand read directly only if it can be for
 // every GEMM of the plan (one kernel variant per launch)
-bool tables_uniform(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T);
+bool tables_uniform(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T, int src_elem);
 int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bool swap,
                   const std::vector<int>& cands);
 
@@ -445,7 +446,7 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family,
     int tile = 0;
     double best = -1;
     const TileModel* models = family == FAMILY_F64 ? kModelsF64 : family == FAMILY_OZAKI ? kModelsOzaki : kModelsBf16;
-    const int nmodels = family == FAMILY_F64 ? 8 : family == FAMILY_OZAKI ? 1 : 3;
+    const int nmodels = family == FAMILY_F64 ? 8 : family == FAMILY_OZAKI ? 1 : 4;
     std::vector<int> cands;
     for (int mi = 0; mi < nmodels; mi++)
     {
@@ -468,7 +469,7 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family,
             !(family == FAMILY_BF16X3 && panel_bf16_tile_fast_only(t)))
             tile = t;
     }
-    if (!tables_uniform(gemms, family, tile, T)) return false;
+    if (!tables_uniform(gemms, family, tile, T, dtype == 0 ? 8 : 4)) return false;
     T.swap = swap;
     return true;
 }
@@ -479,11 +480,11 @@ namespace
 // Tasks grouped by which operands they read straight from the tensors
 // (GEMMs of one batch may differ): the groups become separate launches of
 // the matching kernel variant.
-bool tables_uniform(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T)
+bool tables_uniform(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T, int src_elem)
 {
     const bool allow[2] = {true, true};
     T = PanelTables();
-    if (!tables_for_tile(gemms, family, tile, T, allow)) return false;
+    if (!tables_for_tile(gemms, family, tile, T, allow, src_elem)) return false;
     auto group = [](const PanelTask& t) {
         return (t.a_src != PANEL_MISSING ? 1 : 0) + (t.b_src != PANEL_MISSING ? 2 : 0);
     };
@@ -502,10 +503,23 @@ bool tables_uniform(const std::vector<PanelGemm>& gemms, int family, int tile, P
     return true;
 }
 
+// Many GEMMs (DPD irrep GEMMs, batch instances) read every operand that can
+// be straight from the tensors, so that a call is one GEMM launch; one GEMM
+// reads an operand directly only where that beats packing it.
+// TBLIS_HIP_DIRECT_ALL=1 / =0 forces either rule (read at every plan build).
+bool direct_all(const std::vector<PanelGemm>& gemms)
+{
+    const char* e = std::getenv("TBLIS_HIP_DIRECT_ALL");
+    if (e && e[0] == '0') return false;
+    if (e && e[0] == '1') return true;
+    return gemms.size() > 1;
+}
+
 bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T,
-                     const bool allow_direct[2])
+                     const bool allow_direct[2], int src_elem)
 {
     const Geom G = geom_of(family);
+    const bool general = direct_all(gemms);
     const int64_t BM = panel_tile_m(family, tile), BN = panel_tile_n(family, tile);
 
     Pool pool;
@@ -583,22 +597,32 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
         if (k > 0) T.reads_ab = true;
 
         // Operands read straight from the tensor (FP64 family): a single
-        // block whose 16-row groups are unit-stride and whose 8-k groups
-        // are affine, with non-negative offsets spanning under 2 GiB, and
-        // re-read by at most 24 tiles (beyond that the packed panel's
-        // contiguous 1 KiB groups win).
+        // block with non-negative offsets spanning under 2 GiB.  Each lane
+        // keeps its row offsets for the whole task; k comes from one
+        // (base, stride) pair per 8-k group or, in groups that are not
+        // affine, from the per-element k offsets.  Alone (one GEMM), an
+        // operand is read directly when that is cheaper than a pack pass:
+        // unit-stride 16-row groups with affine k groups re-read by at
+        // most 24 tiles, or affine row groups with unit-stride k groups
+        // re-read by at most 32 tiles within 4 MiB (beyond that the packed
+        // panel's contiguous 1 KiB groups win).  For many GEMMs (DPD irrep
+        // GEMMs, batches) every operand that can be is read directly, so
+        // that the call is one GEMM launch.
         const std::vector<PBlock>* blocks[2] = {&g.a, &g.b};
         const size_t nrp[2] = {NM, NN};
         const int64_t rtot[2] = {m, n};
         bool dir[2] = {false, false};
-        int64_t drt[2] = {0, 0}, dkt[2] = {0, 0}, dsrc[2] = {0, 0};
+        int64_t drt[2] = {0, 0}, dkt[2] = {0, 0}, dkm[2] = {0, 0}, dsrc[2] = {0, 0};
         uint32_t dbytes[2] = {0, 0};
-        for (int o = 0; o < 2 && G.terms == 1 && direct_enabled() && kp > 0; o++)
+        // split-BF16 reads FP32 operands directly (split in registers) on
+        // the tiles that have such variants, for many GEMMs only
+        const bool can_direct = G.terms == 1 || (family != FAMILY_OZAKI && general && panel_bf16_tile_direct(tile));
+        for (int o = 0; o < 2 && can_direct && direct_enabled() && kp > 0; o++)
         {
             if (!allow_direct[o] || nrp[o] != 1 || NK != 1) continue;
             if (o == 0 && !g.kscale.empty() && g.kscale[0] != 1.0) continue;  // scaled while packed
             const int64_t rereads = o == 0 ? ceil_div(n, BN) : ceil_div(m, BM);
-            if (rereads > std::max<int64_t>(direct_rows_rereads(), direct_kfast_rereads())) continue;
+            if (!general && rereads > std::max<int64_t>(direct_rows_rereads(), direct_kfast_rereads())) continue;
             const PBlock& blk = (*blocks[o])[0];
             if (blk.base == PANEL_MISSING) continue;
             const int64_t rm = pool.offsets(blk.rows), km = pool.offsets(blk.k);
@@ -607,50 +631,45 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
             for (int64_t i = 0; i < rtot[o]; i++) lo = std::min(lo, pd[rm + i]), hi = std::max(hi, pd[rm + i]);
             int64_t klo = 0, khi = 0;
             for (int64_t i = 0; i < k; i++) klo = std::min(klo, pd[km + i]), khi = std::max(khi, pd[km + i]);
-            if (lo < 0 || klo < 0 || (hi + khi + 1) * G.elem_bytes >= ((int64_t)1 << 31)) continue;
-            const int64_t rt = pool.group_table(rm, 0, rtot[o], G.gr);
+            if (lo < 0 || klo < 0 || (hi + khi + 1) * src_elem >= ((int64_t)1 << 31)) continue;
             const int64_t kt = pool.group_table(km, 0, k, G.gk);
-            // rows-fast (unit-stride row groups, any affine k groups; re-read
-            // by <= direct_rows_rereads() tiles) or k-fast (affine row groups,
-            // unit-stride k groups; re-read by <= the k-fast limit, <= 4 MiB)
-            bool rows_unit = true, rows_affine = true, k_unit = true, k_affine = true;
-            pd = pool.data().data();
-            for (int64_t gi = 0; gi < ceil_div(rtot[o], G.gr); gi++)
+            if (!general)
             {
-                const int64_t st = pd[rt + 2 * gi + 1];
-                const bool single = std::min<int64_t>(G.gr, rtot[o] - gi * G.gr) == 1;
-                rows_affine &= st != SCATTERED;
-                rows_unit &= st == 1 || (st == 0 && single);
+                const int64_t rt = pool.group_table(rm, 0, rtot[o], G.gr);
+                bool rows_unit = true, rows_affine = true, k_unit = true, k_affine = true;
+                pd = pool.data().data();
+                for (int64_t gi = 0; gi < ceil_div(rtot[o], G.gr); gi++)
+                {
+                    const int64_t st = pd[rt + 2 * gi + 1];
+                    rows_affine &= st != SCATTERED;
+                    rows_unit &= st == 1;
+                }
+                for (int64_t gi = 0; gi < ceil_div(k, G.gk); gi++)
+                {
+                    const int64_t st = pd[kt + 2 * gi + 1];
+                    k_affine &= st != SCATTERED;
+                    k_unit &= st == 1;
+                }
+                // k-fast reads use 32-byte pieces of each row: only worth it
+                // while the re-reads hit in L2 (4 MiB per XCD)
+                const int64_t span = (hi + khi + 1) * src_elem;
+                const bool ok = (rows_unit && k_affine && rereads <= direct_rows_rereads()) ||
+                                (rows_affine && k_unit && rereads <= direct_kfast_rereads() &&
+                                 span <= ((int64_t)4 << 20));
+                if (!ok) continue;
             }
-            for (int64_t gi = 0; gi < ceil_div(k, G.gk); gi++)
-            {
-                const int64_t st = pd[kt + 2 * gi + 1];
-                const bool single = std::min<int64_t>(G.gk, k - gi * G.gk) == 1;
-                k_affine &= st != SCATTERED;
-                k_unit &= st == 1 || (st == 0 && single);
-            }
-            // k-fast reads use 32-byte pieces of each row: only worth it
-            // while the re-reads hit in L2 (4 MiB per XCD)
-            const int64_t span = (hi + khi + 1) * G.elem_bytes;
-            const bool ok = (rows_unit && k_affine && rereads <= direct_rows_rereads()) ||
-                            (rows_affine && k_unit && rereads <= direct_kfast_rereads() &&
-                             span <= ((int64_t)4 << 20));
-            if (!ok) continue;
-            // copies of both tables extended with entries for the padding
-            // groups (their lanes are never valid)
-            auto extend = [&](int64_t tab, int64_t have, int64_t want) {
-                const int64_t at = (int64_t)pool.data().size();
-                pool.data().resize(pool.data().size() + 2 * want, 0);
-                std::copy(pool.data().begin() + tab, pool.data().begin() + tab + 2 * have,
-                          pool.data().begin() + at);
-                return at;
-            };
-            const int64_t rpadded = o == 0 ? mp : np;
+            // the k group table extended with entries for the padding groups
+            // (their lanes are never valid)
+            const int64_t have = ceil_div(k, G.gk);
+            const int64_t at = (int64_t)pool.data().size();
+            pool.data().resize(pool.data().size() + 2 * nkg, 0);
+            std::copy(pool.data().begin() + kt, pool.data().begin() + kt + 2 * have, pool.data().begin() + at);
             dir[o] = true;
-            drt[o] = extend(rt, ceil_div(rtot[o], G.gr), rpadded / G.gr);
-            dkt[o] = extend(kt, ceil_div(k, G.gk), nkg);
+            drt[o] = rm;
+            dkt[o] = at;
+            dkm[o] = km;
             dsrc[o] = blk.base;
-            dbytes[o] = (uint32_t)((hi + khi + 1) * G.elem_bytes);
+            dbytes[o] = (uint32_t)((hi + khi + 1) * src_elem);
         }
 
         // panels
@@ -807,10 +826,12 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                 t.klen = (int32_t)k;
                 t.a_src = dir[0] ? dsrc[0] : PANEL_MISSING;
                 t.b_src = dir[1] ? dsrc[1] : PANEL_MISSING;
-                t.a_rt = ozaki ? ea + tm * BM : drt[0] + 2 * tm * (BM / G.gr);
-                t.b_rt = ozaki ? eb + tn * BN : drt[1] + 2 * tn * (BN / G.gr);
+                t.a_rt = ozaki ? ea + tm * BM : drt[0] + tm * BM;
+                t.b_rt = ozaki ? eb + tn * BN : drt[1] + tn * BN;
                 t.a_kt = dkt[0];
                 t.b_kt = dkt[1];
+                t.a_km = dkm[0];
+                t.b_km = dkm[1];
                 t.a_bytes = dbytes[0];
                 t.b_bytes = dbytes[1];
                 t.nkg = (int32_t)nkg;
@@ -971,9 +992,17 @@ std::string layout_sig(const std::vector<PanelGemm>& gemms)
         {
             sig += bl->size() == 1 ? "|" : "|m";
             if (bl->size() != 1) continue;
-            for (int64_t x : (*bl)[0].rows.stride) sig += std::to_string(x) + ",";
-            sig += "/";
-            for (int64_t x : (*bl)[0].k.stride) sig += std::to_string(x) + ",";
+            for (const Side* sd : {&(*bl)[0].rows, &(*bl)[0].k})
+            {
+                for (int64_t x : sd->stride) sig += std::to_string(x) + ",";
+                if (!sd->offs.empty())
+                {
+                    uint64_t h = 1469598103934665603ULL;
+                    for (int64_t x : sd->offs) h = (h ^ (uint64_t)x) * 1099511628211ULL;
+                    sig += "x" + std::to_string(h);
+                }
+                sig += "/";
+            }
         }
     return sig;
 }
@@ -991,7 +1020,7 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bo
     }
     // time candidates as they will run when the tensors can be read (FP64)
     const TuneOperands ops = t_tune_ops;
-    const bool real = family == FAMILY_F64 && ops.A && ops.B && direct_enabled();
+    const bool real = family != FAMILY_OZAKI && ops.A && ops.B && direct_enabled();
     const void* opA = swap ? ops.B : ops.A;
     const void* opB = swap ? ops.A : ops.B;
     uint64_t h = shape_hash(gemms, family);
@@ -1010,8 +1039,8 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bo
         // with tensors: grouped as the plan will be; without: both operands
         // in panels
         const bool none[2] = {false, false};
-        if (real ? !tables_uniform(gemms, family, cands[i], tabs[i])
-                 : !tables_for_tile(gemms, family, cands[i], tabs[i], none))
+        if (real ? !tables_uniform(gemms, family, cands[i], tabs[i], dtype == 0 ? 8 : 4)
+                 : !tables_for_tile(gemms, family, cands[i], tabs[i], none, dtype == 0 ? 8 : 4))
             return -1;
         panel_bytes = std::max(panel_bytes, (size_t)tabs[i].ws_panel * eb + 256);
         cslot = std::max(cslot, (size_t)panel_tile_m(family, cands[i]) * panel_tile_n(family, cands[i]));

@@ -129,12 +129,12 @@ struct PackChunk
 };
 
 // One C tile of one panel GEMM.  An operand is either read from its panel
-// (a, b) or, when its source block is already MFMA-friendly (unit-stride
-// 16-row groups, affine 8-k groups), straight from the tensor ("direct"):
-// through a buffer resource at tensor + x_src spanning x_bytes bytes, lane
-// offsets built from the row-group and k-group (base, stride) tables;
-// lanes past the valid rows or k read offset 0xffffffff, which the buffer
-// bounds check returns as zero.
+// (a, b) or straight from the tensor ("direct"): through a buffer resource
+// at tensor + x_src spanning x_bytes bytes; each lane takes its row offsets
+// from the explicit row offsets once per task, and per 8-k group adds the
+// group's (base + lane k * stride) or, for a group that is not affine
+// (stride SCATTERED), its own k offset; lanes past the valid rows or k read
+// offset 0xffffffff, which the buffer bounds check returns as zero.
 struct PanelTask
 {
     int64_t a, b;        // workspace offsets (values) of the tile's first row block of A', B'
@@ -151,10 +151,11 @@ struct PanelTask
     int32_t c_ld;
     // direct operands (FP64 family)
     int64_t a_src, b_src;  // element offset of the block in its tensor
-    // pool index of the tile's first row-group (base, stride); in the Ozaki
-    // family the row-maximum array index of the tile's first row of A' / B'
+    // pool index of the offset of the tile's first row; in the Ozaki family
+    // the row-maximum array index of the tile's first row of A' / B'
     int64_t a_rt, b_rt;
     int64_t a_kt, b_kt;    // pool index of k group 0's (base, stride)
+    int64_t a_km, b_km;    // pool index of the per-element k offsets
     uint32_t a_bytes, b_bytes;  // extent of the block from x_src
     int32_t klen;          // valid k
     int32_t pad_;

@@ -251,9 +251,13 @@ void tblis_init_dpd_tensor_s(tblis_dpd_tensor* t, float scalar, unsigned irrep, 
 /* Number of stored elements of a DPD tensor (dpd_varray_base::size). */
 stride_type tblis_dpd_tensor_size(const tblis_dpd_tensor* t);
 
-/* DPD contraction; same semantics as the dense call, block by block.  All
- * symmetry-allowed irrep blocks of every irrep_AB run in one persistent
- * launch. */
+/* DPD contraction; same semantics as the dense call, block by block.  Each
+ * contracted irrep is one GEMM over its symmetry-allowed blocks; when every
+ * block address splits into a row part and a column part (the six MArray
+ * layouts for the usual label orders), the GEMMs of all irreps run as one
+ * persistent GEMM launch that reads A and B and writes C in their DPD
+ * storage.  Otherwise the blocks are packed first (one pack launch) and C
+ * blocks are written from the GEMM epilogue. */
 void tblis_dpd_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
                            const tblis_dpd_tensor* A, const label_type* idx_A,
                            const tblis_dpd_tensor* B, const label_type* idx_B,
@@ -348,9 +352,12 @@ void tblis_indexed_dpd_tensor_mult(const tblis_comm* comm, const tblis_config* c
                                    tblis_indexed_dpd_tensor* C, const label_type* idx_C);
 
 /* ---------------------------------------------------------------------- */
-/* Extension: a batch of independent contractions of uneven shapes, run   */
-/* in one persistent launch.  The reference has no counterpart; its        */
-/* equivalent is a host loop of tblis_tensor_mult calls.                   */
+/* Extension: a batch of independent contractions of uneven shapes.  With */
+/* FP64 or split-BF16 every instance's operands are read straight from    */
+/* the tensors by one persistent GEMM launch over the tiles of all        */
+/* instances (operands beyond a buffer resource's 2 GiB reach are packed  */
+/* by one pack launch first).  The reference has no counterpart; its      */
+/* equivalent is a host loop of tblis_tensor_mult calls.                  */
 /* ---------------------------------------------------------------------- */
 void tblis_tensor_mult_batch(const tblis_comm* comm, const tblis_config* cfg, len_type count,
                              const tblis_tensor* const* A, const label_type* const* idx_A,
@@ -412,6 +419,16 @@ void tblis_hip_debug_fill_block_scatter(int ndim, const len_type* len, const str
                                         len_type BS, len_type off, len_type size,
                                         stride_type* scat, stride_type* bs, int pack_3d,
                                         int elem_size);
+
+/* The panel planner's tables of one matrix side given as (len, stride)
+ * lists: its element offsets in odometer order (first dimension fastest;
+ * the reference's plain scatter vector) into offs, and the (base, stride)
+ * pair of every fragment group of size G with the side placed at panel row
+ * r0 into tab (stride INT64_MIN where a group is not affine; a group of one
+ * element has stride 1, as the reference's block strides give it).
+ * Returns the number of offsets, -2 if max_out is too small, -1 on error. */
+int64_t tblis_hip_debug_panel_side(int ndim, const len_type* len, const stride_type* stride, len_type G,
+                                   len_type r0, stride_type* offs, stride_type* tab, int64_t max_out);
 
 /* DPD block address (offset from the data pointer, lengths, strides) of the
  * block with the given irreps, from this library's DPD layout code. */

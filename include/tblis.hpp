@@ -96,8 +96,11 @@ struct type_tag<double>
     static constexpr type_t value = TYPE_DOUBLE;
 };
 
+// The descriptor refers to the view's own length and stride arrays, so the
+// view must outlive the call.
 template <typename T>
-tblis_tensor make_tensor(T scalar, const varray_view<T>& v)
+tblis_tensor make_tensor_of(T scalar, const void* data, const std::vector<len_type>& len,
+                            const std::vector<stride_type>& stride)
 {
     tblis_tensor t;
     t.type = type_tag<T>::value;
@@ -107,17 +110,23 @@ tblis_tensor make_tensor(T scalar, const varray_view<T>& v)
         t.scalar.data.s = (float)scalar;
     else
         t.scalar.data.d = (double)scalar;
-    t.data = (void*)v.data;
-    t.ndim = (unsigned)v.len.size();
-    t.len = const_cast<len_type*>(v.len.data());
-    t.stride = const_cast<stride_type*>(v.stride.data());
+    t.data = const_cast<void*>(data);
+    t.ndim = (unsigned)len.size();
+    t.len = const_cast<len_type*>(len.data());
+    t.stride = const_cast<stride_type*>(stride.data());
     return t;
+}
+
+template <typename T>
+tblis_tensor make_tensor(T scalar, const varray_view<T>& v)
+{
+    return make_tensor_of<T>(scalar, v.data, v.len, v.stride);
 }
 
 template <typename T>
 tblis_tensor make_tensor(T scalar, const varray_view<const T>& v)
 {
-    return make_tensor(scalar, varray_view<T>{const_cast<T*>(v.data), v.len, v.stride});
+    return make_tensor_of<T>(scalar, v.data, v.len, v.stride);
 }
 
 template <typename T>

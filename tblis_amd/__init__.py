@@ -92,7 +92,8 @@ EXPORTED = [
     "tblis_hip_get_stream", "tblis_hip_set_async", "tblis_hip_set_error_mode",
     "tblis_hip_get_last_error", "tblis_hip_clear_plan_cache", "tblis_hip_last_algorithm",
     "tblis_hip_set_autotune", "tblis_hip_autotune_runs", "tblis_hip_last_tile",
-    "tblis_hip_debug_canonical", "tblis_hip_debug_fill_block_scatter", "tblis_hip_debug_dpd_block",
+    "tblis_hip_debug_canonical", "tblis_hip_debug_fill_block_scatter", "tblis_hip_debug_panel_side",
+    "tblis_hip_debug_dpd_block",
     "tblis_hip_debug_dpd_patches", "tblis_hip_debug_mfma_probe", "tblis_hip_debug_mfma_probe_bf16",
 ]
 
@@ -124,6 +125,9 @@ def lib():
         L.tblis_tensor_mult_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_ssize_t, P(T),
                                               P(C.c_char_p), P(T), P(C.c_char_p), P(T),
                                               P(C.c_char_p)]
+        L.tblis_hip_debug_panel_side.restype = C.c_int64
+        L.tblis_hip_debug_panel_side.argtypes = [C.c_int, P(C.c_ssize_t), P(C.c_ssize_t), C.c_ssize_t,
+                                                 C.c_ssize_t, P(C.c_ssize_t), P(C.c_ssize_t), C.c_int64]
         L.tblis_get_config.restype = C.c_void_p
         L.tblis_get_config.argtypes = [C.c_char_p]
         L.tblis_hip_get_last_error.restype = C.c_char_p

@@ -413,6 +413,25 @@ void tblis_hip_debug_fill_block_scatter(int ndim, const len_type* len, const str
     });
 }
 
+int64_t tblis_hip_debug_panel_side(int ndim, const len_type* len, const stride_type* stride, len_type G,
+                                   len_type r0, stride_type* offs, stride_type* tab, int64_t max_out)
+{
+    return guarded(
+        [&]() -> int64_t {
+            if (ndim < 0 || (ndim > 0 && (!len || !stride)) || G < 1 || r0 < 0) fail("invalid side");
+            Side s;
+            s.len.assign(len, len + ndim);
+            s.stride.assign(stride, stride + ndim);
+            std::vector<int64_t> o, t;
+            debug_side_tables(s, G, r0, o, t);
+            if ((int64_t)o.size() > max_out || (int64_t)t.size() > max_out) return -2;
+            std::copy(o.begin(), o.end(), offs);
+            std::copy(t.begin(), t.end(), tab);
+            return (int64_t)o.size();
+        },
+        (int64_t)-1);
+}
+
 void tblis_hip_debug_dpd_block(const tblis_dpd_tensor* t, const unsigned* irreps, len_type* len,
                                stride_type* offset, stride_type* stride)
 {

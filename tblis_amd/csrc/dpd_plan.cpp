@@ -428,6 +428,7 @@ struct BlockRef
 // (0, q), both shifted so that neither has a negative entry.
 bool merge_blocks(const std::vector<BlockRef>& b, size_t NP, size_t NQ, Side& R, Side& C)
 {
+    if (NP == 0 || NQ == 0 || b.size() != NP * NQ) return false;
     auto same = [](const Side& x, const Side& y) {
         return x.len == y.len && x.stride == y.stride && x.offs == y.offs;
     };
@@ -463,7 +464,11 @@ void merge_irrep_gemm(PanelGemm& g)
     if (const char* e = std::getenv("TBLIS_HIP_DPD_MERGE"))
         if (e[0] == '0') return;
     const size_t NM = g.mlen.size(), NN = g.nlen.size(), NK = g.klen.size();
+    // no k blocks (every contracted extent of this irrep is empty): C <- beta C
+    // over the C blocks as they are
+    if (NM == 0 || NN == 0 || NK == 0) return;
     if (NM * NK != g.a.size() || NN * NK != g.b.size() || NM * NN != g.c.size()) return;
+    if (!g.kscale.empty()) return;  // per-block scales need the blocks
     std::vector<BlockRef> ra, rb, rc;
     for (const PBlock& x : g.a) ra.push_back(BlockRef{x.base, &x.rows, &x.k});
     for (const PBlock& x : g.b) rb.push_back(BlockRef{x.base, &x.rows, &x.k});
@@ -488,7 +493,6 @@ void merge_irrep_gemm(PanelGemm& g)
     out.a = {A};
     out.b = {B};
     out.c = {Cm};
-    if (!g.kscale.empty()) return;  // per-block scales need the blocks
     g = std::move(out);
 }
 

@@ -637,25 +637,31 @@ int persistent_grid(const void* kernel, int block, int ntasks)
     return slots >= 8 && ntasks > slots ? slots : ntasks;
 }
 
-// The Ozaki family has one tile: 64 x 64 per workgroup, 2 x 2 waves.
+// The Ozaki family has one tile: 64 x 64 per workgroup, 2 x 2 waves.  The
+// FP64 family's tiles are the register-fed ones, then the LDS-staged ones
+// (gemm_lds.hip, four waves each).
 int panel_num_tiles(int family)
 {
     if (family == FAMILY_OZAKI) return 1;
-    return family == FAMILY_F64 ? kNumTiles : panel_bf16_num_tiles();
+    return family == FAMILY_F64 ? kNumTiles + lds_num_tiles() : panel_bf16_num_tiles();
 }
+bool panel_tile_lds(int family, int t) { return family == FAMILY_F64 && t >= kNumTiles; }
 int panel_tile_m(int family, int t)
 {
     if (family == FAMILY_OZAKI) return 64;
+    if (panel_tile_lds(family, t)) return lds_tile_m(t - kNumTiles);
     return family == FAMILY_F64 ? kTiles[t].bm : panel_bf16_tile_m(t);
 }
 int panel_tile_n(int family, int t)
 {
     if (family == FAMILY_OZAKI) return 64;
+    if (panel_tile_lds(family, t)) return lds_tile_n(t - kNumTiles);
     return family == FAMILY_F64 ? kTiles[t].bn : panel_bf16_tile_n(t);
 }
 int panel_tile_waves(int family, int t)
 {
     if (family == FAMILY_OZAKI) return 4;
+    if (panel_tile_lds(family, t)) return 4;
     return family == FAMILY_F64 ? kTiles[t].wgm * kTiles[t].wgn * kTiles[t].ks : panel_bf16_tile_waves(t);
 }
 
@@ -676,7 +682,8 @@ hipError_t launch_panel_gemm(const PanelLaunch& L)
 {
     if (L.ntasks == 0) return hipSuccess;
     if (L.family != FAMILY_F64) return launch_panel_gemm_bf16(L);
-    if (L.tile < 0 || L.tile >= kNumTiles) return hipErrorInvalidValue;
+    if (L.tile >= kNumTiles) return launch_gemm_lds(L, L.tile - kNumTiles);
+    if (L.tile < 0) return hipErrorInvalidValue;
     return L.dtype == 0 ? launch_gemm_d<double>(L) : launch_gemm_d<float>(L);
 }
 

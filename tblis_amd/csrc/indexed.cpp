@@ -235,7 +235,7 @@ std::shared_ptr<IndexedPlan> build(const Ix& A, const Ix& B, const Ix& C, int es
             std::memcpy(&bits, &f, sizeof(double));
             tab.push_back(bits);
         }
-        hip_ok(hipMalloc(&ip->d_tab, tab.size() * sizeof(int64_t)), "hipMalloc (indexed table)");
+        hip_ok(device_alloc((void**)&ip->d_tab, tab.size() * sizeof(int64_t)), "hipMalloc (indexed table)");
         hip_ok(hipMemcpy(ip->d_tab, tab.data(), tab.size() * sizeof(int64_t), hipMemcpyHostToDevice),
                "hipMemcpy (indexed table)");
     };

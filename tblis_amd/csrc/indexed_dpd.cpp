@@ -210,7 +210,7 @@ void* scratch(size_t bytes, hipStream_t s)
     if (e.second < bytes)
     {
         void* p = nullptr;
-        hip_ok(hipMalloc(&p, bytes), "hipMalloc (indexed DPD scratch)");
+        hip_ok(device_alloc((void**)&p, bytes), "hipMalloc (indexed DPD scratch)");
         e = {p, bytes};
     }
     return e.first;
@@ -317,7 +317,7 @@ void indexed_dpd_tensor_mult(const tblis_config* cfg, const tblis_indexed_dpd_te
         tb->nw = (int64_t)all.size() - tb->na - tb->nb - tb->nc;
         if (!all.empty())
         {
-            hip_ok(hipMalloc((void**)&tb->d, all.size() * sizeof(BlockCopy)), "hipMalloc (indexed DPD tables)");
+            hip_ok(device_alloc((void**)&tb->d, all.size() * sizeof(BlockCopy)), "hipMalloc (indexed DPD tables)");
             hip_ok(hipMemcpy(tb->d, all.data(), all.size() * sizeof(BlockCopy), hipMemcpyHostToDevice),
                    "hipMemcpy (indexed DPD tables)");
         }

@@ -50,6 +50,9 @@ int panel_num_tiles(int family);
 int panel_tile_m(int family, int tile);
 int panel_tile_n(int family, int tile);
 int panel_tile_waves(int family, int tile);
+// FP64 tiles that stage both operands through LDS (every operand read from
+// the tensors, single-block GEMMs only)
+bool panel_tile_lds(int family, int tile);
 struct PanelLaunch
 {
     const PackChunk* chunks;
@@ -85,6 +88,12 @@ int panel_bf16_tile_n(int tile);
 int panel_bf16_tile_waves(int tile);
 bool panel_bf16_tile_fast_only(int tile);
 bool panel_bf16_tile_direct(int tile);  // has variants reading FP32 operands directly
+// FP64 GEMM reading both operands from the tensors through LDS
+// (gemm_lds.hip); its tiles follow the register-fed FP64 tiles.
+int lds_num_tiles();
+int lds_tile_m(int lt);
+int lds_tile_n(int lt);
+hipError_t launch_gemm_lds(const PanelLaunch& L, int lt);
 // Workgroups of a persistent launch of `kernel` over ntasks tasks (kernels
 // loop over tasks with a stride of the grid size).
 int persistent_grid(const void* kernel, int block, int ntasks);

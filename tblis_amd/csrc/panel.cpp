@@ -238,7 +238,8 @@ struct TileModel
 // Per-wave efficiency by wave tile.  FP64: 64x64 waves are 4x4 MFMA
 // blocks, 32x32 waves 2x2.  Split-BF16: per tile of the BF16 table.
 const TileModel kModelsF64[] = {{0, 0.85, 2}, {2, 0.80, 2}, {3, 0.82, 2}, {4, 0.82, 2},
-                                {1, 0.50, 6}, {5, 0.45, 6}, {6, 0.50, 4}, {7, 0.45, 4}};
+                                {1, 0.50, 6}, {5, 0.45, 6}, {6, 0.50, 4}, {7, 0.45, 4},
+                                {8, 0.88, 1}, {9, 0.55, 2}, {10, 0.80, 1}};
 const TileModel kModelsBf16[] = {{0, 0.55, 1}, {1, 0.55, 1}, {2, 0.45, 2}, {3, 0.35, 4}};
 const TileModel kModelsOzaki[] = {{0, 0.30, 1}};
 
@@ -273,6 +274,18 @@ double tile_cost(const std::vector<PanelGemm>& gemms, const TileModel& tm, int c
 }
 
 }  // namespace
+
+void debug_side_tables(const Side& s, int64_t G, int64_t r0, std::vector<int64_t>& offs,
+                       std::vector<int64_t>& tab)
+{
+    Pool pool;
+    const int64_t at = pool.offsets(s);
+    const int64_t n = side_len(s);
+    offs.assign(pool.data().begin() + at, pool.data().begin() + at + n);
+    const int64_t t = pool.group_table(at, r0, n, G);
+    const int64_t ng = ceil_div(r0 + n, G) - r0 / G;
+    tab.assign(pool.data().begin() + t, pool.data().begin() + t + 2 * ng);
+}
 
 PanelGemm transpose_gemm(const PanelGemm& g)
 {
@@ -414,8 +427,12 @@ struct TuneOperands
     const void* B = nullptr;
 };
 thread_local TuneOperands t_tune_ops;
+thread_local bool t_tune_off = false;  // AutotuneOffScope
 
 }  // namespace
+
+AutotuneOffScope::AutotuneOffScope() : was_(t_tune_off) { t_tune_off = true; }
+AutotuneOffScope::~AutotuneOffScope() { t_tune_off = was_; }
 
 TuneOperandsScope::TuneOperandsScope(const void* A, const void* B) { t_tune_ops = TuneOperands{A, B}; }
 TuneOperandsScope::~TuneOperandsScope() { t_tune_ops = TuneOperands{}; }
@@ -446,12 +463,13 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family,
     int tile = 0;
     double best = -1;
     const TileModel* models = family == FAMILY_F64 ? kModelsF64 : family == FAMILY_OZAKI ? kModelsOzaki : kModelsBf16;
-    const int nmodels = family == FAMILY_F64 ? 8 : family == FAMILY_OZAKI ? 1 : 4;
+    const int nmodels = family == FAMILY_F64 ? 11 : family == FAMILY_OZAKI ? 1 : 4;
     std::vector<int> cands;
     for (int mi = 0; mi < nmodels; mi++)
     {
         const TileModel& tm = models[mi];
         if (family == FAMILY_BF16X3 && panel_bf16_tile_fast_only(tm.tile)) continue;
+        if (panel_tile_lds(family, tm.tile) && !lds_applicable(gemms, dtype == 0 ? 8 : 4)) continue;
         cands.push_back(tm.tile);
         double c = tile_cost(gemms, tm, cus, family);
         if (best < 0 || c < best)
@@ -466,7 +484,8 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family,
     {
         int t = std::atoi(e);
         if (e[0] >= '0' && e[0] <= '9' && t < panel_num_tiles(family) &&
-            !(family == FAMILY_BF16X3 && panel_bf16_tile_fast_only(t)))
+            !(family == FAMILY_BF16X3 && panel_bf16_tile_fast_only(t)) &&
+            !(panel_tile_lds(family, t) && !lds_applicable(gemms, dtype == 0 ? 8 : 4)))
             tile = t;
     }
     if (!tables_uniform(gemms, family, tile, T, dtype == 0 ? 8 : 4)) return false;
@@ -503,6 +522,53 @@ bool tables_uniform(const std::vector<PanelGemm>& gemms, int family, int tile, P
     return true;
 }
 
+// Lowest and highest element offset of a side.
+void side_range(const Side& s, int64_t& lo, int64_t& hi)
+{
+    lo = hi = 0;
+    if (!s.offs.empty())
+    {
+        lo = *std::min_element(s.offs.begin(), s.offs.end());
+        hi = *std::max_element(s.offs.begin(), s.offs.end());
+        return;
+    }
+    for (size_t d = 0; d < s.len.size(); d++)
+    {
+        if (s.len[d] == 0) return;
+        const int64_t span = (s.len[d] - 1) * s.stride[d];
+        (span < 0 ? lo : hi) += span;
+    }
+}
+
+}  // namespace
+
+// Can the LDS-staged tiles run these GEMMs: single A, B and C blocks,
+// unscaled, both operands within a buffer resource's reach (non-negative
+// offsets, under 2 GiB from the block's base).
+bool lds_applicable(const std::vector<PanelGemm>& gemms, int src_elem)
+{
+    for (const PanelGemm& g : gemms)
+    {
+        if (g.mlen.size() != 1 || g.nlen.size() != 1 || g.klen.size() > 1 || g.c.size() != 1) return false;
+        if (g.a.size() != g.klen.size() || g.b.size() != g.klen.size()) return false;
+        for (double x : g.kscale)
+            if (x != 1.0) return false;
+        for (const std::vector<PBlock>* bl : {&g.a, &g.b})
+            for (const PBlock& b : *bl)
+            {
+                if (b.base == PANEL_MISSING) return false;
+                int64_t rlo, rhi, klo, khi;
+                side_range(b.rows, rlo, rhi);
+                side_range(b.k, klo, khi);
+                if (rlo < 0 || klo < 0 || (rhi + khi + 1) * src_elem >= ((int64_t)1 << 31)) return false;
+            }
+    }
+    return true;
+}
+
+namespace
+{
+
 // Many GEMMs (DPD irrep GEMMs, batch instances) read every operand that can
 // be straight from the tensors, so that a call is one GEMM launch; one GEMM
 // reads an operand directly only where that beats packing it.
@@ -519,7 +585,10 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                      const bool allow_direct[2], int src_elem)
 {
     const Geom G = geom_of(family);
-    const bool general = direct_all(gemms);
+    // LDS-staged tiles read both operands from the tensors (lds_applicable
+    // has checked that they can)
+    const bool lds = panel_tile_lds(family, tile);
+    const bool general = direct_all(gemms) || lds;
     const int64_t BM = panel_tile_m(family, tile), BN = panel_tile_n(family, tile);
 
     Pool pool;
@@ -613,6 +682,7 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
         const int64_t rtot[2] = {m, n};
         bool dir[2] = {false, false};
         int64_t drt[2] = {0, 0}, dkt[2] = {0, 0}, dkm[2] = {0, 0}, dsrc[2] = {0, 0};
+        int32_t dkf[2] = {0, 0};
         uint32_t dbytes[2] = {0, 0};
         // split-BF16 reads FP32 operands directly (split in registers) on
         // the tiles that have such variants, for many GEMMs only
@@ -633,6 +703,16 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
             for (int64_t i = 0; i < k; i++) klo = std::min(klo, pd[km + i]), khi = std::max(khi, pd[km + i]);
             if (lo < 0 || klo < 0 || (hi + khi + 1) * src_elem >= ((int64_t)1 << 31)) continue;
             const int64_t kt = pool.group_table(km, 0, k, G.gk);
+            {
+                // lanes of the LDS-staged loads run along k when more k
+                // groups than row groups are unit-stride
+                const int64_t rt = pool.group_table(rm, 0, rtot[o], G.gr);
+                const int64_t* q = pool.data().data();
+                int64_t ru = 0, ku = 0;
+                for (int64_t gi = 0; gi < ceil_div(rtot[o], G.gr); gi++) ru += q[rt + 2 * gi + 1] == 1;
+                for (int64_t gi = 0; gi < ceil_div(k, G.gk); gi++) ku += q[kt + 2 * gi + 1] == 1;
+                dkf[o] = ku * ceil_div(rtot[o], G.gr) > ru * ceil_div(k, G.gk) ? 1 : 0;
+            }
             if (!general)
             {
                 const int64_t rt = pool.group_table(rm, 0, rtot[o], G.gr);
@@ -671,6 +751,8 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
             dsrc[o] = blk.base;
             dbytes[o] = (uint32_t)((hi + khi + 1) * src_elem);
         }
+
+        if (lds && !(dir[0] && dir[1])) fail("internal: LDS-staged tile without both operands read directly");
 
         // panels
         const int64_t oa = ws;
@@ -832,6 +914,7 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                 t.b_kt = dkt[1];
                 t.a_km = dkm[0];
                 t.b_km = dkm[1];
+                t.pad_ = dkf[0] | (dkf[1] << 1);  // LDS-staged loads: lanes along k
                 t.a_bytes = dbytes[0];
                 t.b_bytes = dbytes[1];
                 t.nkg = (int32_t)nkg;
@@ -1010,7 +1093,7 @@ std::string layout_sig(const std::vector<PanelGemm>& gemms)
 int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bool swap,
                   const std::vector<int>& cands)
 {
-    if (!autotune_on() || cands.size() < 2) return -1;
+    if (t_tune_off || !autotune_on() || cands.size() < 2) return -1;
     hipStream_t s = thread_state().stream;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
@@ -1061,7 +1144,7 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bo
         if (e1) (void)hipEventDestroy(e1);
         (void)hipGetLastError();
     };
-    bool ok = hipMalloc(&panels, panel_bytes) == hipSuccess && hipMalloc(&cbuf, SLOTS * cslot * 8) == hipSuccess &&
+    bool ok = device_alloc((void**)&panels, panel_bytes) == hipSuccess && device_alloc((void**)&cbuf, SLOTS * cslot * 8) == hipSuccess &&
               launch_fill_random(panels, (int64_t)(panel_bytes / eb), family != FAMILY_F64, s) == hipSuccess &&
               hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
     std::vector<PanelTask> tasks;
@@ -1074,7 +1157,7 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bo
             tasks[k].c_ld = BN;
             tasks[k].c_src = (int64_t)(k % SLOTS) * BM * BN;
         }
-        ok = hipMalloc(&dtasks[i], tasks.size() * sizeof(PanelTask)) == hipSuccess &&
+        ok = device_alloc((void**)&dtasks[i], tasks.size() * sizeof(PanelTask)) == hipSuccess &&
              hipMemcpyAsync(dtasks[i], tasks.data(), tasks.size() * sizeof(PanelTask), hipMemcpyHostToDevice,
                             s) == hipSuccess;
         if (ok && real)
@@ -1082,9 +1165,9 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bo
             // the pack and direct reads use the plan's own tables
             const std::vector<PackChunk> pc = pack_records(tabs[i], family);
             const std::vector<int64_t>& pool = tabs[i].pool;
-            ok = hipMalloc(&dpool[i], std::max<size_t>(pool.size(), 1) * 8) == hipSuccess &&
+            ok = device_alloc((void**)&dpool[i], std::max<size_t>(pool.size(), 1) * 8) == hipSuccess &&
                  hipMemcpyAsync(dpool[i], pool.data(), pool.size() * 8, hipMemcpyHostToDevice, s) == hipSuccess &&
-                 hipMalloc(&dchunks[i], std::max<size_t>(pc.size(), 1) * sizeof(PackChunk)) == hipSuccess &&
+                 device_alloc((void**)&dchunks[i], std::max<size_t>(pc.size(), 1) * sizeof(PackChunk)) == hipSuccess &&
                  hipMemcpyAsync(dchunks[i], pc.data(), pc.size() * sizeof(PackChunk), hipMemcpyHostToDevice,
                                 s) == hipSuccess;
         }
@@ -1298,7 +1381,7 @@ bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, i
     const size_t o_uc = o_uj + al(s_uj), o_pool = o_uc + al(s_uc);
     plan.bytes = o_pool + s_pool;
     hip_check(hipGetDevice(&plan.device), "hipGetDevice");
-    hip_check(hipMalloc(&plan.dev, plan.bytes), "hipMalloc (plan)");
+    hip_check(device_alloc((void**)&plan.dev, plan.bytes), "hipMalloc (plan)");
     char* base = (char*)plan.dev;
     plan.d_pchunks = (PackChunk*)(base + o_chunks);
     plan.d_ptasks = (PanelTask*)(base + o_tasks);

@@ -37,6 +37,12 @@ struct PanelGemm
 // C^T = B^T A^T.
 PanelGemm transpose_gemm(const PanelGemm& g);
 
+// The planner's offset vector of one side (odometer order, first dimension
+// fastest) and its (base, stride) table at fragment-group size G for the
+// side placed at panel row r0 (introspection for the parity tests).
+void debug_side_tables(const Side& s, int64_t G, int64_t r0, std::vector<int64_t>& offs,
+                       std::vector<int64_t>& tab);
+
 // One PanelGemm per batch element of a GemmDesc (one C patch with its k
 // segments).
 void panel_from_desc(const GemmDesc& g, std::vector<PanelGemm>& out);
@@ -79,6 +85,10 @@ struct PanelTables
     int64_t useful_flops = 0, mfma_flops = 0;
 };
 
+// Can the LDS-staged FP64 tiles run these GEMMs (single blocks, operands in
+// reach of a buffer resource)?  src_elem: bytes per tensor element.
+bool lds_applicable(const std::vector<PanelGemm>& gemms, int src_elem);
+
 // Orientation, tile, pack jobs, tasks and C unpack jobs (host only).
 // Returns false when there is nothing to launch.  dtype: 0 double, 1 float.
 bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family, int dtype);
@@ -97,6 +107,17 @@ struct TuneOperandsScope
 {
     TuneOperandsScope(const void* A, const void* B);
     ~TuneOperandsScope();
+};
+
+// While one of these lives on a thread, plans built there keep the cost
+// model's tile (the autotuner allocates device memory for its timing runs).
+struct AutotuneOffScope
+{
+    AutotuneOffScope();
+    ~AutotuneOffScope();
+
+  private:
+    bool was_;
 };
 
 // Autotuner switch (initially TBLIS_HIP_AUTOTUNE) and the number of tuning

@@ -1,4 +1,5 @@
 // Planner, plan cache and launch.
+#include <atomic>
 #include "plan.hpp"
 
 #include "panel.hpp"
@@ -42,7 +43,17 @@ void hip_check(hipError_t e, const char* what)
     if (e != hipSuccess) fail(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+std::atomic<int64_t> g_device_allocs{0};
+
 }  // namespace
+
+hipError_t device_alloc(void** p, size_t bytes)
+{
+    g_device_allocs.fetch_add(1);
+    return hipMalloc(p, bytes);
+}
+
+int64_t device_alloc_count() { return g_device_allocs.load(); }
 
 Plan::~Plan()
 {
@@ -385,7 +396,7 @@ std::shared_ptr<Plan> PlanBuilder::build()
     size_t off_p = off_s + (sb + 255) / 256 * 256;
     plan->bytes = off_p + pb;
     hip_check(hipGetDevice(&plan->device), "hipGetDevice");
-    hip_check(hipMalloc(&plan->dev, plan->bytes), "hipMalloc (plan)");
+    hip_check(device_alloc(&plan->dev, plan->bytes), "hipMalloc (plan)");
     char* base = (char*)plan->dev;
     plan->d_tasks = (TaskDesc*)base;
     plan->d_segs = (SegDesc*)(base + off_s);
@@ -551,7 +562,7 @@ void* workspace(size_t bytes, hipStream_t s)
         size_t grow = e.size + e.size / 2;
         size_t want = bytes > grow ? bytes : grow;
         void* p = nullptr;
-        hip_check(hipMalloc(&p, want), "hipMalloc (workspace)");
+        hip_check(device_alloc(&p, want), "hipMalloc (workspace)");
         e.ptr = p;
         e.size = want;
     }
@@ -573,6 +584,25 @@ void pin_plan(const std::shared_ptr<Plan>& p)
 }
 
 }  // namespace
+
+void launch_panel_calls(const PanelLaunch& P, const int64_t group_n[4])
+{
+    if (P.use_k) hip_check(launch_panel_pack(P), "pack launch");
+    // one GEMM launch per group of tasks with the same direct operands
+    int64_t at = 0;
+    for (int g = 0; g < 4; g++)
+    {
+        if (group_n[g] == 0) continue;
+        PanelLaunch G = P;
+        G.tasks = P.tasks + at;
+        G.ntasks = (int)group_n[g];
+        G.direct_a = (g & 1) != 0;
+        G.direct_b = (g & 2) != 0;
+        hip_check(launch_panel_gemm(G), "kernel launch");
+        at += group_n[g];
+    }
+    hip_check(launch_panel_unpack(P), "unpack launch");
+}
 
 void run_plan(const std::shared_ptr<Plan>& pp, const void* A, const void* B, void* C, double alpha,
               double beta, bool use_k)
@@ -633,21 +663,7 @@ void run_plan(const std::shared_ptr<Plan>& pp, const void* A, const void* B, voi
             if (!ts.async && !capturing) hip_check(hipStreamSynchronize(ts.stream), "hipStreamSynchronize");
             return;
         }
-        if (P.use_k) hip_check(launch_panel_pack(P), "pack launch");
-        // one GEMM launch per group of tasks with the same direct operands
-        int64_t at = 0;
-        for (int g = 0; g < 4; g++)
-        {
-            if (p.group_n[g] == 0) continue;
-            PanelLaunch G = P;
-            G.tasks = p.d_ptasks + at;
-            G.ntasks = (int)p.group_n[g];
-            G.direct_a = (g & 1) != 0;
-            G.direct_b = (g & 2) != 0;
-            hip_check(launch_panel_gemm(G), "kernel launch");
-            at += p.group_n[g];
-        }
-        hip_check(launch_panel_unpack(P), "unpack launch");
+        launch_panel_calls(P, p.group_n);
     }
     else if (p.packed && L.use_k)
     {

@@ -148,6 +148,16 @@ void* workspace(size_t bytes, hipStream_t s);
 void run_plan(const std::shared_ptr<Plan>& p, const void* A, const void* B, void* C, double alpha,
               double beta, bool use_k);
 
+// The launches of a non-Ozaki panel plan: pack (when k is used), one GEMM
+// launch per group of tasks with the same directly read operands
+// (group_n[g] tasks from P.tasks on, g = direct A + 2 direct B), unpack.
+void launch_panel_calls(const PanelLaunch& P, const int64_t group_n[4]);
+
+// Every device allocation the library makes goes through this (counted, so
+// that a test can show that a call made none).
+hipError_t device_alloc(void** p, size_t bytes);
+int64_t device_alloc_count();
+
 // Device pointer check for a tensor that will be accessed: the memory must
 // be device-accessible on the current device and, for device allocations,
 // element offsets [lo, hi] (in elements of size elem) must lie inside it.

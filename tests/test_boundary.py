@@ -234,6 +234,56 @@ def _product_scatter(lens, strides, BS, off, size, pack_3d, es):
     return np.array(out[:size]), np.array(bs[:size])
 
 
+def _panel_side(lens, strides, G, r0):
+    n = len(lens)
+    cap = int(np.prod(lens)) + 2 * (int(np.prod(lens)) // G + 3)
+    offs = (C.c_ssize_t * cap)()
+    tab = (C.c_ssize_t * cap)()
+    got = T.lib().tblis_hip_debug_panel_side(n, (C.c_ssize_t * max(n, 1))(*lens),
+                                             (C.c_ssize_t * max(n, 1))(*strides), G, r0, offs, tab, cap)
+    assert got == int(np.prod(lens)), got
+    ng = -(-(r0 + got) // G) - r0 // G
+    return np.array(offs[:got]), np.array(tab[:2 * ng]).reshape(ng, 2)
+
+
+@pytest.mark.parametrize("seed", range(120))
+def test_panel_side_tables_bit_exact(seed):
+    """The tables the default (panel) path builds, against the oracle: the
+    offset vector equals the reference's plain scatter vector
+    (block_scatter_matrix.hpp:68-113), and every fragment group's stride is
+    the reference's block stride at that block size (:232-247, 0 where the
+    group is not affine or has stride 0, 1 for one element) with the group's
+    base at its first element; with the side placed at panel row r0 the
+    groups are cut at multiples of G from the panel's row 0."""
+    rng = np.random.default_rng(5000 + seed)
+    nd = int(rng.integers(1, 5))
+    lens = [int(rng.integers(1, 12)) for _ in range(nd)]
+    strides = [int(rng.choice([0, 1, -3, 7, 64, 4096, int(rng.integers(-50, 5000))])) for _ in range(nd)]
+    n = int(np.prod(lens))
+    G = int(rng.choice([8, 16, 32]))
+    r0 = 0 if seed % 2 == 0 else int(rng.integers(0, 3 * G))
+    offs, tab = _panel_side(lens, strides, G, r0)
+    assert (offs == po.fill_scatter(lens, strides, G, 0, n, False)).all()
+    SC = -(1 << 63)
+    for gi in range(len(tab)):
+        lo = max((r0 // G + gi) * G - r0, 0)
+        hi = min((r0 // G + gi) * G - r0 + G, n)
+        blk = offs[lo:hi]
+        if len(blk) == 1:
+            want = 1
+        else:
+            d = np.diff(blk)
+            want = int(d[0]) if (d == d[0]).all() else SC
+        assert tab[gi, 1] == want, (gi, blk, tab[gi])
+        if want != SC:
+            # base + (element index from the group's first panel row) * stride
+            first = (r0 // G + gi) * G
+            assert tab[gi, 0] + (lo + r0 - first) * want == blk[0]
+        if r0 == 0:
+            ref_bs = po.fill_block_stride(G, offs)[lo]
+            assert ref_bs == (0 if want in (SC, 0) else want)
+
+
 @pytest.mark.parametrize("seed", range(200))
 def test_block_scatter_bit_exact(seed):
     rng = np.random.default_rng(2000 + seed)

@@ -184,6 +184,40 @@ def test_float_through_fp64_mfma():
                 config="fp64_mfma", tol=2e-7)
 
 
+@pytest.mark.parametrize("tile", [8, 9, 10])
+def test_lds_staged_tiles(tile, monkeypatch):
+    """The FP64 tiles that read both operands from the tensors through LDS
+    (forced with TBLIS_HIP_PANEL_TILE): ragged edges, k not a multiple of
+    the 32-k stage, operands with lanes along rows and along k, scattered
+    k groups, broadcast dimensions, FP32 data, beta = 0 over NaN and the
+    alpha = 0 rule."""
+    monkeypatch.setenv("TBLIS_HIP_PANEL_TILE", str(tile))
+    T.lib().tblis_hip_clear_plan_cache()
+    try:
+        rng = np.random.default_rng(12000 + tile)
+        _dense_case(rng, "aebf", "fjei", "aibj", dict(a=21, b=13, e=9, f=7, i=25, j=14), ["col"] * 3,
+                    alpha=0.5, beta=1.0)
+        assert T.last_tile() == tile
+        _dense_case(rng, "abef", "efij", "abij", dict(a=37, b=3, e=70, f=1, i=29, j=2),
+                    ["row", "col", "random"], beta=0.0)
+        _dense_case(rng, "aek", "kbe", "ab", dict(a=130, b=67, e=5, k=41), ["random"] * 3, beta=-2.0)
+        _dense_case(rng, "aebf", "fjei", "aibj", dict(a=12, b=10, e=9, f=11, i=7, j=13), ["col"] * 3,
+                    dtype=np.float32, config="fp64_mfma", tol=2e-7)
+        # beta = 0 over NaN, alpha = 0
+        A = torch.rand(70, 33, dtype=torch.float64, device=DEV)
+        B = torch.rand(33, 50, dtype=torch.float64, device=DEV)
+        Cm = torch.full((70, 50), float("nan"), dtype=torch.float64, device=DEV)
+        T.mult(1.0, A, "ak", B, "kb", 0.0, Cm, "ab")
+        assert torch.allclose(Cm, A @ B, rtol=1e-13, atol=1e-13)
+        C0 = torch.rand(70, 50, dtype=torch.float64, device=DEV)
+        Cm = C0.clone()
+        T.mult(0.0, torch.full_like(A, float("nan")), "ak", B, "kb", 3.0, Cm, "ab")
+        assert torch.equal(Cm, 3.0 * C0)
+    finally:
+        monkeypatch.delenv("TBLIS_HIP_PANEL_TILE")
+        T.lib().tblis_hip_clear_plan_cache()
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_broadcast_stride0_dims(dtype):
     """Operands with a stride-0 (broadcast) dimension, which every element of

@@ -15,7 +15,8 @@
 // offsets (PanelTask).  Loads go through buffer resources bounded by the
 // operand's extent; lanes past the valid rows or k read zero.  The epilogue
 // applies alpha / beta through C's offsets (beta == 0 does not read C,
-// src/kernels/3m/gemm.hpp:54-58).
+// src/kernels/3m/gemm.hpp:54-58), or, for a task with c_ld > 0, stores the
+// unscaled sums into a dense C panel in cws.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -115,8 +116,9 @@ __device__ __forceinline__ void stage_store(double* __restrict__ lds, const doub
 template <int BM, int BN, int WGM, int WGN, typename TC>
 __global__ __launch_bounds__(kNT) void gemm_lds_kernel(const PanelTask* __restrict__ tasks, int ntasks,
                                                        const int64_t* __restrict__ pool, const void* A,
-                                                       const void* B, TC* __restrict__ C, double alpha,
-                                                       double beta, int use_k)
+                                                       const void* B, TC* __restrict__ C,
+                                                       double* __restrict__ cws, double alpha, double beta,
+                                                       int use_k)
 {
     static_assert(WGM * WGN * 64 == kNT, "four waves");
     constexpr int WM = BM / WGM, WN = BN / WGN, MB = WM / 16, NB = WN / 16;
@@ -210,6 +212,28 @@ __global__ __launch_bounds__(kNT) void gemm_lds_kernel(const PanelTask* __restri
 
         // D register r of lane l holds D[l / 16 + 4 r][l % 16] of each block
         const int cl = lane & 15, rl = lane >> 4;
+        if (t.c_ld > 0)
+        {
+            // a C panel (leading dimension c_ld at c_src in cws) takes the
+            // unscaled sums: the tile timing runs write there
+#pragma unroll
+            for (int i = 0; i < MB; i++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                {
+                    const int row = wm * WM + i * 16 + rl + 4 * r;
+                    if (row >= t.mlen) continue;
+#pragma unroll
+                    for (int j = 0; j < NB; j++)
+                    {
+                        const int col = wn * WN + j * 16 + cl;
+                        if (col < t.nlen) cws[t.c_src + (int64_t)row * t.c_ld + col] = acc[i][j][r];
+                    }
+                }
+            continue;
+        }
+        // c_ld == 0: C through its row and column offsets (the planner gives
+        // these tiles single-block C only)
         int64_t coff[NB];
 #pragma unroll
         for (int j = 0; j < NB; j++)
@@ -268,7 +292,7 @@ hipError_t launch_lds_t(const PanelLaunch& L, int lt)
         auto* k = &gemm_lds_kernel<bm, bn, wgm, wgn, TC>;                                                \
         const dim3 grid(persistent_grid((const void*)k, kNT, L.ntasks));                                \
         hipLaunchKernelGGL(k, grid, dim3(kNT), 0, L.stream, L.tasks, L.ntasks, L.pool, L.A, L.B, (TC*)L.C, \
-                           L.alpha, L.beta, L.use_k);                                                    \
+                           L.cws, L.alpha, L.beta, L.use_k);                                             \
         return hipGetLastError();                                                                        \
     }
         TBLIS_LDS_TILES(X)

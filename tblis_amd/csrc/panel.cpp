@@ -542,14 +542,17 @@ void side_range(const Side& s, int64_t& lo, int64_t& hi)
 
 }  // namespace
 
-// Can the LDS-staged tiles run these GEMMs: single A, B and C blocks,
-// unscaled, both operands within a buffer resource's reach (non-negative
-// offsets, under 2 GiB from the block's base).
+// Can the LDS-staged tiles run these GEMMs: direct reads enabled, single
+// A, B and C blocks, k > 0, unscaled, both operands within a buffer
+// resource's reach (non-negative offsets, under 2 GiB from the block's
+// base).
 bool lds_applicable(const std::vector<PanelGemm>& gemms, int src_elem)
 {
+    if (!direct_enabled()) return false;
     for (const PanelGemm& g : gemms)
     {
-        if (g.mlen.size() != 1 || g.nlen.size() != 1 || g.klen.size() > 1 || g.c.size() != 1) return false;
+        if (g.mlen.size() != 1 || g.nlen.size() != 1 || g.klen.size() != 1 || g.c.size() != 1) return false;
+        if (g.klen[0] <= 0) return false;
         if (g.a.size() != g.klen.size() || g.b.size() != g.klen.size()) return false;
         for (double x : g.kscale)
             if (x != 1.0) return false;
@@ -752,7 +755,8 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
             dbytes[o] = (uint32_t)((hi + khi + 1) * src_elem);
         }
 
-        if (lds && !(dir[0] && dir[1])) fail("internal: LDS-staged tile without both operands read directly");
+        if (lds && !(dir[0] && dir[1] && g.c.size() == 1))
+            fail("internal: LDS-staged tile without both operands read directly or without a single C block");
 
         // panels
         const int64_t oa = ws;
@@ -1091,9 +1095,10 @@ std::string layout_sig(const std::vector<PanelGemm>& gemms)
 }
 
 int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bool swap,
-                  const std::vector<int>& cands)
+                  const std::vector<int>& cands_in)
 {
-    if (t_tune_off || !autotune_on() || cands.size() < 2) return -1;
+    const std::vector<int>& cands0 = cands_in;
+    if (t_tune_off || !autotune_on() || cands0.size() < 2) return -1;
     hipStream_t s = thread_state().stream;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone)
@@ -1114,6 +1119,13 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bo
     auto it = db.best.find(h ^ (uint64_t)family);
     if (it != db.best.end()) return it->second;
 
+    // without the tensors (or with direct reads off) the candidates run
+    // from packed panels, which the LDS-staged tiles never read
+    std::vector<int> cands_run;
+    for (int c : cands0)
+        if (real || !panel_tile_lds(family, c)) cands_run.push_back(c);
+    if (cands_run.size() < 2) return -1;
+    const std::vector<int>& cands = cands_run;
     std::vector<PanelTables> tabs(cands.size());
     size_t panel_bytes = 256, cslot = 0, ntask = 0;
     const int eb = geom_of(family).elem_bytes;

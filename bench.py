@@ -229,8 +229,10 @@ def dpd_leg(T, dev, stream, steps, warmup):
     lens = [DPD_EXT] * 4
     n = T.dpd_size(0, nirrep, lens)
     g = torch.Generator(device=dev).manual_seed(7)
-    A = torch.rand(n, dtype=torch.float64, device=dev, generator=g) * 2 - 1
-    B = torch.rand(n, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    # FP32 values, held widened for the FP64 leg (the split-BF16 error
+    # against it is then the method's error alone)
+    A = (torch.rand(n, dtype=torch.float64, device=dev, generator=g) * 2 - 1).float().double()
+    B = (torch.rand(n, dtype=torch.float64, device=dev, generator=g) * 2 - 1).float().double()
     Cd = torch.zeros(n, dtype=torch.float64, device=dev)
     Cf = torch.zeros(n, dtype=torch.float32, device=dev)
     s = [sum(DPD_EXT[h] * DPD_EXT[h ^ r] for h in range(nirrep)) for r in range(nirrep)]
@@ -354,14 +356,17 @@ def batch_leg(T, dev, stream, steps, warmup, count=4096, seed=7):
     flops = float(sum(2.0 * np.prod(e.astype(np.float64)) for e in exts))
     views, dims = batch_views(exts)
     g = torch.Generator(device=dev).manual_seed(seed)
+    # one set of FP32 values; the FP64 leg holds the same values widened, so
+    # that the split-BF16 error against it is the method's error alone
+    vals = []
+    for t in range(3):
+        x = (torch.rand(dims[t], dtype=torch.float64, device=dev, generator=g) * 2 - 1).to(torch.float32)
+        if t == 2:
+            x.zero_()
+        vals.append(x)
     insts = {}
     for name, dt in (("fp64", torch.float64), ("split_bf16", torch.float32)):
-        bufs = []
-        for t in range(3):
-            x = torch.rand(dims[t], dtype=torch.float64, device=dev, generator=g) * 2 - 1
-            if t == 2:
-                x.zero_()
-            bufs.append(x.to(dt))
+        bufs = [v.to(dt) if dt != torch.float32 else v for v in vals]
         insts[name] = tuple([torch.as_strided(bufs[t], sh, colmaj(sh), off) for sh, off in views[t]]
                             for t in range(3))
     out = {"workload": "%d independent C[abij] = A[abef]*B[efij], extents uniform in 4..40 "

@@ -252,12 +252,14 @@ void tblis_init_dpd_tensor_s(tblis_dpd_tensor* t, float scalar, unsigned irrep, 
 stride_type tblis_dpd_tensor_size(const tblis_dpd_tensor* t);
 
 /* DPD contraction; same semantics as the dense call, block by block.  Each
- * contracted irrep is one GEMM over its symmetry-allowed blocks; when every
- * block address splits into a row part and a column part (the six MArray
- * layouts for the usual label orders), the GEMMs of all irreps run as one
- * persistent GEMM launch that reads A and B and writes C in their DPD
- * storage.  Otherwise the blocks are packed first (one pack launch) and C
- * blocks are written from the GEMM epilogue. */
+ * contracted irrep is one GEMM over its symmetry-allowed blocks (when every
+ * block address splits into a row part and a column part, the six MArray
+ * layouts for the usual label orders, one GEMM with explicit row and k
+ * offsets).  The GEMMs of all irreps run in one GEMM launch per kernel
+ * variant (one workgroup per C tile), after one pack launch for the
+ * operands that are not read straight from the DPD storage; C blocks are
+ * written from the GEMM epilogue (split-BF16: through a C panel and one
+ * unpack launch). */
 void tblis_dpd_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
                            const tblis_dpd_tensor* A, const label_type* idx_A,
                            const tblis_dpd_tensor* B, const label_type* idx_B,
@@ -352,12 +354,12 @@ void tblis_indexed_dpd_tensor_mult(const tblis_comm* comm, const tblis_config* c
                                    tblis_indexed_dpd_tensor* C, const label_type* idx_C);
 
 /* ---------------------------------------------------------------------- */
-/* Extension: a batch of independent contractions of uneven shapes.  With */
-/* FP64 or split-BF16 every instance's operands are read straight from    */
-/* the tensors by one persistent GEMM launch over the tiles of all        */
-/* instances (operands beyond a buffer resource's 2 GiB reach are packed  */
-/* by one pack launch first).  The reference has no counterpart; its      */
-/* equivalent is a host loop of tblis_tensor_mult calls.                  */
+/* Extension: a batch of independent contractions of uneven shapes.  The */
+/* tiles of all instances run in one GEMM launch per kernel variant (one  */
+/* workgroup per C tile, grouped by which operands are read straight from */
+/* the tensors), after one pack launch for the other operands.  The       */
+/* reference has no counterpart; its equivalent is a host loop of         */
+/* tblis_tensor_mult calls.                                               */
 /* ---------------------------------------------------------------------- */
 void tblis_tensor_mult_batch(const tblis_comm* comm, const tblis_config* cfg, len_type count,
                              const tblis_tensor* const* A, const label_type* const* idx_A,

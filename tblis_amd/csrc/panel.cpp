@@ -359,6 +359,18 @@ int direct_kfast_rereads()
     return v;
 }
 
+// Fragment groups per narrow pack chunk (TBLIS_HIP_PACK_NARROW overrides,
+// read at every plan build).
+int64_t narrow_pack_groups(int family)
+{
+    if (const char* e = std::getenv("TBLIS_HIP_PACK_NARROW"))
+    {
+        const int v = std::atoi(e);
+        if (v >= 1 && v <= 64) return v;
+    }
+    return family == FAMILY_BF16X3 || family == FAMILY_BF16X3_FAST ? 8 : 32;
+}
+
 bool direct_enabled()
 {
     const char* e = std::getenv("TBLIS_HIP_DIRECT");
@@ -572,16 +584,16 @@ bool lds_applicable(const std::vector<PanelGemm>& gemms, int src_elem)
 namespace
 {
 
-// Many GEMMs (DPD irrep GEMMs, batch instances) read every operand that can
-// be straight from the tensors, so that a call is one GEMM launch; one GEMM
-// reads an operand directly only where that beats packing it.
-// TBLIS_HIP_DIRECT_ALL=1 / =0 forces either rule (read at every plan build).
+// An operand is read straight from its tensor only where that beats
+// packing it (unit-stride rows or k, few re-reads).  TBLIS_HIP_DIRECT_ALL=1
+// (read at every plan build) reads every operand that can be read directly
+// in plans of many GEMMs, so that such a call is one GEMM launch without a
+// pack: measured slower on the 8-irrep DPD workload (FP64 0.366 against
+// 0.432 of dense-equivalent) and no faster on the 4096-instance batch.
 bool direct_all(const std::vector<PanelGemm>& gemms)
 {
     const char* e = std::getenv("TBLIS_HIP_DIRECT_ALL");
-    if (e && e[0] == '0') return false;
-    if (e && e[0] == '1') return true;
-    return gemms.size() > 1;
+    return e && e[0] == '1' && gemms.size() > 1;
 }
 
 bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T,
@@ -611,7 +623,10 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
 
     int64_t rows = 0;      // Ozaki family: panel rows (row-maximum array entries)
     const bool ozaki = family == FAMILY_OZAKI;
-    // Pack chunks of one job.  Narrow: 4 row groups x at most 8 k groups.
+    // Pack chunks of one job.  Narrow: at most 8 k groups x as many row
+    // groups as fit the family's budget (narrow_pack_groups: FP64 4 row
+    // groups, 64 rows x 64 k; split-BF16 one, 32 rows x 128 k, the same 4096
+    // elements, since its groups are four times larger).
     // Wide: the job's k groups split evenly into chunks of at most 8, and as
     // many row groups per chunk (4 to 8) as keep it within 32 groups and 16
     // pairs of k groups (one batch of loads per FP64 pack wave), so that
@@ -626,7 +641,8 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
         const PanelJob& j = jobs[id];
         const int64_t rb0 = j.r0 / G.gr, rb1 = ceil_div(j.r0 + j.nr, G.gr);
         const int64_t kg0 = j.k0 / G.gk, kg1 = ceil_div(j.k0 + j.nk, G.gk);
-        int64_t per = 8, rstep = 4;
+        const int64_t budget = narrow_pack_groups(family);
+        int64_t per = std::min<int64_t>(8, budget), rstep = std::max<int64_t>(1, budget / per);
         if (wide)
         {
             per = ceil_div(kg1 - kg0, ceil_div(kg1 - kg0, 8));

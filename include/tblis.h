@@ -396,6 +396,26 @@ int tblis_hip_autotune_runs(void);
  * the direct-gather kernels. */
 int tblis_hip_last_tile(void);
 
+/* No counterpart in the reference: tblis_tensor_mult for callers that
+ * cannot allocate device memory (serving loops and harnesses that forbid
+ * hipMalloc).  The same contraction, with its plan tables and its pack and
+ * C-panel workspace in a 64 MiB device array of the library (one per GPU,
+ * held from the library's first use on it until the process ends); the
+ * call makes no hipMalloc or hipFree.  Real types (float, double) on the
+ * panel path; a contraction whose tables and workspace exceed 64 MiB fails
+ * with an error naming the sizes.  No plan cache and no autotuning: every
+ * call plans on the host and uploads its tables in stream order into the
+ * array.  Calls on one GPU share the array, which keeps the last call's
+ * contents: calls that overlap on one GPU (other streams or host threads)
+ * race. */
+void tblis_hip_tensor_mult_noalloc(const tblis_comm* comm, const tblis_config* cfg,
+                                   const tblis_tensor* A, const label_type* idx_A,
+                                   const tblis_tensor* B, const label_type* idx_B,
+                                   tblis_tensor* C, const label_type* idx_C);
+/* Device allocations (plan tables, workspace, tuning scratch) this process
+ * has made through the library. */
+int64_t tblis_hip_device_alloc_count(void);
+
 /* Introspection (host only, no device work): the canonical form of a
  * contraction as computed by this library's front end.
  * out_groups receives, for AB, AC, BC, ABC in order: ndim, then the folded

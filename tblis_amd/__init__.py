@@ -10,7 +10,7 @@ fallback: a missing library is an error.
 import ctypes as C
 import os
 
-__all__ = ["lib", "mult", "matrix_mult", "dpd_mult", "mult_batch", "indexed_mult", "Indexed",
+__all__ = ["lib", "mult", "mult_noalloc", "device_alloc_count", "matrix_mult", "dpd_mult", "mult_batch", "indexed_mult", "Indexed",
            "indexed_dpd_mult", "IndexedDpd",
            "prepare_indexed_mult", "prepare_mult", "prepare_dpd_mult",
            "prepare_batch", "Tensor", "DpdTensor", "get_config",
@@ -92,6 +92,7 @@ EXPORTED = [
     "tblis_hip_get_stream", "tblis_hip_set_async", "tblis_hip_set_error_mode",
     "tblis_hip_get_last_error", "tblis_hip_clear_plan_cache", "tblis_hip_last_algorithm",
     "tblis_hip_set_autotune", "tblis_hip_autotune_runs", "tblis_hip_last_tile",
+    "tblis_hip_tensor_mult_noalloc", "tblis_hip_device_alloc_count",
     "tblis_hip_debug_canonical", "tblis_hip_debug_fill_block_scatter", "tblis_hip_debug_panel_side",
     "tblis_hip_debug_dpd_block",
     "tblis_hip_debug_dpd_patches", "tblis_hip_debug_mfma_probe", "tblis_hip_debug_mfma_probe_bf16",
@@ -111,6 +112,8 @@ def lib():
                                         C.c_char_p]
         L.tblis_dpd_tensor_mult.argtypes = [C.c_void_p, C.c_void_p, D, C.c_char_p, D, C.c_char_p, D,
                                             C.c_char_p]
+        L.tblis_hip_tensor_mult_noalloc.argtypes = L.tblis_tensor_mult.argtypes
+        L.tblis_hip_device_alloc_count.restype = C.c_int64
         IX = C.POINTER(IndexedTensor)
         L.tblis_indexed_tensor_mult.argtypes = [C.c_void_p, C.c_void_p, IX, C.c_char_p, IX, C.c_char_p,
                                                 IX, C.c_char_p]
@@ -294,6 +297,26 @@ def mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None, conj_A=False, 
     except TblisError as e:
         raise TblisError("tblis_tensor_mult: %s" % e) from None
     return C_
+
+
+def mult_noalloc(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None):
+    """mult() through tblis_hip_tensor_mult_noalloc: no device allocation
+    (plan tables and workspace in the library's 64 MiB device scratch)."""
+    dA, dB, dC = _Desc(A, alpha), _Desc(B), _Desc(C_, beta)
+    cfg = get_config(config) if config else None
+    L = lib()
+    L.tblis_hip_set_error_mode(1)
+    L.tblis_hip_tensor_mult_noalloc(None, cfg, C.byref(dA.d), _labels(idx_A, A.dim(), "A"), C.byref(dB.d),
+                                    _labels(idx_B, B.dim(), "B"), C.byref(dC.d), _labels(idx_C, C_.dim(), "C"))
+    err = last_error()
+    if err:
+        raise TblisError("tblis_hip_tensor_mult_noalloc: %s" % err)
+    return C_
+
+
+def device_alloc_count():
+    """Device allocations the library has made in this process."""
+    return lib().tblis_hip_device_alloc_count()
 
 
 def add(alpha, A, idx_A, beta, B, idx_B):

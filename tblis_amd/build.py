@@ -12,7 +12,7 @@ ARCH = os.environ.get("TBLIS_AMD_ARCH", "gfx950")
 
 SOURCES = ["canon.cpp", "scatter.cpp", "dpd.cpp", "plan.cpp", "contract.cpp", "dpd_plan.cpp",
            "api.cpp", "panel.cpp", "level1.cpp", "indexed.cpp", "indexed_dpd.cpp", "complex.cpp", "gemm_f64.hip", "gemm_bf16x3.hip", "gemm_panel.hip", "gemm_panel_bf16.hip", "gemm_lds.hip",
-           "gemm_ozaki.hip", "level1_kernels.hip"]
+           "gemm_ozaki.hip", "level1_kernels.hip", "noalloc.hip"]
 
 # Per-source device flags.  The panel GEMM keeps its MFMA accumulators in
 # ordinary VGPRs (the unified gfx950 register file); the default AGPR form

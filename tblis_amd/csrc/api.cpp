@@ -244,6 +244,14 @@ void tblis_tensor_mult(const tblis_comm* comm, const tblis_config* cfg, const tb
     guarded_void([&] { tensor_mult(cfg, A, idx_A, B, idx_B, C, idx_C); });
 }
 
+void tblis_hip_tensor_mult_noalloc(const tblis_comm* comm, const tblis_config* cfg, const tblis_tensor* A,
+                                   const label_type* idx_A, const tblis_tensor* B, const label_type* idx_B,
+                                   tblis_tensor* C, const label_type* idx_C)
+{
+    (void)comm;
+    guarded_void([&] { tensor_mult_noalloc(cfg, A, idx_A, B, idx_B, C, idx_C); });
+}
+
 unsigned tblis_get_num_threads(void)
 {
     if (g_num_threads) return g_num_threads;
@@ -387,6 +395,8 @@ void tblis_hip_clear_plan_cache(void) { cache_clear(); }
 void tblis_hip_set_autotune(int on) { set_autotune(on); }
 int tblis_hip_last_tile(void) { return thread_state().last_tile; }
 int tblis_hip_autotune_runs(void) { return autotune_runs(); }
+
+int64_t tblis_hip_device_alloc_count(void) { return device_alloc_count(); }
 const char* tblis_hip_last_algorithm(void) { return thread_state().last_algo.c_str(); }
 
 int64_t tblis_hip_debug_canonical(const tblis_tensor* A, const label_type* idx_A,

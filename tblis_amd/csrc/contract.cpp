@@ -292,6 +292,60 @@ void tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_typ
     reset_scalar(C);
 }
 
+// The same contraction with its plan tables and workspace in the device
+// scratch of noalloc.hip: a plan built for this call alone (no cache, the
+// cost model's tile, no autotuning), uploaded in stream order into the
+// scratch, then the panel launches.  Sizes whose tables and workspace
+// exceed the scratch fail with an error; so do complex types and the
+// non-panel paths.
+void tensor_mult_noalloc(const tblis_config* cfg, const tblis_tensor* A, const label_type* idx_A,
+                         const tblis_tensor* B, const label_type* idx_B, tblis_tensor* C,
+                         const label_type* idx_C)
+{
+    check_tensor(A, idx_A, "A");
+    check_tensor(B, idx_B, "B");
+    check_tensor(C, idx_C, "C");
+    if (A->type != B->type || A->type != C->type) fail("A, B and C must have the same type");
+    if (A->type != TYPE_SINGLE && A->type != TYPE_DOUBLE)
+        fail("the no-allocation entry point takes real (float, double) tensors only");
+    const type_t type = A->type;
+    const Algo algo = resolve_algo(cfg, type);
+    thread_state().last_algo = algo_name(algo);
+    const double alpha = product_scalar(A->scalar, B->scalar, type);
+    const double beta = scalar_value(C->scalar, type);
+
+    Canonical c;
+    std::string why;
+    if (!canonicalize(*A, idx_A, *B, idx_B, *C, idx_C, c, why)) fail("invalid contraction: " + why);
+    GemmDesc g;
+    if (dense_gemm(c, g, 0, 0, 0))
+    {
+        size_t bytes = 0;
+        char* base = noalloc_scratch(bytes);
+        AutotuneOffScope no_tune;
+        StaticStorageScope storage(base, bytes);
+        PlanBuilder pb(elem_size(type), dtype_code(type), algo != Algo::Reference, family_of(algo));
+        if (!pb.panel_route()) fail("the no-allocation entry point runs on the panel path only");
+        pb.add(g);
+        std::shared_ptr<Plan> plan = pb.build();
+        if (plan && plan->ntasks > 0)
+        {
+            const bool use_k = alpha != 0.0;
+            if (use_k || beta != 1.0)
+            {
+                check_data(C, "C");
+                if (use_k && plan->reads_ab)
+                {
+                    check_data(A, "A");
+                    check_data(B, "B");
+                }
+                run_plan(plan, A->data, B->data, C->data, alpha, beta, use_k);
+            }
+        }
+    }
+    reset_scalar(C);
+}
+
 // Matrix multiplication (src/iface/3m/mult.cxx:19-191) is the contraction
 // C[ab] = A[ak] B[kb] with each matrix's (rs, cs) as its two strides; the
 // tensor front end then covers the reference's special cases (empty C,

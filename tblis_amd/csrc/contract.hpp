@@ -12,6 +12,15 @@ void tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_typ
                  const tblis_tensor* B, const label_type* idx_B, tblis_tensor* C,
                  const label_type* idx_C);
 
+// tensor_mult without device allocation: plan tables and workspace in the
+// device scratch of noalloc.hip (real types, panel path; no plan cache,
+// no autotuning).  Calls on one GPU share the scratch and must not overlap.
+void tensor_mult_noalloc(const tblis_config* cfg, const tblis_tensor* A, const label_type* idx_A,
+                         const tblis_tensor* B, const label_type* idx_B, tblis_tensor* C,
+                         const label_type* idx_C);
+// The current device's scratch (noalloc.hip) and its size.
+char* noalloc_scratch(size_t& bytes);
+
 void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tensor* const* A,
                        const label_type* const* idx_A, const tblis_tensor* const* B,
                        const label_type* const* idx_B, tblis_tensor* const* C,

@@ -1409,21 +1409,46 @@ bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, i
     const size_t o_uc = o_uj + al(s_uj), o_pool = o_uc + al(s_uc);
     plan.bytes = o_pool + s_pool;
     hip_check(hipGetDevice(&plan.device), "hipGetDevice");
-    hip_check(device_alloc((void**)&plan.dev, plan.bytes), "hipMalloc (plan)");
+    char* sbase = nullptr;
+    size_t sbytes = 0;
+    static_storage(sbase, sbytes);
+    if (sbase)
+    {
+        // caller storage: tables, then the workspace
+        const size_t need = al(plan.bytes) + plan.ws_bytes;
+        if (need > sbytes)
+            fail("the contraction needs " + std::to_string(need) + " bytes of device scratch (plan tables " +
+                 std::to_string(plan.bytes) + ", workspace " + std::to_string(plan.ws_bytes) +
+                 "); the no-allocation entry point has " + std::to_string(sbytes));
+        plan.dev = sbase;
+        plan.owns_dev = false;
+        plan.ws_static = sbase + al(plan.bytes);
+    }
+    else
+        hip_check(device_alloc((void**)&plan.dev, plan.bytes), "hipMalloc (plan)");
     char* base = (char*)plan.dev;
     plan.d_pchunks = (PackChunk*)(base + o_chunks);
     plan.d_ptasks = (PanelTask*)(base + o_tasks);
     plan.d_ujobs = (UnpackJob*)(base + o_uj);
     plan.d_uchunks = (UnpackChunk*)(base + o_uc);
     plan.d_pool = (int64_t*)(base + o_pool);
+    // Caller storage may still be read by the previous call's kernels on
+    // the thread's stream: the upload is ordered after them on that stream
+    // and waited for before the host tables go away.
+    hipStream_t st = thread_state().stream;
     auto up = [&](void* dst, const void* src, size_t n) {
-        if (n) hip_check(hipMemcpy(dst, src, n, hipMemcpyHostToDevice), "hipMemcpy (plan)");
+        if (!n) return;
+        if (sbase)
+            hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st), "hipMemcpyAsync (plan)");
+        else
+            hip_check(hipMemcpy(dst, src, n, hipMemcpyHostToDevice), "hipMemcpy (plan)");
     };
     up(plan.d_pchunks, pc.data(), s_chunks);
     up(plan.d_ptasks, T.tasks.data(), s_tasks);
     up(plan.d_ujobs, T.ujobs.data(), s_uj);
     up(plan.d_uchunks, T.uchunks.data(), s_uc);
     up(plan.d_pool, T.pool.data(), T.pool.size() * 8);
+    if (sbase) hip_check(hipStreamSynchronize(st), "hipStreamSynchronize (plan)");
     return true;
 }
 

@@ -55,9 +55,28 @@ hipError_t device_alloc(void** p, size_t bytes)
 
 int64_t device_alloc_count() { return g_device_allocs.load(); }
 
+namespace
+{
+struct StaticRegion
+{
+    char* base = nullptr;
+    size_t bytes = 0;
+};
+thread_local StaticRegion t_static;
+}  // namespace
+
+StaticStorageScope::StaticStorageScope(void* base, size_t bytes) { t_static = StaticRegion{(char*)base, bytes}; }
+StaticStorageScope::~StaticStorageScope() { t_static = StaticRegion{}; }
+
+void static_storage(char*& base, size_t& bytes)
+{
+    base = t_static.base;
+    bytes = t_static.bytes;
+}
+
 Plan::~Plan()
 {
-    if (dev)
+    if (dev && owns_dev)
     {
         int cur = 0;
         (void)hipGetDevice(&cur);
@@ -641,7 +660,7 @@ void run_plan(const std::shared_ptr<Plan>& pp, const void* A, const void* B, voi
         P.A = L.A;
         P.B = L.B;
         P.C = C;
-        char* ws = (char*)workspace(p.ws_bytes, ts.stream);
+        char* ws = p.ws_static ? p.ws_static : (char*)workspace(p.ws_bytes, ts.stream);
         P.ws = ws;
         P.cws = (double*)(ws + p.ws_c_offset);
         P.family = p.key.family;

@@ -86,6 +86,8 @@ struct Plan
     }
     int64_t mfma_flops = 0;  // incl. tile padding at 16-row MFMA granularity
     void* dev = nullptr;     // one allocation: tasks | segs | pool
+    bool owns_dev = true;    // false: dev is caller storage (StaticStorageScope)
+    char* ws_static = nullptr;  // workspace in caller storage (else the stream's pool)
     TaskDesc* d_tasks = nullptr;
     SegDesc* d_segs = nullptr;
     int64_t* d_pool = nullptr;
@@ -95,6 +97,19 @@ struct Plan
 };
 
 struct PanelGemm;
+
+// While one lives on a thread, panel plans built there place their device
+// tables and their workspace in [base, base + bytes) of device memory
+// instead of allocating: the tables at base, the workspace after them.  The
+// upload is ordered on the thread's stream and waited for.  A plan that
+// does not fit fails with an error naming the limit.
+struct StaticStorageScope
+{
+    StaticStorageScope(void* base, size_t bytes);
+    ~StaticStorageScope();
+};
+// The active scope's region (base nullptr when none).
+void static_storage(char*& base, size_t& bytes);
 
 class PlanBuilder
 {

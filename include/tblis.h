@@ -198,10 +198,11 @@ void tblis_set_num_threads(unsigned num_threads);
 
 /* Configuration lookup by name (src/util/configs.h:11).  Names:
  *   "mi355x"        default: FP64 MFMA for double, split-BF16 x3 for float
+ *   "reference"     the reference's portable configuration: here the same
+ *                   kernels as the default
  *   "fp64_mfma"     float contractions widened to FP64 MFMA
  *   "bf16x3"        3-term split-BF16 (6 products), FP32 accumulation
  *   "bf16x3_fast"   3-term split-BF16 without the compensated k-blocking
- *   "reference"     plain FMA kernel (no MFMA), for cross-checks
  *   "ozaki"         FP64-accurate for double and float: each row of A and B
  *                   scaled by a power of two and cut into 8 signed 7-bit
  *                   digits (6 or 7 with TBLIS_HIP_OZAKI_SLICES), digit-pair
@@ -392,8 +393,7 @@ const char* tblis_hip_last_algorithm(void);
 void tblis_hip_set_autotune(int on);
 /* Number of tuning measurements this process has made. */
 int tblis_hip_autotune_runs(void);
-/* Panel GEMM tile of the last contraction on this thread, -1 if it ran on
- * the direct-gather kernels. */
+/* Panel GEMM tile of the last contraction on this thread. */
 int tblis_hip_last_tile(void);
 
 /* No counterpart in the reference: tblis_tensor_mult for callers that

@@ -11,7 +11,7 @@ BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("TBLIS_AMD_ARCH", "gfx950")
 
 SOURCES = ["canon.cpp", "scatter.cpp", "dpd.cpp", "plan.cpp", "contract.cpp", "dpd_plan.cpp",
-           "api.cpp", "panel.cpp", "level1.cpp", "indexed.cpp", "indexed_dpd.cpp", "complex.cpp", "gemm_f64.hip", "gemm_bf16x3.hip", "gemm_panel.hip", "gemm_panel_bf16.hip", "gemm_lds.hip",
+           "api.cpp", "panel.cpp", "level1.cpp", "indexed.cpp", "indexed_dpd.cpp", "complex.cpp", "gemm_panel.hip", "gemm_panel_bf16.hip", "gemm_lds.hip",
            "gemm_ozaki.hip", "level1_kernels.hip", "noalloc.hip"]
 
 # Per-source device flags.  The panel GEMM keeps its MFMA accumulators in

@@ -194,8 +194,7 @@ void complex_tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const l
             std::vector<PanelGemm> pg, cg;
             panel_from_desc(g, pg);
             for (const PanelGemm& p : pg) split_gemm(p, terms, cg);
-            PlanBuilder pb(rtype == TYPE_DOUBLE ? 8 : 4, rtype == TYPE_DOUBLE ? 0 : 1, 1, algo_family(algo));
-            if (!pb.panel_route()) fail("complex contractions run on the panel path (TBLIS_HIP_PANEL=0 is set)");
+            PlanBuilder pb(rtype == TYPE_DOUBLE ? 8 : 4, rtype == TYPE_DOUBLE ? 0 : 1, algo_family(algo));
             for (const PanelGemm& p : cg) pb.add_panel(p);
             plan = pb.build();
         }

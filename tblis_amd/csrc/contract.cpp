@@ -189,10 +189,12 @@ const char* algo_name(Algo a)
 
 Algo resolve_algo(const tblis_config* cfg, type_t type)
 {
+    // "reference" (the reference's portable configuration) selects the
+    // same kernels as the default: every configuration runs on MFMA here
     Algo a = cfg ? cfg->algo : Algo::Default;
-    if (a == Algo::Reference || a == Algo::Ozaki) return a;
+    if (a == Algo::Ozaki) return a;
     if (type == TYPE_DOUBLE) return Algo::Fp64Mfma;
-    if (a == Algo::Default) return Algo::Bf16x3;
+    if (a == Algo::Default || a == Algo::Reference) return Algo::Bf16x3;
     return a;
 }
 
@@ -262,7 +264,7 @@ void tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_typ
                 check_data(B, "B");
             }
             TuneOperandsScope ops(reads ? A->data : nullptr, reads ? B->data : nullptr);
-            PlanBuilder pb(elem_size(type), dtype_code(type), algo != Algo::Reference, family_of(algo));
+            PlanBuilder pb(elem_size(type), dtype_code(type), family_of(algo));
             pb.add(g);
             plan = pb.build();
         }
@@ -296,8 +298,7 @@ void tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_typ
 // scratch of noalloc.hip: a plan built for this call alone (no cache, the
 // cost model's tile, no autotuning), uploaded in stream order into the
 // scratch, then the panel launches.  Sizes whose tables and workspace
-// exceed the scratch fail with an error; so do complex types and the
-// non-panel paths.
+// exceed the scratch fail with an error; so do complex types.
 void tensor_mult_noalloc(const tblis_config* cfg, const tblis_tensor* A, const label_type* idx_A,
                          const tblis_tensor* B, const label_type* idx_B, tblis_tensor* C,
                          const label_type* idx_C)
@@ -324,8 +325,7 @@ void tensor_mult_noalloc(const tblis_config* cfg, const tblis_tensor* A, const l
         char* base = noalloc_scratch(bytes);
         AutotuneOffScope no_tune;
         StaticStorageScope storage(base, bytes);
-        PlanBuilder pb(elem_size(type), dtype_code(type), algo != Algo::Reference, family_of(algo));
-        if (!pb.panel_route()) fail("the no-allocation entry point runs on the panel path only");
+        PlanBuilder pb(elem_size(type), dtype_code(type), family_of(algo));
         pb.add(g);
         std::shared_ptr<Plan> plan = pb.build();
         if (plan && plan->ntasks > 0)
@@ -436,7 +436,7 @@ void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tenso
     std::shared_ptr<Plan> plan = cache_get(key);
     if (!plan)
     {
-        PlanBuilder pb(es, dtype_code(type), algo != Algo::Reference, family_of(algo));
+        PlanBuilder pb(es, dtype_code(type), family_of(algo));
         bool reads = false;
         for (int64_t i = 0; i < count; i++)
         {

@@ -623,8 +623,7 @@ void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const l
     {
         DpdProblem P;
         P.init(A, idx_A, B, idx_B, C, idx_C);
-        PlanBuilder pb(type == TYPE_DOUBLE ? 8 : 4, dtype_code(type), algo != Algo::Reference,
-                       algo_family(algo));
+        PlanBuilder pb(type == TYPE_DOUBLE ? 8 : 4, dtype_code(type), algo_family(algo));
         // the autotuner may time its candidates on A and B (read only):
         // check them first
         const bool reads = alpha != 0.0 && total(A) > 0 && total(B) > 0;
@@ -634,7 +633,7 @@ void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const l
             check(B, "B");
         }
         TuneOperandsScope ops(reads ? A->data : nullptr, reads ? B->data : nullptr);
-        if (!pb.panel_route() || !build_panel_gemms(P, pb)) build_gemms(P, pb);
+        if (!build_panel_gemms(P, pb)) build_gemms(P, pb);
         plan = pb.build();
         if (!plan) plan = std::make_shared<Plan>();
         cache_put(key, plan);

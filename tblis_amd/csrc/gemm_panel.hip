@@ -600,7 +600,28 @@ hipError_t launch_gemm_d(const PanelLaunch& L)
     return launch_gemm_t<TC, false, false>(L);
 }
 
+// The v_mfma_f64_16x16x4 lane layout the GEMM relies on (the lane-layout
+// test): lane l supplies A[l%16][l/16] and B[l/16][l%16] of
+// A[i][k] = 1000 i + k + 1 and B[k][j] = (k + 1)(j + 1), and register r of
+// lane l, D[l/16 + 4r][l%16], is stored row-major into out (16 x 16).
+__global__ void mfma_probe_kernel(double* out)
+{
+    const int lane = threadIdx.x;
+    const int i = lane % 16, kq = lane / 16;
+    const double a = 1000.0 * i + kq + 1;
+    const double b = (kq + 1.0) * (i + 1.0);
+    v4d d = {0.0, 0.0, 0.0, 0.0};
+    d = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d, 0, 0, 0);
+    for (int r = 0; r < 4; r++) out[(lane / 16 + 4 * r) * 16 + lane % 16] = d[r];
+}
+
 }  // namespace
+
+hipError_t launch_mfma_probe(double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, s, out);
+    return hipGetLastError();
+}
 
 // Workgroups of a GEMM launch.  By default one per task: the hardware
 // dispatcher then hands tasks to CUs as they free up, which balances the

@@ -642,4 +642,28 @@ hipError_t launch_panel_gemm_bf16(const PanelLaunch& L)
     }
 }
 
+// The v_mfma_f32_32x32x16_bf16 lane layout the GEMM relies on: lane l takes
+// A[l%32][8*(l/32)+j] and B[8*(l/32)+j][l%32] (j = 0..7) from row-major A
+// (32x16) and B (16x32), and register r of lane l,
+// D[8(r/4) + 4(l/32) + r%4][l%32], is stored row-major into D (32x32).
+__global__ void bf16_probe_kernel(const float* A, const float* B, float* D)
+{
+    const int l = threadIdx.x;
+    bf16x8 a, b;
+    for (int j = 0; j < 8; j++)
+    {
+        a[j] = (__bf16)A[(l % 32) * 16 + 8 * (l / 32) + j];
+        b[j] = (__bf16)B[(8 * (l / 32) + j) * 32 + l % 32];
+    }
+    f32x16 d = {};
+    d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, d, 0, 0, 0);
+    for (int r = 0; r < 16; r++) D[(8 * (r / 4) + 4 * (l / 32) + r % 4) * 32 + l % 32] = d[r];
+}
+
+hipError_t launch_bf16_probe(const float* A, const float* B, float* D, hipStream_t s)
+{
+    hipLaunchKernelGGL(bf16_probe_kernel, dim3(1), dim3(64), 0, s, A, B, D);
+    return hipGetLastError();
+}
+
 }  // namespace tblis_amd

@@ -400,8 +400,7 @@ std::shared_ptr<IndexedPlan> build(const Ix& A, const Ix& B, const Ix& C, int es
     upload();
     if (kblocks.empty()) return ip;
 
-    PlanBuilder pb(es, dtype, 1, family);
-    if (!pb.panel_route()) fail("indexed contractions run on the panel path (TBLIS_HIP_PANEL=0 is set)");
+    PlanBuilder pb(es, dtype, family);
     std::vector<int64_t> bi(blen.size(), 0);
     for (int64_t t = 0; t < NBATCH; t++)
     {

@@ -8,42 +8,9 @@
 namespace tblis_amd
 {
 
-struct GemmLaunch
-{
-    const TaskDesc* tasks;
-    int ntasks;
-    const SegDesc* segs;
-    const int64_t* pool;
-    const void* A;
-    const void* B;
-    void* C;
-    double alpha, beta;
-    int use_k;  // 0: ignore k segments (alpha == 0 or k == 0)
-    int grid;
-    hipStream_t stream;
-};
-
-int gemm_f64_num_tiles();
-int gemm_f64_tile_m(int tile);
-int gemm_f64_tile_k(int tile);
-int gemm_f64_tile_n(int tile);
-hipError_t launch_gemm_f64(const GemmLaunch& L, const KernelKey& k);
-int gemm_f64_occupancy(const KernelKey& k);
+// Lane-layout probes of the two MFMA forms the GEMMs use (one wave each).
 hipError_t launch_mfma_probe(double* out, hipStream_t s);
-
-int gemm_bf16_num_tiles();
-int gemm_bf16_tile_m(int tile);
-int gemm_bf16_tile_n(int tile);
-hipError_t launch_gemm_bf16x3(const GemmLaunch& L, const KernelKey& k);
-int gemm_bf16x3_occupancy(const KernelKey& k);
 hipError_t launch_bf16_probe(const float* A, const float* B, float* D, hipStream_t s);
-
-int pack_bf16_rows();
-int pack_bf16_k();
-hipError_t launch_pack_bf16x3(const PackDesc& d, bool kfast, const int64_t* pool, const float* X,
-                              __bf16* out, hipStream_t s);
-hipError_t launch_gemm_bf16x3_packed(const GemmLaunch& L, bool chunked);
-int gemm_bf16x3_packed_occupancy(bool chunked);
 
 // Panel path (FP64 and split-BF16 families)
 int panel_num_tiles(int family);
@@ -103,25 +70,5 @@ hipError_t launch_panel_pack_bf16(const PanelLaunch& L);
 hipError_t launch_panel_ozaki(const PanelLaunch& L, int slices, unsigned long long* rmax, int64_t nrows);
 int ozaki_slices();
 hipError_t launch_panel_gemm_bf16(const PanelLaunch& L);
-
-// family-generic helpers
-inline int tile_m(const KernelKey& k)
-{
-    return k.family == FAMILY_F64 ? gemm_f64_tile_m(k.tile) : gemm_bf16_tile_m(k.tile);
-}
-inline int tile_n(const KernelKey& k)
-{
-    return k.family == FAMILY_F64 ? gemm_f64_tile_n(k.tile) : gemm_bf16_tile_n(k.tile);
-}
-inline int tile_k(const KernelKey& k) { return k.family == FAMILY_F64 ? gemm_f64_tile_k(k.tile) : 16; }
-inline int num_tiles(int family) { return family == FAMILY_F64 ? gemm_f64_num_tiles() : gemm_bf16_num_tiles(); }
-inline int occupancy(const KernelKey& k)
-{
-    return k.family == FAMILY_F64 ? gemm_f64_occupancy(k) : gemm_bf16x3_occupancy(k);
-}
-inline hipError_t launch_gemm(const GemmLaunch& L, const KernelKey& k)
-{
-    return k.family == FAMILY_F64 ? launch_gemm_f64(L, k) : launch_gemm_bf16x3(L, k);
-}
 
 }  // namespace tblis_amd

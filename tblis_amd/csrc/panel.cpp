@@ -411,15 +411,6 @@ int ozaki_slices()
     return s;
 }
 
-bool panel_enabled(int family, int use_mfma)
-{
-    if (!use_mfma) return false;
-    if (family == FAMILY_OZAKI) return true;  // digit planes exist only on the panel path
-    if (const char* e = std::getenv("TBLIS_HIP_PANEL"))
-        if (e[0] == '0') return false;
-    return true;
-}
-
 namespace
 {
 
@@ -1377,18 +1368,15 @@ bool build_panel_plan(Plan& plan, std::vector<PanelGemm> gemms, int elem_size, i
     plan.useful_flops = T.useful_flops;
     plan.mfma_flops = T.mfma_flops;
     plan.reads_ab = T.reads_ab;
-    plan.panel = true;
     plan.panel_tile = T.tile;
     plan.direct[0] = T.direct[0];
     plan.direct[1] = T.direct[1];
     for (int g = 0; g < 4; g++) plan.group_n[g] = T.group_n[g];
     plan.key.family = family;
     plan.key.dtype = dtype;
-    plan.key.use_mfma = 1;
     plan.key.tile = T.tile;
     plan.elem = elem_size;
     plan.ntasks = (int)T.tasks.size();
-    plan.grid = plan.ntasks;
     plan.p_nchunks = (int)T.chunks.size();
     plan.p_nuchunks = (int)T.uchunks.size();
     const size_t pbytes = (size_t)T.ws_panel * geom_of(family).elem_bytes;

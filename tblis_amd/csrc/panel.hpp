@@ -47,9 +47,6 @@ void debug_side_tables(const Side& s, int64_t G, int64_t r0, std::vector<int64_t
 // segments).
 void panel_from_desc(const GemmDesc& g, std::vector<PanelGemm>& out);
 
-// The panel path serves every MFMA family; TBLIS_HIP_PANEL=0 turns it off
-// (direct-gather kernels).
-bool panel_enabled(int family, int use_mfma);
 
 // FP64 operands whose layout the GEMM can read directly skip packing;
 // TBLIS_HIP_DIRECT=0 packs them anyway.

@@ -41,18 +41,12 @@ struct Plan
     KernelKey key{};
     bool swap = false;  // kernel computes C^T = B^T A^T
     int ntasks = 0;
-    int grid = 0;
     int64_t useful_flops = 0;
     bool reads_ab = false;   // some task has k segments
     int elem = 8;            // element size of A, B, C
-    // Packed split-BF16 plan: A' and B' are packed into BF16 panels in a
-    // workspace of ws_bytes, then the panel GEMM runs.
-    bool packed = false;
-    PackDesc pack[2] = {};
-    bool pack_kfast[2] = {false, false};
+    // Panel plan (panel.hpp): pack jobs -> panel GEMM tasks -> C unpack, in
+    // a workspace of ws_bytes
     size_t ws_bytes = 0;
-    // FP64 panel plan (panel.hpp): pack jobs -> panel GEMM tasks -> C unpack
-    bool panel = false;
     int panel_tile = 0;
     size_t ws_c_offset = 0;  // byte offset of the C panels in the workspace
     size_t ws_rmax_offset = 0;  // Ozaki: byte offset of the row-maximum array
@@ -85,11 +79,9 @@ struct Plan
         verified_k = k;
     }
     int64_t mfma_flops = 0;  // incl. tile padding at 16-row MFMA granularity
-    void* dev = nullptr;     // one allocation: tasks | segs | pool
+    void* dev = nullptr;     // one allocation: chunks | tasks | unpack jobs | pool
     bool owns_dev = true;    // false: dev is caller storage (StaticStorageScope)
     char* ws_static = nullptr;  // workspace in caller storage (else the stream's pool)
-    TaskDesc* d_tasks = nullptr;
-    SegDesc* d_segs = nullptr;
     int64_t* d_pool = nullptr;
     size_t bytes = 0;
     int device = 0;
@@ -114,26 +106,19 @@ void static_storage(char*& base, size_t& bytes);
 class PlanBuilder
 {
   public:
-    PlanBuilder(int elem_size, int dtype, int use_mfma, int family = FAMILY_F64);
+    PlanBuilder(int elem_size, int dtype, int family = FAMILY_F64);
     ~PlanBuilder();
     void add(const GemmDesc& g) { gemms_.push_back(g); }
-    // A GEMM assembled from blocks (panel path only).
+    // A GEMM assembled from blocks.
     void add_panel(const PanelGemm& g);
-    // Will build() take the panel path?
-    bool panel_route() const;
-    // Choose swap/tile/thread maps, emit tasks, upload.  Returns nullptr if
+    // Orientation, tile, pack jobs, tasks; upload.  Returns nullptr if
     // there is nothing to launch.
     std::shared_ptr<Plan> build();
 
   private:
-    bool try_packed(Plan& plan, std::vector<TaskDesc>& tasks, std::vector<SegDesc>& segs);
-    SideRef side_ref(const Side& s, int64_t BS, int64_t start);
-    int64_t kblock_table(const Side& s, int64_t BK);
-    int elem_size_, dtype_, use_mfma_, family_;
+    int elem_size_, dtype_, family_;
     std::vector<GemmDesc> gemms_;
     std::unique_ptr<std::vector<PanelGemm>> panel_gemms_;
-    std::vector<int64_t> pool_;
-    std::map<std::vector<int64_t>, int64_t> dedupe_;
 };
 
 // Per-thread execution settings.

@@ -1,15 +1,8 @@
-// Device-side description of a contraction: a list of C tiles (tasks), each
-// with a list of k segments.  Shared by the host planner and the kernels.
-//
-// Every matrix element is addressed as
-//     data + base + row_offset(i) + col_offset(j)
-// where a side's offset is either affine, (start + i) * stride, when that
-// side of the matrix view is a single (folded) dimension, or read from the
-// scatter pool, pool[start + i], which holds the block-scatter vectors of
-// every operand and patch.  This is the reference's block-stride fast path
-// (src/matrix/block_scatter_matrix.hpp:232-247, src/nodes/gemm_ukr.hpp:279-363)
-// applied to whole sides.  A dense contraction has one patch per operand; a
-// DPD contraction one per irrep block; a batch one per instance.
+// Device-side description of a contraction on the panel path: pack
+// chunks, GEMM tasks (C tiles) and C unpack jobs, with the offset tables
+// they index in one int64 pool.  Shared by the host planner and the
+// kernels.  A dense contraction has one block per operand; a DPD
+// contraction one per irrep block; a batch one per instance.
 #pragma once
 
 #include <cstdint>
@@ -18,53 +11,6 @@ namespace tblis_amd
 {
 
 constexpr int64_t SCATTERED = INT64_MIN;
-
-struct SideRef
-{
-    int64_t start;   // pool index (scattered) or first logical index (affine)
-    int64_t stride;  // SCATTERED, or the element stride of the side
-};
-
-struct TaskDesc
-{
-    int64_t c_base;  // element offset of this C patch
-    SideRef c_row;   // C rows of the tile (start already at the tile's first row)
-    SideRef c_col;   // C cols of the tile
-    int32_t mlen;    // valid rows in the tile (1..BM)
-    int32_t nlen;    // valid cols in the tile (1..BN)
-    int32_t seg0;    // first k segment
-    int32_t nseg;    // number of k segments (0: C <- beta*C)
-};
-
-struct SegDesc
-{
-    int64_t a_base, b_base;  // element offsets of the A and B patches
-    SideRef a_row;           // A rows of the tile
-    SideRef a_col;           // A cols (k) of the segment
-    SideRef b_row;           // B rows (k) of the segment
-    SideRef b_col;           // B cols of the tile
-    int64_t klen;            // k extent of the segment (>= 1)
-    // Scattered k sides: pool index of a table of (offset of the block's
-    // first k, block stride or SCATTERED) pairs, one per BK-block of k, so
-    // a k tile inside which the side is affine is addressed without a
-    // per-lane lookup.  Unused (-1) for affine sides.
-    int64_t a_kblk, b_kblk;
-};
-
-// One operand (A or B after the orientation swap) packed into split-BF16
-// panels: rows (m or n) x k, gathered through the scatter description and
-// written as chunks [3 planes][PR rows][PK k] BF16, chunk (p, kt) at
-// out + ((p * nkt + kt) * 3 * PR * PK), zero-padded past nrows and klen.
-struct PackDesc
-{
-    int64_t base;   // element offset of the operand patch in its tensor
-    SideRef rows;   // row side (start 0)
-    SideRef kside;  // k side (start 0)
-    int64_t ktbl;   // k block table (pool index) or -1
-    int64_t nrows, klen;
-    int64_t out;    // BF16 element offset of the panels in the workspace
-    int32_t nrt, nkt;
-};
 
 // ---------------------------------------------------------------- panels
 // FP64 panel path.  Each GEMM operand (A': rows = m, B': rows = n, both
@@ -182,12 +128,9 @@ constexpr int FAMILY_OZAKI = 3;        // FP64-accurate: int8 digit planes (gemm
 // Kernel variant selector.
 struct KernelKey
 {
-    int family;    // FAMILY_*
-    int tile;      // index into the family's tile table
-    int a_mfast;   // A tile loads coalesced along m (1) or k (0)
-    int b_nfast;   // B tile loads coalesced along n (1) or k (0)
-    int dtype;     // 0: double in/out, 1: float in/out (widened to FP64)
-    int use_mfma;  // 1: v_mfma_f64_16x16x4; 0: plain FMA (cross-check)
+    int family;  // FAMILY_*
+    int tile;    // index into the family's tile table
+    int dtype;   // 0: double in/out, 1: float in/out
 };
 
 }  // namespace tblis_amd

@@ -106,26 +106,6 @@ def test_float_outlier_workload_full_sampled():
     assert err <= SPLIT_TOL
 
 
-@pytest.mark.parametrize("tile", range(3))
-def test_every_bf16_tile(tile, monkeypatch):
-    """Each tile of the direct-gather split-BF16 kernel (panel path off)."""
-    monkeypatch.setenv("TBLIS_HIP_PANEL", "0")
-    monkeypatch.setenv("TBLIS_HIP_TILE", str(tile))
-    T.lib().tblis_hip_clear_plan_cache()
-    try:
-        rng = np.random.default_rng(41000 + tile)
-        _float_case(rng, "aebf", "fjei", "aibj", dict(a=13, b=11, e=9, f=7, i=5, j=14), ["col"] * 3,
-                    uniform, alpha=0.5, beta=1.0)
-        _float_case(rng, "abef", "efij", "abij", dict(a=37, b=3, e=70, f=1, i=29, j=2),
-                    ["row", "col", "random"], uniform)
-        ext = {c: [3, 5, 0, 2] for c in "abefij"}
-        _dpd_case(rng, 4, "abef", "efij", "abij", ext, 1, 3, 5, alpha=1.0, beta=0.5,
-                  dtype=np.float32, tol=SPLIT_TOL)
-    finally:
-        monkeypatch.delenv("TBLIS_HIP_TILE")
-        T.lib().tblis_hip_clear_plan_cache()
-
-
 def test_fast_variant_small_k():
     rng = np.random.default_rng(49)
     ext = dict(a=20, b=9, e=12, f=10, i=7, j=11)
@@ -178,22 +158,6 @@ def test_every_bf16_panel_tile(tile, config, monkeypatch):
     finally:
         monkeypatch.delenv("TBLIS_HIP_PANEL_TILE")
         T.lib().tblis_hip_clear_plan_cache()
-
-
-@pytest.mark.parametrize("seed", range(4))
-def test_packed_split_path(seed, monkeypatch):
-    """The earlier single-GEMM packed split-BF16 kernels (panel path off):
-    shapes big enough for them (m, n >= 256), with ragged edges, k not a
-    multiple of 16 and multi-dimensional (scattered) sides."""
-    monkeypatch.setenv("TBLIS_HIP_PANEL", "0")
-    T.lib().tblis_hip_clear_plan_cache()
-    rng = np.random.default_rng(42000 + seed)
-    ext = [dict(a=20, b=15, e=37, f=1, i=13, j=20), dict(a=17, b=19, e=3, f=11, i=16, j=21),
-           dict(a=64, b=5, e=16, f=16, i=3, j=100), dict(a=9, b=31, e=7, f=5, i=29, j=11)][seed]
-    kinds = [["col"] * 3, ["row", "col", "random"], ["random", "random", "col"], ["col", "row", "row"]][seed]
-    _float_case(rng, "aebf", "fjei", "aibj", ext, kinds, outlier_normal, alpha=0.5,
-                beta=[0.0, 1.0, -1.0, 0.25][seed])
-    T.lib().tblis_hip_clear_plan_cache()
 
 
 def test_packed_alpha_zero_and_beta():

@@ -68,8 +68,9 @@ def test_dense_random_fp64(seed):
 
 
 @pytest.mark.parametrize("seed", range(8))
-def test_dense_random_fma_kernel(seed):
-    """The non-MFMA kernel built from the same code path agrees too."""
+def test_dense_random_reference_config(seed):
+    """The reference's portable configuration name ("reference") selects the
+    same MFMA kernels and agrees with the oracle."""
     rng = np.random.default_rng(11000 + seed)
     iA, iB, iC, ext = random_contraction(rng, max_dims=3, max_len=6)
     _dense_case(rng, iA, iB, iC, ext, ["random"] * 3, config="reference")
@@ -351,28 +352,6 @@ def test_batch_uneven():
         assert rel_fro(got, refs[i]) <= FP64_TOL, i
 
 
-@pytest.mark.parametrize("tile", range(8))
-def test_every_tile_variant(tile, monkeypatch):
-    """Each tile shape of the direct-gather FP64 kernel (panel path off,
-    tile forced through TBLIS_HIP_TILE) agrees with the oracle on
-    contractions with ragged edges, scattered k and a DPD problem (several
-    k segments)."""
-    monkeypatch.setenv("TBLIS_HIP_PANEL", "0")
-    monkeypatch.setenv("TBLIS_HIP_TILE", str(tile))
-    T.lib().tblis_hip_clear_plan_cache()
-    try:
-        rng = np.random.default_rng(30000 + tile)
-        _dense_case(rng, "aebf", "fjei", "aibj", dict(a=13, b=11, e=9, f=7, i=5, j=14),
-                    ["col"] * 3, alpha=0.5, beta=1.0)
-        _dense_case(rng, "abef", "efij", "abij", dict(a=37, b=3, e=70, f=1, i=29, j=2),
-                    ["row", "col", "random"])
-        ext = {c: [3, 5, 0, 2] for c in "abefij"}
-        _dpd_case(rng, 4, "abef", "efij", "abij", ext, 1, 3, 5, alpha=1.0, beta=0.5)
-    finally:
-        monkeypatch.delenv("TBLIS_HIP_TILE")
-        T.lib().tblis_hip_clear_plan_cache()
-
-
 @pytest.mark.parametrize("direct", ["1", "0"])
 @pytest.mark.parametrize("tile", range(8))
 def test_every_panel_tile(tile, direct, monkeypatch):
@@ -407,20 +386,6 @@ def test_every_panel_tile(tile, direct, monkeypatch):
     finally:
         monkeypatch.delenv("TBLIS_HIP_PANEL_TILE")
         monkeypatch.delenv("TBLIS_HIP_DIRECT")
-        T.lib().tblis_hip_clear_plan_cache()
-
-
-def test_direct_path_dpd_and_batch(monkeypatch):
-    """The direct-gather kernels (panel path off) still serve DPD and batches."""
-    monkeypatch.setenv("TBLIS_HIP_PANEL", "0")
-    T.lib().tblis_hip_clear_plan_cache()
-    try:
-        rng = np.random.default_rng(32000)
-        ext = {c: [4, 3, 2, 1, 5, 3, 2, 1] for c in "abefij"}
-        _dpd_case(rng, 8, "abef", "efij", "abij", ext, 0, 0, 5, alpha=0.5, beta=1.0)
-        test_batch_uneven()
-    finally:
-        monkeypatch.delenv("TBLIS_HIP_PANEL")
         T.lib().tblis_hip_clear_plan_cache()
 
 

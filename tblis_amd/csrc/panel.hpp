@@ -60,6 +60,9 @@ bool c_unpack_forced();
 int direct_kfast_rereads();
 
 // Host tables of a panel plan.
+// Stages of a plan of many GEMMs: consecutive ranges of GEMMs of about
+// equal flops whose pack launches run on a second stream ahead of the GEMM
+// launches (the pack of stage s + 1 overlaps the GEMMs of stage s).
 struct PanelTables
 {
     std::vector<PanelJob> jobs;
@@ -76,6 +79,11 @@ struct PanelTables
     // tasks come in four consecutive groups by which operands they read from
     // the tensors (none, A, B, both); one GEMM launch per non-empty group
     int64_t group_n[4] = {0, 0, 0, 0};
+    // chunks and tasks ordered by stage: stage_chunks[s] pack chunks and
+    // stage_group_n[s][g] tasks of group g in stage s (group-major within
+    // the stage); task_stage is each task's stage (host bookkeeping)
+    PanelSchedule sched;
+    std::vector<int8_t> task_stage;
     int64_t ws_panel = 0;  // panel values (double for FP64, BF16 for split-BF16, bytes for Ozaki)
     int64_t ws_rows = 0;   // Ozaki: row-maximum entries (one per panel row)
     int64_t ws_c = 0;      // C panel doubles

@@ -264,21 +264,87 @@ void pin_plan(const std::shared_ptr<Plan>& p)
 
 }  // namespace
 
-void launch_panel_calls(const PanelLaunch& P, const int64_t group_n[4])
+namespace
 {
-    if (P.use_k) hip_check(launch_panel_pack(P), "pack launch");
-    // one GEMM launch per group of tasks with the same direct operands
-    int64_t at = 0;
-    for (int g = 0; g < 4; g++)
+
+// The side stream and at least n events for staged packs, or false when
+// they do not exist yet and the stream is being captured (no creation
+// inside a capture: the call then runs its stages unstaged).
+bool stage_resources(ThreadState& ts, hipStream_t main, bool capturing, size_t n, hipStream_t& side)
+{
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    auto it = ts.side.find({dev, main});
+    if (it == ts.side.end() || ts.stage_ev.size() < n)
     {
-        if (group_n[g] == 0) continue;
-        PanelLaunch G = P;
-        G.tasks = P.tasks + at;
-        G.ntasks = (int)group_n[g];
-        G.direct_a = (g & 1) != 0;
-        G.direct_b = (g & 2) != 0;
-        hip_check(launch_panel_gemm(G), "kernel launch");
-        at += group_n[g];
+        if (capturing) return false;
+        if (it == ts.side.end())
+        {
+            hipStream_t s = nullptr;
+            hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+            it = ts.side.emplace(std::make_pair(dev, main), s).first;
+        }
+        while (ts.stage_ev.size() < n)
+        {
+            hipEvent_t e = nullptr;
+            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreateWithFlags");
+            ts.stage_ev.push_back(e);
+        }
+    }
+    side = it->second;
+    return true;
+}
+
+}  // namespace
+
+void launch_panel_calls(const PanelLaunch& P, const PanelSchedule& S, bool overlap)
+{
+    ThreadState& ts = thread_state();
+    hipStream_t side = nullptr;
+    bool staged = false;
+    if (overlap && S.nstages > 1 && P.use_k && P.nchunks > 0)
+    {
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        hip_check(hipStreamIsCapturing(P.stream, &cap), "hipStreamIsCapturing");
+        staged = stage_resources(ts, P.stream, cap != hipStreamCaptureStatusNone, (size_t)S.nstages + 1, side);
+    }
+    if (staged)
+    {
+        // fork: the side stream starts after the work queued on P.stream;
+        // stage s's pack signals stage_ev[s + 1]
+        hip_check(hipEventRecord(ts.stage_ev[0], P.stream), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(side, ts.stage_ev[0], 0), "hipStreamWaitEvent");
+        int64_t c0 = 0;
+        for (int s = 0; s < S.nstages; s++)
+        {
+            PanelLaunch K = P;
+            K.chunks = P.chunks + c0;
+            K.nchunks = (int)S.chunks[s];
+            K.stream = side;
+            hip_check(launch_panel_pack(K), "pack launch");
+            hip_check(hipEventRecord(ts.stage_ev[s + 1], side), "hipEventRecord");
+            c0 += S.chunks[s];
+        }
+    }
+    else if (P.use_k)
+        hip_check(launch_panel_pack(P), "pack launch");
+    // per stage, one GEMM launch per group of tasks with the same direct
+    // operands; the last stage's wait joins the side stream back
+    int64_t at = 0;
+    for (int s = 0; s < S.nstages; s++)
+    {
+        if (staged) hip_check(hipStreamWaitEvent(P.stream, ts.stage_ev[s + 1], 0), "hipStreamWaitEvent");
+        for (int g = 0; g < 4; g++)
+        {
+            if (S.group_n[s][g] == 0) continue;
+            PanelLaunch G = P;
+            G.tasks = P.tasks + at;
+            G.ntasks = (int)S.group_n[s][g];
+            G.direct_a = (g & 1) != 0;
+            G.direct_b = (g & 2) != 0;
+            hip_check(launch_panel_gemm(G), "kernel launch");
+            at += S.group_n[s][g];
+        }
     }
     hip_check(launch_panel_unpack(P), "unpack launch");
 }
@@ -326,7 +392,7 @@ void run_plan(const std::shared_ptr<Plan>& pp, const void* A, const void* B, voi
         hip_check(launch_panel_unpack(P), "unpack launch");
     }
     else
-        launch_panel_calls(P, p.group_n);
+        launch_panel_calls(P, p.sched, p.ws_static == nullptr);
     if (!ts.async && !capturing) hip_check(hipStreamSynchronize(ts.stream), "hipStreamSynchronize");
 }
 

@@ -53,6 +53,7 @@ struct Plan
     int64_t oz_rows = 0;        // Ozaki: its entries
     bool direct[2] = {false, false};  // FP64 operands read from the tensors (every task)
     int64_t group_n[4] = {0, 0, 0, 0};  // panel tasks reading none / A / B / both from the tensors
+    PanelSchedule sched;                // stages of chunks and task groups
     int p_nchunks = 0, p_nuchunks = 0;
     PackChunk* d_pchunks = nullptr;
     PanelTask* d_ptasks = nullptr;
@@ -130,6 +131,10 @@ struct ThreadState
     std::string last_error;
     std::string last_algo;
     int last_tile = -1;  // panel tile of the last launched plan (-1: other kernels)
+    // side streams for staged packs, by (device, main stream), and the
+    // events that order them (reused call after call)
+    std::map<std::pair<int, hipStream_t>, hipStream_t> side;
+    std::vector<hipEvent_t> stage_ev;
 };
 ThreadState& thread_state();
 
@@ -151,7 +156,11 @@ void run_plan(const std::shared_ptr<Plan>& p, const void* A, const void* B, void
 // The launches of a non-Ozaki panel plan: pack (when k is used), one GEMM
 // launch per group of tasks with the same directly read operands
 // (group_n[g] tasks from P.tasks on, g = direct A + 2 direct B), unpack.
-void launch_panel_calls(const PanelLaunch& P, const int64_t group_n[4]);
+// The pack, GEMM and unpack launches of a panel plan.  With overlap and
+// several stages the packs go to a side stream of the calling thread (one
+// per device and main stream, created on first use) and each stage's GEMMs
+// wait for its pack; otherwise one pack launch precedes the GEMMs.
+void launch_panel_calls(const PanelLaunch& P, const PanelSchedule& S, bool overlap);
 
 // Every device allocation the library makes goes through this (counted, so
 // that a test can show that a call made none).

@@ -125,6 +125,17 @@ constexpr int FAMILY_BF16X3 = 1;       // FP32 as 3-term split BF16, chunked acc
 constexpr int FAMILY_BF16X3_FAST = 2;  // same, single accumulator
 constexpr int FAMILY_OZAKI = 3;        // FP64-accurate: int8 digit planes (gemm_ozaki.hip)
 
+// Launch schedule of a panel plan: chunks and tasks ordered by stage;
+// stage s has chunks[s] pack chunks and group_n[s][g] GEMM tasks reading
+// operands directly by group g (0 none, 1 A, 2 B, 3 both), in that order.
+constexpr int kMaxStages = 8;
+struct PanelSchedule
+{
+    int nstages = 1;
+    int64_t chunks[kMaxStages] = {};
+    int64_t group_n[kMaxStages][4] = {};
+};
+
 // Kernel variant selector.
 struct KernelKey
 {

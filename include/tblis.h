@@ -248,6 +248,11 @@ void tblis_init_dpd_tensor_d(tblis_dpd_tensor* t, double scalar, unsigned irrep,
                              unsigned ndim, const len_type* len, double* data, int layout);
 void tblis_init_dpd_tensor_s(tblis_dpd_tensor* t, float scalar, unsigned irrep, unsigned nirrep,
                              unsigned ndim, const len_type* len, float* data, int layout);
+/* Complex DPD tensors: interleaved (re, im) elements in the same layout. */
+void tblis_init_dpd_tensor_c(tblis_dpd_tensor* t, scomplex scalar, unsigned irrep, unsigned nirrep,
+                             unsigned ndim, const len_type* len, scomplex* data, int layout);
+void tblis_init_dpd_tensor_z(tblis_dpd_tensor* t, dcomplex scalar, unsigned irrep, unsigned nirrep,
+                             unsigned ndim, const len_type* len, dcomplex* data, int layout);
 
 /* Number of stored elements of a DPD tensor (dpd_varray_base::size). */
 stride_type tblis_dpd_tensor_size(const tblis_dpd_tensor* t);

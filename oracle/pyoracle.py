@@ -270,6 +270,33 @@ def dpd_mult(alpha, dA, idxA, A, dB, idxB, B, beta, dC, idxC, Cd, mode=1):
     return out
 
 
+def complex_dpd_mult(alpha, dA, idxA, A, dB, idxB, B, beta, dC, idxC, C_, conj_A=False, conj_B=False):
+    """Complex DPD contraction (the reference instantiates DPD mult for its
+    complex types, src/iface/3t/mult.cxx:292-297) on complex128 storage
+    arrays: four real block-wise contractions (dpd_mult mode 1) of the
+    real and imaginary parts, combined by complex arithmetic,
+    P = op(A) op(B), returns alpha P + beta C (op conjugates when conj_X)."""
+    sa = -1.0 if conj_A else 1.0
+    sb = -1.0 if conj_B else 1.0
+    A = np.asarray(A, np.complex128)
+    B = np.asarray(B, np.complex128)
+    nc = int(np.asarray(C_).size)
+    parts = lambda x: (np.ascontiguousarray(x.real), np.ascontiguousarray(x.imag))
+    (ar, ai), (br, bi) = parts(A), parts(B)
+
+    def prod(x, y):
+        out = np.zeros(nc)
+        dpd_mult(1.0, dA, idxA, x, dB, idxB, y, 0.0, dC, idxC, out, mode=1)
+        return out
+
+    P = (prod(ar, br) - sa * sb * prod(ai, bi)) + 1j * (sb * prod(ar, bi) + sa * prod(ai, br))
+    # C's storage is its dpd_size elements; a longer array keeps the rest
+    out = np.array(C_, np.complex128)
+    n = dpd_size(dC)
+    out[:n] = alpha * P[:n] + beta * out[:n]
+    return out
+
+
 def dpd_patch_scatter(dA, idxA, dB, idxB, dC, idxC, which, irrep_ab, BS_row, BS_col,
                       max_scat=1 << 22):
     npatch = (C.c_int * 2)()

@@ -85,7 +85,8 @@ EXPORTED = [
     "tblis_init_matrix_z", "tblis_matrix_mult", "tblis_tensor_add", "tblis_tensor_scale",
     "tblis_tensor_set",
     "tblis_get_num_threads", "tblis_set_num_threads", "tblis_get_config",
-    "tblis_init_dpd_tensor_d", "tblis_init_dpd_tensor_s", "tblis_dpd_tensor_size",
+    "tblis_init_dpd_tensor_d", "tblis_init_dpd_tensor_s", "tblis_init_dpd_tensor_c",
+    "tblis_init_dpd_tensor_z", "tblis_dpd_tensor_size",
     "tblis_dpd_tensor_mult", "tblis_init_indexed_tensor_d", "tblis_init_indexed_tensor_s",
     "tblis_indexed_tensor_mult", "tblis_indexed_dpd_tensor_mult", "tblis_tensor_mult_batch",
     "tblis_hip_set_stream",
@@ -404,11 +405,13 @@ def dpd_size(irrep, nirrep, lens, layout="PREFIX_ROW_MAJOR"):
 
 
 def prepare_dpd_mult(alpha, A, irrep_A, idx_A, B, irrep_B, idx_B, beta, C_, irrep_C, idx_C, nirrep,
-                     lens_A, lens_B, lens_C, layout="PREFIX_ROW_MAJOR", config=None):
+                     lens_A, lens_B, lens_C, layout="PREFIX_ROW_MAJOR", config=None, conj_A=False,
+                     conj_B=False):
     """Build the descriptors of dpd_mult() once; returns a callable."""
     dA = _DpdDesc(A, irrep_A, nirrep, lens_A, layout, alpha)
     dB = _DpdDesc(B, irrep_B, nirrep, lens_B, layout)
     dC = _DpdDesc(C_, irrep_C, nirrep, lens_C, layout, beta)
+    dA.d.conj, dB.d.conj = int(bool(conj_A)), int(bool(conj_B))
     cfg = get_config(config) if config else None
     args = (None, cfg, C.byref(dA.d), idx_A.encode("latin1"), C.byref(dB.d),
             idx_B.encode("latin1"), C.byref(dC.d), idx_C.encode("latin1"))
@@ -422,12 +425,13 @@ def prepare_dpd_mult(alpha, A, irrep_A, idx_A, B, irrep_B, idx_B, beta, C_, irre
 
 
 def dpd_mult(alpha, A, irrep_A, idx_A, B, irrep_B, idx_B, beta, C_, irrep_C, idx_C, nirrep,
-             lens_A, lens_B, lens_C, layout="PREFIX_ROW_MAJOR", config=None):
+             lens_A, lens_B, lens_C, layout="PREFIX_ROW_MAJOR", config=None, conj_A=False, conj_B=False):
     """DPD contraction over 1-D storage tensors A, B, C_ (tblis::mult over
-    dpd_varray_view, src/iface/3t/mult.h:53-72)."""
+    dpd_varray_view, src/iface/3t/mult.h:53-72); real or complex dtypes,
+    conj_A / conj_B conjugate a complex operand."""
     try:
         prepare_dpd_mult(alpha, A, irrep_A, idx_A, B, irrep_B, idx_B, beta, C_, irrep_C, idx_C,
-                         nirrep, lens_A, lens_B, lens_C, layout, config)()
+                         nirrep, lens_A, lens_B, lens_C, layout, config, conj_A, conj_B)()
     except TblisError as e:
         raise TblisError("tblis_dpd_tensor_mult: %s" % e) from None
     return C_

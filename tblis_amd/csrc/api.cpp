@@ -301,6 +301,36 @@ void tblis_init_dpd_tensor_s(tblis_dpd_tensor* t, float scalar, unsigned irrep, 
     t->len = len;
 }
 
+void tblis_init_dpd_tensor_c(tblis_dpd_tensor* t, scomplex scalar, unsigned irrep, unsigned nirrep,
+                             unsigned ndim, const len_type* len, scomplex* data, int layout)
+{
+    std::memset(t, 0, sizeof(*t));
+    t->type = TYPE_SCOMPLEX;
+    std::memcpy(&t->scalar.data, &scalar, sizeof(scalar));
+    t->scalar.type = TYPE_SCOMPLEX;
+    t->data = data;
+    t->ndim = ndim;
+    t->irrep = irrep;
+    t->nirrep = nirrep;
+    t->layout = layout;
+    t->len = len;
+}
+
+void tblis_init_dpd_tensor_z(tblis_dpd_tensor* t, dcomplex scalar, unsigned irrep, unsigned nirrep,
+                             unsigned ndim, const len_type* len, dcomplex* data, int layout)
+{
+    std::memset(t, 0, sizeof(*t));
+    t->type = TYPE_DCOMPLEX;
+    std::memcpy(&t->scalar.data, &scalar, sizeof(scalar));
+    t->scalar.type = TYPE_DCOMPLEX;
+    t->data = data;
+    t->ndim = ndim;
+    t->irrep = irrep;
+    t->nirrep = nirrep;
+    t->layout = layout;
+    t->len = len;
+}
+
 stride_type tblis_dpd_tensor_size(const tblis_dpd_tensor* t)
 {
     return guarded(

@@ -152,6 +152,69 @@ Level1Desc c_desc(const tblis_tensor& C, const label_type* idx_C)
 
 }  // namespace
 
+void complex_split_gemms(const std::vector<PanelGemm>& in, double sa, double sb, std::vector<PanelGemm>& out)
+{
+    std::vector<Term> terms[2];
+    terms[0] = {{0, 0, 1.0}, {1, 1, -sa * sb}};
+    terms[1] = {{0, 1, sb}, {1, 0, sa}};
+    for (const PanelGemm& p : in) split_gemm(p, terms, out);
+}
+
+void run_complex_plan(const std::shared_ptr<Plan>& plan, const void* A, const void* B, void* C, double alpha_re,
+                      double alpha_im, double beta_re, double beta_im, const Level1Desc& cd, int dt)
+{
+    const Cplx alpha{alpha_re, alpha_im}, beta{beta_re, beta_im};
+    const bool use_k = alpha.re != 0.0 || alpha.im != 0.0;
+    const bool one = beta.re == 1.0 && beta.im == 0.0;
+    const bool bzero = beta.re == 0.0 && beta.im == 0.0;
+    const bool calpha = alpha.im != 0.0;
+    // C <- pre C before the GEMMs (pre: beta, or beta / alpha for a complex
+    // alpha), C <- alpha C after them for a complex alpha
+    Cplx pre = beta;
+    if (calpha && !bzero)
+    {
+        const double d = alpha.re * alpha.re + alpha.im * alpha.im;
+        pre = Cplx{(beta.re * alpha.re + beta.im * alpha.im) / d, (beta.im * alpha.re - beta.re * alpha.im) / d};
+    }
+    const bool pass = use_k ? (calpha ? !bzero : beta.im != 0.0) : !one;
+    if (plan->ntasks == 0 || !(use_k || !one)) return;
+    ThreadState& ts = thread_state();
+    if (pass)
+    {
+        const hipError_t e = launch_level1_cscale(dt, cd, pre.re, pre.im, C, ts.stream);
+        if (e != hipSuccess) fail(std::string("complex beta pass: ") + hipGetErrorString(e));
+    }
+    auto finish = [&]() {
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(ts.stream, &cap) != hipSuccess) fail("hipStreamIsCapturing failed");
+        if (!ts.async && cap == hipStreamCaptureStatusNone && hipStreamSynchronize(ts.stream) != hipSuccess)
+            fail("hipStreamSynchronize failed");
+    };
+    if (use_k && calpha)
+    {
+        // GEMMs accumulate P (over C scaled by beta / alpha, or over
+        // nothing when beta == 0), then C <- alpha C; the synchronising
+        // variant of run_plan must not return before the last pass
+        {
+            struct Restore
+            {
+                ThreadState& t;
+                bool async;
+                ~Restore() { t.async = async; }
+            } restore{ts, ts.async};
+            ts.async = true;
+            run_plan(plan, A, B, C, 1.0, bzero ? 0.0 : 1.0, true);
+        }
+        const hipError_t e = launch_level1_cscale(dt, cd, alpha.re, alpha.im, C, ts.stream);
+        if (e != hipSuccess) fail(std::string("complex alpha pass: ") + hipGetErrorString(e));
+        finish();
+    }
+    else if (use_k)
+        run_plan(plan, A, B, C, alpha.re, pass ? 1.0 : beta.re, true);
+    else
+        finish();
+}
+
 void complex_tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_type* idx_A,
                          const tblis_tensor* B, const label_type* idx_B, tblis_tensor* C,
                          const label_type* idx_C)
@@ -188,14 +251,9 @@ void complex_tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const l
         GemmDesc g;
         if (dense_gemm(c, g, 0, 0, 0))
         {
-            std::vector<Term> terms[2];
-            terms[0] = {{0, 0, 1.0}, {1, 1, -sa * sb}};
-            terms[1] = {{0, 1, sb}, {1, 0, sa}};
-            std::vector<PanelGemm> pg, cg;
-            panel_from_desc(g, pg);
-            for (const PanelGemm& p : pg) split_gemm(p, terms, cg);
             PlanBuilder pb(rtype == TYPE_DOUBLE ? 8 : 4, rtype == TYPE_DOUBLE ? 0 : 1, algo_family(algo));
-            for (const PanelGemm& p : cg) pb.add_panel(p);
+            pb.set_complex(sa, sb);
+            pb.add(g);
             plan = pb.build();
         }
         if (!plan) plan = std::make_shared<Plan>();
@@ -204,17 +262,6 @@ void complex_tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const l
 
     const bool use_k = alpha.re != 0.0 || alpha.im != 0.0;
     const bool one = beta.re == 1.0 && beta.im == 0.0;
-    const bool bzero = beta.re == 0.0 && beta.im == 0.0;
-    const bool calpha = alpha.im != 0.0;
-    // C <- pre C before the GEMMs (pre: beta, or beta / alpha for a complex
-    // alpha), C <- alpha C after them for a complex alpha
-    Cplx pre = beta;
-    if (calpha && !bzero)
-    {
-        const double d = alpha.re * alpha.re + alpha.im * alpha.im;
-        pre = Cplx{(beta.re * alpha.re + beta.im * alpha.im) / d, (beta.im * alpha.re - beta.re * alpha.im) / d};
-    }
-    const bool pass = use_k ? (calpha ? !bzero : beta.im != 0.0) : !one;
     if (plan->ntasks > 0 && (use_k || !one))
     {
         const bool k = use_k && plan->reads_ab;
@@ -228,45 +275,8 @@ void complex_tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const l
             }
             plan->mark_verified(A->data, B->data, C->data, k);
         }
-        ThreadState& ts = thread_state();
-        const Level1Desc cd = c_desc(*C, idx_C);
-        const int dt = rtype == TYPE_DOUBLE ? 0 : 1;
-        if (pass)
-        {
-            const hipError_t e = launch_level1_cscale(dt, cd, pre.re, pre.im, C->data, ts.stream);
-            if (e != hipSuccess) fail(std::string("complex beta pass: ") + hipGetErrorString(e));
-        }
-        if (use_k && calpha)
-        {
-            // GEMMs accumulate P (over C scaled by beta / alpha, or over
-            // nothing when beta == 0), then C <- alpha C; the synchronising
-            // variant of run_plan must not return before the last pass
-            {
-                struct Restore
-                {
-                    ThreadState& t;
-                    bool async;
-                    ~Restore() { t.async = async; }
-                } restore{ts, ts.async};
-                ts.async = true;
-                run_plan(plan, A->data, B->data, C->data, 1.0, bzero ? 0.0 : 1.0, true);
-            }
-            const hipError_t e = launch_level1_cscale(dt, cd, alpha.re, alpha.im, C->data, ts.stream);
-            if (e != hipSuccess) fail(std::string("complex alpha pass: ") + hipGetErrorString(e));
-            hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-            if (hipStreamIsCapturing(ts.stream, &cap) != hipSuccess) fail("hipStreamIsCapturing failed");
-            if (!ts.async && cap == hipStreamCaptureStatusNone && hipStreamSynchronize(ts.stream) != hipSuccess)
-                fail("hipStreamSynchronize failed");
-        }
-        else if (use_k)
-            run_plan(plan, A->data, B->data, C->data, alpha.re, pass ? 1.0 : beta.re, true);
-        else
-        {
-            hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-            if (hipStreamIsCapturing(ts.stream, &cap) != hipSuccess) fail("hipStreamIsCapturing failed");
-            if (!ts.async && cap == hipStreamCaptureStatusNone && hipStreamSynchronize(ts.stream) != hipSuccess)
-                fail("hipStreamSynchronize failed");
-        }
+        run_complex_plan(plan, A->data, B->data, C->data, alpha.re, alpha.im, beta.re, beta.im, c_desc(*C, idx_C),
+                         rtype == TYPE_DOUBLE ? 0 : 1);
     }
     reset_scalar(C);
 }

@@ -1,6 +1,8 @@
 // High-level contraction entry points behind the C ABI.
 #pragma once
 
+#include <memory>
+
 #include "common.hpp"
 
 namespace tblis_amd
@@ -47,6 +49,17 @@ int debug_dpd_patches(const tblis_dpd_tensor* A, const label_type* idx_A,
 
 // Complex (scomplex / dcomplex) contraction: two real panel GEMMs per
 // complex GEMM (complex.cpp).
+struct PanelGemm;
+struct Plan;
+struct Level1Desc;
+// The two real GEMMs (Re C, Im C) of each complex GEMM of in (offsets in
+// complex elements; sa / sb = -1 conjugates A / B).
+void complex_split_gemms(const std::vector<PanelGemm>& in, double sa, double sb, std::vector<PanelGemm>& out);
+// Runs a plan of split complex GEMMs with complex alpha and beta: real
+// ones go to the GEMM epilogue, complex ones to level-1 passes over the C
+// elements cd describes (complex elements; dt 0 double, 1 float).
+void run_complex_plan(const std::shared_ptr<Plan>& plan, const void* A, const void* B, void* C, double alpha_re,
+                      double alpha_im, double beta_re, double beta_im, const Level1Desc& cd, int dt);
 void complex_tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_type* idx_A,
                          const tblis_tensor* B, const label_type* idx_B, tblis_tensor* C,
                          const label_type* idx_C);

@@ -18,6 +18,7 @@
 #include "contract.hpp"
 #include "dpd.hpp"
 #include "panel.hpp"
+#include "level1.hpp"
 #include "plan.hpp"
 
 namespace tblis_amd
@@ -593,22 +594,50 @@ void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const l
 {
     if (!A || !B || !C) fail("DPD: null descriptor");
     if (A->type != B->type || A->type != C->type) fail("DPD: A, B and C must have the same type");
-    if (A->type != TYPE_SINGLE && A->type != TYPE_DOUBLE)
-        fail("complex tensor types are not supported by this library yet");
     const type_t type = A->type;
-    Algo algo = resolve_algo(cfg, type);
+    // complex types (src/iface/3t/mult.cxx:292-297 instantiates all four):
+    // each irrep GEMM as two real GEMMs over the interleaved parts
+    const bool cplx = type == TYPE_SCOMPLEX || type == TYPE_DCOMPLEX;
+    const type_t rtype = type == TYPE_DOUBLE || type == TYPE_DCOMPLEX ? TYPE_DOUBLE : TYPE_SINGLE;
+    Algo algo = resolve_algo(cfg, rtype);
     thread_state().last_algo = algo_name(algo);
+    const double sa = A->conj ? -1.0 : 1.0, sb = B->conj ? -1.0 : 1.0;
 
     std::string key;
     put(key, (int32_t)algo);
+    put(key, (int32_t)type);
+    put(key, sa);
+    put(key, sb);
     dpd_key(key, *A, idx_A);
     dpd_key(key, *B, idx_B);
     dpd_key(key, *C, idx_C);
 
-    double alpha = type == TYPE_SINGLE ? (double)(A->scalar.data.s * B->scalar.data.s)
-                                       : A->scalar.data.d * B->scalar.data.d;
-    const double beta = scalar_of(C->scalar, type);
-    const int es = type == TYPE_DOUBLE ? 8 : 4;
+    // alpha = alpha_A alpha_B and beta in the tensor type
+    double alpha = 0, alpha_im = 0, beta = 0, beta_im = 0;
+    if (type == TYPE_SINGLE)
+        alpha = (double)(A->scalar.data.s * B->scalar.data.s), beta = C->scalar.data.s;
+    else if (type == TYPE_DOUBLE)
+        alpha = A->scalar.data.d * B->scalar.data.d, beta = C->scalar.data.d;
+    else if (type == TYPE_SCOMPLEX)
+    {
+        const float ar = A->scalar.data.c[0], ai = A->scalar.data.c[1], br = B->scalar.data.c[0],
+                    bi = B->scalar.data.c[1];
+        alpha = (double)(ar * br - ai * bi);
+        alpha_im = (double)(ar * bi + ai * br);
+        beta = C->scalar.data.c[0];
+        beta_im = C->scalar.data.c[1];
+    }
+    else
+    {
+        const double ar = A->scalar.data.z[0], ai = A->scalar.data.z[1], br = B->scalar.data.z[0],
+                     bi = B->scalar.data.z[1];
+        alpha = ar * br - ai * bi;
+        alpha_im = ar * bi + ai * br;
+        beta = C->scalar.data.z[0];
+        beta_im = C->scalar.data.z[1];
+    }
+    const int rs = rtype == TYPE_DOUBLE ? 8 : 4;  // bytes of a real part
+    const int es = cplx ? 2 * rs : rs;
     auto total = [&](const tblis_dpd_tensor* t) {
         DpdLayout L;
         L.init(t->irrep, t->nirrep, t->ndim, t->len, t->layout);
@@ -623,10 +652,11 @@ void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const l
     {
         DpdProblem P;
         P.init(A, idx_A, B, idx_B, C, idx_C);
-        PlanBuilder pb(type == TYPE_DOUBLE ? 8 : 4, dtype_code(type), algo_family(algo));
+        PlanBuilder pb(rs, dtype_code(rtype), algo_family(algo));
+        if (cplx) pb.set_complex(sa, sb);
         // the autotuner may time its candidates on A and B (read only):
         // check them first
-        const bool reads = alpha != 0.0 && total(A) > 0 && total(B) > 0;
+        const bool reads = (alpha != 0.0 || alpha_im != 0.0) && total(A) > 0 && total(B) > 0;
         if (reads)
         {
             check(A, "A");
@@ -650,10 +680,39 @@ void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const l
             for (unsigned j = 0; j < B->ndim; j++)
                 for (unsigned k = 0; k < C->ndim; k++)
                     if (idx_A[i] == idx_B[j] && idx_A[i] == idx_C[k]) has_abc = true;
-        if (!has_abc && (A->irrep ^ (unsigned)(B->irrep != C->irrep)) != 0) alpha = 0.0;
+        if (!has_abc && (A->irrep ^ (unsigned)(B->irrep != C->irrep)) != 0) alpha = alpha_im = 0.0;
     }
 
-    if (plan->ntasks > 0)
+    if (cplx && plan->ntasks > 0)
+    {
+        const bool use_k = alpha != 0.0 || alpha_im != 0.0;
+        if (use_k || beta != 1.0 || beta_im != 0.0)
+        {
+            const bool k = use_k && plan->reads_ab;
+            if (!plan->already_verified(A->data, B->data, C->data, k))
+            {
+                check(C, "C");
+                if (k)
+                {
+                    check(A, "A");
+                    check(B, "B");
+                }
+                plan->mark_verified(A->data, B->data, C->data, k);
+            }
+            // C's storage holds exactly its symmetry-allowed blocks, all of
+            // which the contraction writes: complex scalings run over it flat
+            Level1Desc cd;
+            const int64_t n = total(C);
+            cd.nb = n;
+            cd.ndb = 1;
+            cd.len_b[0] = n;
+            cd.sb[0] = 1;
+            cd.sa_b[0] = 0;
+            run_complex_plan(plan, A->data, B->data, C->data, alpha, alpha_im, beta, beta_im, cd,
+                             rtype == TYPE_DOUBLE ? 0 : 1);
+        }
+    }
+    else if (plan->ntasks > 0)
     {
         const bool use_k = alpha != 0.0;
         if (use_k || beta != 1.0)
@@ -674,8 +733,12 @@ void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const l
     }
     if (type == TYPE_SINGLE)
         C->scalar.data.s = 1.0f;
-    else
+    else if (type == TYPE_DOUBLE)
         C->scalar.data.d = 1.0;
+    else if (type == TYPE_SCOMPLEX)
+        C->scalar.data.c[0] = 1.0f, C->scalar.data.c[1] = 0.0f;
+    else
+        C->scalar.data.z[0] = 1.0, C->scalar.data.z[1] = 0.0;
     C->conj = 0;
 }
 

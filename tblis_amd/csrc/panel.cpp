@@ -389,6 +389,18 @@ int plan_stages(size_t ngemms, int family)
     return (int)std::max<size_t>(1, std::min<size_t>((size_t)s, ngemms));
 }
 
+// GEMM count from which each GEMM of a plan is one XCD region
+// (TBLIS_HIP_WHOLE_GEMM_REGIONS=<n>, read at every plan build; 0: never).
+int64_t whole_gemm_regions()
+{
+    if (const char* e = std::getenv("TBLIS_HIP_WHOLE_GEMM_REGIONS"))
+    {
+        const long v = std::atol(e);
+        return v <= 0 ? INT64_MAX : v;
+    }
+    return 64;
+}
+
 bool direct_enabled()
 {
     const char* e = std::getenv("TBLIS_HIP_DIRECT");
@@ -934,9 +946,13 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
                 }
         }
         // XCD regions: the tile grid cut into RM x RN rectangles (RM * RN <= 8),
-        // the cut that loads the fewest panel rows per region
+        // the cut that loads the fewest panel rows per region.  In plans of
+        // many GEMMs (batches) each GEMM is one region: its tiles then share
+        // one XCD's L2, where a cut GEMM has each XCD fetch its own copy of
+        // the operand panels, and the many GEMMs balance the XCDs.
         const int64_t MT = mp / BM, NT = np / BN;
         int64_t RM = 1, RN = 1;
+        if ((int64_t)gemms.size() < whole_gemm_regions())
         {
             int64_t best = -1;
             for (int64_t rm = 1; rm <= kXcds; rm *= 2)

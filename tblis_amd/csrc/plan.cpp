@@ -2,6 +2,7 @@
 #include <atomic>
 #include "plan.hpp"
 
+#include "contract.hpp"
 #include "panel.hpp"
 
 #include <algorithm>
@@ -85,6 +86,12 @@ std::shared_ptr<Plan> PlanBuilder::build()
     auto plan = std::make_shared<Plan>();
     std::vector<PanelGemm> pg(*panel_gemms_);
     for (auto& g : gemms_) panel_from_desc(g, pg);
+    if (complex_)
+    {
+        std::vector<PanelGemm> cg;
+        complex_split_gemms(pg, sa_, sb_, cg);
+        pg.swap(cg);
+    }
     if (!build_panel_plan(*plan, std::move(pg), elem_size_, dtype_, family_)) return nullptr;
     return plan;
 }

@@ -112,12 +112,22 @@ class PlanBuilder
     void add(const GemmDesc& g) { gemms_.push_back(g); }
     // A GEMM assembled from blocks.
     void add_panel(const PanelGemm& g);
+    // The GEMMs added are complex (offsets in complex elements, element
+    // size that of the real part): build() runs each as two real GEMMs.
+    void set_complex(double sa, double sb)
+    {
+        complex_ = true;
+        sa_ = sa;
+        sb_ = sb;
+    }
     // Orientation, tile, pack jobs, tasks; upload.  Returns nullptr if
     // there is nothing to launch.
     std::shared_ptr<Plan> build();
 
   private:
     int elem_size_, dtype_, family_;
+    bool complex_ = false;
+    double sa_ = 1.0, sb_ = 1.0;
     std::vector<GemmDesc> gemms_;
     std::unique_ptr<std::vector<PanelGemm>> panel_gemms_;
 };

@@ -104,3 +104,54 @@ def test_complex_scalar_reset():
                               ctypes.byref(d.d), b"ab")
     torch.cuda.synchronize()
     assert (d.d.scalar.data.z[0], d.d.scalar.data.z[1], d.d.conj) == (1.0, 0.0, 0)
+
+
+# ---------------------------------------------------------------- DPD
+def _dpd_complex_case(rng, nirrep, iA, iB, iC, ext, irA, irB, layout, dtype, alpha, beta, conj_A=False,
+                      conj_B=False):
+    irC = irA ^ irB
+    lens = lambda idx: [ext[c] for c in idx]
+    dA, dB, dC = (po.dpd_init(irA, nirrep, lens(iA), layout), po.dpd_init(irB, nirrep, lens(iB), layout),
+                  po.dpd_init(irC, nirrep, lens(iC), layout))
+    cx = lambda d: (rng.uniform(-1, 1, max(po.dpd_size(d), 1)) + 1j * rng.uniform(-1, 1, max(po.dpd_size(d), 1)))
+    A, B, C0 = cx(dA), cx(dB), cx(dC)
+    if dtype == torch.complex64:
+        A, B, C0 = (x.astype(np.complex64).astype(np.complex128) for x in (A, B, C0))
+    want = po.complex_dpd_mult(alpha, dA, iA, A, dB, iB, B, beta, dC, iC, C0, conj_A, conj_B)
+    At, Bt, Ct = (torch.from_numpy(x).to(dtype).to(DEV) for x in (A, B, C0))
+    T.dpd_mult(alpha, At, irA, iA, Bt, irB, iB, beta, Ct, irC, iC, nirrep, lens(iA), lens(iB), lens(iC),
+               layout=layout, conj_A=conj_A, conj_B=conj_B)
+    torch.cuda.synchronize()
+    got = Ct.cpu().to(torch.complex128).numpy()
+    nrm = np.linalg.norm(want)
+    err = np.linalg.norm(got - want) / (nrm if nrm > 0 else 1.0)
+    assert err <= TOL[dtype], (nirrep, iA, iB, iC, dtype, alpha, beta, conj_A, conj_B, err)
+    return err
+
+
+@pytest.mark.parametrize("seed", range(8))
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_complex_dpd_random(seed, dtype):
+    """Complex DPD contractions (the reference instantiates DPD mult for its
+    complex types, src/iface/3t/mult.cxx:292-297) against the oracle's
+    complex DPD restatement: random irreps, all six layouts, real and
+    complex alpha and beta, conjugated operands."""
+    rng = np.random.default_rng(66000 + seed)
+    nirrep = [1, 2, 4, 8][seed % 4]
+    cases = [("abef", "efij", "abij"), ("aebf", "fjei", "aibj"), ("abe", "eij", "abij"),
+             ("aeb", "ebcd", "acbd")]
+    iA, iB, iC = cases[seed % 4]
+    ext = {c: [int(rng.integers(0, 6)) for _ in range(nirrep)] for c in sorted(set(iA + iB + iC))}
+    irA, irB = int(rng.integers(0, nirrep)), int(rng.integers(0, nirrep))
+    alpha, beta = SCALARS[seed % len(SCALARS)]
+    _dpd_complex_case(rng, nirrep, iA, iB, iC, ext, irA, irB, seed % 6, dtype, alpha, beta,
+                      conj_A=bool(seed & 1), conj_B=bool(seed & 2))
+
+
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_complex_dpd_8_irreps(dtype):
+    """The 8-irrep DPD structure (XOR product table, PREFIX_ROW_MAJOR) at
+    per-irrep extents [6,5,4,3,6,4,3,2], complex alpha and beta."""
+    rng = np.random.default_rng(66100)
+    ext = {c: [6, 5, 4, 3, 6, 4, 3, 2] for c in "abefij"}
+    _dpd_complex_case(rng, 8, "abef", "efij", "abij", ext, 0, 0, 5, dtype, 0.75 + 0.5j, -0.5 + 1j)

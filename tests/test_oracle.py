@@ -480,6 +480,54 @@ def test_dpd_mult_blockwise_matches_full_expansion(seed):
     np.testing.assert_allclose(C1, C2, rtol=1e-12, atol=1e-12)
 
 
+def _dpd_dense(d, data, nirrep, lens):
+    """Dense expansion of a DPD tensor: every symmetry-allowed block (from
+    dpd_get_block) placed at its irreps' offsets, zeros elsewhere."""
+    import itertools
+    starts = [np.concatenate([[0], np.cumsum(l)]) for l in lens]
+    out = np.zeros([int(sum(l)) for l in lens], np.asarray(data).dtype)
+    for irr in itertools.product(range(nirrep), repeat=len(lens)):
+        if np.bitwise_xor.reduce(np.array(irr, np.int64)) != d.irrep:
+            continue
+        bl, off, st = po.dpd_get_block(d, list(irr))
+        if any(x == 0 for x in bl):
+            continue
+        blk = np.lib.stride_tricks.as_strided(np.asarray(data)[off:], bl,
+                                              [x * np.asarray(data).itemsize for x in st])
+        out[tuple(slice(starts[i][irr[i]], starts[i][irr[i]] + bl[i]) for i in range(len(lens)))] = blk
+    return out
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_complex_dpd_mult_matches_dense_expansion(seed):
+    """complex_dpd_mult (four real block-wise contractions) against numpy's
+    complex einsum over the dense expansions of A and B, read back at C's
+    blocks; conjugated operands and complex alpha, beta."""
+    rng = np.random.default_rng(650 + seed)
+    nirrep = [1, 2, 4, 8][seed % 4]
+    iA, iB, iC, ext, dA, dB, dC = _dpd_problem(rng, nirrep, layout=seed % 6,
+                                               ndims=(2, 2, 2) if seed % 3 else (3, 3, 2))
+    cx = lambda n: rng.uniform(-1, 1, max(n, 1)) + 1j * rng.uniform(-1, 1, max(n, 1))
+    A, B, C0 = cx(po.dpd_size(dA)), cx(po.dpd_size(dB)), cx(po.dpd_size(dC))
+    cA, cB = bool(seed & 1), bool(seed & 2)
+    alpha, beta = 0.5 - 0.25j, 1.5 + 0.75j
+    got = po.complex_dpd_mult(alpha, dA, iA, A, dB, iB, B, beta, dC, iC, C0, cA, cB)
+    lens = lambda idx: [ext[c] for c in idx]
+    DA = _dpd_dense(dA, A, nirrep, lens(iA))
+    DB = _dpd_dense(dB, B, nirrep, lens(iB))
+    DA, DB = (np.conj(DA) if cA else DA), (np.conj(DB) if cB else DB)
+    P = np.einsum("%s,%s->%s" % (iA, iB, iC), DA, DB)
+    # C's storage positions of the dense product: expand a running index
+    pos = _dpd_dense(dC, np.arange(len(C0), dtype=np.float64), nirrep, lens(iC))
+    mask = _dpd_dense(dC, np.ones(len(C0)), nirrep, lens(iC)) > 0
+    n = po.dpd_size(dC)
+    want = C0.copy()
+    want[:n] *= beta  # an empty C leaves its placeholder storage alone
+    if n > 0:
+        want[pos[mask].astype(np.int64)] += alpha * P[mask]
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
 @pytest.mark.parametrize("seed", range(8))
 def test_dpd_patch_scatter_addresses(seed):
     """Every patch of the irrep_AB GEMM addresses exactly the elements of

@@ -257,7 +257,9 @@ void tblis_init_dpd_tensor_z(tblis_dpd_tensor* t, dcomplex scalar, unsigned irre
 /* Number of stored elements of a DPD tensor (dpd_varray_base::size). */
 stride_type tblis_dpd_tensor_size(const tblis_dpd_tensor* t);
 
-/* DPD contraction; same semantics as the dense call, block by block.  Each
+/* DPD contraction; same semantics as the dense call, block by block, for
+ * all four types (complex: two real GEMMs per irrep GEMM over the
+ * interleaved parts, conj honoured).  Each
  * contracted irrep is one GEMM over its symmetry-allowed blocks (when every
  * block address splits into a row part and a column part, the six MArray
  * layouts for the usual label orders, one GEMM with explicit row and k
@@ -365,7 +367,9 @@ void tblis_indexed_dpd_tensor_mult(const tblis_comm* comm, const tblis_config* c
 /* workgroup per C tile, grouped by which operands are read straight from */
 /* the tensors), after one pack launch for the other operands.  The       */
 /* reference has no counterpart; its equivalent is a host loop of         */
-/* tblis_tensor_mult calls.                                               */
+/* tblis_tensor_mult calls.  Every instance shares alpha, beta, the type  */
+/* and the conj flags; complex types run as two real GEMMs per instance,  */
+/* a complex alpha or beta adds one scaling pass per instance's C.        */
 /* ---------------------------------------------------------------------- */
 void tblis_tensor_mult_batch(const tblis_comm* comm, const tblis_config* cfg, len_type count,
                              const tblis_tensor* const* A, const label_type* const* idx_A,

@@ -591,12 +591,16 @@ def indexed_dpd_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None):
     return C_
 
 
-def prepare_batch(alpha, As, idx_As, Bs, idx_Bs, beta, Cs, idx_Cs, config=None):
+def prepare_batch(alpha, As, idx_As, Bs, idx_Bs, beta, Cs, idx_Cs, config=None, conj_A=False, conj_B=False):
     """Build the descriptor arrays of mult_batch() once; returns a callable."""
     n = len(As)
     dA = [_Desc(a, alpha) for a in As]
     dB = [_Desc(b) for b in Bs]
     dC = [_Desc(c, beta) for c in Cs]
+    for d in dA:
+        d.d.conj = int(bool(conj_A))
+    for d in dB:
+        d.d.conj = int(bool(conj_B))
     T = C.POINTER(Tensor)
     pa = (T * n)(*[C.pointer(d.d) for d in dA])
     pb = (T * n)(*[C.pointer(d.d) for d in dB])
@@ -616,10 +620,12 @@ def prepare_batch(alpha, As, idx_As, Bs, idx_Bs, beta, Cs, idx_Cs, config=None):
     return _Prepared(run, (), (dA, dB, dC, pa, pb, pc, la, lb, lc))
 
 
-def mult_batch(alpha, As, idx_As, Bs, idx_Bs, beta, Cs, idx_Cs, config=None):
-    """Independent contractions C_i <- alpha*A_i.B_i + beta*C_i, one launch."""
+def mult_batch(alpha, As, idx_As, Bs, idx_Bs, beta, Cs, idx_Cs, config=None, conj_A=False, conj_B=False):
+    """Independent contractions C_i <- alpha*A_i.B_i + beta*C_i, one GEMM
+    launch per kernel variant; real or complex dtypes (conj_A / conj_B
+    conjugate a complex operand of every instance)."""
     try:
-        prepare_batch(alpha, As, idx_As, Bs, idx_Bs, beta, Cs, idx_Cs, config)()
+        prepare_batch(alpha, As, idx_As, Bs, idx_Bs, beta, Cs, idx_Cs, config, conj_A, conj_B)()
     except TblisError as e:
         raise TblisError("tblis_tensor_mult_batch: %s" % e) from None
     return Cs

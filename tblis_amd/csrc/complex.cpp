@@ -124,6 +124,8 @@ void split_gemm(const PanelGemm& g, const std::vector<Term> (&terms)[2], std::ve
 // descriptor in complex elements: C's diagonal (a repeated label addresses
 // one element per value, src/iface/3t/mult.cxx:44-48), length-1 dimensions
 // dropped, smallest stride first, contiguous dimensions folded.
+}  // namespace
+
 Level1Desc c_desc(const tblis_tensor& C, const label_type* idx_C)
 {
     Group g;
@@ -150,8 +152,6 @@ Level1Desc c_desc(const tblis_tensor& C, const label_type* idx_C)
     return d;
 }
 
-}  // namespace
-
 void complex_split_gemms(const std::vector<PanelGemm>& in, double sa, double sb, std::vector<PanelGemm>& out)
 {
     std::vector<Term> terms[2];
@@ -161,7 +161,7 @@ void complex_split_gemms(const std::vector<PanelGemm>& in, double sa, double sb,
 }
 
 void run_complex_plan(const std::shared_ptr<Plan>& plan, const void* A, const void* B, void* C, double alpha_re,
-                      double alpha_im, double beta_re, double beta_im, const Level1Desc& cd, int dt)
+                      double alpha_im, double beta_re, double beta_im, const std::vector<CPass>& cps, int dt)
 {
     const Cplx alpha{alpha_re, alpha_im}, beta{beta_re, beta_im};
     const bool use_k = alpha.re != 0.0 || alpha.im != 0.0;
@@ -179,11 +179,14 @@ void run_complex_plan(const std::shared_ptr<Plan>& plan, const void* A, const vo
     const bool pass = use_k ? (calpha ? !bzero : beta.im != 0.0) : !one;
     if (plan->ntasks == 0 || !(use_k || !one)) return;
     ThreadState& ts = thread_state();
-    if (pass)
-    {
-        const hipError_t e = launch_level1_cscale(dt, cd, pre.re, pre.im, C, ts.stream);
-        if (e != hipSuccess) fail(std::string("complex beta pass: ") + hipGetErrorString(e));
-    }
+    auto scale_c = [&](Cplx f, const char* what) {
+        for (const CPass& cp : cps)
+        {
+            const hipError_t e = launch_level1_cscale(dt, cp.desc, f.re, f.im, cp.data, ts.stream);
+            if (e != hipSuccess) fail(std::string(what) + ": " + hipGetErrorString(e));
+        }
+    };
+    if (pass) scale_c(pre, "complex beta pass");
     auto finish = [&]() {
         hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
         if (hipStreamIsCapturing(ts.stream, &cap) != hipSuccess) fail("hipStreamIsCapturing failed");
@@ -205,8 +208,7 @@ void run_complex_plan(const std::shared_ptr<Plan>& plan, const void* A, const vo
             ts.async = true;
             run_plan(plan, A, B, C, 1.0, bzero ? 0.0 : 1.0, true);
         }
-        const hipError_t e = launch_level1_cscale(dt, cd, alpha.re, alpha.im, C, ts.stream);
-        if (e != hipSuccess) fail(std::string("complex alpha pass: ") + hipGetErrorString(e));
+        scale_c(alpha, "complex alpha pass");
         finish();
     }
     else if (use_k)
@@ -275,8 +277,8 @@ void complex_tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const l
             }
             plan->mark_verified(A->data, B->data, C->data, k);
         }
-        run_complex_plan(plan, A->data, B->data, C->data, alpha.re, alpha.im, beta.re, beta.im, c_desc(*C, idx_C),
-                         rtype == TYPE_DOUBLE ? 0 : 1);
+        run_complex_plan(plan, A->data, B->data, C->data, alpha.re, alpha.im, beta.re, beta.im,
+                         {CPass{C->data, c_desc(*C, idx_C)}}, rtype == TYPE_DOUBLE ? 0 : 1);
     }
     reset_scalar(C);
 }

@@ -44,7 +44,7 @@ int elem_size(type_t t)
 void check_real(type_t t)
 {
     if (t != TYPE_SINGLE && t != TYPE_DOUBLE)
-        fail("complex tensor types are not supported by this library yet");
+        fail("complex tensor types are supported by the dense, DPD and batched contractions only");
 }
 
 double scalar_value(const tblis_scalar& s, type_t t)
@@ -391,9 +391,13 @@ void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tenso
     check_tensor(B[0], idx_B[0], "B");
     check_tensor(C[0], idx_C[0], "C");
     const type_t type = A[0]->type;
-    check_real(type);
+    // complex types: each instance's GEMM as two real GEMMs over the
+    // interleaved parts (the same split as the dense complex call)
+    const bool cplx = type == TYPE_SCOMPLEX || type == TYPE_DCOMPLEX;
+    const type_t rtype = type == TYPE_DOUBLE || type == TYPE_DCOMPLEX ? TYPE_DOUBLE : TYPE_SINGLE;
     const int es = elem_size(type);
-    double alpha = 0, beta = 0;
+    double alpha = 0, beta = 0, alpha_im = 0, beta_im = 0;
+    const double sa = A[0]->conj ? -1.0 : 1.0, sb = B[0]->conj ? -1.0 : 1.0;
     for (int64_t i = 0; i < count; i++)
     {
         check_tensor(A[i], idx_A[i], "A");
@@ -401,17 +405,38 @@ void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tenso
         check_tensor(C[i], idx_C[i], "C");
         if (A[i]->type != type || B[i]->type != type || C[i]->type != type)
             fail("batch: every tensor must have the same type");
-        double al = product_scalar(A[i]->scalar, B[i]->scalar, type);
-        double be = scalar_value(C[i]->scalar, type);
+        if ((A[i]->conj ? -1.0 : 1.0) != sa || (B[i]->conj ? -1.0 : 1.0) != sb)
+            fail("batch: every instance must conjugate the same operands");
+        double al, ali = 0, be, bei = 0;
+        if (cplx)
+        {
+            const bool z = type == TYPE_DCOMPLEX;
+            const double ar = z ? A[i]->scalar.data.z[0] : A[i]->scalar.data.c[0];
+            const double ai = z ? A[i]->scalar.data.z[1] : A[i]->scalar.data.c[1];
+            const double br = z ? B[i]->scalar.data.z[0] : B[i]->scalar.data.c[0];
+            const double bi = z ? B[i]->scalar.data.z[1] : B[i]->scalar.data.c[1];
+            al = ar * br - ai * bi;
+            ali = ar * bi + ai * br;
+            if (!z) al = (double)(float)al, ali = (double)(float)ali;
+            be = z ? C[i]->scalar.data.z[0] : C[i]->scalar.data.c[0];
+            bei = z ? C[i]->scalar.data.z[1] : C[i]->scalar.data.c[1];
+        }
+        else
+        {
+            al = product_scalar(A[i]->scalar, B[i]->scalar, type);
+            be = scalar_value(C[i]->scalar, type);
+        }
         if (i == 0)
         {
             alpha = al;
             beta = be;
+            alpha_im = ali;
+            beta_im = bei;
         }
-        else if (al != alpha || be != beta)
+        else if (al != alpha || be != beta || ali != alpha_im || bei != beta_im)
             fail("batch: every instance must use the same alpha and beta");
     }
-    Algo algo = resolve_algo(cfg, type);
+    Algo algo = resolve_algo(cfg, rtype);
     thread_state().last_algo = algo_name(algo);
 
     auto delta = [&](const void* p, const void* p0) -> int64_t {
@@ -423,6 +448,8 @@ void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tenso
     std::string key;
     put(key, (int32_t)algo);
     put(key, count);
+    put(key, sa);
+    put(key, sb);
     for (int64_t i = 0; i < count; i++)
     {
         tensor_key(key, *A[i], idx_A[i]);
@@ -436,7 +463,8 @@ void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tenso
     std::shared_ptr<Plan> plan = cache_get(key);
     if (!plan)
     {
-        PlanBuilder pb(es, dtype_code(type), family_of(algo));
+        PlanBuilder pb(cplx ? es / 2 : es, dtype_code(rtype), family_of(algo));
+        if (cplx) pb.set_complex(sa, sb);
         bool reads = false;
         for (int64_t i = 0; i < count; i++)
         {
@@ -454,7 +482,7 @@ void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tenso
         }
         // the autotuner may time its candidates on the A and B instances
         // (read only): check them first
-        reads &= alpha != 0.0;
+        reads &= alpha != 0.0 || alpha_im != 0.0;
         if (reads)
             for (int64_t i = 0; i < count; i++)
             {
@@ -468,8 +496,8 @@ void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tenso
     }
     if (plan->ntasks > 0)
     {
-        const bool use_k = alpha != 0.0;
-        if (use_k || beta != 1.0)
+        const bool use_k = alpha != 0.0 || alpha_im != 0.0;
+        if (use_k || beta != 1.0 || beta_im != 0.0)
         {
             const bool k = use_k && plan->reads_ab;
             if (!plan->already_verified(A[0]->data, B[0]->data, C[0]->data, k))
@@ -485,7 +513,17 @@ void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tenso
                 }
                 plan->mark_verified(A[0]->data, B[0]->data, C[0]->data, k);
             }
-            run_plan(plan, A[0]->data, B[0]->data, C[0]->data, alpha, beta, use_k);
+            if (cplx)
+            {
+                // complex alpha or beta: one level-1 pass per instance's C
+                std::vector<CPass> cps;
+                if (alpha_im != 0.0 || beta_im != 0.0 || !use_k)
+                    for (int64_t i = 0; i < count; i++) cps.push_back(CPass{C[i]->data, c_desc(*C[i], idx_C[i])});
+                run_complex_plan(plan, A[0]->data, B[0]->data, C[0]->data, alpha, alpha_im, beta, beta_im, cps,
+                                 rtype == TYPE_DOUBLE ? 0 : 1);
+            }
+            else
+                run_plan(plan, A[0]->data, B[0]->data, C[0]->data, alpha, beta, use_k);
         }
     }
     for (int64_t i = 0; i < count; i++) reset_scalar(C[i]);

@@ -4,6 +4,7 @@
 #include <memory>
 
 #include "common.hpp"
+#include "level1.hpp"
 
 namespace tblis_amd
 {
@@ -51,15 +52,24 @@ int debug_dpd_patches(const tblis_dpd_tensor* A, const label_type* idx_A,
 // complex GEMM (complex.cpp).
 struct PanelGemm;
 struct Plan;
-struct Level1Desc;
+// One complex scaling pass of run_complex_plan: C elements (complex)
+// described by desc at data.
+struct CPass
+{
+    void* data;
+    Level1Desc desc;
+};
+// The elements of C a contraction writes (C's diagonal for repeated
+// labels) as a level-1 descriptor in complex elements.
+Level1Desc c_desc(const tblis_tensor& C, const label_type* idx_C);
 // The two real GEMMs (Re C, Im C) of each complex GEMM of in (offsets in
 // complex elements; sa / sb = -1 conjugates A / B).
 void complex_split_gemms(const std::vector<PanelGemm>& in, double sa, double sb, std::vector<PanelGemm>& out);
 // Runs a plan of split complex GEMMs with complex alpha and beta: real
 // ones go to the GEMM epilogue, complex ones to level-1 passes over the C
-// elements cd describes (complex elements; dt 0 double, 1 float).
+// elements each pass describes (dt 0 double, 1 float).
 void run_complex_plan(const std::shared_ptr<Plan>& plan, const void* A, const void* B, void* C, double alpha_re,
-                      double alpha_im, double beta_re, double beta_im, const Level1Desc& cd, int dt);
+                      double alpha_im, double beta_re, double beta_im, const std::vector<CPass>& cps, int dt);
 void complex_tensor_mult(const tblis_config* cfg, const tblis_tensor* A, const label_type* idx_A,
                          const tblis_tensor* B, const label_type* idx_B, tblis_tensor* C,
                          const label_type* idx_C);

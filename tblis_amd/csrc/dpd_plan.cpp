@@ -708,7 +708,7 @@ void dpd_tensor_mult(const tblis_config* cfg, const tblis_dpd_tensor* A, const l
             cd.len_b[0] = n;
             cd.sb[0] = 1;
             cd.sa_b[0] = 0;
-            run_complex_plan(plan, A->data, B->data, C->data, alpha, alpha_im, beta, beta_im, cd,
+            run_complex_plan(plan, A->data, B->data, C->data, alpha, alpha_im, beta, beta_im, {CPass{C->data, cd}},
                              rtype == TYPE_DOUBLE ? 0 : 1);
         }
     }

@@ -155,3 +155,37 @@ def test_complex_dpd_8_irreps(dtype):
     rng = np.random.default_rng(66100)
     ext = {c: [6, 5, 4, 3, 6, 4, 3, 2] for c in "abefij"}
     _dpd_complex_case(rng, 8, "abef", "efij", "abij", ext, 0, 0, 5, dtype, 0.75 + 0.5j, -0.5 + 1j)
+
+
+# ---------------------------------------------------------------- batch
+@pytest.mark.parametrize("scalars", [(1.0, 0.0), (0.5 - 0.25j, 1.5 + 0.5j), (0.0, 0.5j)])
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_complex_batch(dtype, scalars):
+    """A batch of uneven complex contractions (tblis_tensor_mult_batch):
+    every instance against the oracle's complex restatement, with real and
+    complex alpha / beta, alpha = 0, and conjugated operands."""
+    alpha, beta = scalars
+    rng = np.random.default_rng(67000 + int(abs(alpha) * 10))
+    cA, cB = alpha != 1.0, alpha == 0.0
+    As, Bs, Cs, host = [], [], [], []
+    for _ in range(24):
+        e = {c: int(rng.integers(1, 9)) for c in "abefij"}
+        lA, lB, lC = [e[c] for c in "abef"], [e[c] for c in "efij"], [e[c] for c in "abij"]
+        mk = lambda l: (rng.uniform(-1, 1, int(np.prod(l))) + 1j * rng.uniform(-1, 1, int(np.prod(l))))
+        A, B, C0 = mk(lA), mk(lB), mk(lC)
+        if dtype == torch.complex64:
+            A, B, C0 = (x.astype(np.complex64).astype(np.complex128) for x in (A, B, C0))
+        cm = lambda l: [int(np.prod(l[:i])) for i in range(len(l))]
+        host.append((A, B, C0, lA, lB, lC, cm(lA), cm(lB), cm(lC)))
+        As.append(torch.as_strided(torch.from_numpy(A).to(dtype).to(DEV), lA, cm(lA)))
+        Bs.append(torch.as_strided(torch.from_numpy(B).to(dtype).to(DEV), lB, cm(lB)))
+        Cs.append(torch.from_numpy(C0.copy()).to(dtype).to(DEV))
+    Cv = [torch.as_strided(c, h[5], h[8]) for c, h in zip(Cs, host)]
+    T.mult_batch(alpha, As, ["abef"] * 24, Bs, ["efij"] * 24, beta, Cv, ["abij"] * 24, conj_A=cA, conj_B=cB)
+    torch.cuda.synchronize()
+    for c, (A, B, C0, lA, lB, lC, sA, sB, sC) in zip(Cs, host):
+        want = po.complex_tensor_mult(alpha, A, lA, sA, "abef", B, lB, sB, "efij", beta, C0, lC, sC, "abij",
+                                      cA, cB)
+        got = c.cpu().to(torch.complex128).numpy()
+        err = np.linalg.norm(got - want) / max(np.linalg.norm(want), 1e-300)
+        assert err <= TOL[dtype], (lA, lB, alpha, beta, err)

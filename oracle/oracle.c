@@ -28,7 +28,9 @@
  *     (batch labels, several MC/KC/NR blocks, beta==0 over NaN) and the
  *     column limit of the bounded CPU sample;
  *   - orc_dpd_mult_d: block-wise vs full dense expansion (mode 2), which are
- *     two different algorithms for src/internal/3t/dpd/mult.cxx:29-74.
+ *     two different algorithms for src/internal/3t/dpd/mult.cxx:29-74, and
+ *     (through pyoracle.complex_dpd_mult, four block-wise calls) numpy's
+ *     complex einsum over dense expansions built from orc_dpd_get_block.
  * tests/test_oracle.py holds each of these checks.
  */
 #include "oracle.h"

@@ -6,10 +6,12 @@
 // workgroup's 256 threads load the tile's A' (BM x 32) and B' (BN x 32)
 // elements with lanes along whichever side of the operand is unit-stride
 // (rows, or k), so that every wave reads whole cache lines, and store them
-// in LDS as [k][row] with a one-element pad per k row.  The next stage's
-// loads are in flight while the waves run this stage's MFMAs
+// in LDS as [k][row] with a one-element pad per k row (one stage per
+// operand in LDS).  The next stage's loads are in flight, into registers,
+// while the waves run this stage's MFMAs
 // (v_mfma_f64_16x16x4, operands read from LDS with one 8-byte load per lane
-// and block), one barrier per stage.  Element (r, k) of an operand is at
+// and block), two barriers per stage (before the next stage's stores and
+// after them).  Element (r, k) of an operand is at
 // tensor + src + rows[r] + koff(k), koff from the (base, stride) pair of
 // k's 8-k group or, where the group is not affine, the per-element k
 // offsets (PanelTask).  Loads go through buffer resources bounded by the
@@ -66,15 +68,32 @@ __device__ __forceinline__ void stage_load(double (&v)[R * kBK / kNT], const Lds
     constexpr int NE = R * kBK / kNT;
     if (!s.kfast)
     {
+        // k = k0 + tid / R + KT j: element j lies in the stage's k group
+        // (KT j) / 8 (tid / R < KT), so the four groups' (base, stride)
+        // pairs are loaded once per stage (wave-uniform addresses); only
+        // groups that are not affine read per-element k offsets
+        constexpr int KT = kNT / R;
+        static_assert(KT <= 8 && NE * KT == kBK, "stage shape");
         const int r = tid % R;
         const bool rin = r < s.rows;
         const uint32_t ro = srow[r];
+        const int g0 = k0 >> 3;
+        int64_t kb[kBK / 8], ks[kBK / 8];
+#pragma unroll
+        for (int q = 0; q < kBK / 8; q++)
+        {
+            const bool gin = k0 + 8 * q < klen;
+            kb[q] = gin ? s.kt[2 * (g0 + q)] : 0;
+            ks[q] = gin ? s.kt[2 * (g0 + q) + 1] : 0;
+        }
 #pragma unroll
         for (int j = 0; j < NE; j++)
         {
-            const int k = k0 + tid / R + (kNT / R) * j;
+            const int q = (KT * j) >> 3;
+            const int k = k0 + tid / R + KT * j;
             const bool in = rin && k < klen;
-            const uint32_t ko = k < klen ? k_offset(s, k) : 0u;
+            uint32_t ko = 0u;
+            if (k < klen) ko = ks[q] != SCATTERED ? (uint32_t)(kb[q] + (k & 7) * ks[q]) : (uint32_t)s.km[k];
             v[j] = lds_src_load<TC>(s.rsrc, in ? (ro + ko) * (uint32_t)sizeof(TC) : 0xffffffffu);
         }
     }
@@ -123,8 +142,11 @@ __global__ __launch_bounds__(kNT) void gemm_lds_kernel(const PanelTask* __restri
     static_assert(WGM * WGN * 64 == kNT, "four waves");
     constexpr int WM = BM / WGM, WN = BN / WGN, MB = WM / 16, NB = WN / 16;
     constexpr int NEA = BM * kBK / kNT, NEB = BN * kBK / kNT;
-    __shared__ double As[2][kBK * (BM + 1)];
-    __shared__ double Bs[2][kBK * (BN + 1)];
+    // one stage of each operand: the next stage waits in registers while
+    // the waves compute from LDS (half the LDS of double buffering, twice
+    // the resident workgroups)
+    __shared__ double As[kBK * (BM + 1)];
+    __shared__ double Bs[kBK * (BN + 1)];
     __shared__ uint32_t srow_a[BM], srow_b[BN];
 
     const int tid = threadIdx.x;
@@ -165,20 +187,19 @@ __global__ __launch_bounds__(kNT) void gemm_lds_kernel(const PanelTask* __restri
         {
             stage_load<BM, TC>(va, sa, srow_a, 0, t.klen, tid);
             stage_load<BN, TC>(vb, sb, srow_b, 0, t.klen, tid);
-            stage_store<BM>(As[0], va, sa.kfast, tid);
-            stage_store<BN>(Bs[0], vb, sb.kfast, tid);
+            stage_store<BM>(As, va, sa.kfast, tid);
+            stage_store<BN>(Bs, vb, sb.kfast, tid);
         }
         __syncthreads();
         for (int s = 0; s < nst; s++)
         {
-            const int cur = s & 1;
             if (s + 1 < nst)
             {
                 stage_load<BM, TC>(va, sa, srow_a, (s + 1) * kBK, t.klen, tid);
                 stage_load<BN, TC>(vb, sb, srow_b, (s + 1) * kBK, t.klen, tid);
             }
-            const double* __restrict__ a_s = As[cur] + wm * WM + lr;
-            const double* __restrict__ b_s = Bs[cur] + wn * WN + lr;
+            const double* __restrict__ a_s = As + wm * WM + lr;
+            const double* __restrict__ b_s = Bs + wn * WN + lr;
             double fa[2][MB], fb[2][NB];
 #pragma unroll
             for (int i = 0; i < MB; i++) fa[0][i] = a_s[lk * (BM + 1) + i * 16];
@@ -204,8 +225,9 @@ __global__ __launch_bounds__(kNT) void gemm_lds_kernel(const PanelTask* __restri
             }
             if (s + 1 < nst)
             {
-                stage_store<BM>(As[cur ^ 1], va, sa.kfast, tid);
-                stage_store<BN>(Bs[cur ^ 1], vb, sb.kfast, tid);
+                __syncthreads();  // every wave has read this stage
+                stage_store<BM>(As, va, sa.kfast, tid);
+                stage_store<BN>(Bs, vb, sb.kfast, tid);
             }
             __syncthreads();
         }

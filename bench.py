@@ -802,9 +802,10 @@ def main():
                                          "core; 256 sampled entries of its output within %.1e of "
                                          "the oracle" % (args.cpu_cols, fl, runs, dt, threads, err),
                                "host": host,
-                               "calibration": "this port on an 8-core Xeon @ 2.7 GHz (AVX2): C2 "
-                                              "154-158 GFLOP/s where the reference's own build ran "
-                                              "126.2 (1.22-1.26x), C1 107.7 against 95.5 (1.13x)"}
+                               "calibration": "this port on an 8-core Xeon @ 2.7 GHz (AVX2, 8 "
+                                              "threads, full contraction): C2 150-158 GFLOP/s in most "
+                                              "runs (one 218) where the reference's own build ran "
+                                              "126.2 (1.19-1.26x), C1 107.7 against 95.5 (1.13x)"}
     print(json.dumps(out))
     if tune_tmp:
         os.remove(tune_tmp)

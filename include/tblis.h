@@ -285,8 +285,12 @@ void tblis_dpd_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
  * dense_stride); the others are indexed.  Entry i is the dense sub-tensor at
  * data[i] (device memory), placed at the index values
  * indices[i * (ndim - dense_ndim) + j] of the indexed dimensions and
- * multiplied by factor[i] (NULL: every factor is 1).  Positions that no
- * entry lists are zero.  Entries must list distinct index values. */
+ * multiplied by factor[i] (NULL: every factor is 1; of the tensor's own
+ * type, so complex for scomplex / dcomplex entries).  Positions that no
+ * entry lists are zero.  Entries must list distinct index values.  All four
+ * types are accepted, as the reference instantiates them
+ * (src/iface/3t/mult.cxx:405-409); conj conjugates a complex operand's
+ * entries (not its factors). */
 typedef struct tblis_indexed_tensor
 {
     type_t type;
@@ -310,6 +314,14 @@ void tblis_init_indexed_tensor_s(tblis_indexed_tensor* t, float scalar, unsigned
                                  unsigned dense_ndim, const len_type* len,
                                  const stride_type* dense_stride, len_type num_indices,
                                  float* const* data, const len_type* indices, const float* factor);
+void tblis_init_indexed_tensor_c(tblis_indexed_tensor* t, scomplex scalar, unsigned ndim,
+                                 unsigned dense_ndim, const len_type* len,
+                                 const stride_type* dense_stride, len_type num_indices,
+                                 scomplex* const* data, const len_type* indices, const scomplex* factor);
+void tblis_init_indexed_tensor_z(tblis_indexed_tensor* t, dcomplex scalar, unsigned ndim,
+                                 unsigned dense_ndim, const len_type* len,
+                                 const stride_type* dense_stride, len_type num_indices,
+                                 dcomplex* const* data, const len_type* indices, const dcomplex* factor);
 
 /* Indexed contraction C <- alpha A.B + beta C with alpha = A.scalar *
  * B.scalar and beta = C.scalar, following the reference
@@ -330,7 +342,9 @@ void tblis_indexed_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
  * irrep ^ idx_irrep[0] ^ ... in the given layout (lengths len[d*nirrep + r]);
  * indexed dimension j has the fixed irrep idx_irrep[j] and its index values
  * count within that irrep.  Entry i is the DPD storage at data[i] (device
- * memory), at indices[i * (ndim - dense_ndim) + j], times factor[i]. */
+ * memory), at indices[i * (ndim - dense_ndim) + j], times factor[i] (of
+ * the tensor's type).  Real and complex types, as the reference
+ * instantiates them (src/iface/3t/mult.cxx:539-543). */
 typedef struct tblis_indexed_dpd_tensor
 {
     type_t type;

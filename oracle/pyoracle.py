@@ -387,28 +387,44 @@ def complex_tensor_mult(alpha, A, lenA, strA, idxA, B, lenB, strB, idxB, beta, C
 
 # ---------------------------------------------------------------- indexed
 
-def indexed_full(entries, indices, factors, dense_len, idx_len):
+def _dt(*xs):
+    """complex128 when any of xs (arrays, lists of arrays, scalars) is
+    complex, else float64."""
+    def cx(x):
+        if isinstance(x, (list, tuple)):
+            return any(cx(y) for y in x)
+        return np.iscomplexobj(x)
+    return np.complex128 if any(cx(x) for x in xs) else np.float64
+
+
+def indexed_full(entries, indices, factors, dense_len, idx_len, conj=False):
     """The full tensor an indexed tensor describes (dense dimensions first),
     as block_to_full builds it (src/internal/1t/indexed/util.hpp:14-43):
-    each entry, times its factor, written at its index values."""
-    full = np.zeros(list(dense_len) + list(idx_len))
+    each entry, times its factor, written at its index values (conj: the
+    complex conjugate of each entry, the factor as given)."""
+    dt = _dt(entries, factors)
+    full = np.zeros(list(dense_len) + list(idx_len), dt)
     nd = len(dense_len)
     for e, ix, f in zip(entries, indices, factors):
-        full[(slice(None),) * nd + tuple(ix)] = f * np.asarray(e, np.float64)
+        e = np.asarray(e, dt)
+        full[(slice(None),) * nd + tuple(ix)] = f * (np.conj(e) if conj else e)
     return full
 
 
-def indexed_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C):
+def indexed_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, conj_A=False, conj_B=False):
     """Indexed contraction on host arrays, the reference's FULL algorithm
     (src/internal/3t/indexed/mult.cxx:15-59) behind its front end
     (src/iface/3t/mult.cxx:299-402) and scaling rule
     (src/internal/3t/indexed/mult.cxx:288-296, src/internal/1t/indexed/
     scale.cxx:12-30, set.cxx:12-20).  Operands are dicts with "entries"
     (dense numpy arrays, logical shape), "indices" (tuples), "factors",
-    "dense_len", "idx_len"; returns C's new entries (float64)."""
+    "dense_len", "idx_len"; returns C's new entries (float64, or complex128
+    when any input is complex; conj_X conjugates X's entries)."""
+    dt = _dt(alpha, beta, A["entries"], A["factors"], B["entries"], B["factors"], C_["entries"],
+             C_["factors"])
     out = []
     for e, f in zip(C_["entries"], C_["factors"]):
-        e = np.asarray(e, np.float64)
+        e = np.asarray(e, dt)
         if beta == 0.0 or (beta != 1.0 and f == 0.0):
             out.append(np.zeros_like(e))
         elif beta != 1.0:
@@ -418,9 +434,9 @@ def indexed_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C):
     if alpha == 0.0:
         return out
     full = {}
-    for name, X in (("A", A), ("B", B)):
+    for name, X, cj in (("A", A, conj_A), ("B", B, conj_B)):
         full[name] = indexed_full(X["entries"], X["indices"], X["factors"], X["dense_len"],
-                                  X["idx_len"])
+                                  X["idx_len"], cj)
     prod = alpha * np.einsum("%s,%s->%s" % (idx_A, idx_B, idx_C), full["A"], full["B"])
     nd = len(C_["dense_len"])
     for i, (ix, f) in enumerate(zip(C_["indices"], C_["factors"])):
@@ -450,25 +466,26 @@ def indexed_dpd_expand(X):
         dense_irrep ^= r
     d = dpd_init(dense_irrep, nir, [lens[i] for i in range(nd)], X["layout"]) if nd else None
     entries = []
+    dt = _dt(X["entries"])
     for e in X["entries"]:
-        out = np.zeros(full[:nd])
+        out = np.zeros(full[:nd], dt)
+        ea = np.ascontiguousarray(np.asarray(e, dt))
         if nd:
             for irr in _irrep_blocks(nir, nd, dense_irrep):
                 bl, off, st = dpd_get_block(d, list(irr))
                 if 0 in bl:
                     continue
-                src = np.lib.stride_tricks.as_strided(np.asarray(e, np.float64)[off:], bl,
-                                                      [8 * x for x in st])
+                src = np.lib.stride_tricks.as_strided(ea[off:], bl, [ea.itemsize * x for x in st])
                 out[tuple(slice(ioff[i][irr[i]], ioff[i][irr[i]] + bl[i]) for i in range(nd))] = src
         else:
-            out = np.asarray(e, np.float64).reshape(())
+            out = ea.reshape(())
         entries.append(out)
     idx = [tuple(ioff[nd + j][X["idx_irrep"][j]] + v for j, v in enumerate(ix)) for ix in X["indices"]]
     return dict(entries=entries, indices=idx, factors=list(X["factors"]), dense_len=full[:nd],
                 idx_len=full[nd:]), d, dense_irrep, ioff
 
 
-def indexed_dpd_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C):
+def indexed_dpd_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, conj_A=False, conj_B=False):
     """Indexed DPD contraction on host storage by the reference's FULL
     algorithm (src/internal/3t/indexed_dpd/mult.cxx:68-118): A, B and C
     expanded to full indexed tensors, the indexed oracle above, and C's
@@ -479,17 +496,18 @@ def indexed_dpd_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C):
     ea, _, _, _ = indexed_dpd_expand(A)
     eb, _, _, _ = indexed_dpd_expand(B)
     ec, dc, cirr, ioff = indexed_dpd_expand(C_)
-    res = indexed_mult(alpha, ea, idx_A, eb, idx_B, beta, ec, idx_C)
+    res = indexed_mult(alpha, ea, idx_A, eb, idx_B, beta, ec, idx_C, conj_A, conj_B)
     nd, nir = C_["dense_ndim"], C_["nirrep"]
     out = []
+    dt = _dt(alpha, beta, A["entries"], B["entries"], C_["entries"], A["factors"], B["factors"], C_["factors"])
     for e0, full in zip(C_["entries"], res):
-        e = np.array(e0, np.float64)
+        e = np.array(e0, dt)
         if nd:
             for irr in _irrep_blocks(nir, nd, cirr):
                 bl, off, st = dpd_get_block(dc, list(irr))
                 if 0 in bl:
                     continue
-                dst = np.lib.stride_tricks.as_strided(e[off:], bl, [8 * x for x in st])
+                dst = np.lib.stride_tricks.as_strided(e[off:], bl, [e.itemsize * x for x in st])
                 dst[...] = full[tuple(slice(ioff[i][irr[i]], ioff[i][irr[i]] + bl[i]) for i in range(nd))]
         else:
             e[...] = full

@@ -88,6 +88,7 @@ EXPORTED = [
     "tblis_init_dpd_tensor_d", "tblis_init_dpd_tensor_s", "tblis_init_dpd_tensor_c",
     "tblis_init_dpd_tensor_z", "tblis_dpd_tensor_size",
     "tblis_dpd_tensor_mult", "tblis_init_indexed_tensor_d", "tblis_init_indexed_tensor_s",
+    "tblis_init_indexed_tensor_c", "tblis_init_indexed_tensor_z",
     "tblis_indexed_tensor_mult", "tblis_indexed_dpd_tensor_mult", "tblis_tensor_mult_batch",
     "tblis_hip_set_stream",
     "tblis_hip_get_stream", "tblis_hip_set_async", "tblis_hip_set_error_mode",
@@ -447,7 +448,8 @@ class Indexed:
         self.entries = list(entries)
         self.indices = [tuple(int(v) for v in ix) for ix in indices]
         self.idx_len = [int(x) for x in idx_len]
-        self.factors = None if factors is None else [float(f) for f in factors]
+        self.factors = None if factors is None else [complex(f) if isinstance(f, complex) else float(f)
+                                                    for f in factors]
         if len(self.indices) != len(self.entries):
             raise TblisError("Indexed: one index tuple per entry")
         if self.entries:
@@ -495,14 +497,20 @@ class _IxDesc:
         self.d.indices = C.cast(self.idx, C.POINTER(C.c_ssize_t))
         self.fac = None
         if X.factors is not None:
-            ft = C.c_double if self.d.type == TYPE_DOUBLE else C.c_float
-            self.fac = (ft * max(n, 1))(*X.factors)
+            ft = C.c_double if self.d.type in (TYPE_DOUBLE, TYPE_DCOMPLEX) else C.c_float
+            if self.d.type in (TYPE_SCOMPLEX, TYPE_DCOMPLEX):
+                # interleaved (re, im) per entry
+                vals = [v for f in X.factors for v in (complex(f).real, complex(f).imag)]
+                self.fac = (ft * max(2 * n, 2))(*vals)
+            else:
+                self.fac = (ft * max(n, 1))(*[float(f) for f in X.factors])
             self.d.factor = C.cast(self.fac, C.c_void_p)
 
 
-def prepare_indexed_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None):
+def prepare_indexed_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None, conj_A=False, conj_B=False):
     """Build the descriptors of indexed_mult() once; returns a callable."""
     dA, dB, dC = _IxDesc(A, alpha), _IxDesc(B), _IxDesc(C_, beta)
+    dA.d.conj, dB.d.conj = int(bool(conj_A)), int(bool(conj_B))
     cfg = get_config(config) if config else None
     args = (None, cfg, C.byref(dA.d), _labels(idx_A, A.ndim(), "A"), C.byref(dB.d),
             _labels(idx_B, B.ndim(), "B"), C.byref(dC.d), _labels(idx_C, C_.ndim(), "C"))
@@ -515,12 +523,13 @@ def prepare_indexed_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None
     return _Prepared(run, (), (dA, dB, dC))
 
 
-def indexed_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None):
+def indexed_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None, conj_A=False, conj_B=False):
     """Indexed contraction (tblis::mult over indexed_varray_view,
-    src/iface/3t/mult.h:74-86) over Indexed operands; C_'s entries are
+    src/iface/3t/mult.h:74-86) over Indexed operands (real or complex;
+    conj_A / conj_B conjugate a complex operand's entries); C_'s entries are
     updated in place."""
     try:
-        prepare_indexed_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config)()
+        prepare_indexed_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config, conj_A, conj_B)()
     except TblisError as e:
         raise TblisError("tblis_indexed_tensor_mult: %s" % e) from None
     return C_
@@ -540,7 +549,8 @@ class IndexedDpd:
         self.idx_irrep = [int(r) for r in idx_irrep]
         self.irrep, self.nirrep, self.dense_ndim = int(irrep), int(nirrep), int(dense_ndim)
         self.lens = [[int(x) for x in row] for row in lens]
-        self.factors = None if factors is None else [float(f) for f in factors]
+        self.factors = None if factors is None else [complex(f) if isinstance(f, complex) else float(f)
+                                                    for f in factors]
         self.layout = DPD_LAYOUTS[layout] if isinstance(layout, str) else int(layout)
         if len(self.indices) != len(self.entries):
             raise TblisError("IndexedDpd: one index tuple per entry")
@@ -573,15 +583,21 @@ class _IxDpdDesc:
         self.d.indices = C.cast(self.idx, C.POINTER(C.c_ssize_t))
         self.fac = None
         if X.factors is not None:
-            ft = C.c_double if self.d.type == TYPE_DOUBLE else C.c_float
-            self.fac = (ft * max(n, 1))(*X.factors)
+            ft = C.c_double if self.d.type in (TYPE_DOUBLE, TYPE_DCOMPLEX) else C.c_float
+            if self.d.type in (TYPE_SCOMPLEX, TYPE_DCOMPLEX):
+                vals = [v for f in X.factors for v in (complex(f).real, complex(f).imag)]
+                self.fac = (ft * max(2 * n, 2))(*vals)
+            else:
+                self.fac = (ft * max(n, 1))(*[float(f) for f in X.factors])
             self.d.factor = C.cast(self.fac, C.c_void_p)
 
 
-def indexed_dpd_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None):
+def indexed_dpd_mult(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, config=None, conj_A=False, conj_B=False):
     """Indexed DPD contraction (tblis::mult over indexed_dpd_varray_view,
-    src/iface/3t/mult.h:95-114) over IndexedDpd operands."""
+    src/iface/3t/mult.h:95-114) over IndexedDpd operands, real or complex
+    (conj_A / conj_B conjugate a complex operand)."""
     dA, dB, dC = _IxDpdDesc(A, alpha), _IxDpdDesc(B), _IxDpdDesc(C_, beta)
+    dA.d.conj, dB.d.conj = int(bool(conj_A)), int(bool(conj_B))
     cfg = get_config(config) if config else None
     L = lib()
     L.tblis_hip_set_error_mode(1)

@@ -389,6 +389,26 @@ void tblis_init_indexed_tensor_s(tblis_indexed_tensor* t, float scalar, unsigned
     t->scalar.data.s = scalar;
 }
 
+void tblis_init_indexed_tensor_c(tblis_indexed_tensor* t, scomplex scalar, unsigned ndim,
+                                 unsigned dense_ndim, const len_type* len,
+                                 const stride_type* dense_stride, len_type num_indices,
+                                 scomplex* const* data, const len_type* indices, const scomplex* factor)
+{
+    init_indexed(t, TYPE_SCOMPLEX, ndim, dense_ndim, len, dense_stride, num_indices, (void* const*)data,
+                 indices, factor);
+    std::memcpy(&t->scalar.data, &scalar, sizeof(scalar));
+}
+
+void tblis_init_indexed_tensor_z(tblis_indexed_tensor* t, dcomplex scalar, unsigned ndim,
+                                 unsigned dense_ndim, const len_type* len,
+                                 const stride_type* dense_stride, len_type num_indices,
+                                 dcomplex* const* data, const len_type* indices, const dcomplex* factor)
+{
+    init_indexed(t, TYPE_DCOMPLEX, ndim, dense_ndim, len, dense_stride, num_indices, (void* const*)data,
+                 indices, factor);
+    std::memcpy(&t->scalar.data, &scalar, sizeof(scalar));
+}
+
 void tblis_indexed_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
                                const tblis_indexed_tensor* A, const label_type* idx_A,
                                const tblis_indexed_tensor* B, const label_type* idx_B,

@@ -77,37 +77,62 @@ PBlock part(const PBlock& b, int im)
     return r;
 }
 
+// One real GEMM's k blocks: the parts of A and B they multiply (0 real,
+// 1 imaginary) and the real scale of each original k block.
 struct Term
 {
-    int a_im, b_im;  // 0: real parts, 1: imaginary parts
+    int a_im, b_im;
     double scale;
 };
 
-// One complex GEMM as the two real GEMMs of Re C and Im C.
-void split_gemm(const PanelGemm& g, const std::vector<Term> (&terms)[2], std::vector<PanelGemm>& out)
+// One complex GEMM (k block q scaled by s_q = kscale[q] + i kscale_im[q])
+// as the two real GEMMs of Re C and Im C.  With P_q = op(A_q) op(B_q),
+//   Re(s P) = sr (Ar Br - sa sb Ai Bi) - si (sb Ar Bi + sa Ai Br)
+//   Im(s P) = sr (sb Ar Bi + sa Ai Br) + si (Ar Br - sa sb Ai Bi);
+// terms whose scale is zero are left out.
+void split_gemm(const PanelGemm& g, double sa, double sb, std::vector<PanelGemm>& out)
 {
     const size_t NM = g.mlen.size(), NN = g.nlen.size(), NK = g.klen.size();
     for (int ci = 0; ci < 2; ci++)
     {
-        const std::vector<Term>& ts = terms[ci];
         PanelGemm r;
         r.mlen = g.mlen;
         r.nlen = g.nlen;
-        const size_t NK2 = ts.size() * NK;
-        for (const Term& t : ts)
+        // (k block, term) pairs in term-major order, as the real GEMM's k blocks
+        std::vector<std::pair<size_t, Term>> ks;
+        for (int pass = 0; pass < 4; pass++)
             for (size_t q = 0; q < NK; q++)
             {
-                r.klen.push_back(g.klen[q]);
-                r.kscale.push_back(t.scale);
+                const double sr = g.kscale.empty() ? 1.0 : g.kscale[q];
+                const double si = g.kscale_im.empty() ? 0.0 : g.kscale_im[q];
+                Term t{};
+                if (ci == 0)
+                {
+                    const Term all[4] = {{0, 0, sr}, {1, 1, -sa * sb * sr}, {0, 1, -sb * si}, {1, 0, -sa * si}};
+                    t = all[pass];
+                }
+                else
+                {
+                    const Term all[4] = {{0, 1, sb * sr}, {1, 0, sa * sr}, {0, 0, si}, {1, 1, -sa * sb * si}};
+                    t = all[pass];
+                }
+                if (t.scale != 0.0) ks.push_back({q, t});
             }
+        const size_t NK2 = ks.size();
+        for (const auto& kt : ks)
+        {
+            r.klen.push_back(g.klen[kt.first]);
+            r.kscale.push_back(kt.second.scale);
+        }
         r.a.resize(NM * NK2);
         r.b.resize(NN * NK2);
-        for (size_t ti = 0; ti < ts.size(); ti++)
-            for (size_t q = 0; q < NK; q++)
-            {
-                for (size_t p = 0; p < NM; p++) r.a[p * NK2 + ti * NK + q] = part(g.a[p * NK + q], ts[ti].a_im);
-                for (size_t s = 0; s < NN; s++) r.b[s * NK2 + ti * NK + q] = part(g.b[s * NK + q], ts[ti].b_im);
-            }
+        for (size_t x = 0; x < NK2; x++)
+        {
+            const size_t q = ks[x].first;
+            const Term& t = ks[x].second;
+            for (size_t p = 0; p < NM; p++) r.a[p * NK2 + x] = part(g.a[p * NK + q], t.a_im);
+            for (size_t s = 0; s < NN; s++) r.b[s * NK2 + x] = part(g.b[s * NK + q], t.b_im);
+        }
         for (const CBlock& c : g.c)
         {
             CBlock d;
@@ -154,10 +179,7 @@ Level1Desc c_desc(const tblis_tensor& C, const label_type* idx_C)
 
 void complex_split_gemms(const std::vector<PanelGemm>& in, double sa, double sb, std::vector<PanelGemm>& out)
 {
-    std::vector<Term> terms[2];
-    terms[0] = {{0, 0, 1.0}, {1, 1, -sa * sb}};
-    terms[1] = {{0, 1, sb}, {1, 0, sa}};
-    for (const PanelGemm& p : in) split_gemm(p, terms, out);
+    for (const PanelGemm& p : in) split_gemm(p, sa, sb, out);
 }
 
 void run_complex_plan(const std::shared_ptr<Plan>& plan, const void* A, const void* B, void* C, double alpha_re,

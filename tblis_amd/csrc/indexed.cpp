@@ -60,6 +60,8 @@ struct Ix
     int64_t n = 0;
     std::vector<int64_t> off;  // element offset of each entry from data[0]
     std::vector<double> fac;
+    std::vector<double> fim;   // imaginary parts of the factors (complex types)
+    bool zero(int64_t e) const { return fac[e] == 0.0 && fim[e] == 0.0; }
     int64_t value(int64_t e, int pos) const { return t->indices[e * nix + (pos - ndd)]; }
     int64_t stride(int pos) const { return t->dense_stride[pos]; }
     int pos_of(label_type l) const
@@ -98,6 +100,7 @@ void load(Ix& x, const tblis_indexed_tensor* t, const label_type* idx, const cha
     }
     x.off.resize(x.n);
     x.fac.resize(x.n);
+    x.fim.assign(x.n, 0.0);
     for (int64_t e = 0; e < x.n; e++)
     {
         if (!t->data[e]) fail(name + ": null entry pointer");
@@ -106,7 +109,11 @@ void load(Ix& x, const tblis_indexed_tensor* t, const label_type* idx, const cha
         x.off[e] = (int64_t)(d / es);
         x.fac[e] = !t->factor ? 1.0
                    : type == TYPE_DOUBLE ? ((const double*)t->factor)[e]
-                                         : (double)((const float*)t->factor)[e];
+                   : type == TYPE_SINGLE ? (double)((const float*)t->factor)[e]
+                   : type == TYPE_DCOMPLEX ? ((const double*)t->factor)[2 * e]
+                                           : (double)((const float*)t->factor)[2 * e];
+        if (t->factor && type == TYPE_DCOMPLEX) x.fim[e] = ((const double*)t->factor)[2 * e + 1];
+        if (t->factor && type == TYPE_SCOMPLEX) x.fim[e] = (double)((const float*)t->factor)[2 * e + 1];
         for (int p = x.ndd; p < x.nd; p++)
         {
             const int64_t v = x.value(e, p);
@@ -126,6 +133,7 @@ void key_of(std::string& k, const Ix& x)
     if (x.n > 0 && x.nix > 0) k.append((const char*)x.t->indices, sizeof(len_type) * x.n * x.nix);
     k.append((const char*)x.off.data(), sizeof(int64_t) * x.n);
     k.append((const char*)x.fac.data(), sizeof(double) * x.n);
+    k.append((const char*)x.fim.data(), sizeof(double) * x.n);
 }
 
 struct VecHash
@@ -148,7 +156,8 @@ struct IndexedPlan
     int64_t nunc = 0;            // C entries no GEMM writes whole (beta == 0 zeroes only these)
     bool whole = false;          // every GEMM writes whole C entries: beta == 0 goes to the epilogue
     void* d_tab = nullptr;       // C entry offsets (int64), factors (double), then the same for
-                                 // the uncovered entries
+                                 // the uncovered entries; complex: offsets, real and imaginary
+                                 // parts of the factors
     bool pinned = false;         // used while capturing: never freed
     ~IndexedPlan()
     {
@@ -208,7 +217,12 @@ Level1Desc dense_desc(const Ix& c)
     return d;
 }
 
-std::shared_ptr<IndexedPlan> build(const Ix& A, const Ix& B, const Ix& C, int es, int dtype, int family)
+// cplx: interleaved complex tensors (offsets in complex elements, es the
+// size of a real part); each GEMM then runs as two real GEMMs with the
+// complex coefficients f_a f_b f_c as k-block scales (sa / sb = -1
+// conjugate A / B).
+std::shared_ptr<IndexedPlan> build(const Ix& A, const Ix& B, const Ix& C, int es, int dtype, int family, bool cplx,
+                                   double sa, double sb)
 {
     auto ip = std::make_shared<IndexedPlan>();
     ip->cdesc = dense_desc(C);
@@ -222,6 +236,20 @@ std::shared_ptr<IndexedPlan> build(const Ix& A, const Ix& B, const Ix& C, int es
         {
             tab[e] = C.off[e];
             std::memcpy(&tab[C.n + e], &C.fac[e], sizeof(double));
+        }
+        if (cplx)
+        {
+            // offsets, real parts, imaginary parts
+            for (int64_t e = 0; e < C.n; e++)
+            {
+                int64_t bits;
+                std::memcpy(&bits, &C.fim[e], sizeof(double));
+                tab.push_back(bits);
+            }
+            hip_ok(device_alloc((void**)&ip->d_tab, tab.size() * sizeof(int64_t)), "hipMalloc (indexed table)");
+            hip_ok(hipMemcpy(ip->d_tab, tab.data(), tab.size() * sizeof(int64_t), hipMemcpyHostToDevice),
+                   "hipMemcpy (indexed table)");
+            return;
         }
         std::vector<int64_t> unc_off;
         std::vector<double> unc_fac;
@@ -333,7 +361,7 @@ std::shared_ptr<IndexedPlan> build(const Ix& A, const Ix& B, const Ix& C, int es
     struct KBlk
     {
         int64_t oa, ob;
-        double coef;
+        double coef, coef_im;
     };
     std::map<int64_t, size_t> where;
     std::vector<int64_t> cbase;
@@ -342,14 +370,14 @@ std::shared_ptr<IndexedPlan> build(const Ix& A, const Ix& B, const Ix& C, int es
     std::vector<char> fixed(labs.size());
     for (int64_t a = 0; a < A.n; a++)
     {
-        if (A.fac[a] == 0.0) continue;
+        if (A.zero(a)) continue;
         key.clear();
         for (const Lab* x : kb) key.push_back(A.value(a, x->pa));
         auto bit = bmap.find(key);
         if (bit == bmap.end()) continue;
         for (int64_t b : bit->second)
         {
-            if (B.fac[b] == 0.0) continue;
+            if (B.zero(b)) continue;
             for (size_t i = 0; i < labs.size(); i++)
             {
                 const Lab& x = labs[i];
@@ -365,8 +393,12 @@ std::shared_ptr<IndexedPlan> build(const Ix& A, const Ix& B, const Ix& C, int es
             if (cit == cmap.end()) continue;
             for (int64_t c : cit->second)
             {
-                const double coef = A.fac[a] * B.fac[b] * C.fac[c];
-                if (coef == 0.0) continue;
+                // f_a f_b f_c (complex for complex types)
+                const double abr = A.fac[a] * B.fac[b] - A.fim[a] * B.fim[b];
+                const double abi = A.fac[a] * B.fim[b] + A.fim[a] * B.fac[b];
+                const double coef = abr * C.fac[c] - abi * C.fim[c];
+                const double coef_im = abr * C.fim[c] + abi * C.fac[c];
+                if (coef == 0.0 && coef_im == 0.0) continue;
                 int64_t oa = A.off[a], ob = B.off[b], oc = C.off[c];
                 for (size_t i = 0; i < labs.size(); i++)
                 {
@@ -390,7 +422,7 @@ std::shared_ptr<IndexedPlan> build(const Ix& A, const Ix& B, const Ix& C, int es
                 }
                 else
                     gi = w->second;
-                kblocks[gi].push_back(KBlk{oa, ob, coef});
+                kblocks[gi].push_back(KBlk{oa, ob, coef, coef_im});
             }
         }
     }
@@ -401,6 +433,7 @@ std::shared_ptr<IndexedPlan> build(const Ix& A, const Ix& B, const Ix& C, int es
     if (kblocks.empty()) return ip;
 
     PlanBuilder pb(es, dtype, family);
+    if (cplx) pb.set_complex(sa, sb);
     std::vector<int64_t> bi(blen.size(), 0);
     for (int64_t t = 0; t < NBATCH; t++)
     {
@@ -422,6 +455,7 @@ std::shared_ptr<IndexedPlan> build(const Ix& A, const Ix& B, const Ix& C, int es
                 g.a.push_back(PBlock{kbk.oa + da, mA, kA});
                 g.b.push_back(PBlock{kbk.ob + db, nB, kB});
                 g.kscale.push_back(kbk.coef);
+                if (cplx) g.kscale_im.push_back(kbk.coef_im);
             }
             CBlock cb;
             cb.base = cbase[gi] + dc;
@@ -467,6 +501,80 @@ void check_entries(const Ix& x, const char* nm, int es)
     for (int64_t e = 0; e < x.n; e++) check_device_pointer(x.t->data[e], nm, lo, hi, es);
 }
 
+// A complex call: C_e <- g_e C_e before the GEMMs (g_e = f_e beta, or 1
+// when beta == 1, as the real rule; divided by alpha when alpha is not
+// real), the GEMMs accumulating sum f_a f_b f_c a.b, then C_e <- alpha C_e
+// for a complex alpha (a real alpha goes to the GEMM epilogue).
+void run_complex(IndexedPlan& ip, const Ix& a, const Ix& b, const Ix& c, const tblis_indexed_tensor* A,
+                 const tblis_indexed_tensor* B, tblis_indexed_tensor* C, int es, int dt)
+{
+    const bool z = A->type == TYPE_DCOMPLEX;
+    auto part = [&](const tblis_scalar& s, int i) { return z ? s.data.z[i] : (double)s.data.c[i]; };
+    const double ar0 = part(A->scalar, 0), ai0 = part(A->scalar, 1), br0 = part(B->scalar, 0),
+                 bi0 = part(B->scalar, 1);
+    double ar = ar0 * br0 - ai0 * bi0, ai = ar0 * bi0 + ai0 * br0;
+    if (!z) ar = (double)(float)ar, ai = (double)(float)ai;
+    const double betar = part(C->scalar, 0), betai = part(C->scalar, 1);
+    ThreadState& ts = thread_state();
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    hip_ok(hipStreamIsCapturing(ts.stream, &cap), "hipStreamIsCapturing");
+    const bool capturing = cap != hipStreamCaptureStatusNone;
+    if (capturing) ip.pinned = true;
+    const bool gemm = (ar != 0.0 || ai != 0.0) && ip.gemm && ip.gemm->ntasks > 0;
+    const bool beta_one = betar == 1.0 && betai == 0.0;
+    const bool calpha = gemm && ai != 0.0;
+    const int64_t n = ip.nent;
+    if (!gemm && (beta_one || n == 0 || ip.cdesc.nb == 0)) return;
+    const void* a0 = a.n ? A->data[0] : nullptr;
+    const void* b0 = b.n ? B->data[0] : nullptr;
+    void* c0 = C->data[0];
+    Plan* vp = ip.gemm ? ip.gemm.get() : nullptr;
+    if (!vp || !vp->already_verified(a0, b0, c0, gemm))
+    {
+        check_entries(c, "C", es);
+        if (gemm)
+        {
+            check_entries(a, "A", es);
+            check_entries(b, "B", es);
+        }
+        if (vp) vp->mark_verified(a0, b0, c0, gemm);
+    }
+    const int64_t* offs = (const int64_t*)ip.d_tab;
+    const double* fre = offs ? (const double*)(offs + n) : nullptr;
+    const double* fim = fre ? fre + n : nullptr;
+    // pre pass
+    double sr = betar, si = betai;
+    if (calpha)
+    {
+        const double d = ar * ar + ai * ai, ir = ar / d, ii = -ai / d;
+        if (beta_one)
+            sr = ir, si = ii;
+        else
+            sr = betar * ir - betai * ii, si = betar * ii + betai * ir;
+    }
+    if (calpha || !beta_one)
+        hip_ok(launch_level1_centries(dt, ip.cdesc, offs, calpha && beta_one ? nullptr : fre, fim, n, sr, si, c0,
+                                      ts.stream),
+               "indexed beta pass");
+    if (gemm)
+    {
+        {
+            struct Restore
+            {
+                ThreadState& t;
+                bool async;
+                ~Restore() { t.async = async; }
+            } restore{ts, ts.async};
+            if (calpha) ts.async = true;
+            run_plan(ip.gemm, a0, b0, c0, calpha ? 1.0 : ar, 1.0, true);
+        }
+        if (calpha)
+            hip_ok(launch_level1_centries(dt, ip.cdesc, offs, nullptr, nullptr, n, ar, ai, c0, ts.stream),
+                   "indexed alpha pass");
+    }
+    if ((!gemm || calpha) && !ts.async && !capturing) hip_ok(hipStreamSynchronize(ts.stream), "hipStreamSynchronize");
+}
+
 }  // namespace
 
 void indexed_tensor_mult(const tblis_config* cfg, const tblis_indexed_tensor* A, const label_type* idx_A,
@@ -475,8 +583,11 @@ void indexed_tensor_mult(const tblis_config* cfg, const tblis_indexed_tensor* A,
 {
     if (!A || !B || !C) fail("indexed: null descriptor");
     const type_t type = A->type;
-    check_real(type);
+    // complex types (src/iface/3t/mult.cxx:405-409 instantiates all four)
+    const bool cplx = type == TYPE_SCOMPLEX || type == TYPE_DCOMPLEX;
+    const type_t rtype = type == TYPE_DOUBLE || type == TYPE_DCOMPLEX ? TYPE_DOUBLE : TYPE_SINGLE;
     const int es = elem_size(type);
+    const double sa = A->conj ? -1.0 : 1.0, sb = B->conj ? -1.0 : 1.0;
     Ix a, b, c;
     load(a, A, idx_A, "A", type, es);
     load(b, B, idx_B, "B", type, es);
@@ -499,27 +610,39 @@ void indexed_tensor_mult(const tblis_config* cfg, const tblis_indexed_tensor* A,
             }
         }
 
-    Algo algo = resolve_algo(cfg, type);
-    if (algo == Algo::Reference) algo = type == TYPE_DOUBLE ? Algo::Fp64Mfma : Algo::Bf16x3;
+    Algo algo = resolve_algo(cfg, rtype);
+    if (algo == Algo::Reference) algo = rtype == TYPE_DOUBLE ? Algo::Fp64Mfma : Algo::Bf16x3;
     thread_state().last_algo = algo_name(algo);
     thread_state().last_tile = -1;
 
     std::string key;
     put(key, (int32_t)algo);
     put(key, (int32_t)type);
+    put(key, sa);
+    put(key, sb);
     key_of(key, a);
     key_of(key, b);
     key_of(key, c);
     std::shared_ptr<IndexedPlan> ip = cache_find(key);
     if (!ip)
     {
-        ip = build(a, b, c, es, type == TYPE_DOUBLE ? 0 : 1, algo_family(algo));
+        ip = build(a, b, c, cplx ? es / 2 : es, rtype == TYPE_DOUBLE ? 0 : 1, algo_family(algo), cplx, sa, sb);
         cache_store(key, ip);
     }
 
+    ThreadState& ts = thread_state();
+    if (cplx)
+    {
+        run_complex(*ip, a, b, c, A, B, C, es, rtype == TYPE_DOUBLE ? 0 : 1);
+        if (type == TYPE_DCOMPLEX)
+            C->scalar.data.z[0] = 1.0, C->scalar.data.z[1] = 0.0;
+        else
+            C->scalar.data.c[0] = 1.0f, C->scalar.data.c[1] = 0.0f;
+        C->conj = 0;
+        return;
+    }
     const double alpha = product_scalar(A->scalar, B->scalar, type);
     const double beta = scalar_value(C->scalar, type);
-    ThreadState& ts = thread_state();
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     hip_ok(hipStreamIsCapturing(ts.stream, &cap), "hipStreamIsCapturing");
     const bool capturing = cap != hipStreamCaptureStatusNone;

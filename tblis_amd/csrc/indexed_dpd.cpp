@@ -224,7 +224,9 @@ void indexed_dpd_tensor_mult(const tblis_config* cfg, const tblis_indexed_dpd_te
 {
     if (!A || !B || !C) fail("indexed DPD: null descriptor");
     const type_t type = A->type;
-    check_real(type);
+    // all four types (src/iface/3t/mult.cxx:539-543); complex entries are
+    // copied as (re, im) pairs and contracted by the complex indexed path
+    const bool cplx = type == TYPE_SCOMPLEX || type == TYPE_DCOMPLEX;
     const int es = elem_size(type);
     Op a, b, c;
     load(a, A, idx_A, "A", type, es);
@@ -269,6 +271,7 @@ void indexed_dpd_tensor_mult(const tblis_config* cfg, const tblis_indexed_dpd_te
         }
         std::memset(&p.d, 0, sizeof(p.d));
         p.d.type = type;
+        p.d.conj = x.t->conj;
         p.d.scalar = x.t->scalar;
         p.d.ndim = (unsigned)x.nd;
         p.d.dense_ndim = (unsigned)x.ndd;
@@ -285,6 +288,7 @@ void indexed_dpd_tensor_mult(const tblis_config* cfg, const tblis_indexed_dpd_te
 
     std::string key;
     put(key, (int32_t)type);
+    put(key, (int32_t)(A->conj * 2 + B->conj));
     for (const Op* x : {&a, &b, &c})
     {
         put(key, (int32_t)x->nd);
@@ -335,7 +339,10 @@ void indexed_dpd_tensor_mult(const tblis_config* cfg, const tblis_indexed_dpd_te
     const bool capturing = cap != hipStreamCaptureStatusNone;
     if (capturing) tb->pinned = true;
 
-    const int dt = type == TYPE_DOUBLE ? 0 : 1;
+    // real type of the zero fill; block-copy element type
+    const int dt = type == TYPE_DOUBLE || type == TYPE_DCOMPLEX ? 0 : 1;
+    const int ct = type == TYPE_DCOMPLEX ? 2 : type == TYPE_SCOMPLEX ? 3 : dt;
+    const int64_t rl = cplx ? 2 : 1;  // reals per element
     auto check = [&](const Op& x, const char* nm) {
         if (x.ndd == 0)
         {
@@ -354,17 +361,17 @@ void indexed_dpd_tensor_mult(const tblis_config* cfg, const tblis_indexed_dpd_te
         // zero the scratch (positions outside the allowed blocks), expand
         Level1Desc z;
         z.ndb = 1;
-        z.len_b[0] = na + nb + nc;
+        z.len_b[0] = rl * (na + nb + nc);
         z.sb[0] = 1;
         z.sa_b[0] = 0;
-        z.nb = na + nb + nc;
+        z.nb = rl * (na + nb + nc);
         hip_ok(launch_level1(LEVEL1_SET, dt, z, nullptr, base, 0.0, 0.0, 0.0, nullptr, ts.stream),
                "indexed DPD zero fill");
-        if (tb->na) hip_ok(launch_block_copy(dt, tb->d, tb->na, A->data[0], base, ts.stream), "indexed DPD expand A");
+        if (tb->na) hip_ok(launch_block_copy(ct, tb->d, tb->na, A->data[0], base, ts.stream), "indexed DPD expand A");
         if (tb->nb)
-            hip_ok(launch_block_copy(dt, tb->d + tb->na, tb->nb, B->data[0], base, ts.stream), "indexed DPD expand B");
+            hip_ok(launch_block_copy(ct, tb->d + tb->na, tb->nb, B->data[0], base, ts.stream), "indexed DPD expand B");
         if (tb->nc)
-            hip_ok(launch_block_copy(dt, tb->d + tb->na + tb->nb, tb->nc, C->data[0], base, ts.stream),
+            hip_ok(launch_block_copy(ct, tb->d + tb->na + tb->nb, tb->nc, C->data[0], base, ts.stream),
                    "indexed DPD expand C");
     }
     {
@@ -378,13 +385,17 @@ void indexed_dpd_tensor_mult(const tblis_config* cfg, const tblis_indexed_dpd_te
         indexed_tensor_mult(cfg, &pa.d, idx_A, &pb.d, idx_B, &pc.d, idx_C);
     }
     if (tb->nw)
-        hip_ok(launch_block_copy(dt, tb->d + tb->na + tb->nb + tb->nc, tb->nw, base, C->data[0], ts.stream),
+        hip_ok(launch_block_copy(ct, tb->d + tb->na + tb->nb + tb->nc, tb->nw, base, C->data[0], ts.stream),
                "indexed DPD write-back");
     if (!ts.async && !capturing) hip_ok(hipStreamSynchronize(ts.stream), "hipStreamSynchronize");
     if (type == TYPE_SINGLE)
         C->scalar.data.s = 1.0f;
-    else
+    else if (type == TYPE_DOUBLE)
         C->scalar.data.d = 1.0;
+    else if (type == TYPE_SCOMPLEX)
+        C->scalar.data.c[0] = 1.0f, C->scalar.data.c[1] = 0.0f;
+    else
+        C->scalar.data.z[0] = 1.0, C->scalar.data.z[1] = 0.0;
     C->conj = 0;
 }
 

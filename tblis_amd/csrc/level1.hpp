@@ -29,6 +29,11 @@ struct Level1Desc
 // entry offsets and factors in device memory).
 hipError_t launch_level1_entries(int dtype, const Level1Desc& d, const int64_t* offs, const double* fac,
                                  int64_t nent, double beta, void* B, hipStream_t s);
+// C_e <- (f_e s) C_e over the entries of an interleaved complex indexed
+// tensor of real type dtype (d and offs in complex elements; f_e from
+// fre / fim, or 1 when fre is null; s = sr + i si).
+hipError_t launch_level1_centries(int dtype, const Level1Desc& d, const int64_t* offs, const double* fre,
+                                  const double* fim, int64_t nent, double sr, double si, void* B, hipStream_t s);
 // C <- (br + i bi) C over an interleaved complex tensor of real type dtype
 // (d in complex elements).
 hipError_t launch_level1_cscale(int dtype, const Level1Desc& d, double br, double bi, void* B, hipStream_t s);
@@ -42,6 +47,7 @@ struct BlockCopy
     int32_t nd, pad_;
     int64_t len[BLOCK_COPY_MAXDIM], ss[BLOCK_COPY_MAXDIM], ds[BLOCK_COPY_MAXDIM];
 };
+// dtype: 0 double, 1 float, 2 dcomplex, 3 scomplex (offsets in elements).
 hipError_t launch_block_copy(int dtype, const BlockCopy* tasks, int64_t ntasks, const void* src, void* dst,
                              hipStream_t s);
 // Device workspace a call needs (partial sums of a split A-only sum), bytes.

@@ -414,6 +414,49 @@ __global__ __launch_bounds__(kThreads) void level1_cscale_kernel(Level1Desc d, d
     }
 }
 
+// C_e <- (f_e s) C_e over the entries of an interleaved complex indexed
+// tensor (s = sr + i si; f_e = fre[e] + i fim[e], or 1 when fre is null);
+// a zero product writes zeros without reading.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void level1_centries_kernel(Level1Desc d, const int64_t* __restrict__ offs,
+                                                                   const double* __restrict__ fre,
+                                                                   const double* __restrict__ fim, int64_t nent,
+                                                                   double sr, double si, T* __restrict__ B)
+{
+    const int64_t l0 = d.ndb > 0 ? d.len_b[0] : 1, sb0 = d.ndb > 0 ? d.sb[0] : 0;
+    const int64_t nrows = d.nb / l0, nseg = (l0 + kSeg - 1) / kSeg;
+    const int64_t per = nrows * nseg, nblocks = nent * per;
+    for (int64_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x)
+    {
+        const int64_t e = blk / per, r = blk - e * per;
+        const int64_t row = r / nseg, seg = r - row * nseg;
+        int64_t ob, oa;
+        row_offsets(d, row, 1, -1, ob, oa);
+        T* base = B + 2 * (offs[e] + ob);
+        const double fr = fre ? fre[e] : 1.0, fi = fre ? fim[e] : 0.0;
+        const double cr = fr * sr - fi * si, ci = fr * si + fi * sr;
+        const bool zero = cr == 0.0 && ci == 0.0;
+#pragma unroll
+        for (int j = 0; j < kPerThread; j++)
+        {
+            const int64_t col = seg * kSeg + threadIdx.x + j * kThreads;
+            if (col >= l0) break;
+            T* p = base + 2 * col * sb0;
+            if (zero)
+            {
+                p[0] = (T)0;
+                p[1] = (T)0;
+            }
+            else
+            {
+                const double re = (double)p[0], im = (double)p[1];
+                p[0] = (T)(cr * re - ci * im);
+                p[1] = (T)(cr * im + ci * re);
+            }
+        }
+    }
+}
+
 // Block copies between layouts (indexed DPD expansion and write-back): one
 // workgroup per piece of a block, the elements decomposed over the block's
 // dimensions.
@@ -580,6 +623,21 @@ hipError_t launch_level1_entries(int dtype, const Level1Desc& d, const int64_t* 
     return hipGetLastError();
 }
 
+hipError_t launch_level1_centries(int dtype, const Level1Desc& d, const int64_t* offs, const double* fre,
+                                  const double* fim, int64_t nent, double sr, double si, void* B, hipStream_t s)
+{
+    if (nent == 0 || d.nb == 0) return hipSuccess;
+    const int64_t l0 = d.ndb > 0 ? d.len_b[0] : 1;
+    const unsigned grid = grid_for(nent * (d.nb / l0) * ((l0 + kSeg - 1) / kSeg));
+    if (dtype == 0)
+        hipLaunchKernelGGL((level1_centries_kernel<double>), dim3(grid), dim3(kThreads), 0, s, d, offs, fre, fim,
+                           nent, sr, si, (double*)B);
+    else
+        hipLaunchKernelGGL((level1_centries_kernel<float>), dim3(grid), dim3(kThreads), 0, s, d, offs, fre, fim,
+                           nent, sr, si, (float*)B);
+    return hipGetLastError();
+}
+
 hipError_t launch_level1_cscale(int dtype, const Level1Desc& d, double br, double bi, void* B, hipStream_t s)
 {
     if (d.nb == 0) return hipSuccess;
@@ -600,9 +658,15 @@ hipError_t launch_block_copy(int dtype, const BlockCopy* tasks, int64_t ntasks, 
     if (dtype == 0)
         hipLaunchKernelGGL((block_copy_kernel<double>), dim3(grid), dim3(kThreads), 0, s, tasks, ntasks,
                            (const double*)src, (double*)dst);
-    else
+    else if (dtype == 1)
         hipLaunchKernelGGL((block_copy_kernel<float>), dim3(grid), dim3(kThreads), 0, s, tasks, ntasks,
                            (const float*)src, (float*)dst);
+    else if (dtype == 2)  // dcomplex: (re, im) pairs
+        hipLaunchKernelGGL((block_copy_kernel<double2>), dim3(grid), dim3(kThreads), 0, s, tasks, ntasks,
+                           (const double2*)src, (double2*)dst);
+    else
+        hipLaunchKernelGGL((block_copy_kernel<float2>), dim3(grid), dim3(kThreads), 0, s, tasks, ntasks,
+                           (const float2*)src, (float2*)dst);
     return hipGetLastError();
 }
 

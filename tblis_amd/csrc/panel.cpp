@@ -296,6 +296,7 @@ PanelGemm transpose_gemm(const PanelGemm& g)
     t.a = g.b;
     t.b = g.a;
     t.kscale = g.kscale;
+    t.kscale_im = g.kscale_im;
     const size_t nm = g.mlen.size(), nn = g.nlen.size();
     t.c.resize(g.c.size());
     for (size_t p = 0; p < nm; p++)

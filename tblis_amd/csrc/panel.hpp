@@ -32,6 +32,9 @@ struct PanelGemm
     std::vector<PBlock> a, b;
     std::vector<CBlock> c;
     std::vector<double> kscale;  // per k block, applied to A while packing (empty: all 1)
+    // imaginary parts of the k-block scales of a complex GEMM (empty: all
+    // 0); complex_split_gemms turns them into real term scales
+    std::vector<double> kscale_im;
 };
 
 // C^T = B^T A^T.
@@ -59,10 +62,9 @@ bool c_unpack_forced();
 // read directly (TBLIS_HIP_DIRECT_KFAST, default 32).
 int direct_kfast_rereads();
 
-// Host tables of a panel plan.
-// Stages of a plan of many GEMMs: consecutive ranges of GEMMs of about
-// equal flops whose pack launches run on a second stream ahead of the GEMM
-// launches (the pack of stage s + 1 overlaps the GEMMs of stage s).
+// Host tables of a panel plan.  Stages (plans of many GEMMs): consecutive
+// ranges of GEMMs of about equal flops whose pack launches can run on a
+// second stream ahead of the GEMM launches.
 struct PanelTables
 {
     std::vector<PanelJob> jobs;
@@ -79,8 +81,8 @@ struct PanelTables
     // tasks come in four consecutive groups by which operands they read from
     // the tensors (none, A, B, both); one GEMM launch per non-empty group
     int64_t group_n[4] = {0, 0, 0, 0};
-    // chunks and tasks ordered by stage: stage_chunks[s] pack chunks and
-    // stage_group_n[s][g] tasks of group g in stage s (group-major within
+    // chunks and tasks ordered by stage: sched.chunks[s] pack chunks and
+    // sched.group_n[s][g] tasks of group g in stage s (group-major within
     // the stage); task_stage is each task's stage (host bookkeeping)
     PanelSchedule sched;
     std::vector<int8_t> task_stage;

@@ -92,7 +92,7 @@ def random_indexed(rng, labels, ext, dense_ndim, fill=0.5, factors=(1.0, 0.5, 0.
     return dict(entries=entries, indices=pick, factors=facs, dense_len=dense_len, idx_len=idx_len)
 
 
-def indexed_brute_force(alpha, A, idx_A, B, idx_B, beta, C_, idx_C):
+def indexed_brute_force(alpha, A, idx_A, B, idx_B, beta, C_, idx_C, conj_A=False, conj_B=False):
     """The indexed contraction by its definition, one entry triple and one
     assignment of every label at a time (pure Python, tiny sizes only):
     C entry c gains alpha f_a f_b f_c a[..] b[..] wherever the labels' values
@@ -104,9 +104,12 @@ def indexed_brute_force(alpha, A, idx_A, B, idx_B, beta, C_, idx_C):
     for X, idx in ((A, idx_A), (B, idx_B), (C_, idx_C)):
         for c, n in zip(idx, list(X["dense_len"]) + list(X["idx_len"])):
             ext[c] = n
+    cx = any(np.iscomplexobj(x) for X in (A, B, C_) for x in list(X["entries"]) + list(X["factors"]))
+    cx = cx or np.iscomplexobj(alpha) or np.iscomplexobj(beta)
+    dt = np.complex128 if cx else np.float64
     out = []
     for e, f in zip(C_["entries"], C_["factors"]):
-        e = np.array(e, np.float64)
+        e = np.array(e, dt)
         if beta == 0.0 or (beta != 1.0 and f == 0.0):
             e[...] = 0.0
         elif beta != 1.0:
@@ -133,7 +136,9 @@ def indexed_brute_force(alpha, A, idx_A, B, idx_B, beta, C_, idx_C):
                 for k, (xc, fc) in enumerate(zip(C_["indices"], C_["factors"])):
                     if tuple(xc) != ic:
                         continue
-                    out[k][dc] += alpha * fa * fb * fc * ea[da] * eb[db]
+                    va = np.conj(ea[da]) if conj_A else ea[da]
+                    vb = np.conj(eb[db]) if conj_B else eb[db]
+                    out[k][dc] += alpha * fa * fb * fc * va * vb
     return out
 
 

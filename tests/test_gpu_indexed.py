@@ -17,7 +17,7 @@ from tests.helpers import random_indexed
 pytestmark = pytest.mark.gpu
 
 DEV = "cuda:0"
-TOL = {np.float64: 1e-12, np.float32: 1e-6}
+TOL = {np.float64: 1e-12, np.float32: 1e-6, np.complex128: 1e-12, np.complex64: 1e-6}
 
 CASES = [
     # (idx_A, dense_A, idx_B, dense_B, idx_C, dense_C)
@@ -61,7 +61,8 @@ def _device(X, dtype, layout, rng, separate):
 
 
 def _torch(dtype):
-    return torch.float64 if dtype == np.float64 else torch.float32
+    return {np.float64: torch.float64, np.float32: torch.float32, np.complex128: torch.complex128,
+            np.complex64: torch.complex64}[dtype]
 
 
 FACTORS = (1.0, 0.5, -0.75, 2.0)
@@ -126,3 +127,38 @@ def test_indexed_larger_blocks():
     case = ("abef", 3, "eijf", 3, "abij", 3)
     for dtype in (np.float64, np.float32):
         assert _run(case, dtype, 63600, 1.0, 0.0, ext_range=(24, 40), fill=0.7)
+
+
+@pytest.mark.parametrize("ci", [1, 2, 3, 5, 6, 7])
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_indexed_complex(ci, dtype):
+    """Complex indexed contractions (the reference instantiates indexed mult
+    for its complex types, src/iface/3t/mult.cxx:405-409): complex entries
+    and factors, real and complex alpha / beta, beta = 0 and 1, alpha = 0,
+    conjugated operands, against the oracle's FULL algorithm in complex
+    arithmetic (pinned against the per-triple definition)."""
+    case = CASES[ci]
+    iA, nA, iB, nB, iC, nC = case
+    for t, (alpha, beta, cA, cB) in enumerate([(1.0, 0.0, False, False), (0.75 - 0.5j, 0.5 + 1j, True, False),
+                                               (1j, 1.0, False, True), (0.0, 0.5 - 0.5j, False, False)]):
+        rng = np.random.default_rng(64000 + 10 * ci + t)
+        ext = {c: int(rng.integers(2, 8)) for c in sorted(set(iA + iB + iC))}
+        for idx, nd in ((iA, nA), (iB, nB), (iC, nC)):
+            for c in idx[nd:]:
+                ext[c] = min(ext[c], 4)
+        X = [random_indexed(rng, i, ext, n, 0.8, FACTORS) for i, n in ((iA, nA), (iB, nB), (iC, nC))]
+        for Y in X:
+            Y["entries"] = [(np.asarray(e) + 1j * rng.uniform(-1, 1, np.shape(e))).astype(dtype) for e in Y["entries"]]
+            Y["factors"] = [f * complex(1.0, float(rng.uniform(-1, 1))) for f in Y["factors"]]
+        A, B, C0 = X
+        want = po.indexed_mult(alpha, A, iA, B, iB, beta, C0, iC, cA, cB)
+        dA, kA = _device(A, dtype, "col", rng, False)
+        dB, kB = _device(B, dtype, "row", rng, t == 1)
+        dC, kC = _device(C0, dtype, "col", rng, False)
+        T.indexed_mult(alpha, dA, iA, dB, iB, beta, dC, iC, conj_A=cA, conj_B=cB)
+        torch.cuda.synchronize()
+        got = np.concatenate([e.cpu().to(torch.complex128).numpy().ravel() for e in dC.entries])
+        ref = np.concatenate([w.ravel() for w in want])
+        nrm = np.linalg.norm(ref)
+        err = np.linalg.norm(got - ref) / (nrm if nrm > 0 else 1.0)
+        assert err <= TOL[dtype], (case, dtype, t, alpha, beta, err)

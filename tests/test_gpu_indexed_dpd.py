@@ -17,7 +17,7 @@ from tests.helpers import random_indexed_dpd
 pytestmark = pytest.mark.gpu
 
 DEV = "cuda:0"
-TOL = {np.float64: 1e-12, np.float32: 1e-6}
+TOL = {np.float64: 1e-12, np.float32: 1e-6, np.complex128: 1e-12, np.complex64: 1e-6}
 
 CASES = [
     # (idx_A, dense_A, idx_B, dense_B, idx_C, dense_C)
@@ -46,10 +46,11 @@ def _device(X, dtype, separate):
 
 
 def _torch(dtype):
-    return torch.float64 if dtype == np.float64 else torch.float32
+    return {np.float64: torch.float64, np.float32: torch.float32, np.complex128: torch.complex128,
+            np.complex64: torch.complex64}[dtype]
 
 
-def _run(case, dtype, seed, alpha, beta, nirrep, layout, separate=False, maxlen=4):
+def _run(case, dtype, seed, alpha, beta, nirrep, layout, separate=False, maxlen=4, conj=(False, False)):
     iA, nA, iB, nB, iC, nC = case
     rng = np.random.default_rng(seed)
     labels = set(iA + iB + iC)
@@ -61,17 +62,22 @@ def _run(case, dtype, seed, alpha, beta, nirrep, layout, separate=False, maxlen=
     A = random_indexed_dpd(rng, iA, lens, nA, irA, nirrep, layout)
     B = random_indexed_dpd(rng, iB, lens, nB, irB, nirrep, layout)
     C0 = random_indexed_dpd(rng, iC, lens, nC, irA ^ irB, nirrep, layout)
+    cplx = np.iscomplexobj(np.zeros(1, dtype))
     for X in (A, B, C0):
+        if cplx:
+            X["entries"] = [np.asarray(e) + 1j * rng.uniform(-1, 1, np.shape(e)) for e in X["entries"]]
+            X["factors"] = [f * complex(1.0, float(rng.uniform(-1, 1))) for f in X["factors"]]
         X["entries"] = [np.asarray(e, dtype) for e in X["entries"]]
-    want = po.indexed_dpd_mult(alpha, A, iA, B, iB, beta, C0, iC)
+    want = po.indexed_dpd_mult(alpha, A, iA, B, iB, beta, C0, iC, conj[0], conj[1])
     dA, kA = _device(A, dtype, separate)
     dB, kB = _device(B, dtype, separate)
     dC, kC = _device(C0, dtype, separate)
-    T.indexed_dpd_mult(alpha, dA, iA, dB, iB, beta, dC, iC)
+    T.indexed_dpd_mult(alpha, dA, iA, dB, iB, beta, dC, iC, conj_A=conj[0], conj_B=conj[1])
     torch.cuda.synchronize()
     if not want:
         return
-    got = np.concatenate([e.cpu().double().numpy().ravel() for e in dC.entries])
+    got = np.concatenate([e.cpu().to(torch.complex128 if cplx else torch.float64).numpy().ravel()
+                          for e in dC.entries])
     ref = np.concatenate([w.ravel() for w in want])
     nrm = np.linalg.norm(ref)
     err = np.linalg.norm(got - ref) / (nrm if nrm > 0 else 1.0)
@@ -91,3 +97,18 @@ def test_indexed_dpd_alpha_zero_and_layouts():
         _run(CASES[1], np.float64, 67500 + layout, 1.0, 0.25, 4, layout)
     _run(CASES[1], np.float64, 67510, 0.0, 0.5, 2, 5)     # only the beta rule
     _run(CASES[3], np.float32, 67511, 0.0, 0.0, 2, 5)     # C set to zero
+
+
+@pytest.mark.parametrize("ci", range(len(CASES)))
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_indexed_dpd_complex(ci, dtype):
+    """Complex indexed DPD contractions (the reference instantiates them for
+    its complex types, src/iface/3t/mult.cxx:539-543): complex entries and
+    factors, real and complex alpha / beta, conjugated operands, 1 to 8
+    irreps, against the oracle's FULL algorithm in complex arithmetic."""
+    for t, (nirrep, layout, alpha, beta, conj) in enumerate([
+            (1, 5, 1.0, 0.0, (False, False)), (2, 0, 0.75 - 0.5j, 0.5 + 1j, (True, False)),
+            (4, 3, 1j, 1.0, (False, True)), (8, 5, -1.0, 0.25, (True, True)),
+            (2, 5, 0.0, 0.5 - 0.5j, (False, False))]):
+        _run(CASES[ci], dtype, 67700 + 10 * ci + t, alpha, beta, nirrep, layout, separate=(t % 2 == 1),
+             conj=conj)

@@ -665,6 +665,34 @@ def test_indexed_oracle_against_brute_force(case):
             np.testing.assert_allclose(g, w, rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("case", range(4))
+def test_indexed_oracle_complex_matches_brute_force(case):
+    """The indexed oracle with complex entries, factors and scalars and
+    conjugated operands against the per-triple definition."""
+    from tests.helpers import indexed_brute_force, random_indexed
+    iA, nA, iB, nB, iC, nC = INDEXED_CASES[case]
+    rng = np.random.default_rng(62500 + case)
+
+    def cplx(X):
+        X["entries"] = [e + 1j * rng.uniform(-1, 1, np.shape(e)) for e in X["entries"]]
+        X["factors"] = [f * (0.5 + 1j * rng.uniform(-1, 1)) for f in X["factors"]]
+        return X
+
+    for trial in range(3):
+        ext = {c: int(rng.integers(1, 4)) for c in sorted(set(iA + iB + iC))}
+        A = cplx(random_indexed(rng, iA, ext, nA))
+        B = cplx(random_indexed(rng, iB, ext, nB))
+        C0 = cplx(random_indexed(rng, iC, ext, nC))
+        alpha, beta = [(0.75 - 0.5j, 0.5 + 1j), (1.0, 0.0), (1j, 1.0)][trial]
+        cA, cB = bool(trial & 1), trial == 2
+        got = po.indexed_mult(alpha, A, iA, B, iB, beta, C0, iC, cA, cB)
+        want = indexed_brute_force(alpha, A, iA, B, iB, beta, C0, iC, cA, cB)
+        assert len(got) == len(want)
+        for g, w in zip(got, want):
+            assert np.iscomplexobj(g)
+            np.testing.assert_allclose(g, w, rtol=1e-12, atol=1e-12)
+
+
 def test_indexed_oracle_fully_populated_is_dense_contraction():
     """Every index value present with factor 1: the indexed contraction is
     the dense one (checked against the dense oracle, oracle.c)."""

@@ -362,18 +362,26 @@ __device__ __forceinline__ void panel_task(const PanelTask& t, const int64_t* __
         double2 a[PF][MB], b[PF][NB];
 #pragma unroll
         for (int d = 0; d < PF; d++) load(a[d], b[d], d < nk ? d : nk - 1);
-        for (int kg = 0; kg < nk; kg += PF)
+        // whole passes of PF groups with no exit inside the pass, so that
+        // the loop header is reached with the same loads in flight on every
+        // path and each group waits only for its own loads (an exit inside
+        // the pass made the compiler drain every load at the header)
+        const int nmain = nk - nk % PF;
+        for (int kg = 0; kg < nmain; kg += PF)
         {
 #pragma unroll
             for (int d = 0; d < PF; d++)
             {
-                if (kg + d >= nk) break;  // the last pass may be partial
                 mma_group<MB, NB>(acc, a[d], b[d]);
                 // past the end: re-read the final group (in bounds, unused)
                 const int nx = kg + d + PF < nk ? kg + d + PF : nk - 1;
                 load(a[d], b[d], nx);
             }
         }
+        // the last nk % PF groups sit in a[0 ..], b[0 ..]
+#pragma unroll
+        for (int d = 0; d < PF; d++)
+            if (d < nk - nmain) mma_group<MB, NB>(acc, a[d], b[d]);
     }
     if constexpr (KS > 1)
     {

@@ -482,14 +482,30 @@ def indexed_dpd_leg(T, dev, stream, steps, warmup, seed=13):
                      lens_b, 3)
     Cx = T.IndexedDpd([sc], [()], [], 0, nir, lens_c, 4)
     fn = lambda: T.indexed_dpd_mult(1.0, A, "abef", B, "eijf", 0.0, Cx, "abij")
-    ms_step, _, _ = time_steps(fn, steps, warmup, stream)
     full = 36
     dense = 2.0 * full ** 5 * len(match)
-    return {"workload": "indexed DPD C[abij] += A[abef]*B[efij], 4 irreps, extents %s per irrep, f indexed at "
-                        "irrep 0 (%d values, A and B list %d, %d match; numpy default_rng(%d)), FP64, FULL "
-                        "algorithm" % (ext, nf, listed, len(match), seed),
-            "useful_flops": flops, "ms": round(ms_step, 4), "useful_tflops": tflops(flops, ms_step),
-            "expanded_dense_flops": dense, "expanded_tflops": tflops(dense, ms_step)}
+    out = {"workload": "indexed DPD C[abij] += A[abef]*B[efij], 4 irreps, extents %s per irrep, f indexed at "
+                       "irrep 0 (%d values, A and B list %d, %d match; numpy default_rng(%d)), FP64"
+                       % (ext, nf, listed, len(match), seed),
+           "useful_flops": flops, "expanded_dense_flops": dense}
+    # the blocked algorithm (default) and the FULL one (dense expansion)
+    results = {}
+    for name, env in (("blocked", None), ("full", "full")):
+        old = os.environ.pop("TBLIS_HIP_INDEXED_DPD", None)
+        if env:
+            os.environ["TBLIS_HIP_INDEXED_DPD"] = env
+        try:
+            ms_step, _, _ = time_steps(fn, steps, warmup, stream)
+            results[name] = sc.clone()
+        finally:
+            os.environ.pop("TBLIS_HIP_INDEXED_DPD", None)
+            if old is not None:
+                os.environ["TBLIS_HIP_INDEXED_DPD"] = old
+        out[name] = {"algorithm": name, "ms": round(ms_step, 4), "useful_tflops": tflops(flops, ms_step),
+                     "expanded_tflops": tflops(dense, ms_step)}
+    out["blocked_vs_full_rel_err"] = float("%.3e" % rel_fro(results["blocked"], results["full"]))
+    out["speedup_blocked_vs_full"] = round(out["full"]["ms"] / out["blocked"]["ms"], 3)
+    return out
 
 
 # ---------------------------------------------------------------- level-1

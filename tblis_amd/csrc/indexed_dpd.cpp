@@ -1,5 +1,7 @@
-// Indexed DPD contractions by the reference's FULL algorithm
-// (src/internal/3t/indexed_dpd/mult.cxx:68-118): each operand's entries are
+// Indexed DPD contractions.  By default the blocked algorithm
+// (indexed_dpd_blocked, indexed.cpp); with TBLIS_HIP_INDEXED_DPD=full the
+// reference's FULL algorithm
+// (src/internal/3t/indexed_dpd/mult.cxx:68-118), here: each operand's entries are
 // expanded from DPD storage to dense arrays over the summed per-irrep
 // extents (index values shifted by their irrep's offset), the indexed
 // contraction (indexed.cpp) runs on those plain indexed tensors, and C's
@@ -10,6 +12,7 @@
 // (src/iface/3t/mult.cxx:512); the FULL algorithm needs no such test, since
 // products of mismatched irreps land outside C's allowed blocks.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -216,6 +219,13 @@ void* scratch(size_t bytes, hipStream_t s)
     return e.first;
 }
 
+// TBLIS_HIP_INDEXED_DPD=full selects the FULL algorithm (read at each call).
+bool full_algorithm()
+{
+    const char* e = std::getenv("TBLIS_HIP_INDEXED_DPD");
+    return e && std::strcmp(e, "full") == 0;
+}
+
 }  // namespace
 
 void indexed_dpd_tensor_mult(const tblis_config* cfg, const tblis_indexed_dpd_tensor* A, const label_type* idx_A,
@@ -242,6 +252,19 @@ void indexed_dpd_tensor_mult(const tblis_config* cfg, const tblis_indexed_dpd_te
                         for (int r = 0; r < a.nir; r++)
                             if (x->len(i, r) != y->len(j, r))
                                 fail("indexed DPD: label '" + std::string(1, x->idx[i]) + "' has unequal lengths");
+    for (const Op* x : {&a, &b, &c})
+        for (int i = 0; i < x->nd; i++)
+        {
+            int n = 0;
+            for (const Op* y : {&a, &b, &c})
+                for (int j = 0; j < y->nd; j++) n += y->idx[j] == x->idx[i];
+            if (n < 2) fail("indexed DPD: label '" + std::string(1, x->idx[i]) + "' appears in only one tensor");
+        }
+    if (!full_algorithm())
+    {
+        indexed_dpd_blocked(cfg, A, idx_A, B, idx_B, C, idx_C);
+        return;
+    }
 
     // the plain indexed operands over the expanded entries
     const int64_t na = a.n * a.dense_elems, nb = b.n * b.dense_elems, nc = c.n * c.dense_elems;

@@ -1,6 +1,7 @@
 """GPU parity of indexed DPD contractions (tblis_indexed_dpd_tensor_mult; the
 reference's tblis::mult over indexed_dpd_varray_view,
-src/iface/3t/mult.cxx:413-540) against the oracle's FULL algorithm
+src/iface/3t/mult.cxx:413-540), run by the blocked algorithm (the default)
+and by the FULL one, against the oracle's FULL algorithm
 (oracle/pyoracle.py indexed_dpd_mult, pinned against the DPD and the
 indexed oracles): 1, 2, 4 and 8 irreps, every layout, labels indexed in one
 or both operands or in C only, nonzero tensor irreps, zero-length irreps,
@@ -112,3 +113,14 @@ def test_indexed_dpd_complex(ci, dtype):
             (2, 5, 0.0, 0.5 - 0.5j, (False, False))]):
         _run(CASES[ci], dtype, 67700 + 10 * ci + t, alpha, beta, nirrep, layout, separate=(t % 2 == 1),
              conj=conj)
+
+
+def test_indexed_dpd_full_algorithm(monkeypatch):
+    """The FULL algorithm (TBLIS_HIP_INDEXED_DPD=full, the reference's
+    dpd_impl == FULL, src/internal/3t/indexed_dpd/mult.cxx:1282-1287) on
+    every case, real and complex; the tests above run the blocked one."""
+    monkeypatch.setenv("TBLIS_HIP_INDEXED_DPD", "full")
+    for ci in range(len(CASES)):
+        _run(CASES[ci], np.float64, 67800 + ci, 0.75, 0.5, 4, 5, separate=True)
+        _run(CASES[ci], np.float32, 67820 + ci, 1.0, 0.0, 8, 0)
+        _run(CASES[ci], np.complex128, 67840 + ci, 0.5 - 0.25j, 1.0, 2, 3, conj=(True, False))

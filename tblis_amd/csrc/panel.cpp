@@ -505,6 +505,10 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family,
         if (family == FAMILY_BF16X3 && panel_bf16_tile_fast_only(tm.tile)) continue;
         if (panel_tile_lds(family, tm.tile) && !lds_applicable(gemms, dtype == 0 ? 8 : 4)) continue;
         cands.push_back(tm.tile);
+        // the LDS-staged tiles measured slower than the register tiles on
+        // every workload (permuted 64/32, 8-irrep DPD, 4096-instance batch):
+        // the tuner times them, the cost model does not pick them
+        if (panel_tile_lds(family, tm.tile)) continue;
         double c = tile_cost(gemms, tm, cus, family);
         if (best < 0 || c < best)
         {

@@ -566,7 +566,8 @@ struct PanelTile
     X(4, 64, 128, 1, 2, 1)    \
     X(5, 32, 32, 1, 1, 1)     \
     X(6, 64, 64, 2, 2, 2)     \
-    X(7, 32, 32, 1, 1, 2)
+    X(7, 32, 32, 1, 1, 2)     \
+    X(8, 32, 32, 1, 1, 4)
 
 constexpr PanelTile kTiles[] = {
 #define X(id, bm, bn, wgm, wgn, ks) {bm, bn, wgm, wgn, ks},

@@ -238,11 +238,11 @@ struct TileModel
 // Per-wave efficiency by wave tile.  FP64: 64x64 waves are 4x4 MFMA
 // blocks, 32x32 waves 2x2; set so that without the autotuner the model
 // picks what the tuner measured fastest on the permuted 64/32 GEMM (tile 5),
-// the 8-irrep DPD GEMMs (5) and the 4096-instance batch (3).  Split-BF16:
-// per tile of the BF16 table.
+// the 8-irrep DPD GEMMs (5), the 4096-instance batch (3) and the dense 24^4
+// GEMM (8).  Split-BF16: per tile of the BF16 table.
 const TileModel kModelsF64[] = {{0, 0.85, 2}, {2, 0.70, 2}, {3, 0.82, 2}, {4, 0.82, 2},
                                 {1, 0.62, 6}, {5, 0.72, 6}, {6, 0.62, 4}, {7, 0.62, 4},
-                                {8, 0.88, 1}, {9, 0.55, 2}, {10, 0.80, 1}};
+                                {8, 0.50, 4}, {9, 0.88, 1}, {10, 0.55, 2}, {11, 0.80, 1}};
 const TileModel kModelsBf16[] = {{0, 0.55, 1}, {1, 0.55, 1}, {2, 0.45, 2}, {3, 0.35, 4}};
 const TileModel kModelsOzaki[] = {{0, 0.30, 1}};
 
@@ -500,7 +500,8 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family,
     int tile = 0;
     double best = -1;
     const TileModel* models = family == FAMILY_F64 ? kModelsF64 : family == FAMILY_OZAKI ? kModelsOzaki : kModelsBf16;
-    const int nmodels = family == FAMILY_F64 ? 11 : family == FAMILY_OZAKI ? 1 : 4;
+    const int nmodels = family == FAMILY_F64 ? (int)(sizeof(kModelsF64) / sizeof(kModelsF64[0]))
+                        : family == FAMILY_OZAKI ? 1 : 4;
     std::vector<int> cands;
     for (int mi = 0; mi < nmodels; mi++)
     {
@@ -1175,7 +1176,7 @@ void load_tune_file(TuneDb& db)
     {
         int fam = -1, tile = -1;
         unsigned long long h = 0;
-        if (std::sscanf(line, "{\"format\": \"tblis-tune-v1\", \"family\": %d, \"shapes\": \"%llx\", \"tile\": %d",
+        if (std::sscanf(line, "{\"format\": \"tblis-tune-v2\", \"family\": %d, \"shapes\": \"%llx\", \"tile\": %d",
                         &fam, &h, &tile) == 3 &&
             fam >= 0 && fam <= FAMILY_BF16X3_FAST && tile >= 0 && tile < panel_num_tiles(fam))
             db.best[(uint64_t)h ^ (uint64_t)fam] = tile;
@@ -1189,7 +1190,7 @@ void save_tune(int family, uint64_t h, int tile)
     if (!path || !path[0]) return;
     FILE* f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "{\"format\": \"tblis-tune-v1\", \"family\": %d, \"shapes\": \"%016llx\", \"tile\": %d}\n",
+    std::fprintf(f, "{\"format\": \"tblis-tune-v2\", \"family\": %d, \"shapes\": \"%016llx\", \"tile\": %d}\n",
                  family, (unsigned long long)h, tile);
     std::fclose(f);
 }

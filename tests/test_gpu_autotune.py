@@ -38,7 +38,7 @@ def test_autotune_picks_and_runs(tmp_path, monkeypatch):
         assert T.autotune_runs() >= r1 + 2
         lines = [json.loads(l) for l in path.read_text().splitlines()]
         assert len(lines) == T.autotune_runs() - r0
-        assert all(l["format"] == "tblis-tune-v1" and l["tile"] >= 0 for l in lines)
+        assert all(l["format"] == "tblis-tune-v2" and l["tile"] >= 0 for l in lines)
     finally:
         T.set_autotune(False)
         T.lib().tblis_hip_clear_plan_cache()

@@ -185,7 +185,7 @@ def test_float_through_fp64_mfma():
                 config="fp64_mfma", tol=2e-7)
 
 
-@pytest.mark.parametrize("tile", [8, 9, 10])
+@pytest.mark.parametrize("tile", [9, 10, 11])
 def test_lds_staged_tiles(tile, monkeypatch):
     """The FP64 tiles that read both operands from the tensors through LDS
     (forced with TBLIS_HIP_PANEL_TILE): ragged edges, k not a multiple of
@@ -353,10 +353,11 @@ def test_batch_uneven():
 
 
 @pytest.mark.parametrize("direct", ["1", "0"])
-@pytest.mark.parametrize("tile", range(8))
+@pytest.mark.parametrize("tile", range(9))
 def test_every_panel_tile(tile, direct, monkeypatch):
     """Each tile of the FP64 panel GEMM (forced through TBLIS_HIP_PANEL_TILE;
-    tiles 6 and 7 split k between two sets of waves inside the workgroup),
+    tiles 6 and 7 split k between two sets of waves inside the workgroup,
+    tile 8 between four),
     with operands read straight from the tensors where their layout allows
     (direct) and with every operand packed: dense contractions with ragged
     edges and scattered k, FP32 data through FP64 MFMA, the 8-irrep DPD

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -470,33 +471,69 @@ std::shared_ptr<IndexedPlan> build(const Ix& A, const Ix& B, const Ix& C, int es
     std::map<int64_t, size_t> where;
     std::vector<Grp> groups;
 
-    int64_t ncomb = 1;
-    for (size_t i = 0; i < gl.size(); i++) ncomb *= nir;
-    if (!consistent || A.n == 0 || B.n == 0 || C.n == 0) ncomb = 0;
+    // The assignments of irreps to the labels dense everywhere that give each
+    // tensor its irrep, depth first: a tensor's product is checked as soon
+    // as its last such label is set, which prunes the search by a factor of
+    // nir per tensor (the products of the other labels are fixed).
+    std::vector<std::vector<unsigned>> assigns;
+    if (consistent && A.n > 0 && B.n > 0 && C.n > 0)
+    {
+        const unsigned want[3] = {A.irrep, B.irrep, C.irrep};
+        unsigned fx[3] = {0, 0, 0};
+        int last[3] = {-1, -1, -1};
+        std::vector<char> is_g(labs.size(), 0);
+        for (size_t g : gl) is_g[g] = 1;
+        for (size_t i = 0; i < labs.size(); i++)
+        {
+            const int pos[3] = {labs[i].pa, labs[i].pb, labs[i].pc};
+            for (int x = 0; x < 3; x++)
+                if (pos[x] >= 0 && !is_g[i]) fx[x] ^= lirr[i];
+        }
+        for (size_t d = 0; d < gl.size(); d++)
+        {
+            const int pos[3] = {labs[gl[d]].pa, labs[gl[d]].pb, labs[gl[d]].pc};
+            for (int x = 0; x < 3; x++)
+                if (pos[x] >= 0) last[x] = (int)d;
+        }
+        bool ok0 = true;
+        for (int x = 0; x < 3; x++)
+            if (last[x] < 0 && fx[x] != want[x]) ok0 = false;
+        std::vector<unsigned> cur(gl.size(), 0);
+        std::function<void(size_t, unsigned, unsigned, unsigned)> dfs = [&](size_t d, unsigned p0, unsigned p1,
+                                                                            unsigned p2) {
+            if (d == gl.size())
+            {
+                assigns.push_back(cur);
+                return;
+            }
+            const Lab& x = labs[gl[d]];
+            for (int r = 0; r < nir; r++)
+            {
+                const unsigned q[3] = {x.pa >= 0 ? p0 ^ r : p0, x.pb >= 0 ? p1 ^ r : p1, x.pc >= 0 ? p2 ^ r : p2};
+                bool ok = true;
+                for (int t = 0; t < 3; t++)
+                    if (last[t] == (int)d && q[t] != want[t]) ok = false;
+                if (!ok) continue;
+                cur[d] = (unsigned)r;
+                dfs(d + 1, q[0], q[1], q[2]);
+            }
+        };
+        if (ok0) dfs(0, fx[0], fx[1], fx[2]);
+    }
     std::vector<int64_t> val(labs.size());
     std::vector<char> fixed(labs.size());
     std::vector<unsigned> irr_a(A.ndd), irr_b(B.ndd), irr_c(C.ndd);
     std::vector<int64_t> alen(A.ndd), ast(A.ndd), blen_(B.ndd), bst(B.ndd), clen(C.ndd), cst(C.ndd);
-    for (int64_t cmb = 0; cmb < ncomb; cmb++)
+    for (const std::vector<unsigned>& asg : assigns)
     {
-        int64_t rest = cmb;
-        for (size_t g : gl)
-        {
-            lirr[g] = (unsigned)(rest % nir);
-            rest /= nir;
-        }
-        unsigned xa = 0, xb = 0, xc = 0;
+        for (size_t d = 0; d < gl.size(); d++) lirr[gl[d]] = asg[d];
         for (size_t i = 0; i < labs.size(); i++)
         {
             const Lab& x = labs[i];
-            if (x.pa >= 0) xa ^= lirr[i];
-            if (x.pb >= 0) xb ^= lirr[i];
-            if (x.pc >= 0) xc ^= lirr[i];
             if (x.pa >= 0 && A.dense(x.pa)) irr_a[x.pa] = lirr[i];
             if (x.pb >= 0 && B.dense(x.pb)) irr_b[x.pb] = lirr[i];
             if (x.pc >= 0 && C.dense(x.pc)) irr_c[x.pc] = lirr[i];
         }
-        if (xa != A.irrep || xb != B.irrep || xc != C.irrep) continue;
         int64_t boa, bob, boc;
         A.block(irr_a.data(), alen.data(), ast.data(), boa);
         B.block(irr_b.data(), blen_.data(), bst.data(), bob);

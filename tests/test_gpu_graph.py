@@ -1,5 +1,6 @@
 """hipGraph capture of library calls: a prepared dense, packed split-BF16,
-DPD, batched, complex, indexed and FP64-accurate (int8 digit) contraction is
+DPD, batched, complex, indexed, indexed DPD and FP64-accurate (int8 digit)
+contraction is
 captured once and replayed
 several times,
 with the inputs changed by eager work between replays; every replay must
@@ -224,6 +225,52 @@ def test_graph_indexed_with_beta_pass():
 
     fn = T.prepare_indexed_mult(1.25, A, "abef", B, "eijf", 0.5, Cx, "abij")
     graph = _capture(fn)
+    T.lib().tblis_hip_clear_plan_cache()
+    for rep in range(3):
+        sa.mul_(1.5 if rep % 2 == 0 else -0.75)
+        sc.copy_(sc0)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        got = sc.clone()
+        want = eager()[0]
+        torch.cuda.synchronize()
+        assert rel(got, want) <= 1e-12, rep
+
+
+def test_graph_indexed_dpd_blocked():
+    """Indexed DPD contraction (the blocked algorithm: beta pass, then one
+    GEMM launch over the C blocks) captured and replayed."""
+    from tests.helpers import random_indexed_dpd
+    rng = np.random.default_rng(15)
+    lens = {"a": [3, 2, 1, 2], "b": [2, 2, 1, 3], "e": [2, 3, 1, 2], "i": [1, 2, 3, 2], "j": [2, 1, 2, 3],
+            "f": [3, 2, 2, 1]}
+    A0 = random_indexed_dpd(rng, "abef", lens, 3, 1, 4)
+    B0 = random_indexed_dpd(rng, "eijf", lens, 3, 2, 4)
+    C0 = random_indexed_dpd(rng, "abij", lens, 4, 3, 4)
+
+    def store(X):
+        n = len(X["entries"][0])
+        s = torch.zeros(max(len(X["entries"]), 1) * n, dtype=torch.float64, device=DEV)
+        for i, e in enumerate(X["entries"]):
+            s[i * n:(i + 1) * n] = torch.from_numpy(np.asarray(e)).to(DEV)
+        return s, n
+
+    def tensor(X, s, n):
+        return T.IndexedDpd([s[i * n:(i + 1) * n] for i in range(len(X["entries"]))], X["indices"],
+                            X["idx_irrep"], X["irrep"], X["nirrep"], X["lens"], X["dense_ndim"], X["factors"],
+                            X["layout"], dtype=torch.float64)
+
+    (sa, na), (sb, nb), (sc0, nc) = store(A0), store(B0), store(C0)
+    sc = sc0.clone()
+    A, B, Cx = tensor(A0, sa, na), tensor(B0, sb, nb), tensor(C0, sc, nc)
+
+    def eager():
+        ce = sc0.clone()
+        T.indexed_dpd_mult(0.75, A, "abef", B, "eijf", 0.5, tensor(C0, ce, nc), "abij")
+        return [ce]
+
+    graph = _capture(lambda: T.indexed_dpd_mult(0.75, A, "abef", B, "eijf", 0.5, Cx, "abij"))
     T.lib().tblis_hip_clear_plan_cache()
     for rep in range(3):
         sa.mul_(1.5 if rep % 2 == 0 else -0.75)

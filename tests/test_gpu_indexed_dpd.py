@@ -27,6 +27,8 @@ CASES = [
     ("abef", 3, "efij", 4, "abij", 4),   # f indexed in A only (B sliced at its irrep and value)
     ("abef", 4, "efij", 4, "abij", 3),   # j indexed in C only
     ("aek", 2, "ebk", 2, "abk", 2),      # k indexed everywhere
+    ("aek", 3, "ebk", 3, "abk", 3),      # k a dense batch label (in all three)
+    ("akef", 3, "ebkf", 3, "abk", 3),    # dense batch label k, f indexed in A and B
 ]
 
 

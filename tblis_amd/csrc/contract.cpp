@@ -490,6 +490,7 @@ void tensor_mult_batch(const tblis_config* cfg, int64_t count, const tblis_tenso
                 check_data(B[i], "B");
             }
         TuneOperandsScope ops(reads ? A[0]->data : nullptr, reads ? B[0]->data : nullptr);
+        DirectAllScope direct;
         plan = pb.build();
         if (!plan) plan = empty_plan();
         cache_put(key, plan);

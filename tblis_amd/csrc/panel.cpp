@@ -465,9 +465,13 @@ struct TuneOperands
     const void* B = nullptr;
 };
 thread_local TuneOperands t_tune_ops;
-thread_local bool t_tune_off = false;  // AutotuneOffScope
+thread_local bool t_tune_off = false;    // AutotuneOffScope
+thread_local bool t_direct_all = false;  // DirectAllScope
 
 }  // namespace
+
+DirectAllScope::DirectAllScope() : was_(t_direct_all) { t_direct_all = true; }
+DirectAllScope::~DirectAllScope() { t_direct_all = was_; }
 
 AutotuneOffScope::AutotuneOffScope() : was_(t_tune_off) { t_tune_off = true; }
 AutotuneOffScope::~AutotuneOffScope() { t_tune_off = was_; }
@@ -640,7 +644,8 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
     // LDS-staged tiles read both operands from the tensors (lds_applicable
     // has checked that they can)
     const bool lds = panel_tile_lds(family, tile);
-    const bool general = direct_all(gemms) || lds;
+    const bool scoped = t_direct_all && family == FAMILY_F64;
+    const bool general = direct_all(gemms) || lds || scoped;
     const int64_t BM = panel_tile_m(family, tile), BN = panel_tile_n(family, tile);
 
     Pool pool;
@@ -777,6 +782,7 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
             if (o == 0 && !g.kscale.empty() && g.kscale[0] != 1.0) continue;  // scaled while packed
             const int64_t rereads = o == 0 ? ceil_div(n, BN) : ceil_div(m, BM);
             if (!general && rereads > std::max<int64_t>(direct_rows_rereads(), direct_kfast_rereads())) continue;
+            if (scoped && !lds && rereads > 64) continue;
             const PBlock& blk = (*blocks[o])[0];
             if (blk.base == PANEL_MISSING) continue;
             const int64_t rm = pool.offsets(blk.rows), km = pool.offsets(blk.k);

@@ -127,6 +127,21 @@ struct AutotuneOffScope
     bool was_;
 };
 
+// While one of these lives on a thread, FP64 plans built there read every
+// operand that the GEMM can read straight from its tensor (re-read by at
+// most 64 tiles) instead of weighing it against a pack pass: a batch of
+// small instances is then one GEMM launch with no pack (the 4096-instance
+// batch: 23.15 against 23.37 ms with the pack; the merged DPD irrep GEMMs
+// are slower that way and do not use it).
+struct DirectAllScope
+{
+    DirectAllScope();
+    ~DirectAllScope();
+
+  private:
+    bool was_;
+};
+
 // Autotuner switch (initially TBLIS_HIP_AUTOTUNE) and the number of tuning
 // measurements made by this process.
 void set_autotune(int on);

@@ -379,8 +379,10 @@ void tblis_indexed_dpd_tensor_mult(const tblis_comm* comm, const tblis_config* c
 /* Extension: a batch of independent contractions of uneven shapes.  The */
 /* tiles of all instances run in one GEMM launch per kernel variant (one  */
 /* workgroup per C tile, grouped by which operands are read straight from */
-/* the tensors), after one pack launch for the other operands.  The       */
-/* reference has no counterpart; its equivalent is a host loop of         */
+/* the tensors), after one pack launch for the other operands; in FP64    */
+/* every operand the GEMM can read directly is read so (re-read by at     */
+/* most 64 tiles), and a batch of small instances is a single launch.     */
+/* The reference has no counterpart; its equivalent is a host loop of     */
 /* tblis_tensor_mult calls.  Every instance shares alpha, beta, the type  */
 /* and the conj flags; complex types run as two real GEMMs per instance,  */
 /* a complex alpha or beta adds one scaling pass per instance's C.        */

@@ -208,6 +208,23 @@ def fp32_leg(T, dev, stream, steps, warmup):
                     "mfma_frac": round(pairs * flops / (ms_o * 1e-3) / 1e12 / PEAK_INT8, 4),
                     "inputs": "the FP32 values held as FP64"}
     del Ad, Bd, Co
+    # the same comparison at extent 64 (the low end of the 64-96 range)
+    n64, L64 = 64 ** 4, [64] * 4
+    A6, B6 = outlier_normal(dev, n64, 43), outlier_normal(dev, n64, 143)
+    a6, b6 = strided(A6, L64), strided(B6, L64)
+    f64 = 2.0 * 4096 ** 3
+    ref64 = (a6.double().permute(0, 2, 1, 3).reshape(4096, 4096) @
+             b6.double().permute(0, 2, 1, 3).reshape(4096, 4096)).reshape(64, 64, 64, 64)
+    e64 = {"workload": "C[abcd] = A[aebf]*B[ecfd], extents 64, FP32 normal(0,1) with 1% x1e4"}
+    for name, cfg in (("split_bf16", None), ("fp64_mfma", "fp64_mfma")):
+        C6 = torch.zeros(n64, dtype=torch.float32, device=dev)
+        fn = T.prepare_mult(1.0, a6, "aebf", b6, "ecfd", 0.0, strided(C6, L64), "abcd", config=cfg)
+        ms6, _, _ = time_steps(fn, steps, warmup, stream)
+        e64[name] = {"tile": T.last_tile(), "ms": round(ms6, 4), "useful_tflops": tflops(f64, ms6),
+                     "rel_err": float("%.3e" % rel_fro(strided(C6, L64), ref64))}
+    e64["speedup_split_vs_fp64"] = round(e64["fp64_mfma"]["ms"] / e64["split_bf16"]["ms"], 3)
+    out["extent_64"] = e64
+    del A6, B6, ref64
     ms_s, ms_d = out["split_bf16"]["ms"], out["fp64_mfma"]["ms"]
     out["speedup_ozaki_vs_fp64"] = round(ms_d / ms_o, 3)
     out["speedup_split_vs_fp64"] = round(ms_d / ms_s, 3)

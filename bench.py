@@ -837,8 +837,10 @@ def main():
                                "host": host,
                                "calibration": "this port on an 8-core Xeon @ 2.7 GHz (AVX2, 8 "
                                               "threads, full contraction): C2 150-158 GFLOP/s in most "
-                                              "runs (one 218) where the reference's own build ran "
-                                              "126.2 (1.19-1.26x), C1 107.7 against 95.5 (1.13x)"}
+                                              "runs, 218-224 in others on the same host, where the "
+                                              "reference's own build ran 126.2 (1.19-1.77x: the port "
+                                              "is at least as fast as the reference), C1 107.7 "
+                                              "against 95.5 (1.13x)"}
     print(json.dumps(out))
     if tune_tmp:
         os.remove(tune_tmp)

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void oz_rowmax_kernel(const PackChunk* __restr
                                                         const T* __restrict__ A, const T* __restrict__ B,
                                                         unsigned long long* __restrict__ rmax)
 {
-    const PackChunk& j = chunks[blockIdx.x];
+    const PackChunk& j = chunks[xcd_contiguous_chunk(blockIdx.x, gridDim.x)];
     if (j.src == PANEL_MISSING) return;
     const T* __restrict__ X = j.operand ? B : A;
     const int lane = threadIdx.x & 63;
@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void oz_pack_kernel(const PackChunk* __restric
                                                       const unsigned long long* __restrict__ rmax,
                                                       int8_t* __restrict__ ws)
 {
-    const PackChunk& j = chunks[blockIdx.x];
+    const PackChunk& j = chunks[xcd_contiguous_chunk(blockIdx.x, gridDim.x)];
     const T* __restrict__ X = j.operand ? B : A;
     const bool zero = j.src == PANEL_MISSING;
     const int lane = threadIdx.x & 63;

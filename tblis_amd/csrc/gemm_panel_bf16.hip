@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void pack_bf16_panel_kernel(const PackChunk* _
                                                               const float* __restrict__ B,
                                                               __bf16* __restrict__ ws)
 {
-    const PackChunk& j = chunks[blockIdx.x];
+    const PackChunk& j = chunks[xcd_contiguous_chunk(blockIdx.x, gridDim.x)];
     const float* __restrict__ X = j.operand ? B : A;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);

@@ -8,6 +8,17 @@
 namespace tblis_amd
 {
 
+// The chunk a pack workgroup takes: workgroup b of a launch runs on XCD
+// b % 8 (round-robin dispatch), and XCD x takes the x-th contiguous eighth
+// of the n chunks, so that neighbouring chunks, which share the cache lines
+// at their edges, read them through one L2.
+__device__ __forceinline__ int xcd_contiguous_chunk(int b, int n)
+{
+    constexpr int kX = 8;
+    const int x = b % kX, i = b / kX, per = n / kX, rem = n % kX;
+    return x * per + (x < rem ? x : rem) + i;
+}
+
 // Lane-layout probes of the two MFMA forms the GEMMs use (one wave each).
 hipError_t launch_mfma_probe(double* out, hipStream_t s);
 hipError_t launch_bf16_probe(const float* A, const float* B, float* D, hipStream_t s);

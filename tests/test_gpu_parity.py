@@ -384,6 +384,9 @@ def test_every_panel_tile(tile, direct, monkeypatch):
         # permuted workload shape at reduced size: A's rows and k are affine
         _dense_case(rng, "aebf", "fjei", "aibj", dict(a=32, b=5, e=16, f=3, i=7, j=9),
                     ["col"] * 3, alpha=0.5, beta=1.0)
+        # k = 3 (two k groups): the k-split tiles have wave sets with no k
+        # group, which add zeros in the reduction
+        _dense_case(rng, "ae", "eb", "ab", dict(a=70, b=45, e=3), ["col", "row", "col"], alpha=1.5, beta=0.5)
     finally:
         monkeypatch.delenv("TBLIS_HIP_PANEL_TILE")
         monkeypatch.delenv("TBLIS_HIP_DIRECT")

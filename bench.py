@@ -84,23 +84,34 @@ def strided(flat, lens):
 
 
 def time_steps(fn, steps, warmup, stream):
-    """Wall ms per step, and mean/min of HIP-event time around each call on
-    the library's stream."""
+    """Wall ms per step over the timed region; HIP-event ms per step over the
+    same region (one event pair on the library's stream around all steps:
+    an event record between calls costs ~8 us of GPU time per call, 30% of a
+    dense 24^4 call); and the fastest single call of a short run after the
+    timed region, with events around each call."""
     import torch
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
-    for i in range(steps):
+    start.record(stream)
+    for _ in range(steps):
+        fn()
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    region = start.elapsed_time(end) / steps
+    n = min(steps, 10)
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+    for i in range(n):
         starts[i].record(stream)
         fn()
         ends[i].record(stream)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern = [s.elapsed_time(e) for s, e in zip(starts, ends)]
-    return wall / steps * 1e3, float(np.mean(kern)), float(np.min(kern))
+    single = min(s.elapsed_time(e) for s, e in zip(starts, ends))
+    return wall / steps * 1e3, float(region), float(single)
 
 
 def rel_fro(got, want):

@@ -146,12 +146,19 @@ def permuted_leg(T, dev, stream, steps, warmup):
         "config": {"workload": "C[aibj] += 0.5*A[aebf]*B[fjei], a=b=e=f=64, i=j=32, column-major, "
                                "FP64 MFMA (tblis_tensor_mult)",
                    "m": 4096, "n": 1024, "k": 4096, "algorithm": T.last_algorithm(),
-                   "tile": T.last_tile()},
+                   "tile": T.last_tile(), "read_direct": direct_names(T)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64, 4), "traffic": None,
                      "kernel_ms_mean": round(ms_kern, 4), "kernel_ms_min": round(ms_min, 4),
                      "algorithmic_flops": flops, "algorithmic_bytes": nbytes},
     }, (lA, lB, lC)
+
+
+def direct_names(T):
+    """Operands the last call's GEMM read straight from the tensors (the
+    others were packed into panels first)."""
+    d = T.last_direct()
+    return [n for n, bit in (("A", 1), ("B", 2)) if d & bit]
 
 
 # ---------------------------------------------------------------- dense 24^4
@@ -168,6 +175,7 @@ def dense24_leg(T, dev, stream, steps, warmup):
     ms_graph, _, _ = time_steps(graph.replay, steps, warmup, torch.cuda.current_stream())
     return {"workload": "C[abij] = A[abef]*B[efij], extents 24, FP64, column-major, alpha 1, beta 0",
             "useful_flops": flops, "algorithm": T.last_algorithm(), "tile": T.last_tile(),
+            "read_direct": direct_names(T),
             "ms": round(ms_step, 4), "useful_tflops": tflops(flops, ms_step), "call_ms_events": round(ms_kern, 4),
             "graph_ms": round(ms_graph, 4), "graph_useful_tflops": tflops(flops, ms_graph)}
 

@@ -412,7 +412,9 @@ void tblis_hip_clear_plan_cache(void);
 const char* tblis_hip_last_algorithm(void);
 /* Autotuner (off unless TBLIS_HIP_AUTOTUNE=1): when on, building a plan for
  * a new set of GEMM shapes times every candidate tile of the panel GEMM on
- * scratch buffers and keeps the fastest; results are cached per process and,
+ * scratch buffers (a tile that reads an operand straight from its tensor
+ * also with every operand packed) and keeps the fastest; results are cached
+ * per process and,
  * with TBLIS_HIP_TUNE_FILE=<path>, appended to / read from that file.  No
  * measurement happens while the stream is being captured. */
 void tblis_hip_set_autotune(int on);
@@ -420,6 +422,10 @@ void tblis_hip_set_autotune(int on);
 int tblis_hip_autotune_runs(void);
 /* Panel GEMM tile of the last contraction on this thread. */
 int tblis_hip_last_tile(void);
+/* Operands that contraction's panel GEMM read straight from the tensors
+ * instead of packing them: bit 0 A, bit 1 B (set when every GEMM task reads
+ * it so); 0 for other kernels. */
+int tblis_hip_last_direct(void);
 
 /* No counterpart in the reference: tblis_tensor_mult for callers that
  * cannot allocate device memory (serving loops and harnesses that forbid

@@ -93,7 +93,7 @@ EXPORTED = [
     "tblis_hip_set_stream",
     "tblis_hip_get_stream", "tblis_hip_set_async", "tblis_hip_set_error_mode",
     "tblis_hip_get_last_error", "tblis_hip_clear_plan_cache", "tblis_hip_last_algorithm",
-    "tblis_hip_set_autotune", "tblis_hip_autotune_runs", "tblis_hip_last_tile",
+    "tblis_hip_set_autotune", "tblis_hip_autotune_runs", "tblis_hip_last_tile", "tblis_hip_last_direct",
     "tblis_hip_tensor_mult_noalloc", "tblis_hip_device_alloc_count",
     "tblis_hip_debug_canonical", "tblis_hip_debug_fill_block_scatter", "tblis_hip_debug_panel_side",
     "tblis_hip_debug_dpd_block",
@@ -199,6 +199,12 @@ def autotune_runs():
 
 def last_tile():
     return lib().tblis_hip_last_tile()
+
+
+def last_direct():
+    """Operands the last contraction's panel GEMM read straight from the
+    tensors instead of packing them: bit 0 A, bit 1 B."""
+    return lib().tblis_hip_last_direct()
 
 
 def _type_code(dtype):

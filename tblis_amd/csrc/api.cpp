@@ -444,6 +444,7 @@ const char* tblis_hip_get_last_error(void) { return thread_state().last_error.c_
 void tblis_hip_clear_plan_cache(void) { cache_clear(); }
 void tblis_hip_set_autotune(int on) { set_autotune(on); }
 int tblis_hip_last_tile(void) { return thread_state().last_tile; }
+int tblis_hip_last_direct(void) { return thread_state().last_direct; }
 int tblis_hip_autotune_runs(void) { return autotune_runs(); }
 
 int64_t tblis_hip_device_alloc_count(void) { return device_alloc_count(); }

@@ -829,6 +829,7 @@ void contract(const tblis_config* cfg, const Ix& a, const Ix& b, const Ix& c, ty
     if (algo == Algo::Reference) algo = rtype == TYPE_DOUBLE ? Algo::Fp64Mfma : Algo::Bf16x3;
     thread_state().last_algo = algo_name(algo);
     thread_state().last_tile = -1;
+    thread_state().last_direct = 0;
 
     std::string key;
     put(key, (int32_t)algo);

@@ -62,6 +62,7 @@ void tensor_add(const tblis_config* cfg, const tblis_tensor* A, const label_type
     if (A->type != B->type) fail("A and B must have the same type");
     check_real(A->type);
     thread_state().last_tile = -1;
+    thread_state().last_direct = 0;
     Level1Form f;
     std::string why;
     if (!canonicalize_add(*A, idx_A, *B, idx_B, f, why)) fail("invalid add: " + why);

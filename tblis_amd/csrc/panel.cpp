@@ -453,10 +453,16 @@ namespace
 // operands)
 bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T,
                      const bool allow_direct[2], int src_elem);
-// tables_for_tile with each operand read directly only if it can be for
-// every GEMM of the plan (one kernel variant per launch)
-bool tables_uniform(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T, int src_elem);
-// model_tile: the cost model's pick, preferred among finalists that tie
+// tables_for_tile with the tasks grouped by which operands they read
+// directly (one kernel variant per launch); packed: every operand packed
+bool tables_uniform(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T, int src_elem,
+                    bool packed = false);
+// Autotuner candidates: a tile, or with kPackedBit the same tile with every
+// operand packed (timed next to the planner's direct reads)
+constexpr int kPackedBit = 256;
+// model_tile: the cost model's pick, preferred among finalists that tie.
+// Returns the measured choice (a tile, + kPackedBit for its every-operand-
+// packed variant) or -1.
 int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bool swap,
                   const std::vector<int>& cands, int model_tile);
 
@@ -526,17 +532,25 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family,
             tile = tm.tile;
         }
     }
+    bool packed = false;
     const int tuned = autotune_tile(gemms, family, dtype, swap, cands, tile);
-    if (tuned >= 0) tile = tuned;
+    if (tuned >= 0)
+    {
+        tile = tuned % kPackedBit;
+        packed = tuned >= kPackedBit;
+    }
     if (const char* e = std::getenv("TBLIS_HIP_PANEL_TILE"))
     {
         int t = std::atoi(e);
         if (e[0] >= '0' && e[0] <= '9' && t < panel_num_tiles(family) &&
             !(family == FAMILY_BF16X3 && panel_bf16_tile_fast_only(t)) &&
             !(panel_tile_lds(family, t) && !lds_applicable(gemms, dtype == 0 ? 8 : 4)))
+        {
             tile = t;
+            packed = false;
+        }
     }
-    if (!tables_uniform(gemms, family, tile, T, dtype == 0 ? 8 : 4)) return false;
+    if (!tables_uniform(gemms, family, tile, T, dtype == 0 ? 8 : 4, packed)) return false;
     T.swap = swap;
     return true;
 }
@@ -547,9 +561,10 @@ namespace
 // Tasks grouped by which operands they read straight from the tensors
 // (GEMMs of one batch may differ): the groups become separate launches of
 // the matching kernel variant.
-bool tables_uniform(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T, int src_elem)
+bool tables_uniform(const std::vector<PanelGemm>& gemms, int family, int tile, PanelTables& T, int src_elem,
+                    bool packed)
 {
-    const bool allow[2] = {true, true};
+    const bool allow[2] = {!packed, !packed};
     T = PanelTables();
     if (!tables_for_tile(gemms, family, tile, T, allow, src_elem)) return false;
     auto group = [](const PanelTask& t) {
@@ -1182,24 +1197,29 @@ void load_tune_file(TuneDb& db)
     char line[256];
     while (std::fgets(line, sizeof line, f))
     {
-        int fam = -1, tile = -1;
+        int fam = -1, tile = -1, packed = -1;
         unsigned long long h = 0;
-        if (std::sscanf(line, "{\"format\": \"tblis-tune-v2\", \"family\": %d, \"shapes\": \"%llx\", \"tile\": %d",
-                        &fam, &h, &tile) == 3 &&
-            fam >= 0 && fam <= FAMILY_BF16X3_FAST && tile >= 0 && tile < panel_num_tiles(fam))
-            db.best[(uint64_t)h ^ (uint64_t)fam] = tile;
+        if (std::sscanf(line,
+                        "{\"format\": \"tblis-tune-v3\", \"family\": %d, \"shapes\": \"%llx\", \"tile\": %d, "
+                        "\"packed\": %d",
+                        &fam, &h, &tile, &packed) == 4 &&
+            fam >= 0 && fam <= FAMILY_BF16X3_FAST && tile >= 0 && tile < panel_num_tiles(fam) &&
+            (packed == 0 || packed == 1))
+            db.best[(uint64_t)h ^ (uint64_t)fam] = tile + (packed ? kPackedBit : 0);
     }
     std::fclose(f);
 }
 
-void save_tune(int family, uint64_t h, int tile)
+void save_tune(int family, uint64_t h, int code)
 {
     const char* path = std::getenv("TBLIS_HIP_TUNE_FILE");
     if (!path || !path[0]) return;
     FILE* f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "{\"format\": \"tblis-tune-v2\", \"family\": %d, \"shapes\": \"%016llx\", \"tile\": %d}\n",
-                 family, (unsigned long long)h, tile);
+    std::fprintf(f,
+                 "{\"format\": \"tblis-tune-v3\", \"family\": %d, \"shapes\": \"%016llx\", \"tile\": %d, "
+                 "\"packed\": %d}\n",
+                 family, (unsigned long long)h, code % kPackedBit, code >= kPackedBit ? 1 : 0);
     std::fclose(f);
 }
 
@@ -1255,24 +1275,43 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bo
 
     // without the tensors (or with direct reads off) the candidates run
     // from packed panels, which the LDS-staged tiles never read
-    std::vector<int> cands_run;
+    // With the tensors, a tile whose plan reads an operand straight from
+    // its tensor is timed a second time with every operand packed: for
+    // short GEMMs the pack can win (dense 24^4: 26.2 against 28.1 us per
+    // call on the four-way k-split tile).  Plans under DirectAllScope keep
+    // their direct reads (one GEMM launch).
+    std::vector<int> cands;  // tile, + kPackedBit: every operand packed
+    std::vector<PanelTables> tabs;
     for (int c : cands0)
-        if (real || !panel_tile_lds(family, c)) cands_run.push_back(c);
-    if (cands_run.size() < 2) return -1;
-    const std::vector<int>& cands = cands_run;
-    std::vector<PanelTables> tabs(cands.size());
+    {
+        if (!real && panel_tile_lds(family, c)) continue;
+        // with tensors: grouped as the plan will be; without: both operands
+        // in panels
+        const bool none[2] = {false, false};
+        PanelTables T1;
+        if (real ? !tables_uniform(gemms, family, c, T1, dtype == 0 ? 8 : 4)
+                 : !tables_for_tile(gemms, family, c, T1, none, dtype == 0 ? 8 : 4))
+            return -1;
+        const bool reads = T1.group_n[1] + T1.group_n[2] + T1.group_n[3] > 0;
+        cands.push_back(c);
+        tabs.push_back(std::move(T1));
+        if (real && reads && !t_direct_all && !panel_tile_lds(family, c))
+        {
+            PanelTables T2;
+            if (!tables_uniform(gemms, family, c, T2, dtype == 0 ? 8 : 4, true)) return -1;
+            cands.push_back(c + kPackedBit);
+            tabs.push_back(std::move(T2));
+        }
+    }
+    if (cands.size() < 2) return -1;
+    auto tile_of = [](int code) { return code % kPackedBit; };
     size_t panel_bytes = 256, cslot = 0, ntask = 0;
     const int eb = geom_of(family).elem_bytes;
     for (size_t i = 0; i < cands.size(); i++)
     {
-        // with tensors: grouped as the plan will be; without: both operands
-        // in panels
-        const bool none[2] = {false, false};
-        if (real ? !tables_uniform(gemms, family, cands[i], tabs[i], dtype == 0 ? 8 : 4)
-                 : !tables_for_tile(gemms, family, cands[i], tabs[i], none, dtype == 0 ? 8 : 4))
-            return -1;
+        const int ti = tile_of(cands[i]);
         panel_bytes = std::max(panel_bytes, (size_t)tabs[i].ws_panel * eb + 256);
-        cslot = std::max(cslot, (size_t)panel_tile_m(family, cands[i]) * panel_tile_n(family, cands[i]));
+        cslot = std::max(cslot, (size_t)panel_tile_m(family, ti) * panel_tile_n(family, ti));
         ntask = std::max(ntask, tabs[i].tasks.size());
     }
     constexpr size_t SLOTS = 1024;
@@ -1296,7 +1335,7 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bo
     std::vector<PanelTask> tasks;
     for (size_t i = 0; ok && i < nc; i++)
     {
-        const int BM = panel_tile_m(family, cands[i]), BN = panel_tile_n(family, cands[i]);
+        const int BM = panel_tile_m(family, tile_of(cands[i])), BN = panel_tile_n(family, tile_of(cands[i]));
         tasks = tabs[i].tasks;
         for (size_t k = 0; k < tasks.size(); k++)
         {
@@ -1335,7 +1374,7 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bo
         L.use_k = 1;
         L.dtype = real ? dtype : (family == FAMILY_F64 ? 0 : 1);
         L.family = family;
-        L.tile = cands[i];
+        L.tile = tile_of(cands[i]);
         L.stream = s;
         if (!real) return launch_panel_gemm(L) == hipSuccess;
         // pack into the scratch panels, then one GEMM launch per group
@@ -1435,10 +1474,13 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bo
             if (v[0] == '1')
             {
                 std::fprintf(stderr, "tblis autotune family %d:", family);
-                for (size_t i = 0; i < nc; i++) std::fprintf(stderr, " tile %d %.4f ms", cands[i], t[i]);
+                auto name = [&](int code) { return code >= kPackedBit ? " packed" : ""; };
+                for (size_t i = 0; i < nc; i++)
+                    std::fprintf(stderr, " tile %d%s %.4f ms", tile_of(cands[i]), name(cands[i]), t[i]);
                 for (size_t f = 0; f < steady.size(); f++)
-                    std::fprintf(stderr, "%s tile %d %.4f ms", f ? "," : "; back to back:", cands[fin[f]], steady[f]);
-                std::fprintf(stderr, " -> tile %d\n", best_tile);
+                    std::fprintf(stderr, "%s tile %d%s %.4f ms", f ? "," : "; back to back:", tile_of(cands[fin[f]]),
+                                 name(cands[fin[f]]), steady[f]);
+                std::fprintf(stderr, " -> tile %d%s\n", tile_of(best_tile), name(best_tile));
             }
     }
     if (hipStreamSynchronize(s) != hipSuccess) best_tile = -1;

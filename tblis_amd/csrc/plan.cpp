@@ -366,6 +366,8 @@ void run_plan(const std::shared_ptr<Plan>& pp, const void* A, const void* B, voi
     const bool capturing = cap != hipStreamCaptureStatusNone;
     if (capturing) pin_plan(pp);
     ts.last_tile = p.panel_tile;
+    // p.direct is in the GEMM's operand order, swapped when it computes C^T
+    ts.last_direct = (p.direct[p.swap ? 1 : 0] ? 1 : 0) | (p.direct[p.swap ? 0 : 1] ? 2 : 0);
     PanelLaunch P{};
     P.chunks = p.d_pchunks;
     P.nchunks = p.p_nchunks;

@@ -141,6 +141,7 @@ struct ThreadState
     std::string last_error;
     std::string last_algo;
     int last_tile = -1;  // panel tile of the last launched plan (-1: other kernels)
+    int last_direct = 0;  // its operands read from the tensors: 1 A, 2 B
     // side streams for staged packs, by (device, main stream), and the
     // events that order them (reused call after call)
     std::map<std::pair<int, hipStream_t>, hipStream_t> side;

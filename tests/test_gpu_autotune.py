@@ -38,7 +38,47 @@ def test_autotune_picks_and_runs(tmp_path, monkeypatch):
         assert T.autotune_runs() >= r1 + 2
         lines = [json.loads(l) for l in path.read_text().splitlines()]
         assert len(lines) == T.autotune_runs() - r0
-        assert all(l["format"] == "tblis-tune-v2" and l["tile"] >= 0 for l in lines)
+        assert all(l["format"] == "tblis-tune-v3" and l["tile"] >= 0 and l["packed"] in (0, 1) for l in lines)
+    finally:
+        T.set_autotune(False)
+        T.lib().tblis_hip_clear_plan_cache()
+
+
+def test_packed_variant_recorded_and_reused(tmp_path, monkeypatch):
+    """Without the tuner the col-major 24^4 contraction reads both operands
+    straight from the tensors (last_direct 3), and TBLIS_HIP_DIRECT=0 packs
+    both (0).  With the tuner, a tile that reads an operand directly is also
+    timed with every operand packed: the choice lands in the tuning file as
+    (tile, packed), the plan follows it (packed 1: last_direct 0), a new
+    plan of the same shapes reuses it, and every form matches the oracle."""
+    from tests.test_gpu_parity import _dense_case
+    ext = dict(a=24, b=24, e=24, f=24, i=24, j=24)
+    rng = np.random.default_rng(33100)
+    T.lib().tblis_hip_clear_plan_cache()
+    _dense_case(rng, "abef", "efij", "abij", ext, ["col"] * 3, beta=0.0)
+    assert T.last_direct() == 3
+    monkeypatch.setenv("TBLIS_HIP_DIRECT", "0")
+    T.lib().tblis_hip_clear_plan_cache()
+    _dense_case(rng, "abef", "efij", "abij", ext, ["col"] * 3, beta=0.0)
+    assert T.last_direct() == 0
+    monkeypatch.delenv("TBLIS_HIP_DIRECT")
+    path = tmp_path / "tune.txt"
+    monkeypatch.setenv("TBLIS_HIP_TUNE_FILE", str(path))
+    T.set_autotune(True)
+    T.lib().tblis_hip_clear_plan_cache()
+    try:
+        r0 = T.autotune_runs()
+        _dense_case(rng, "abef", "efij", "abij", ext, ["col"] * 3, beta=0.0)
+        assert T.autotune_runs() == r0 + 1
+        line = json.loads(path.read_text().splitlines()[-1])
+        assert line["format"] == "tblis-tune-v3" and line["tile"] == T.last_tile()
+        if line["packed"] == 1:
+            assert T.last_direct() == 0
+        tile, direct = T.last_tile(), T.last_direct()
+        T.lib().tblis_hip_clear_plan_cache()
+        _dense_case(rng, "abef", "efij", "abij", ext, ["col"] * 3, alpha=-1.25, beta=0.5)
+        assert T.autotune_runs() == r0 + 1
+        assert (T.last_tile(), T.last_direct()) == (tile, direct)
     finally:
         T.set_autotune(False)
         T.lib().tblis_hip_clear_plan_cache()

@@ -1,11 +1,14 @@
 """The panel-tile autotuner: with tuning on, a new GEMM shape is measured
 once (the tuning counter moves, the tuning file gets a line), later plans of
 the same shapes reuse the result, and the contraction it then runs matches
-the oracle (FP64: 1e-12, split-BF16: 1e-6 relative Frobenius)."""
+the oracle (FP64: 1e-12, split-BF16: 1e-6 relative Frobenius); the
+operands a plan reads straight from the tensors are reported per caller
+operand."""
 import json
 
 import numpy as np
 import pytest
+import torch
 
 import tblis_amd as T
 from tests.test_gpu_bf16x3 import _float_case, uniform
@@ -82,3 +85,22 @@ def test_packed_variant_recorded_and_reused(tmp_path, monkeypatch):
     finally:
         T.set_autotune(False)
         T.lib().tblis_hip_clear_plan_cache()
+
+
+def test_last_direct_names_the_callers_operands():
+    """last_direct() reports the caller's A and B, whichever way round the
+    GEMM runs: A with no unit stride (strides (2e, 2)) is packed, B row-major
+    (unit stride along its columns) is read from the tensor, with C row-major
+    (the GEMM computes C) and column-major (it computes C^T = B^T A^T)."""
+    torch.manual_seed(33200)
+    a, e, b = 64, 40, 48
+    A = torch.rand(a, 2 * e, dtype=torch.float64, device="cuda")[:, ::2]
+    B = torch.rand(e, b, dtype=torch.float64, device="cuda")
+    want = A.cpu() @ B.cpu()
+    for C in (torch.zeros(a, b, dtype=torch.float64, device="cuda"),
+              torch.zeros(b, a, dtype=torch.float64, device="cuda").t()):
+        T.lib().tblis_hip_clear_plan_cache()
+        T.mult(1.0, A, "ae", B, "eb", 0.0, C, "ab")
+        torch.cuda.synchronize()
+        assert torch.allclose(C.cpu(), want, rtol=1e-13, atol=1e-13)
+        assert T.last_direct() == 2, (C.stride(), T.last_direct())

@@ -394,7 +394,11 @@ __device__ __forceinline__ void stage_frags(bf16x8 (&x)[NBLK][3], const Stage<NB
             for (int t = 0; t < 3; t++) x[i][t] = st.f[i][t];
 }
 
-template <int BM, int BN, int WGM, int WGN, bool CHUNKED, bool DA, bool DB>
+// One C tile.  KS > 1 splits k inside the workgroup: KS sets of WGM x WGN
+// waves take interleaved pairs of k groups (pair p goes to set p % KS) and
+// their sums meet in LDS before set 0 runs the epilogue; every wave reaches
+// that barrier.
+template <int BM, int BN, int WGM, int WGN, bool CHUNKED, bool DA, bool DB, int KS>
 __device__ __forceinline__ void bf16_task(const PanelTask& t, const int64_t* __restrict__ pool,
                                           const __bf16* __restrict__ ws, double* __restrict__ cws,
                                           const float* A, const float* B, float* __restrict__ C, double alpha,
@@ -403,9 +407,11 @@ __device__ __forceinline__ void bf16_task(const PanelTask& t, const int64_t* __r
     constexpr int WM = BM / WGM, WN = BN / WGN, MB = WM / 32, NB = WN / 32;
     constexpr int CHUNK = 16;  // k16 steps per accumulation chunk (256 k)
     const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wave_all = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int kw = wave_all / (WGM * WGN), wave = wave_all % (WGM * WGN);
     const int wm = wave % WGM, wn = wave / WGM;
-    if (wm * WM >= t.mlen || wn * WN >= t.nlen) return;  // no barriers below
+    const bool active = !(wm * WM >= t.mlen || wn * WN >= t.nlen);
+    if (KS == 1 && !active) return;  // no barriers below when KS == 1
 
     f32x16 acc[MB][NB], tot[CHUNKED ? MB : 1][CHUNKED ? NB : 1];
 #pragma unroll
@@ -420,8 +426,9 @@ __device__ __forceinline__ void bf16_task(const PanelTask& t, const int64_t* __r
             for (int jb = 0; jb < (CHUNKED ? NB : 1); jb++)
                 for (int r = 0; r < 16; r++) tot[i][jb][r] = 0.0f;
 
-    const int nk = use_k ? t.nkg : 0;  // even
-    if (nk > 0)
+    // this set's pairs of k groups: kw, kw + KS, ... (t.nkg is even)
+    const int np = use_k && active ? (t.nkg / 2 - kw + KS - 1) / KS : 0;
+    if (np > 0)
     {
         const int lr = lane & 31, lh = lane >> 5;
         const int64_t rs = (int64_t)t.nkg * 192;  // uint4 per row block
@@ -435,14 +442,14 @@ __device__ __forceinline__ void bf16_task(const PanelTask& t, const int64_t* __r
             direct_f32_init<NB>(db, B, t.b_src, t.b_bytes, pool, t.b_rt, t.b_kt, t.b_km, wn * WN, t.nlen, lr);
         Stage<MB, DA> a0, a1;
         Stage<NB, DB> b0, b1;
-        stage_load<MB, DA>(a0, pa, rs, da, 0, lh, t.klen);
-        stage_load<NB, DB>(b0, pb, rs, db, 0, lh, t.klen);
-        stage_load<MB, DA>(a1, pa, rs, da, 1, lh, t.klen);
-        stage_load<NB, DB>(b1, pb, rs, db, 1, lh, t.klen);
-        for (int ks = 0; ks < nk; ks += 2)
+        stage_load<MB, DA>(a0, pa, rs, da, 2 * kw, lh, t.klen);
+        stage_load<NB, DB>(b0, pb, rs, db, 2 * kw, lh, t.klen);
+        stage_load<MB, DA>(a1, pa, rs, da, 2 * kw + 1, lh, t.klen);
+        stage_load<NB, DB>(b1, pb, rs, db, 2 * kw + 1, lh, t.klen);
+        for (int q = 0; q < np; q++)
         {
-            // the last iteration re-reads the final groups (in bounds, unused)
-            const int nx = ks + 2 < nk ? ks + 2 : nk - 2;
+            // the last iteration re-reads its own groups (in bounds, unused)
+            const int nx = 2 * (kw + (q + 1 < np ? q + 1 : q) * KS);
             bf16x8 fa[MB][3], fb[NB][3];
             stage_frags<MB, DA>(fa, a0);
             stage_frags<NB, DB>(fb, b0);
@@ -454,7 +461,7 @@ __device__ __forceinline__ void bf16_task(const PanelTask& t, const int64_t* __r
             mma_frags<MB, NB>(acc, fa, fb);
             stage_load<MB, DA>(a1, pa, rs, da, nx + 1, lh, t.klen);
             stage_load<NB, DB>(b1, pb, rs, db, nx + 1, lh, t.klen);
-            if (CHUNKED && ((ks + 2) % CHUNK == 0 || ks + 2 >= nk))
+            if (CHUNKED && ((2 * q + 2) % CHUNK == 0 || q + 1 >= np))
             {
 #pragma unroll
                 for (int i = 0; i < MB; i++)
@@ -466,6 +473,32 @@ __device__ __forceinline__ void bf16_task(const PanelTask& t, const int64_t* __r
                     }
             }
         }
+    }
+    if constexpr (KS > 1)
+    {
+        // the other sets' sums (the chunk totals when CHUNKED) through LDS
+        __shared__ f32x16 red[KS - 1][WGM * WGN][MB * NB][64];
+        if (kw > 0 && active)
+#pragma unroll
+            for (int i = 0; i < MB; i++)
+#pragma unroll
+                for (int jb = 0; jb < NB; jb++)
+                    red[kw - 1][wave][i * NB + jb][lane] =
+                        CHUNKED ? tot[i % (CHUNKED ? MB : 1)][jb % (CHUNKED ? NB : 1)] : acc[i][jb];
+        __syncthreads();
+        if (kw > 0 || !active) return;
+#pragma unroll
+        for (int s = 0; s < KS - 1; s++)
+#pragma unroll
+            for (int i = 0; i < MB; i++)
+#pragma unroll
+                for (int jb = 0; jb < NB; jb++)
+                {
+                    if constexpr (CHUNKED)
+                        tot[i % (CHUNKED ? MB : 1)][jb % (CHUNKED ? NB : 1)] += red[s][wave][i * NB + jb][lane];
+                    else
+                        acc[i][jb] += red[s][wave][i * NB + jb][lane];
+                }
     }
 
     // D register r of lane l holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32]
@@ -537,8 +570,8 @@ __device__ __forceinline__ void bf16_task(const PanelTask& t, const int64_t* __r
 }
 
 // Persistent over the task list, as the FP64 panel GEMM.
-template <int BM, int BN, int WGM, int WGN, bool CHUNKED, bool DA, bool DB>
-__global__ __launch_bounds__(64 * WGM * WGN) void gemm_bf16_panel_kernel(
+template <int BM, int BN, int WGM, int WGN, bool CHUNKED, bool DA, bool DB, int KS>
+__global__ __launch_bounds__(64 * WGM * WGN * KS) void gemm_bf16_panel_kernel(
     const PanelTask* __restrict__ tasks, int ntasks, const int64_t* __restrict__ pool,
     const __bf16* __restrict__ ws, double* __restrict__ cws, const float* A, const float* B,
     float* __restrict__ C, double alpha, double beta, int use_k)
@@ -546,7 +579,9 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_bf16_panel_kernel(
     for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x)
     {
         const PanelTask t = tasks[ti];
-        bf16_task<BM, BN, WGM, WGN, CHUNKED, DA, DB>(t, pool, ws, cws, A, B, C, alpha, beta, use_k);
+        bf16_task<BM, BN, WGM, WGN, CHUNKED, DA, DB, KS>(t, pool, ws, cws, A, B, C, alpha, beta, use_k);
+        // the next task's partial sums reuse the LDS that set 0 just read
+        if constexpr (KS > 1) __syncthreads();
     }
 }
 
@@ -557,6 +592,7 @@ struct Bf16Tile
     int bm, bn, wgm, wgn;
     bool fast_only;  // the chunked accumulator does not fit the register file
     bool direct;     // variants that read FP32 operands from the tensors
+    int ks;          // k sets inside the workgroup
 };
 
 // id, BM, BN, waves (WGM x WGN), fast only, direct variants.  Registers
@@ -565,42 +601,46 @@ struct Bf16Tile
 // registers for the single accumulator and spills; it is not offered.)
 // Direct operands hold two 16-k steps of raw FP32 per block and split them
 // in registers, so they are offered on the 64x64 and 32x32 wave tiles.
-#define TBLIS_PANEL_BF16_TILES(X)    \
-    X(0, 256, 128, 2, 2, false, false) \
-    X(1, 128, 256, 2, 2, false, false) \
-    X(2, 128, 128, 2, 2, false, true)  \
-    X(3, 64, 64, 2, 2, false, true)
+// Tiles 4 and 5 are tile 3 with k split between two and four sets of waves
+// (short k: DPD irrep GEMMs), reading packed panels only.
+#define TBLIS_PANEL_BF16_TILES(X)        \
+    X(0, 256, 128, 2, 2, false, false, 1) \
+    X(1, 128, 256, 2, 2, false, false, 1) \
+    X(2, 128, 128, 2, 2, false, true, 1)  \
+    X(3, 64, 64, 2, 2, false, true, 1)    \
+    X(4, 64, 64, 2, 2, false, false, 2)   \
+    X(5, 64, 64, 2, 2, false, false, 4)
 
 constexpr Bf16Tile kBTiles[] = {
-#define X(id, bm, bn, wgm, wgn, fo, dir) {bm, bn, wgm, wgn, fo, dir},
+#define X(id, bm, bn, wgm, wgn, fo, dir, ks) {bm, bn, wgm, wgn, fo, dir, ks},
     TBLIS_PANEL_BF16_TILES(X)
 #undef X
 };
 constexpr int kNumBTiles = sizeof(kBTiles) / sizeof(kBTiles[0]);
 
-template <int BM, int BN, int WGM, int WGN, bool CHUNKED, bool DA, bool DB>
+template <int BM, int BN, int WGM, int WGN, bool CHUNKED, bool DA, bool DB, int KS>
 hipError_t launch_bf16_t(const PanelLaunch& L)
 {
-    auto* k = &gemm_bf16_panel_kernel<BM, BN, WGM, WGN, CHUNKED, DA, DB>;
-    const int threads = 64 * WGM * WGN;
+    auto* k = &gemm_bf16_panel_kernel<BM, BN, WGM, WGN, CHUNKED, DA, DB, KS>;
+    const int threads = 64 * WGM * WGN * KS;
     const dim3 grid(persistent_grid((const void*)k, threads, L.ntasks));
     hipLaunchKernelGGL(k, grid, dim3(threads), 0, L.stream, L.tasks, L.ntasks, L.pool, (const __bf16*)L.ws,
                        L.cws, (const float*)L.A, (const float*)L.B, (float*)L.C, L.alpha, L.beta, L.use_k);
     return hipGetLastError();
 }
 
-template <int BM, int BN, int WGM, int WGN, bool CHUNKED, bool DIR>
+template <int BM, int BN, int WGM, int WGN, bool CHUNKED, bool DIR, int KS>
 hipError_t launch_bf16_d(const PanelLaunch& L)
 {
     if constexpr (DIR)
     {
-        if (L.direct_a && L.direct_b) return launch_bf16_t<BM, BN, WGM, WGN, CHUNKED, true, true>(L);
-        if (L.direct_a) return launch_bf16_t<BM, BN, WGM, WGN, CHUNKED, true, false>(L);
-        if (L.direct_b) return launch_bf16_t<BM, BN, WGM, WGN, CHUNKED, false, true>(L);
+        if (L.direct_a && L.direct_b) return launch_bf16_t<BM, BN, WGM, WGN, CHUNKED, true, true, KS>(L);
+        if (L.direct_a) return launch_bf16_t<BM, BN, WGM, WGN, CHUNKED, true, false, KS>(L);
+        if (L.direct_b) return launch_bf16_t<BM, BN, WGM, WGN, CHUNKED, false, true, KS>(L);
     }
     else if (L.direct_a || L.direct_b)
         return hipErrorInvalidValue;
-    return launch_bf16_t<BM, BN, WGM, WGN, CHUNKED, false, false>(L);
+    return launch_bf16_t<BM, BN, WGM, WGN, CHUNKED, false, false, KS>(L);
 }
 
 }  // namespace
@@ -608,7 +648,7 @@ hipError_t launch_bf16_d(const PanelLaunch& L)
 int panel_bf16_num_tiles() { return kNumBTiles; }
 int panel_bf16_tile_m(int t) { return kBTiles[t].bm; }
 int panel_bf16_tile_n(int t) { return kBTiles[t].bn; }
-int panel_bf16_tile_waves(int t) { return kBTiles[t].wgm * kBTiles[t].wgn; }
+int panel_bf16_tile_waves(int t) { return kBTiles[t].wgm * kBTiles[t].wgn * kBTiles[t].ks; }
 bool panel_bf16_tile_fast_only(int t) { return kBTiles[t].fast_only; }
 bool panel_bf16_tile_direct(int t) { return t >= 0 && t < kNumBTiles && kBTiles[t].direct; }
 
@@ -629,11 +669,11 @@ hipError_t launch_panel_gemm_bf16(const PanelLaunch& L)
     if (chunked && kBTiles[L.tile].fast_only) return hipErrorInvalidValue;
     switch (L.tile * 2 + (chunked ? 1 : 0))
     {
-#define X(id, bm, bn, wgm, wgn, fo, dir)                           \
+#define X(id, bm, bn, wgm, wgn, fo, dir, ks)                       \
     case 2 * id:                                                   \
-        return launch_bf16_d<bm, bn, wgm, wgn, false, dir>(L);     \
+        return launch_bf16_d<bm, bn, wgm, wgn, false, dir, ks>(L); \
     case 2 * id + 1:                                               \
-        if constexpr (!fo) return launch_bf16_d<bm, bn, wgm, wgn, true, dir>(L); \
+        if constexpr (!fo) return launch_bf16_d<bm, bn, wgm, wgn, true, dir, ks>(L); \
         return hipErrorInvalidValue;
         TBLIS_PANEL_BF16_TILES(X)
 #undef X

@@ -244,7 +244,8 @@ struct TileModel
 const TileModel kModelsF64[] = {{0, 0.85, 2}, {2, 0.70, 2}, {3, 0.82, 2}, {4, 0.80, 2},
                                 {1, 0.62, 6}, {5, 0.72, 6}, {6, 0.62, 4}, {7, 0.62, 4},
                                 {8, 0.50, 4}, {9, 0.88, 1}, {10, 0.55, 2}, {11, 0.80, 1}};
-const TileModel kModelsBf16[] = {{0, 0.55, 1}, {1, 0.55, 1}, {2, 0.45, 2}, {3, 0.35, 4}};
+const TileModel kModelsBf16[] = {{0, 0.55, 1}, {1, 0.55, 1}, {2, 0.45, 2},
+                                 {3, 0.35, 4}, {4, 0.30, 4}, {5, 0.25, 4}};
 const TileModel kModelsOzaki[] = {{0, 0.30, 1}};
 
 // Estimated run time of the GEMM tasks for a tile.
@@ -512,8 +513,9 @@ bool plan_panel_tables(std::vector<PanelGemm> gemms, PanelTables& T, int family,
     int tile = 0;
     double best = -1;
     const TileModel* models = family == FAMILY_F64 ? kModelsF64 : family == FAMILY_OZAKI ? kModelsOzaki : kModelsBf16;
-    const int nmodels = family == FAMILY_F64 ? (int)(sizeof(kModelsF64) / sizeof(kModelsF64[0]))
-                        : family == FAMILY_OZAKI ? 1 : 4;
+    const int nmodels = family == FAMILY_F64     ? (int)(sizeof(kModelsF64) / sizeof(kModelsF64[0]))
+                        : family == FAMILY_OZAKI ? 1
+                                                 : (int)(sizeof(kModelsBf16) / sizeof(kModelsBf16[0]));
     std::vector<int> cands;
     for (int mi = 0; mi < nmodels; mi++)
     {

@@ -136,11 +136,12 @@ def test_float_dpd_and_batch():
 
 
 @pytest.mark.parametrize("config", ["bf16x3", "bf16x3_fast"])
-@pytest.mark.parametrize("tile", range(3))
+@pytest.mark.parametrize("tile", range(6))
 def test_every_bf16_panel_tile(tile, config, monkeypatch):
-    """Each tile of the split-BF16 panel GEMM, chunked and single
-    accumulator: ragged dense cases, scattered k, DPD (merged irrep GEMMs
-    with C panels) and a batch."""
+    """Each tile of the split-BF16 panel GEMM (tiles 4 and 5 split k
+    between two and four sets of waves), chunked and single accumulator:
+    ragged dense cases, scattered k (incl. k longer than one 256-k
+    accumulation chunk per set), DPD (merged irrep GEMMs with C panels)."""
     monkeypatch.setenv("TBLIS_HIP_PANEL_TILE", str(tile))
     T.lib().tblis_hip_clear_plan_cache()
     try:
@@ -149,6 +150,8 @@ def test_every_bf16_panel_tile(tile, config, monkeypatch):
                     uniform, alpha=0.5, beta=1.0, config=config)
         _float_case(rng, "abef", "efij", "abij", dict(a=37, b=3, e=70, f=1, i=29, j=2),
                     ["row", "col", "random"], outlier_normal, config=config)
+        _float_case(rng, "aek", "kbe", "ab", dict(a=70, b=45, e=3, k=700), ["col", "random", "col"],
+                    uniform, alpha=-0.75, beta=0.25, config=config)
         ext = {c: [4, 3, 2, 1, 5, 3, 2, 1] for c in "abefij"}
         _dpd_case(rng, 8, "abef", "efij", "abij", ext, 0, 0, 5, alpha=0.5, beta=1.0,
                   dtype=np.float32, tol=SPLIT_TOL, config=config)

@@ -1275,13 +1275,22 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bo
     auto it = db.best.find(h ^ (uint64_t)family);
     if (it != db.best.end()) return it->second;
 
-    // without the tensors (or with direct reads off) the candidates run
-    // from packed panels, which the LDS-staged tiles never read
+    // Without the tensors (or with direct reads off) the candidates run
+    // from packed panels, which the LDS-staged tiles never read.
     // With the tensors, a tile whose plan reads an operand straight from
     // its tensor is timed a second time with every operand packed: for
     // short GEMMs the pack can win (dense 24^4: 26.2 against 28.1 us per
     // call on the four-way k-split tile).  Plans under DirectAllScope keep
-    // their direct reads (one GEMM launch).
+    // their direct reads (one GEMM launch), and so do GEMMs whose panels
+    // would take more than a quarter of the free device memory (the
+    // measurement would not fit, and neither would the plan).
+    const int eb = geom_of(family).elem_bytes;
+    size_t free_bytes = 0, total_bytes = 0;
+    if (hipMemGetInfo(&free_bytes, &total_bytes) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        free_bytes = 0;
+    }
     std::vector<int> cands;  // tile, + kPackedBit: every operand packed
     std::vector<PanelTables> tabs;
     for (int c : cands0)
@@ -1301,14 +1310,16 @@ int autotune_tile(const std::vector<PanelGemm>& gemms, int family, int dtype, bo
         {
             PanelTables T2;
             if (!tables_uniform(gemms, family, c, T2, dtype == 0 ? 8 : 4, true)) return -1;
-            cands.push_back(c + kPackedBit);
-            tabs.push_back(std::move(T2));
+            if ((size_t)T2.ws_panel * eb <= free_bytes / 4)
+            {
+                cands.push_back(c + kPackedBit);
+                tabs.push_back(std::move(T2));
+            }
         }
     }
     if (cands.size() < 2) return -1;
     auto tile_of = [](int code) { return code % kPackedBit; };
     size_t panel_bytes = 256, cslot = 0, ntask = 0;
-    const int eb = geom_of(family).elem_bytes;
     for (size_t i = 0; i < cands.size(); i++)
     {
         const int ti = tile_of(cands[i]);

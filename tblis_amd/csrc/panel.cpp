@@ -365,15 +365,22 @@ int direct_kfast_rereads()
     return v;
 }
 
-// Fragment groups per narrow pack chunk (TBLIS_HIP_PACK_NARROW overrides,
-// read at every plan build).
-int64_t narrow_pack_groups(int family)
+// Fragment groups per narrow pack chunk set by TBLIS_HIP_PACK_NARROW (read
+// at every plan build), or 0.
+int64_t narrow_pack_forced()
 {
     if (const char* e = std::getenv("TBLIS_HIP_PACK_NARROW"))
     {
         const int v = std::atoi(e);
         if (v >= 1 && v <= 64) return v;
     }
+    return 0;
+}
+
+// Fragment groups per narrow pack chunk.
+int64_t narrow_pack_groups(int family)
+{
+    if (const int64_t v = narrow_pack_forced()) return v;
     return family == FAMILY_BF16X3 || family == FAMILY_BF16X3_FAST ? 8 : 32;
 }
 
@@ -722,11 +729,10 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
     // it they only lengthen each wave's work (indexed 48 workload, split-BF16:
     // 81 TFLOP/s against 95 with narrow chunks).  A 64-group budget (two
     // batches per wave, 1 792 DPD chunks) took 30.7 us.
-    auto chunk_job = [&](int32_t id, bool wide) {
+    auto chunk_job = [&](int32_t id, bool wide, int64_t budget) {
         const PanelJob& j = jobs[id];
         const int64_t rb0 = j.r0 / G.gr, rb1 = ceil_div(j.r0 + j.nr, G.gr);
         const int64_t kg0 = j.k0 / G.gk, kg1 = ceil_div(j.k0 + j.nk, G.gk);
-        const int64_t budget = narrow_pack_groups(family);
         int64_t per = std::min<int64_t>(8, budget), rstep = std::max<int64_t>(1, budget / per);
         if (wide)
         {
@@ -750,7 +756,7 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
         const int32_t id = (int32_t)jobs.size();
         jobs.push_back(j);
         job_stage.push_back(cur_stage);
-        chunk_job(id, false);
+        chunk_job(id, false, narrow_pack_groups(family));
     };
 
     for (size_t gidx = 0; gidx < gemms.size(); gidx++)
@@ -1061,10 +1067,34 @@ bool tables_for_tile(const std::vector<PanelGemm>& gemms, int family, int tile, 
     }
     if (tasks.empty()) return false;
     const int forced = pack_chunks_forced();
-    if (forced > 0 || (forced == 0 && chunks.size() > (size_t)8 * device_cu_count()))
+    const size_t slots = (size_t)8 * device_cu_count();  // four-wave pack workgroups resident at once
+    if (forced == 0 && chunks.size() <= slots && !narrow_pack_forced() &&
+        (family == FAMILY_F64 || family == FAMILY_BF16X3 || family == FAMILY_BF16X3_FAST))
+    {
+        // The pack runs as one round of workgroups: split it into smaller
+        // chunks while they still fit in one round, so that more waves share
+        // the memory latency (8-irrep DPD FP64: 1 288 chunks of up to 32
+        // groups 0.0627-0.0637 ms per call, about 1 700 of up to 24 groups
+        // 0.0600-0.0609, same box).  FP64 chunks keep at least 8 groups and
+        // split-BF16 chunks (groups four times larger) at least 4.
+        const int64_t floor = family == FAMILY_F64 ? 8 : 4;
+        for (const int64_t b : {24, 16, 12, 8, 6, 4})
+        {
+            if (b >= narrow_pack_groups(family) || b < floor) continue;
+            std::vector<PanelChunk> kept;
+            kept.swap(chunks);
+            for (size_t id = 0; id < jobs.size(); id++) chunk_job((int32_t)id, false, b);
+            if (chunks.size() > slots)
+            {
+                chunks.swap(kept);
+                break;
+            }
+        }
+    }
+    if (forced > 0 || (forced == 0 && chunks.size() > slots))
     {
         chunks.clear();
-        for (size_t id = 0; id < jobs.size(); id++) chunk_job((int32_t)id, true);
+        for (size_t id = 0; id < jobs.size(); id++) chunk_job((int32_t)id, true, narrow_pack_groups(family));
     }
     if (tasks.size() > (size_t)INT32_MAX || chunks.size() > (size_t)INT32_MAX ||
         uchunks.size() > (size_t)INT32_MAX)

@@ -88,28 +88,37 @@ def time_steps(fn, steps, warmup, stream):
     same region (one event pair on the library's stream around all steps:
     an event record between calls costs ~8 us of GPU time per call, 30% of a
     dense 24^4 call); and the fastest single call of a short run after the
-    timed region, with events around each call."""
+    timed region, with events around each call.  Python's garbage collector
+    is off while the calls run, as in timeit: a collection inside the loop
+    (the indexed calls build their entry tables as Python objects) stalled
+    one call for up to 38 ms with the GPU idle, 0.82 ms per call over 50."""
+    import gc
     import torch
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    start.record(stream)
-    for _ in range(steps):
-        fn()
-    end.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    region = start.elapsed_time(end) / steps
-    n = min(steps, 10)
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
-    for i in range(n):
-        starts[i].record(stream)
-        fn()
-        ends[i].record(stream)
-    torch.cuda.synchronize()
+    gc.collect()
+    gc.disable()
+    try:
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        start.record(stream)
+        for _ in range(steps):
+            fn()
+        end.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        region = start.elapsed_time(end) / steps
+        n = min(steps, 10)
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+        for i in range(n):
+            starts[i].record(stream)
+            fn()
+            ends[i].record(stream)
+        torch.cuda.synchronize()
+    finally:
+        gc.enable()
     single = min(s.elapsed_time(e) for s, e in zip(starts, ends))
     return wall / steps * 1e3, float(region), float(single)
 

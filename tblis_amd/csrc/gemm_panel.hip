@@ -354,11 +354,14 @@ __device__ __forceinline__ void panel_task(const PanelTask& t, const int64_t* __
                 for (int j = 0; j < NB; j++) b[j] = pb[j * rs + kg * 64];
         };
         // k groups in flight: five for waves of at most 32x32 (their MFMA
-        // work per group is short; 126 registers, still four waves per SIMD;
-        // 1-2% faster than four on the permuted workload), three for 64x64
-        // waves (as many as fit in the registers left next to the
-        // accumulators at two waves per SIMD); 1-18% faster than two
-        constexpr int PF = MB * NB <= 4 ? 5 : 3;
+        // work per group is short; 1-2% faster than four on the permuted
+        // workload; 160 registers, three waves per SIMD), three for 32x32
+        // waves with k split four ways (128 registers, four waves per SIMD:
+        // their short per-set k runs gain more from resident waves, 8-irrep
+        // DPD 0.0630 -> 0.0609 ms per call) and for 64x64 waves (as many as
+        // fit in the registers left next to the accumulators at two waves
+        // per SIMD; 1-18% faster than two)
+        constexpr int PF = MB * NB <= 4 && KS < 4 ? 5 : 3;
         double2 a[PF][MB], b[PF][NB];
 #pragma unroll
         for (int d = 0; d < PF; d++) load(a[d], b[d], d < nk ? d : nk - 1);

@@ -767,6 +767,10 @@ def main():
     T.set_stream(stream.cuda_stream)
     T.set_async(True)
     T.set_autotune(not args.no_autotune)
+    # legs of long calls (milliseconds each) time a tenth of the steps; legs
+    # of short calls time all of them, since a handful of sub-0.1 ms calls
+    # right after another leg measures the clocks ramping up (indexed DPD:
+    # 0.086 ms per call over 20 calls, 0.035 over 200)
     few = max(3, args.steps // 10)
 
     head, lens = None, None
@@ -790,16 +794,13 @@ def main():
                                       min(args.warmup, 2))
         torch.cuda.empty_cache()
     if args.only in (None, "indexed"):
-        legs["indexed"] = indexed_leg(T, dev, stream, few if args.only is None else args.steps,
-                                      min(args.warmup, 2))
+        legs["indexed"] = indexed_leg(T, dev, stream, args.steps, args.warmup)
         torch.cuda.empty_cache()
     if args.only in (None, "indexed_dpd"):
-        legs["indexed_dpd"] = indexed_dpd_leg(T, dev, stream, few if args.only is None else args.steps,
-                                              min(args.warmup, 2))
+        legs["indexed_dpd"] = indexed_dpd_leg(T, dev, stream, args.steps, args.warmup)
         torch.cuda.empty_cache()
     if args.only in (None, "level1"):
-        legs["level1"] = level1_leg(T, dev, stream, few if args.only is None else args.steps,
-                                    min(args.warmup, 2))
+        legs["level1"] = level1_leg(T, dev, stream, args.steps, args.warmup)
         torch.cuda.empty_cache()
     T.set_async(False)
     if args.calibrate_copy:

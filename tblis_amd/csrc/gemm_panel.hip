@@ -353,15 +353,17 @@ __device__ __forceinline__ void panel_task(const PanelTask& t, const int64_t* __
 #pragma unroll
                 for (int j = 0; j < NB; j++) b[j] = pb[j * rs + kg * 64];
         };
-        // k groups in flight: five for waves of at most 32x32 (their MFMA
-        // work per group is short; 1-2% faster than four on the permuted
-        // workload; 160 registers, three waves per SIMD), three for 32x32
-        // waves with k split four ways (128 registers, four waves per SIMD:
-        // their short per-set k runs gain more from resident waves, 8-irrep
-        // DPD 0.0630 -> 0.0609 ms per call) and for 64x64 waves (as many as
-        // fit in the registers left next to the accumulators at two waves
-        // per SIMD; 1-18% faster than two)
-        constexpr int PF = MB * NB <= 4 && KS < 4 ? 5 : 3;
+        // k groups in flight: five for one-wave 32x32 tiles with one or two
+        // k sets (their MFMA work per group is short; 160 registers, three
+        // waves per SIMD; three in flight ties on the permuted workload),
+        // three otherwise: 32x32 waves in four-wave tiles or with k split
+        // four ways then hold 128 registers and four waves per SIMD, which
+        // gain more from resident waves (permuted workload on the 2x2-wave
+        // 64x64 tile 49.4-49.9 -> 53.3-53.7 TFLOP/s, 8-irrep DPD on the
+        // four-set tile 0.0630 -> 0.0609 ms per call), and 64x64 waves hold
+        // as many as fit next to their accumulators at two waves per SIMD
+        // (1-18% faster than two)
+        constexpr int PF = MB * NB <= 4 && WGM * WGN == 1 && KS < 4 ? 5 : 3;
         double2 a[PF][MB], b[PF][NB];
 #pragma unroll
         for (int d = 0; d < PF; d++) load(a[d], b[d], d < nk ? d : nk - 1);
